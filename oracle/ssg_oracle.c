@@ -1,0 +1,1026 @@
+/*
+ * ssg_oracle.c -- serial CPU restatement of SimilaritySearch.jl's query hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libssg_hip.so.  Only tests/, the
+ * smoke check and the benchmark's CPU-baseline leg load this library; the
+ * product never does.  Built with -ffp-contract=off so that every FP64
+ * operation below is exactly the one written.
+ *
+ * Parity status of each function (how it is pinned) is listed at its
+ * definition.  A function whose behaviour no known answer, invariant or brute
+ * force pins says "parity unpinned" there.
+ */
+#include "ssg_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Canonical distance.                                                       */
+/* Follows src/dist/vecs.jl:93-102 (SqL2), :59-67 (L2), src/dist/cos.jl:78-85 */
+/* (NormCosine).  The reference sums in FP32 in a CPU-width-dependent order  */
+/* (@fastmath @simd); the contract here fixes one order: the padded dim      */
+/* Dp = roundup(dim, 16) is cut into 16 contiguous runs of Dp/16 elements,   */
+/* each run is accumulated in FP64 with fma in increasing index, and the 16  */
+/* partial sums are combined by the xor-butterfly 8,4,2,1 (the GPU's 16-lane */
+/* reduction).  The FP64 result is rounded once to Float32.                  */
+/* Pinned by: closed forms of test/testdistances.jl:34-76 and an FP64        */
+/* recompute (tests/test_oracle_distances.py).                               */
+/* ------------------------------------------------------------------------ */
+static double tree16(int metric, const float* a, const float* b, int dim)
+{
+    const int dp = (dim + 15) & ~15;
+    const int v = dp / 16;
+    double p[16], t[16];
+    for (int j = 0; j < 16; j++) {
+        double s = 0.0;
+        for (int u = 0; u < v; u++) {
+            const int i = j * v + u;
+            const double x = i < dim ? (double)a[i] : 0.0;
+            const double y = i < dim ? (double)b[i] : 0.0;
+            if (metric == OR_NORMCOS) {
+                s = fma(x, y, s);
+            } else {
+                const double m = x - y;
+                s = fma(m, m, s);
+            }
+        }
+        p[j] = s;
+    }
+    for (int w = 8; w >= 1; w >>= 1) {
+        for (int j = 0; j < 16; j++) t[j] = p[j] + p[j ^ w];
+        memcpy(p, t, sizeof(p));
+    }
+    return p[0];
+}
+
+static inline float finish(int metric, double s)
+{
+    if (metric == OR_L2) return (float)sqrt(s);
+    if (metric == OR_NORMCOS) return (float)(1.0 - s);
+    return (float)s;
+}
+
+float or_dist(int metric, const float* a, const float* b, int dim)
+{
+    return finish(metric, tree16(metric, a, b, dim));
+}
+
+double or_dist_f64(int metric, const float* a, const float* b, int dim)
+{
+    double s = 0.0;
+    for (int i = 0; i < dim; i++) {
+        if (metric == OR_NORMCOS) {
+            s += (double)a[i] * (double)b[i];
+        } else {
+            const double m = (double)a[i] - (double)b[i];
+            s += m * m;
+        }
+    }
+    if (metric == OR_L2) return sqrt(s);
+    if (metric == OR_NORMCOS) return 1.0 - s;
+    return s;
+}
+
+void or_dist_batch(int metric, const float* X, int64_t n, int dim, const float* Q, int64_t m,
+                   float* out)
+{
+    for (int64_t j = 0; j < m; j++)
+        for (int64_t i = 0; i < n; i++)
+            out[j * n + i] = or_dist(metric, X + i * dim, Q + j * dim, dim);
+}
+
+/* ------------------------------------------------------------------------ */
+/* KnnSorted, restating src/pqueue/knnsorted.jl:31-233 with 0-based slots.   */
+/* Active range is [sp, ep]; empty when ep < sp.                             */
+/* Pinned by: test/testresults.jl:57-85 (sort-based gold under random        */
+/* pushes) plus stable-tie and full-queue rejection cases                    */
+/* (tests/test_oracle_knn.py).                                               */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint32_t* ids;
+    float* d;
+    int32_t sp, ep, maxlen, buflen;
+} knn_t;
+
+static inline void knn_init(knn_t* r, uint32_t* ids, float* d, int32_t cap)
+{
+    r->ids = ids;
+    r->d = d;
+    r->sp = 0;
+    r->ep = -1;
+    r->maxlen = cap;
+    r->buflen = cap;
+}
+
+static inline int32_t knn_len(const knn_t* r) { return r->ep - r->sp + 1; }
+static inline float knn_max(const knn_t* r) { return r->d[r->ep]; }
+
+/* sort_last_item! knnsorted.jl:68-101: binary search for the first slot whose
+ * distance is > the new one (equal distances stay in front), then shift. */
+static void sort_last_item(uint32_t* ids, float* d, int32_t sp, int32_t ep)
+{
+    if (sp == ep) return;
+    const uint32_t id = ids[ep];
+    const float x = d[ep];
+    if (d[ep - 1] <= x) return;
+    int32_t lo = sp, hi = ep - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (d[mid] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    for (int32_t i = ep; i > lo; i--) {
+        ids[i] = ids[i - 1];
+        d[i] = d[i - 1];
+    }
+    ids[lo] = id;
+    d[lo] = x;
+}
+
+/* push_item! knnsorted.jl:140-168 */
+static int knn_push(knn_t* r, uint32_t id, float x)
+{
+    const int32_t len = knn_len(r);
+    if (len < r->maxlen) {
+        if (r->ep == r->buflen - 1) { /* compaction, knnsorted.jl:145-153 */
+            for (int32_t i = 0; i < len; i++) {
+                r->ids[i] = r->ids[r->sp + i];
+                r->d[i] = r->d[r->sp + i];
+            }
+            r->sp = 0;
+            r->ep = len - 1;
+        }
+        r->ep++;
+        r->ids[r->ep] = id;
+        r->d[r->ep] = x;
+        sort_last_item(r->ids, r->d, r->sp, r->ep);
+        return 1;
+    }
+    if (x >= knn_max(r)) return 0;
+    r->ids[r->ep] = id;
+    r->d[r->ep] = x;
+    sort_last_item(r->ids, r->d, r->sp, r->ep);
+    return 1;
+}
+
+/* pop_min! knnsorted.jl:191-196 */
+static inline void knn_popmin(knn_t* r, uint32_t* id, float* x)
+{
+    *id = r->ids[r->sp];
+    *x = r->d[r->sp];
+    r->sp++;
+}
+
+int or_knn_run(int32_t cap, int64_t nops, const int8_t* ops, const uint32_t* ids_in,
+               const float* d_in, uint8_t* accepted, uint32_t* ids_out, float* d_out,
+               int32_t* len_out)
+{
+    uint32_t* ib = (uint32_t*)calloc((size_t)cap + 1, sizeof(uint32_t));
+    float* db = (float*)calloc((size_t)cap + 1, sizeof(float));
+    knn_t r;
+    knn_init(&r, ib, db, cap);
+    for (int64_t i = 0; i < nops; i++) {
+        if (ops[i] == 0) {
+            accepted[i] = (uint8_t)knn_push(&r, ids_in[i], d_in[i]);
+        } else {
+            uint32_t id;
+            float x;
+            if (knn_len(&r) <= 0) {
+                free(ib);
+                free(db);
+                return -1;
+            }
+            knn_popmin(&r, &id, &x);
+            accepted[i] = 1;
+        }
+    }
+    const int32_t len = knn_len(&r);
+    for (int32_t i = 0; i < len; i++) {
+        ids_out[i] = r.ids[r.sp + i];
+        d_out[i] = r.d[r.sp + i];
+    }
+    *len_out = len;
+    free(ib);
+    free(db);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ExhaustiveSearch.search, src/exact/sequential-exhaustive.jl:68-80:        */
+/* ascending scan i = 1..n, d = evaluate(dist, db[i], q), push_item!.        */
+/* Pinned by: brute-force sort on tiny inputs, self-query (testexactseq.jl:  */
+/* 27-29) and parallel == sequential (testparallelexhaustive.jl:21-64).      */
+/* ------------------------------------------------------------------------ */
+static void exhaustive_one(int metric, const float* X, int64_t n, int dim, const float* q,
+                           int32_t k, uint32_t* ids, float* dists)
+{
+    knn_t r;
+    for (int32_t i = 0; i < k; i++) {
+        ids[i] = 0;
+        dists[i] = INFINITY;
+    }
+    knn_init(&r, ids, dists, k);
+    for (int64_t i = 0; i < n; i++) knn_push(&r, (uint32_t)(i + 1), or_dist(metric, X + i * dim, q, dim));
+}
+
+void or_exhaustive(int metric, const float* X, int64_t n, int dim, const float* Q, int64_t m,
+                   int32_t k, uint32_t* ids, float* dists, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+#endif
+    for (int64_t j = 0; j < m; j++)
+        exhaustive_one(metric, X, n, dim, Q + j * dim, k, ids + j * k, dists + j * k);
+    (void)nthreads;
+}
+
+/* ParallelExhaustiveSearch.search, src/exact/parallel-exhaustive.jl:64-102:
+ * contiguous chunks of minbatch = ceil(n / nbatches) rows, one private
+ * KnnSorted per chunk, merged in chunk order (@END block :95-97). */
+void or_parallel_exhaustive(int metric, const float* X, int64_t n, int dim, const float* Q,
+                            int64_t m, int32_t k, int32_t nbatches, uint32_t* ids, float* dists)
+{
+    if (nbatches < 1) nbatches = 1;
+    const int64_t mb = (n + nbatches - 1) / nbatches;
+    const int64_t nb = (n + mb - 1) / mb;
+    uint32_t* rid = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)k * (size_t)nb);
+    float* rd = (float*)malloc(sizeof(float) * (size_t)k * (size_t)nb);
+    int32_t* used = (int32_t*)malloc(sizeof(int32_t) * (size_t)nb);
+    for (int64_t j = 0; j < m; j++) {
+        const float* q = Q + j * dim;
+        for (int64_t b = 0; b < nb; b++) {
+            knn_t r;
+            knn_init(&r, rid + b * k, rd + b * k, k);
+            const int64_t lo = b * mb, hi = (lo + mb < n) ? lo + mb : n;
+            for (int64_t i = lo; i < hi; i++)
+                knn_push(&r, (uint32_t)(i + 1), or_dist(metric, X + i * dim, q, dim));
+            /* KnnSorted never pops here, so sp stays 0 */
+            used[b] = knn_len(&r);
+        }
+        knn_t res;
+        for (int32_t i = 0; i < k; i++) {
+            ids[j * k + i] = 0;
+            dists[j * k + i] = INFINITY;
+        }
+        knn_init(&res, ids + j * k, dists + j * k, k);
+        for (int64_t b = 0; b < nb; b++)
+            for (int32_t t = 0; t < used[b]; t++) knn_push(&res, rid[b * k + t], rd[b * k + t]);
+    }
+    free(rid);
+    free(rd);
+    free(used);
+}
+
+/* ------------------------------------------------------------------------ */
+/* BeamSearch, src/searchgraph/beamsearch.jl:12-110 with the beam capped by  */
+/* getbeam (context.jl:220-225) and the UInt64 visited bitmap of             */
+/* visitedvertices.jl:8-43.  The graph is the StaticAdjList CSR              */
+/* (adjstatic.jl:27-30,102-120) with an explicit offs[0] = 0.                */
+/* Pinned by: n < 64 equals brute force exactly (beamsearch.jl:94-101), and  */
+/* hand-traced small graphs with known visit order (tests/test_oracle_beam). */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int metric;
+    const float* X;
+    int64_t n; /* length(index) */
+    int dim;
+    const int64_t* offs;
+    const uint32_t* adj; /* 1-based ids */
+} gidx_t;
+
+static inline int visit_test_set(uint64_t* vs, uint32_t id1)
+{
+    const uint64_t i = (uint64_t)id1 - 1;
+    const uint64_t w = i >> 6, b = i & 63;
+    const int v = (int)((vs[w] >> b) & 1);
+    if (!v) vs[w] |= (1ull << b);
+    return v;
+}
+
+static inline int visited_only(const uint64_t* vs, uint32_t id1)
+{
+    const uint64_t i = (uint64_t)id1 - 1;
+    return (int)((vs[i >> 6] >> (i & 63)) & 1);
+}
+
+/* enqueue_item! beamsearch.jl:12-18 */
+static inline void enqueue_item(const gidx_t* g, const float* q, knn_t* res, uint32_t id1,
+                                uint64_t* vs, int64_t* cd)
+{
+    if (visit_test_set(vs, id1)) return;
+    const float d = or_dist(g->metric, q, g->X + (int64_t)(id1 - 1) * g->dim, g->dim);
+    knn_push(res, id1, d);
+    (*cd)++;
+}
+
+/* search(bs::BeamSearch, ...) beamsearch.jl:88-110 */
+static void beam_search(const gidx_t* g, const float* q, knn_t* res, const uint32_t* hints,
+                        int64_t nh, int32_t bsize, float delta, int64_t maxvisits,
+                        int32_t beamcap, uint64_t* vs, uint32_t* bids, float* bd, int64_t* cd,
+                        int64_t* cb)
+{
+    const int64_t n = g->n;
+    if (n == 0) return;
+    if (n < 64) {
+        for (int64_t i = 0; i < n; i++)
+            knn_push(res, (uint32_t)(i + 1), or_dist(g->metric, q, g->X + i * g->dim, g->dim));
+        *cd += n;
+        return;
+    }
+    /* beamsearch_init :26-34 */
+    for (int64_t h = 0; h < nh; h++) enqueue_item(g, q, res, hints[h], vs, cd);
+    if (knn_len(res) == 0) {
+        const int64_t L = (int64_t)ceil(log((double)(1 + n)) / log(2.0));
+        for (int64_t id = 1; id <= L; id++) enqueue_item(g, q, res, (uint32_t)id, vs, cd);
+    }
+    /* beamsearch_inner_beam :44-73 */
+    const int32_t nsize = bsize < beamcap ? bsize : beamcap;
+    knn_t beam;
+    knn_init(&beam, bids, bd, nsize);
+    knn_push(&beam, res->ids[res->sp], res->d[res->sp]);
+    int64_t costdists = 0, costblocks = 0;
+    while (knn_len(&beam) > 0) {
+        uint32_t pid;
+        float pd;
+        knn_popmin(&beam, &pid, &pd);
+        const int64_t s = g->offs[pid - 1], e = g->offs[pid];
+        costblocks++;
+        for (int64_t t = s; t < e; t++) {
+            const uint32_t child = g->adj[t];
+            if (visit_test_set(vs, child)) continue;
+            const float d = or_dist(g->metric, q, g->X + (int64_t)(child - 1) * g->dim, g->dim);
+            knn_push(res, child, d);
+            costdists++;
+            if (costdists > maxvisits) goto finish;
+            const int64_t deg = g->offs[child] - g->offs[child - 1];
+            const float thr = delta * knn_max(res);
+            if (deg > 1 && d <= thr) knn_push(&beam, child, d);
+        }
+    }
+finish:
+    *cd += costdists;
+    *cb += costblocks;
+}
+
+void or_beamsearch(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
+                   const uint32_t* adj, const uint32_t* hints, int64_t nhints,
+                   const uint32_t* qhints, const int32_t* qh_count, int32_t qh_stride,
+                   const float* Q, int64_t m, int32_t k, const int32_t* kq, int32_t kstride,
+                   int32_t bsize, const int32_t* bq, float delta, int64_t maxvisits,
+                   int32_t beamcap, uint32_t* ids, float* dists, int64_t* costdists,
+                   int64_t* costblocks, int nthreads)
+{
+    gidx_t g = {metric, X, n, dim, offs, adj};
+    const int64_t nw = (n + 63) / 64 + 1;
+    if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        uint64_t* vs = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)nw);
+        uint32_t* bids = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(beamcap + 1));
+        float* bd = (float*)malloc(sizeof(float) * (size_t)(beamcap + 1));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 8)
+#endif
+        for (int64_t j = 0; j < m; j++) {
+            const int32_t kk = kq ? kq[j] : k;
+            const int32_t bs = bq ? bq[j] : bsize;
+            uint32_t* oid = ids + j * kstride;
+            float* od = dists + j * kstride;
+            for (int32_t i = 0; i < kstride; i++) {
+                oid[i] = 0;
+                od[i] = INFINITY;
+            }
+            memset(vs, 0, sizeof(uint64_t) * (size_t)nw); /* getvstate, context.jl:209-211 */
+            knn_t res;
+            knn_init(&res, oid, od, kk);
+            int64_t cd = 0, cb = 0;
+            const uint32_t* h = hints;
+            int64_t nh = nhints;
+            if (qhints) {
+                h = qhints + j * (int64_t)qh_stride;
+                nh = qh_count[j];
+            }
+            beam_search(&g, Q + j * dim, &res, h, nh, bs, delta, maxvisits, beamcap, vs, bids, bd,
+                        &cd, &cb);
+            if (costdists) costdists[j] = cd;
+            if (costblocks) costblocks[j] = cb;
+        }
+        free(vs);
+        free(bids);
+        free(bd);
+    }
+}
+
+/* allknn_single_search!(g::SearchGraph, ...) src/allknn.jl:93-106: each
+ * neighbour h of i seeds one search; all of them share one visited set. */
+void or_allknn_graph(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
+                     const uint32_t* adj, int32_t k, int32_t bsize, float delta,
+                     int64_t maxvisits, int32_t beamcap, uint32_t* ids, float* dists,
+                     int nthreads)
+{
+    gidx_t g = {metric, X, n, dim, offs, adj};
+    const int64_t nw = (n + 63) / 64 + 1;
+    if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        uint64_t* vs = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)nw);
+        uint32_t* bids = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(beamcap + 1));
+        float* bd = (float*)malloc(sizeof(float) * (size_t)(beamcap + 1));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 8)
+#endif
+        for (int64_t j = 0; j < n; j++) {
+            uint32_t* oid = ids + j * k;
+            float* od = dists + j * k;
+            for (int32_t i = 0; i < k; i++) {
+                oid[i] = 0;
+                od[i] = INFINITY;
+            }
+            memset(vs, 0, sizeof(uint64_t) * (size_t)nw);
+            knn_t res;
+            knn_init(&res, oid, od, k);
+            int64_t cd = 0, cb = 0;
+            for (int64_t t = offs[j]; t < offs[j + 1]; t++) {
+                uint32_t h = adj[t];
+                if (visited_only(vs, h)) continue;
+                beam_search(&g, X + j * dim, &res, &h, 1, bsize, delta, maxvisits, beamcap, vs,
+                            bids, bd, &cd, &cb);
+            }
+        }
+        free(vs);
+        free(bids);
+        free(bd);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* rerank! src/rerank.jl:22-41 (vector form) applied per column (:77-86):    */
+/* rescore the prefix before the first 0 id, then a stable sort by distance  */
+/* (Julia's default sortperm is stable).                                     */
+/* Pinned by: tests compare with numpy's stable argsort of FP64 recomputes.  */
+/* ------------------------------------------------------------------------ */
+void or_rerank(int metric, const float* X, int dim, const float* Q, int64_t m, int32_t k,
+               uint32_t* ids, float* dists)
+{
+    uint32_t* ti = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)k);
+    float* td = (float*)malloc(sizeof(float) * (size_t)k);
+    for (int64_t j = 0; j < m; j++) {
+        uint32_t* ci = ids + j * k;
+        float* cd = dists + j * k;
+        int32_t len = 0;
+        for (int32_t i = 0; i < k; i++) {
+            if (ci[i] == 0) break;
+            len = i + 1;
+            cd[i] = or_dist(metric, X + (int64_t)(ci[i] - 1) * dim, Q + j * dim, dim);
+        }
+        /* stable insertion sort */
+        for (int32_t i = 0; i < len; i++) {
+            ti[i] = ci[i];
+            td[i] = cd[i];
+        }
+        for (int32_t i = 1; i < len; i++) {
+            const uint32_t a = ti[i];
+            const float x = td[i];
+            int32_t p = i - 1;
+            while (p >= 0 && td[p] > x) {
+                ti[p + 1] = ti[p];
+                td[p + 1] = td[p];
+                p--;
+            }
+            ti[p + 1] = a;
+            td[p + 1] = x;
+        }
+        for (int32_t i = 0; i < len; i++) {
+            ci[i] = ti[i];
+            cd[i] = td[i];
+        }
+    }
+    free(ti);
+    free(td);
+}
+
+/* ------------------------------------------------------------------------ */
+/* macrorecall(goldI, resI, k) src/perf.jl:31-33,71-79: mean over columns of */
+/* |Set(gold[1:k]) ∩ Set(res[1:k])| / k (0 ids take part like any id).       */
+/* Pinned by: identical inputs give 1.0; disjoint give 0; numpy set algebra. */
+/* ------------------------------------------------------------------------ */
+static int cmp_u32(const void* a, const void* b)
+{
+    const uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    return x < y ? -1 : (x > y);
+}
+
+double or_macrorecall(const uint32_t* gold, int32_t kg, const uint32_t* res, int32_t kr,
+                      int64_t m, int32_t k)
+{
+    uint32_t* a = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)k);
+    uint32_t* b = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)k);
+    double s = 0.0;
+    for (int64_t j = 0; j < m; j++) {
+        memcpy(a, gold + j * kg, sizeof(uint32_t) * (size_t)k);
+        memcpy(b, res + j * kr, sizeof(uint32_t) * (size_t)k);
+        qsort(a, (size_t)k, sizeof(uint32_t), cmp_u32);
+        qsort(b, (size_t)k, sizeof(uint32_t), cmp_u32);
+        int32_t na = 0, nb = 0;
+        for (int32_t i = 0; i < k; i++)
+            if (i == 0 || a[i] != a[i - 1]) a[na++] = a[i];
+        for (int32_t i = 0; i < k; i++)
+            if (i == 0 || b[i] != b[i - 1]) b[nb++] = b[i];
+        int32_t x = 0, y = 0, c = 0;
+        while (x < na && y < nb) {
+            if (a[x] == b[y]) {
+                c++;
+                x++;
+                y++;
+            } else if (a[x] < b[y]) {
+                x++;
+            } else {
+                y++;
+            }
+        }
+        s += (double)c / (double)k;
+    }
+    free(a);
+    free(b);
+    return m > 0 ? s / (double)m : 0.0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Deterministic RNG for the randomised callbacks (splitmix64).  Julia's     */
+/* Xoshiro streams cannot be reproduced outside Julia, so draws never match  */
+/* the reference; only the algorithms around them do.                        */
+/* ------------------------------------------------------------------------ */
+static inline uint64_t sm64(uint64_t* s)
+{
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static inline int64_t rand_1n(uint64_t* s, int64_t n) { return 1 + (int64_t)(sm64(s) % (uint64_t)n); }
+
+/* fft, src/kcenters/fft.jl:53-87: k rounds of (record ε, add imax, relax all
+ * nndists against the pivot with strict <, findmax -> first maximum).
+ * Pinned by: brute-force k-centers invariants on tiny inputs (every nn is the
+ * closest chosen center, ε non-increasing, the next center is a farthest point). */
+void or_fft(int metric, const float* X, int64_t n, int dim, int32_t k, int64_t start,
+            uint64_t seed, uint32_t* centers, uint32_t* nn, float* nndists, float* eps_list)
+{
+    uint64_t s = seed;
+    for (int64_t i = 0; i < n; i++) {
+        nndists[i] = INFINITY;
+        nn[i] = 0;
+    }
+    if (n == 0) return;
+    int64_t imax = start == 0 ? rand_1n(&s, n) : start;
+    float eps = INFINITY;
+    for (int32_t c = 0; c < k; c++) {
+        eps_list[c] = eps;
+        centers[c] = (uint32_t)imax;
+        const float* piv = X + (imax - 1) * dim;
+        for (int64_t i = 0; i < n; i++) {
+            const float d = or_dist(metric, X + i * dim, piv, dim);
+            if (d < nndists[i]) {
+                nndists[i] = d;
+                nn[i] = (uint32_t)imax;
+            }
+        }
+        float best = nndists[0];
+        int64_t bi = 0;
+        for (int64_t i = 1; i < n; i++)
+            if (nndists[i] > best) {
+                best = nndists[i];
+                bi = i;
+            }
+        eps = best;
+        imax = bi + 1;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Growable adjacency (AdjList, src/adj/adjlist.jl:106-161)                  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint32_t* v;
+    int32_t len, cap;
+} alist_t;
+
+static void alist_push(alist_t* a, uint32_t x)
+{
+    if (a->len == a->cap) {
+        a->cap = a->cap ? a->cap * 2 : 8;
+        a->v = (uint32_t*)realloc(a->v, sizeof(uint32_t) * (size_t)a->cap);
+    }
+    a->v[a->len++] = x;
+}
+
+/* CSR snapshot of the first len lists (the graph as a search sees it) */
+typedef struct {
+    int64_t* offs;
+    uint32_t* adj;
+    int64_t cap_adj;
+} csr_t;
+
+static void snapshot(const alist_t* L, int64_t len, csr_t* c)
+{
+    int64_t nnz = 0;
+    for (int64_t i = 0; i < len; i++) nnz += L[i].len;
+    if (nnz > c->cap_adj) {
+        c->cap_adj = nnz + nnz / 2 + 16;
+        c->adj = (uint32_t*)realloc(c->adj, sizeof(uint32_t) * (size_t)c->cap_adj);
+    }
+    c->offs[0] = 0;
+    for (int64_t i = 0; i < len; i++) {
+        memcpy(c->adj + c->offs[i], L[i].v, sizeof(uint32_t) * (size_t)L[i].len);
+        c->offs[i + 1] = c->offs[i] + L[i].len;
+    }
+}
+
+/* RandomHints callback, src/searchgraph/hints.jl:96-120.  The inner
+ * `child ∈ V || union!(...)` of :110-112 never fires because N was just
+ * united into V, so V gains exactly {objID} ∪ N(objID). */
+static int64_t random_hints_csr(int64_t n, const int64_t* offs, const uint32_t* adj,
+                                float logbase, uint64_t* s, uint32_t* out, int64_t cap)
+{
+    const int64_t m = (int64_t)ceilf(logf((float)n) / logf(logbase));
+    /* lists may already name node n+1 (the sequential loop fires callbacks
+     * before length(index) is bumped), so V covers one extra id */
+    uint8_t* V = (uint8_t*)calloc((size_t)n + 2, 1);
+    int64_t nh = 0;
+    for (int64_t it = 0; it < n; it++) {
+        const int64_t obj = rand_1n(s, n);
+        if (V[obj]) continue;
+        const int64_t st = offs[obj - 1], en = offs[obj];
+        if (en - st <= 2) continue;
+        V[obj] = 1;
+        for (int64_t t = st; t < en; t++) V[adj[t]] = 1;
+        if (nh < cap) out[nh] = (uint32_t)obj;
+        nh++;
+        if (nh >= m) break;
+    }
+    free(V);
+    return nh < cap ? nh : cap;
+}
+
+int64_t or_random_hints(int64_t n, const int64_t* offs, const uint32_t* adj, float logbase,
+                        uint64_t seed, uint32_t* hints_out, int64_t cap)
+{
+    uint64_t s = seed;
+    return random_hints_csr(n, offs, adj, logbase, &s, hints_out, cap);
+}
+
+/* ------------------------------------------------------------------------ */
+/* optimize_index!(index, ctx, MinRecall(r)), src/opt.jl:220-317 with the    */
+/* error function of :104-163 and BeamSearch's setconfig!/runconfig          */
+/* (src/searchgraph/optbs.jl:141-156).  The reference drives the search with */
+/* SearchModels.jl 0.5 (evolutionary, not vendored here); this restatement   */
+/* scores every point of the BeamSearchSpace grid (bsize 2:2:16, Δ           */
+/* 0.9:0.025:1.1) with the same cost function and keeps the first minimum.   */
+/* Per-query costs stand for the per-batch counters (one query per batch).  */
+/* parity unpinned: SearchModels' population dynamics are not reproduced.    */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double recall, vmean;
+    int64_t vmax;
+    int valid;
+} perf_t;
+
+static double set_recall(const uint32_t* g, int32_t ng, const uint32_t* r, int32_t nr)
+{
+    int32_t c = 0;
+    for (int32_t i = 0; i < ng; i++)
+        for (int32_t t = 0; t < nr; t++)
+            if (g[i] == r[t]) {
+                c++;
+                break;
+            }
+    return ng > 0 ? (double)c / (double)ng : 0.0;
+}
+
+static double optimize_core(const gidx_t* g, const uint32_t* hints, int64_t nh, const float* Q,
+                            int64_t m, int32_t ks, float minrecall, int64_t cur_maxvisits,
+                            int32_t beamcap, int32_t* bsize_out, float* delta_out,
+                            int64_t* maxvisits_out, int nthreads)
+{
+    /* gold: ExhaustiveSearch over db[1:len] (opt.jl:251-256) */
+    uint32_t* gid = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(ks * m));
+    float* gd = (float*)malloc(sizeof(float) * (size_t)(ks * m));
+    int32_t* glen = (int32_t*)malloc(sizeof(int32_t) * (size_t)m);
+    or_exhaustive(g->metric, g->X, g->n, g->dim, Q, m, ks, gid, gd, nthreads);
+    for (int64_t j = 0; j < m; j++) {
+        int32_t l = 0;
+        while (l < ks && gid[j * ks + l] != 0) l++;
+        glen[j] = l;
+    }
+    enum { NB = 8, ND = 9 };
+    perf_t P[NB * ND];
+    uint32_t* rid = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(ks * m));
+    float* rd = (float*)malloc(sizeof(float) * (size_t)(ks * m));
+    int64_t* cdv = (int64_t*)malloc(sizeof(int64_t) * (size_t)m);
+    int64_t* cbv = (int64_t*)malloc(sizeof(int64_t) * (size_t)m);
+    for (int bi = 0; bi < NB; bi++) {
+        for (int di = 0; di < ND; di++) {
+            const int32_t bs = 2 + 2 * bi;
+            const float delta = (float)((900.0 + 25.0 * di) / 1000.0);
+            or_beamsearch(g->metric, g->X, g->n, g->dim, g->offs, g->adj, hints, nh, NULL, NULL,
+                          0, Q, m, ks, NULL, ks, bs, NULL, delta, 2 * cur_maxvisits, beamcap, rid,
+                          rd, cdv, cbv, nthreads);
+            perf_t* p = &P[bi * ND + di];
+            int64_t full = 0;
+            double rs = 0.0, vs = 0.0;
+            int64_t vmax = 0, act = 0;
+            for (int64_t j = 0; j < m; j++) {
+                int32_t l = 0;
+                while (l < ks && rid[j * ks + l] != 0) l++;
+                if (l == ks) full++;
+                rs += set_recall(gid + j * ks, glen[j], rid + j * ks, l);
+                if (cdv[j] != 0) {
+                    vs += (double)cdv[j];
+                    act++;
+                    if (cdv[j] > vmax) vmax = cdv[j];
+                }
+            }
+            p->valid = full > 3;
+            p->recall = rs / (double)m;
+            p->vmean = act ? vs / (double)act : 0.0;
+            p->vmax = vmax;
+        }
+    }
+    double M = 0.0;
+    for (int i = 0; i < NB * ND; i++)
+        if (P[i].valid && (double)P[i].vmax > M) M = (double)P[i].vmax;
+    int best = -1;
+    double bestc = INFINITY;
+    for (int i = 0; i < NB * ND; i++) {
+        if (!P[i].valid) continue;
+        const double c = P[i].recall < (double)minrecall
+                             ? 1.0 + ((double)minrecall - P[i].recall)
+                             : (M > 0 ? P[i].vmean / M : 0.0);
+        if (c < bestc) {
+            bestc = c;
+            best = i;
+        }
+    }
+    double rec = -1.0;
+    if (best >= 0) {
+        *bsize_out = 2 + 2 * (best / ND);
+        *delta_out = (float)((900.0 + 25.0 * (best % ND)) / 1000.0);
+        *maxvisits_out = (int64_t)ceil(2.0 * (double)P[best].vmax);
+        if (*maxvisits_out < 1) *maxvisits_out = 1;
+        rec = P[best].recall;
+    }
+    free(gid);
+    free(gd);
+    free(glen);
+    free(rid);
+    free(rd);
+    free(cdv);
+    free(cbv);
+    return rec;
+}
+
+double or_optimize_minrecall(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
+                             const uint32_t* adj, const uint32_t* hints, int64_t nhints,
+                             const float* Q, int64_t m, int32_t ksearch, float minrecall,
+                             int64_t cur_maxvisits, int32_t beamcap, int32_t* bsize_out,
+                             float* delta_out, int64_t* maxvisits_out, int nthreads)
+{
+    gidx_t g = {metric, X, n, dim, offs, adj};
+    return optimize_core(&g, hints, nhints, Q, m, ksearch, minrecall, cur_maxvisits, beamcap,
+                         bsize_out, delta_out, maxvisits_out, nthreads);
+}
+
+/* ------------------------------------------------------------------------ */
+/* index!(index::SearchGraph, ctx), src/searchgraph/insertions.jl:35-113,    */
+/* with find_neighborhood! (neighborhood.jl:25-45), the SAT/HSP filter       */
+/* (hsp.jl:8-33,103-119, neardup = typemin(Float32) so the random branch     */
+/* never fires), connect_reverse_links! (neighborhood.jl:52-89; the          */
+/* reference appends from parallel tasks in unspecified order, here in       */
+/* ascending node order) and execute_callbacks! (callbacks.jl:20-25).        */
+/* Pinned by: structural invariants (every list sorted by insertion, HSP     */
+/* property of every forward list, symmetric reverse links) and n < 64       */
+/* closed cases.  The callbacks' random draws (RandomHints, the query sample */
+/* of optimize_index!) and the SearchModels optimiser are parity unpinned.   */
+/* ------------------------------------------------------------------------ */
+static int32_t nb_size(const or_build_params* p, int64_t n)
+{
+    if (n == 0) return p->nb_minsize;
+    return (int32_t)ceilf((float)p->nb_minsize + logf((float)n) / logf(p->nb_logbase));
+}
+
+/* hsp_proximal_neighborhood_filter! hsp.jl:103-119 over the sorted tmp queue */
+static void hsp_filter(int metric, const float* X, int dim, const float* center, const knn_t* tmp,
+                       knn_t* out)
+{
+    (void)center;
+    const int32_t len = knn_len(tmp);
+    knn_push(out, tmp->ids[tmp->sp], tmp->d[tmp->sp]);
+    for (int32_t i = 1; i < len; i++) {
+        const uint32_t pid = tmp->ids[tmp->sp + i];
+        const float pd = tmp->d[tmp->sp + i];
+        const float* pt = X + (int64_t)(pid - 1) * dim;
+        int push = 1;
+        const int32_t ol = knn_len(out);
+        for (int32_t t = 0; t < ol; t++) { /* hsp_should_push hsp.jl:8-33 */
+            const uint32_t hid = out->ids[out->sp + t];
+            const float dph = or_dist(metric, pt, X + (int64_t)(hid - 1) * dim, dim);
+            if (dph < pd) {
+                push = 0;
+                break;
+            }
+        }
+        if (push) knn_push(out, pid, pd);
+    }
+}
+
+typedef struct {
+    int32_t bsize;
+    float delta;
+    int64_t maxvisits;
+} algo_t;
+
+static void run_callbacks(const or_build_params* p, int metric, const float* X, int dim,
+                          alist_t* L, int64_t len, int64_t nlists, csr_t* c, uint32_t** hints,
+                          int64_t* nh, int64_t* hcap, algo_t* algo, uint64_t* rng)
+{
+    snapshot(L, nlists, c);
+    if (p->hints_logbase > 0) {
+        const int64_t want = (int64_t)ceilf(logf((float)len) / logf(p->hints_logbase)) + 1;
+        if (want > *hcap) {
+            *hcap = want;
+            *hints = (uint32_t*)realloc(*hints, sizeof(uint32_t) * (size_t)want);
+        }
+        *nh = random_hints_csr(len, c->offs, c->adj, p->hints_logbase, rng, *hints, *hcap);
+    }
+    if (p->opt_minrecall > 0) {
+        /* sample = rand(rng, 1:length(index), numqueries) |> unique  (opt.jl:242) */
+        const int32_t nq = p->opt_numqueries;
+        int64_t* smp = (int64_t*)malloc(sizeof(int64_t) * (size_t)nq);
+        int32_t m = 0;
+        for (int32_t i = 0; i < nq; i++) {
+            const int64_t x = rand_1n(rng, len);
+            int dup = 0;
+            for (int32_t t = 0; t < m; t++)
+                if (smp[t] == x) dup = 1;
+            if (!dup) smp[m++] = x;
+        }
+        float* Q = (float*)malloc(sizeof(float) * (size_t)m * (size_t)dim);
+        for (int32_t i = 0; i < m; i++)
+            memcpy(Q + (int64_t)i * dim, X + (smp[i] - 1) * dim, sizeof(float) * (size_t)dim);
+        gidx_t g = {metric, X, len, dim, c->offs, c->adj};
+        int32_t bs;
+        float dl;
+        int64_t mv;
+        if (optimize_core(&g, *hints, *nh, Q, m, p->opt_ksearch, p->opt_minrecall, algo->maxvisits,
+                          p->beamcap, &bs, &dl, &mv, 1) >= 0.0) {
+            algo->bsize = bs;
+            algo->delta = dl;
+            algo->maxvisits = mv;
+        }
+        free(Q);
+        free(smp);
+    }
+}
+
+static int cb_fires(const or_build_params* p, int64_t n, int64_t m)
+{
+    if (n < p->cb_starting) return 0;
+    const float lb = logf(p->cb_logbase);
+    return (int64_t)ceilf(logf((float)n) / lb) != (int64_t)ceilf(logf((float)m) / lb);
+}
+
+int or_build_graph(int metric, const float* X, int64_t n, int dim, const or_build_params* p,
+                   or_graph* out)
+{
+    if (n <= 0) return -1;
+    alist_t* L = (alist_t*)calloc((size_t)n, sizeof(alist_t));
+    csr_t c = {(int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1)), NULL, 0};
+    uint32_t* hints = NULL;
+    int64_t nh = 0, hcap = 0;
+    algo_t algo = {4, 1.0f, 1000000};
+    uint64_t rng = p->seed;
+    const int64_t nw = (n + 63) / 64 + 1;
+    uint64_t* vs = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)nw);
+    const int32_t ksmax = nb_size(p, n) + 2;
+    uint32_t* ti = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)ksmax);
+    float* td = (float*)malloc(sizeof(float) * (size_t)ksmax);
+    uint32_t* oi = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)ksmax);
+    float* od = (float*)malloc(sizeof(float) * (size_t)ksmax);
+    uint32_t* bids = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(p->beamcap + 1));
+    float* bd = (float*)malloc(sizeof(float) * (size_t)(p->beamcap + 1));
+    int64_t len = 0;
+    const int32_t PB = p->parallel_block < 1 ? 1 : p->parallel_block;
+
+    if (PB == 1) {
+        /* _sequential_append_items_loop! insertions.jl:35-44 + push_item! :142-161 */
+        for (int64_t sp = 1; sp <= n; sp++) {
+            const int32_t ks = nb_size(p, sp);
+            knn_t tmp, nbr;
+            knn_init(&tmp, ti, td, ks);
+            knn_init(&nbr, oi, od, ks);
+            const float* item = X + (sp - 1) * dim;
+            if (len > 0) {
+                snapshot(L, len, &c);
+                gidx_t g = {metric, X, len, dim, c.offs, c.adj};
+                memset(vs, 0, sizeof(uint64_t) * (size_t)nw);
+                int64_t cd = 0, cb = 0;
+                beam_search(&g, item, &tmp, hints, nh, algo.bsize, algo.delta, algo.maxvisits,
+                            p->beamcap, vs, bids, bd, &cd, &cb);
+            }
+            if (knn_len(&tmp) > 0) hsp_filter(metric, X, dim, item, &tmp, &nbr);
+            const int64_t node = len + 1;
+            for (int32_t t = 0; t < knn_len(&nbr); t++) alist_push(&L[node - 1], nbr.ids[nbr.sp + t]);
+            if (node > 1) {
+                for (int32_t t = 0; t < L[node - 1].len; t++) {
+                    const uint32_t rel = L[node - 1].v[t];
+                    if ((int64_t)rel != node) alist_push(&L[rel - 1], (uint32_t)node);
+                }
+                if (cb_fires(p, len, len + 1))
+                    run_callbacks(p, metric, X, dim, L, len, len + 1, &c, &hints, &nh, &hcap,
+                                  &algo, &rng);
+            }
+            len = node;
+        }
+    } else {
+        /* _parallel_append_items_loop! insertions.jl:46-79 */
+        for (int64_t sp = 1; sp <= n;) {
+            const int64_t ep = (sp + PB - 1 < n) ? sp + PB - 1 : n;
+            const int32_t ks = nb_size(p, ep);
+            if (len > 0) snapshot(L, len, &c);
+            gidx_t g = {metric, X, len, dim, c.offs, c.adj};
+            for (int64_t obj = sp; obj <= ep; obj++) {
+                knn_t tmp, nbr;
+                knn_init(&tmp, ti, td, ks);
+                knn_init(&nbr, oi, od, ks);
+                const float* item = X + (obj - 1) * dim;
+                if (len > 0) {
+                    memset(vs, 0, sizeof(uint64_t) * (size_t)nw);
+                    int64_t cd = 0, cb = 0;
+                    beam_search(&g, item, &tmp, hints, nh, algo.bsize, algo.delta, algo.maxvisits,
+                                p->beamcap, vs, bids, bd, &cd, &cb);
+                }
+                for (int64_t i = sp; i < obj; i++) { /* interblock neighbours */
+                    const float d = or_dist(metric, item, X + (i - 1) * dim, dim);
+                    if (d <= -INFINITY) continue;
+                    knn_push(&tmp, (uint32_t)i, d);
+                }
+                if (knn_len(&tmp) > 0) hsp_filter(metric, X, dim, item, &tmp, &nbr);
+                for (int32_t t = 0; t < knn_len(&nbr); t++)
+                    alist_push(&L[obj - 1], nbr.ids[nbr.sp + t]);
+            }
+            /* connect_reverse_links!(adj, sp, ep) neighborhood.jl:73-89 */
+            for (int64_t node = sp; node <= ep; node++)
+                for (int32_t t = 0; t < L[node - 1].len; t++) {
+                    const uint32_t rel = L[node - 1].v[t];
+                    if ((int64_t)rel < sp) alist_push(&L[rel - 1], (uint32_t)node);
+                }
+            int32_t* Lc = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ep - sp + 1));
+            for (int64_t node = sp; node <= ep; node++) Lc[node - sp] = L[node - 1].len;
+            for (int64_t node = sp; node <= ep; node++)
+                for (int32_t t = 0; t < Lc[node - sp]; t++) {
+                    const uint32_t rel = L[node - 1].v[t];
+                    if ((int64_t)rel >= sp && (int64_t)rel != node)
+                        alist_push(&L[rel - 1], (uint32_t)node);
+                }
+            free(Lc);
+            len = ep;
+            if (cb_fires(p, sp, ep))
+                run_callbacks(p, metric, X, dim, L, len, len, &c, &hints, &nh, &hcap, &algo, &rng);
+            sp = ep + 1;
+        }
+    }
+
+    snapshot(L, n, &c);
+    out->offs = c.offs;
+    out->nnz = c.offs[n];
+    out->adj = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(out->nnz + 1));
+    memcpy(out->adj, c.adj, sizeof(uint32_t) * (size_t)out->nnz);
+    out->hints = hints ? hints : (uint32_t*)malloc(sizeof(uint32_t));
+    out->nhints = nh;
+    out->bsize = algo.bsize;
+    out->delta = algo.delta;
+    out->maxvisits = algo.maxvisits;
+    for (int64_t i = 0; i < n; i++) free(L[i].v);
+    free(L);
+    free(c.adj);
+    free(vs);
+    free(ti);
+    free(td);
+    free(oi);
+    free(od);
+    free(bids);
+    free(bd);
+    return 0;
+}
+
+void or_free(void* p) { free(p); }
