@@ -1,0 +1,132 @@
+/*
+ * ssg.h -- C ABI of libssg_hip.so, the MI355X (gfx950) query hot path of
+ * SimilaritySearch.jl: searchbatch/allknn over SearchGraph beam search and over
+ * ExhaustiveSearch / ParallelExhaustiveSearch, plus rerank!.
+ *
+ * The reference has no FFI; its plug-in surface is Julia multiple dispatch.  A
+ * GPU index type subtypes AbstractSearchIndex (src/SimilaritySearch.jl:4) and
+ * overrides the batch entry points (src/SimilaritySearch.jl:331, :384,
+ * src/allknn.jl:66) with one ccall each into the functions below.
+ *
+ * Conventions (all of them the reference's):
+ *   - databases and query sets are column-major (dim, n) Float32 matrices,
+ *     i.e. object i is the contiguous run X[i*dim .. i*dim+dim-1];
+ *   - results are (kstride, m) column-major: query j owns
+ *     ids[j*kstride .. j*kstride+k-1];
+ *   - ids are 1-based UInt32, 0 marks an empty slot, empty distances are +Inf;
+ *   - each column is sorted by ascending distance, ties keep the lower id
+ *     (KnnSorted's stable insertion, src/pqueue/knnsorted.jl:68-101,140-168).
+ * Every entry point returns 0 on success and a negative SSG_E* code on error;
+ * ssg_last_error() then describes the failure.  Calls on one handle must not
+ * overlap; different handles may run concurrently.  Functions without the
+ * _dev suffix take host pointers and block until the results are in host
+ * memory; _dev functions take device pointers and enqueue on `stream`
+ * (a hipStream_t, NULL = the handle's own stream) without synchronising.
+ */
+#ifndef SSG_H
+#define SSG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ssg_handle ssg_handle;
+
+enum {
+    SSG_OK = 0,
+    SSG_E_ARG = -1,     /* invalid argument (the reference throws ArgumentError) */
+    SSG_E_STATE = -2,   /* missing upload (no database / no graph) */
+    SSG_E_HIP = -3,     /* HIP runtime failure */
+    SSG_E_NOMEM = -4,   /* device allocation failed */
+    SSG_E_LIMIT = -5    /* size beyond a documented kernel limit */
+};
+
+/* Distance functions (src/dist/vecs.jl:52-102, src/dist/cos.jl:39,76-85). */
+enum { SSG_SQL2 = 0, SSG_L2 = 1, SSG_NORMCOS = 2 };
+
+/* Storage precision of the device copy of the database. */
+enum { SSG_F32 = 0, SSG_BF16 = 1 };
+
+/* Lifecycle.  Replaces constructing the Julia index object; device = HIP ordinal. */
+int ssg_create(ssg_handle** h, int device);
+int ssg_destroy(ssg_handle* h);
+/* Thread-local message of the last failed call (h may be NULL). */
+const char* ssg_last_error(const ssg_handle* h);
+/* Library version string. */
+const char* ssg_version(void);
+
+/* Upload the database (MatrixDatabase, src/db/matrixdatabase.jl:27-59) and fix the
+ * index's distance (the `dist` field of ExhaustiveSearch / SearchGraph).
+ * X: column-major (dim, n) Float32 host matrix. */
+int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t metric);
+
+/* Upload a frozen graph in StaticAdjList form (src/adj/adjstatic.jl:27-30,102-120):
+ * offset[i] is the inclusive end of node i+1's list in end_point (offset[0] of the
+ * reference is implicit 0), end_point holds 1-based ids.  hints are SearchGraph.hints
+ * (src/searchgraph/SearchGraph.jl:147-158), 1-based. */
+int ssg_upload_graph(ssg_handle* h, const int64_t* offset, int64_t n, const uint32_t* end_point,
+                     int64_t nnz, const uint32_t* hints, int64_t nhints);
+
+/* searchbatch!(::ExhaustiveSearch / ::ParallelExhaustiveSearch, ctx, Q, ids, dists)
+ * (src/SimilaritySearch.jl:331-348 over src/exact/sequential-exhaustive.jl:68-80 and
+ * src/exact/parallel-exhaustive.jl:64-102).  Q: column-major (dim, m). */
+int ssg_exhaustive_knn(ssg_handle* h, const float* Q, int64_t m, int32_t k, uint32_t* ids,
+                       float* dists);
+int ssg_exhaustive_knn_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, uint32_t* dids,
+                           float* ddists, void* stream);
+
+/* Beam-search parameters (BeamSearch, src/searchgraph/SearchGraph.jl:94-100) plus the
+ * rows of the context's beam buffers (getbeam clamps bsize to them,
+ * src/searchgraph/context.jl:119-126,220-225). */
+typedef struct {
+    int32_t bsize;
+    float delta;
+    int64_t maxvisits;
+    int32_t beamcap; /* 32 in the default SearchGraphContext; at most 256 */
+} ssg_beam_params;
+
+/* Per-query optional inputs of the queue-vector form searchbatch!(index, ctx, Q, knns)
+ * (src/SimilaritySearch.jl:384-398): per-query k and bsize, and per-query hint lists
+ * that replace SearchGraph.hints (the approx_by_hints! injection point,
+ * src/searchgraph/hints.jl:8-14).  Every pointer may be NULL. */
+typedef struct {
+    const int32_t* k;        /* [m], each 1 <= k[j] <= kstride */
+    const int32_t* bsize;    /* [m] */
+    const uint32_t* hints;   /* [m * hint_stride], 1-based */
+    const int32_t* nhints;   /* [m] */
+    int32_t hint_stride;
+} ssg_query_opts;
+
+/* searchbatch!(::SearchGraph, ctx, Q, ids, dists) with search(::BeamSearch, ...)
+ * (src/searchgraph/beamsearch.jl:88-110, src/searchgraph/SearchGraph.jl:210-213).
+ * costdists / costblocks (nullable, [m]) receive the per-query counters that the
+ * reference accumulates in ctx.costdists / ctx.costblocks
+ * (src/SimilaritySearch.jl:224-225). */
+int ssg_beamsearch(ssg_handle* h, const float* Q, int64_t m, int32_t k, int32_t kstride,
+                   const ssg_beam_params* bp, const ssg_query_opts* qo, uint32_t* ids,
+                   float* dists, int64_t* costdists, int64_t* costblocks);
+/* Device-pointer form; qo's arrays are device pointers too. */
+int ssg_beamsearch_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, int32_t kstride,
+                       const ssg_beam_params* bp, const ssg_query_opts* qo, uint32_t* dids,
+                       float* ddists, int64_t* dcostdists, int64_t* dcostblocks, void* stream);
+
+/* allknn!(::SearchGraph, ctx, ids, dists) (src/allknn.jl:66-106): every database object
+ * is a query; the neighbours of object i seed one search each, sharing one visited set. */
+int ssg_allknn_graph(ssg_handle* h, int32_t k, const ssg_beam_params* bp, uint32_t* ids,
+                     float* dists);
+/* allknn!(::ExhaustiveSearch, ...) (src/allknn.jl:66-91,108-110): exact, self included. */
+int ssg_allknn_exhaustive(ssg_handle* h, int32_t k, uint32_t* ids, float* dists);
+
+/* rerank!(dist, db, queries, ids, dists) batch form (src/rerank.jl:77-86): recompute the
+ * non-zero-id prefix of each column with the handle's distance and stable-sort it.
+ * ids / dists are updated in place. */
+int ssg_rerank(ssg_handle* h, const float* Q, int64_t m, int32_t k, uint32_t* ids, float* dists);
+int ssg_rerank_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, uint32_t* dids,
+                   float* ddists, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,351 @@
+// ssg_beam.hip -- wavefront-per-query graph beam search for gfx950.
+//
+// Restates search(bs::BeamSearch, ...) of src/searchgraph/beamsearch.jl:88-110:
+//   n == 0 -> nothing; n < 64 -> brute force over 1..n (no visited marks);
+//   otherwise beamsearch_init (:26-34, every hint through enqueue_item! :12-18,
+//   then ids 1:ceil(log2(1+n)) if the result is still empty) and
+//   beamsearch_inner_beam (:44-73): a KnnSorted beam of min(bsize, beamcap)
+//   slots seeded with nearest(res); pop_min, expand the CSR list, test-and-set
+//   visited, evaluate, push into res, stop once costdists > maxvisits, and admit
+//   the child into the beam when deg(child) > 1 and d <= Δ * maximum(res).
+//
+// GPU shape: one wave64 per query, persistent over an atomic work counter.
+//   * res (k <= 64) and the beam (<= 256 slots) live in registers (RegQueue).
+//   * Neighbour lists are expanded 64 ids at a time: one coalesced id load, one
+//     visited-byte load per id, then the unvisited ids are compacted in LDS and
+//     their vectors gathered 32 at a time (4 per 16-lane group, 8 rounds), with
+//     the child degrees loaded in the same round trip.
+//   * The order-dependent part (pushes into res and beam, the maxvisits cut) is
+//     replayed serially over the children in list order with readlane, skipping
+//     children that provably change neither queue.
+//   * Visited set: one epoch byte per (wave slot, object).  A query takes the
+//     next epoch of its slot; the slot is cleared once every 255 queries.  A
+//     byte equal to the current epoch means visited, so no per-query reset.
+#include "ssg_device.h"
+#include "ssg_kernels.h"
+
+namespace ssg {
+
+constexpr int ROUNDS = 8;           // gather rounds per batch -> 32 children
+constexpr int BATCH = 4 * ROUNDS;
+constexpr int WAVES = 4;            // waves per block
+
+enum { MODE_HINT = 0, MODE_EXPAND = 1, MODE_BRUTE = 2 };
+
+__device__ __forceinline__ void lds_order()
+{
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Wait for every outstanding vector memory operation of this wave (gfx9 vmcnt
+// covers stores), so visited marks are visible to the next visited loads.
+__device__ __forceinline__ void vm_drain()
+{
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    asm volatile("" ::: "memory");
+}
+
+template <int V, int NB>
+struct Searcher {
+    const BeamArgs& a;
+    float qv[V];
+    RegQueue<1, false> res;
+    RegQueue<NB, true> beam;
+    int64_t cd_init;   // distance evaluations charged by enqueue_item!
+    int64_t cd_inner;  // costdists of beamsearch_inner_beam
+    int64_t cb;        // costblocks
+    uint32_t epoch;
+    volatile uint8_t* vis;
+    uint32_t* cidbuf;  // 64 ids, wave-private LDS
+    int lane, g, j;
+
+    __device__ __forceinline__ Searcher(const BeamArgs& args) : a(args) {}
+
+    // Evaluate cidbuf[ub .. ub+cnt) (cnt <= BATCH); lane u receives child u's
+    // distance, id, list start and degree.
+    __device__ __forceinline__ void eval_batch(int ub, int cnt, float& dd, uint32_t& cid,
+                                               uint32_t& st, uint32_t& dg, bool need_deg)
+    {
+        const int dp = 16 * V;
+        cid = lane < cnt ? cidbuf[ub + lane] : 0u;
+        int64_t o0 = 0, o1 = 0;
+        if (need_deg && lane < cnt) {
+            o0 = a.offs[cid];
+            o1 = a.offs[cid + 1];
+        }
+        float xv[ROUNDS][V];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++) {
+            if (4 * r < cnt) {
+                const int u = 4 * r + g;
+                if (u < cnt) {
+                    const uint32_t c = cidbuf[ub + u];
+                    load_run<V>(a.X + (size_t)c * dp + j * V, xv[r]);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < V; t++) xv[r][t] = 0.f;
+                }
+            }
+        }
+        float dr[ROUNDS];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++) {
+            dr[r] = __builtin_inff();
+            if (4 * r < cnt) dr[r] = finish_dist(a.metric, group16_sum(run_partial<V>(a.metric, xv[r], qv)));
+        }
+        dd = __builtin_inff();
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++) {
+            if (4 * r < cnt) {
+                const float t = __shfl(dr[r], (lane & 3) * 16);
+                if ((lane >> 2) == r) dd = t;
+            }
+        }
+        st = (uint32_t)o0;
+        dg = (uint32_t)(o1 - o0);
+    }
+
+    // One chunk of up to 64 candidate ids (lane-indexed, `valid` lanes).  Returns
+    // true when the maxvisits cut ends the search.
+    __device__ __forceinline__ bool chunk(int mode, uint32_t cid, bool valid, bool dedupe)
+    {
+        bool unv = valid;
+        if (mode != MODE_BRUTE) {
+            vm_drain();
+            if (valid) unv = vis[cid] != (uint8_t)epoch;
+            if (dedupe) {
+                const uint64_t vm = __ballot(valid);
+                const int cnt = 64 - __clzll(vm);
+                for (int t = 0; t < cnt; t++) {
+                    const uint32_t c = rl_u(cid, t);
+                    if (lane > t && cid == c) unv = false;
+                }
+            }
+            if (unv) vis[cid] = (uint8_t)epoch;
+        }
+        const uint64_t mask = __ballot(unv);
+        const int U = __popcll(mask);
+        const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+        if (unv) cidbuf[pos] = cid;
+        lds_order();
+        bool stopped = false;
+        for (int ub = 0; ub < U && !stopped; ub += BATCH) {
+            const int cnt = U - ub < BATCH ? U - ub : BATCH;
+            float dd;
+            uint32_t c, st, dg;
+            eval_batch(ub, cnt, dd, c, st, dg, mode == MODE_EXPAND);
+            int lim = cnt, stop = -1;
+            if (mode == MODE_EXPAND && cd_inner + cnt > a.maxvisits) {
+                stop = (int)(a.maxvisits - cd_inner);
+                lim = stop + 1;
+            }
+            const bool full = res.len == res.cap;
+            const float mx0 = full ? res.maximum() : __builtin_inff();
+            const float th0 = a.delta * mx0;
+            bool cand = lane < lim;
+            if (full) cand = cand && (dd < mx0 || (mode == MODE_EXPAND && dd <= th0));
+            uint64_t cm = __ballot(cand);
+            while (cm) {
+                const int u = __ffsll((unsigned long long)cm) - 1;
+                cm &= cm - 1;
+                const float du = rl_f(dd, u);
+                const uint32_t cu = rl_u(c, u);
+                res.push(du, cu, 0, 0);
+                if (mode != MODE_EXPAND) continue;
+                if (u == stop) break;
+                const uint32_t gu = rl_u(dg, u);
+                const float thr = a.delta * res.maximum();
+                if (gu > 1 && du <= thr) beam.push(du, cu, rl_u(st, u), gu);
+            }
+            if (mode == MODE_EXPAND) {
+                cd_inner += lim;
+                if (stop >= 0) {
+                    // children after the cut were marked ahead of time: unmark them
+                    if (lane > ub + stop && lane < U) vis[cidbuf[lane]] = 0;
+                    stopped = true;
+                }
+            } else {
+                cd_init += cnt;
+            }
+        }
+        lds_order();
+        return stopped;
+    }
+
+    __device__ __forceinline__ void hint_list(const uint32_t* h, int nh, bool one_based)
+    {
+        for (int hb = 0; hb < nh; hb += 64) {
+            const bool valid = hb + lane < nh;
+            uint32_t c = valid ? h[hb + lane] : 1u;
+            if (one_based) c -= 1u;
+            chunk(MODE_HINT, c, valid, true);
+        }
+    }
+
+    // search(bs::BeamSearch, ...) for the current query with the given hints.
+    __device__ __forceinline__ void search(const uint32_t* h, int nh, bool one_based, int bsize)
+    {
+        const int64_t n = a.n;
+        if (n == 0) return;
+        if (n < 64) {
+            chunk(MODE_BRUTE, (uint32_t)lane, lane < n, false);
+            return;
+        }
+        hint_list(h, nh, one_based);
+        if (res.len == 0) chunk(MODE_HINT, (uint32_t)lane, lane < a.log2n, false);
+        const int cap = bsize < a.beamcap ? bsize : a.beamcap;
+        beam.init(cap);
+        {
+            const uint32_t nid = res.id_at(0);
+            const float nd = res.dist_at(0);
+            const int64_t o0 = a.offs[nid], o1 = a.offs[nid + 1];
+            beam.push(nd, nid, (uint32_t)o0, (uint32_t)(o1 - o0));
+        }
+        while (beam.len > 0) {
+            float pd;
+            uint32_t pid, pst, pdeg;
+            beam.pop_min(pd, pid, pst, pdeg);
+            cb++;
+            bool stop = false;
+            for (uint32_t base = 0; base < pdeg && !stop; base += 64) {
+                const uint32_t rem = pdeg - base;
+                const bool valid = (uint32_t)lane < rem;
+                const uint32_t c = valid ? a.adj[(size_t)pst + base + lane] : 0u;
+                stop = chunk(MODE_EXPAND, c, valid, a.has_dups != 0);
+            }
+            if (stop) break;
+        }
+    }
+};
+
+template <int V, int NB>
+__global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
+{
+    __shared__ uint32_t s_cid[WAVES][64];
+    const int w = threadIdx.x >> 6;
+    const int64_t slot = (int64_t)blockIdx.x * WAVES + w;
+    Searcher<V, NB> S(a);
+    S.lane = lane_id();
+    S.g = S.lane >> 4;
+    S.j = S.lane & 15;
+    S.cidbuf = s_cid[w];
+    S.vis = a.vis + slot * a.vis_stride;
+    uint32_t epoch = rfl_u(a.epochs[slot]);
+    const int lane = S.lane;
+    const int dp = 16 * V;
+
+    for (;;) {
+        uint32_t qi = 0;
+        if (lane == 0) qi = atomicAdd(a.counter, 1u);
+        qi = rl_u(qi, 0);
+        if ((int64_t)qi >= a.m) break;
+        epoch++;
+        if (epoch > 255) {
+            uint8_t* vp = (uint8_t*)(a.vis + slot * a.vis_stride);
+            for (int64_t off = (int64_t)lane * 16; off < a.vis_stride; off += 64 * 16)
+                *reinterpret_cast<uint4*>(vp + off) = make_uint4(0, 0, 0, 0);
+            vm_drain();
+            epoch = 1;
+        }
+        S.epoch = epoch;
+        if (a.allknn) {
+            load_run<V>(a.X + (size_t)qi * dp + S.j * V, S.qv);
+        } else {
+#pragma unroll
+            for (int t = 0; t < V; t++) {
+                const int i = S.j * V + t;
+                S.qv[t] = i < a.qdim ? a.Q[(size_t)qi * a.qdim + i] : 0.f;
+            }
+        }
+        const int kk = a.kq ? a.kq[qi] : a.k;
+        const int bs = a.bq ? a.bq[qi] : a.bsize;
+        S.res.init(kk);
+        S.cd_init = 0;
+        S.cd_inner = 0;
+        S.cb = 0;
+        if (a.allknn) {
+            const int64_t o0 = a.offs[qi], o1 = a.offs[qi + 1];
+            for (int64_t t = o0; t < o1; t++) {
+                const uint32_t h = a.adj[t];
+                vm_drain();
+                const uint32_t seen = rfl_u((uint32_t)S.vis[h]);
+                if (seen == epoch) continue;
+                const uint32_t h1 = h + 1u;
+                S.search(&h1, 1, true, bs);
+                S.cd_inner = 0;
+            }
+        } else if (a.qhints) {
+            S.search(a.qhints + (size_t)qi * a.qhstride, a.qhcount[qi], true, bs);
+        } else {
+            S.search(a.hints, a.nhints, true, bs);
+        }
+        if (lane < a.kstride) {
+            const bool has = lane < S.res.len;
+            a.ids[(size_t)qi * a.kstride + lane] = has ? S.res.id[0] + 1u : 0u;
+            a.dists[(size_t)qi * a.kstride + lane] = has ? S.res.d[0] : __builtin_inff();
+        }
+        if (lane == 0) {
+            if (a.costd) a.costd[qi] = S.cd_init + S.cd_inner;
+            if (a.costb) a.costb[qi] = S.cb;
+        }
+    }
+    if (lane == 0) a.epochs[slot] = (uint8_t)epoch;
+}
+
+__global__ void zero_u32_kernel(unsigned int* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(unsigned int* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+__global__ void fill_f32_kernel(float* p, int64_t n, float v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+template <int V>
+static hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
+{
+    const dim3 grid((unsigned)(nslots / WAVES)), block(256);
+    if (a.beamcap <= 64)
+        hipLaunchKernelGGL((beam_kernel<V, 1>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((beam_kernel<V, 4>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_beam(const BeamArgs& a, int nslots, hipStream_t s)
+{
+    switch (a.dp / 16) {
+    case 1: return launch_v<1>(a, nslots, s);
+    case 2: return launch_v<2>(a, nslots, s);
+    case 3: return launch_v<3>(a, nslots, s);
+    case 4: return launch_v<4>(a, nslots, s);
+    case 5: return launch_v<5>(a, nslots, s);
+    case 6: return launch_v<6>(a, nslots, s);
+    case 7: return launch_v<7>(a, nslots, s);
+    case 8: return launch_v<8>(a, nslots, s);
+    case 12: return launch_v<12>(a, nslots, s);
+    case 16: return launch_v<16>(a, nslots, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace ssg

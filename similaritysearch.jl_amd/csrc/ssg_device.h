@@ -1,0 +1,258 @@
+// ssg_device.h -- device-side building blocks shared by the gfx950 kernels.
+//
+// * Canonical distance: a 16-lane group evaluates one (object, query) pair.
+//   Lane j of the group owns the contiguous run [j*V, (j+1)*V) of the padded
+//   dimension Dp = 16*V, accumulates it in FP64 with fma in increasing index,
+//   and the group combines the 16 partial sums by the xor-butterfly 8,4,2,1.
+//   The FP64 result is rounded once to Float32.  The CPU oracle evaluates the
+//   identical expression tree, so both sides produce the same bits.
+// * RegQueue: KnnSorted (src/pqueue/knnsorted.jl:31-233) held in registers,
+//   slot e = 64*b + lane.  push keeps equal distances in insertion order and a
+//   full queue rejects d >= maximum (knnsorted.jl:140-168); pop_min removes the
+//   first slot (knnsorted.jl:191-196).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ssg {
+
+enum { M_SQL2 = 0, M_L2 = 1, M_NORMCOS = 2 };
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+__device__ __forceinline__ float rl_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ uint32_t rl_u(uint32_t v, int l)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ uint32_t rfl_u(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// Orders a wave's LDS and global accesses before the ones that follow.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int V>
+__device__ __forceinline__ void load_run(const float* __restrict__ p, float (&x)[V])
+{
+    if constexpr (V % 4 == 0) {
+#pragma unroll
+        for (int t = 0; t < V / 4; t++) {
+            const float4 w = reinterpret_cast<const float4*>(p)[t];
+            x[4 * t + 0] = w.x;
+            x[4 * t + 1] = w.y;
+            x[4 * t + 2] = w.z;
+            x[4 * t + 3] = w.w;
+        }
+    } else if constexpr (V % 2 == 0) {
+#pragma unroll
+        for (int t = 0; t < V / 2; t++) {
+            const float2 w = reinterpret_cast<const float2*>(p)[t];
+            x[2 * t + 0] = w.x;
+            x[2 * t + 1] = w.y;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < V; t++) x[t] = p[t];
+    }
+}
+
+// Partial sum of one lane's run.
+template <int V>
+__device__ __forceinline__ double run_partial(int metric, const float (&x)[V], const float (&q)[V])
+{
+    double s = 0.0;
+    if (metric == M_NORMCOS) {
+#pragma unroll
+        for (int t = 0; t < V; t++) s = __builtin_fma((double)x[t], (double)q[t], s);
+    } else {
+#pragma unroll
+        for (int t = 0; t < V; t++) {
+            const double m = (double)x[t] - (double)q[t];
+            s = __builtin_fma(m, m, s);
+        }
+    }
+    return s;
+}
+
+// xor-butterfly over the 16 lanes of a group; every lane ends with the total.
+__device__ __forceinline__ double group16_sum(double s)
+{
+    s = s + __shfl_xor(s, 8);
+    s = s + __shfl_xor(s, 4);
+    s = s + __shfl_xor(s, 2);
+    s = s + __shfl_xor(s, 1);
+    return s;
+}
+
+__device__ __forceinline__ float finish_dist(int metric, double s)
+{
+    if (metric == M_L2) return (float)__builtin_sqrt(s);
+    if (metric == M_NORMCOS) return (float)(1.0 - s);
+    return (float)s;
+}
+
+// ---------------------------------------------------------------------------
+// KnnSorted in registers.  NB registers per field, capacity cap <= 64*NB.
+// PAY adds two uint32 payload fields carried with each entry.
+// ---------------------------------------------------------------------------
+template <int NB, bool PAY>
+struct RegQueue {
+    float d[NB];
+    uint32_t id[NB];
+    uint32_t pa[PAY ? NB : 1];
+    uint32_t pb[PAY ? NB : 1];
+    int len;
+    int cap;
+
+    __device__ __forceinline__ void init(int capacity)
+    {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            d[b] = __builtin_inff();
+            id[b] = 0xffffffffu;
+            if constexpr (PAY) {
+                pa[b] = 0;
+                pb[b] = 0;
+            }
+        }
+        len = 0;
+        cap = capacity;
+    }
+
+    // distance of slot e (e wave-uniform)
+    __device__ __forceinline__ float dist_at(int e) const
+    {
+        float r = 0.f;
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+            if ((e >> 6) == b) r = rl_f(d[b], e & 63);
+        return r;
+    }
+    __device__ __forceinline__ uint32_t id_at(int e) const
+    {
+        uint32_t r = 0;
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+            if ((e >> 6) == b) r = rl_u(id[b], e & 63);
+        return r;
+    }
+    __device__ __forceinline__ float maximum() const { return dist_at(len - 1); }
+
+    // push_item! with wave-uniform arguments; returns whether the item entered.
+    __device__ __forceinline__ bool push(float x, uint32_t xid, uint32_t xa, uint32_t xb)
+    {
+        const int lane = lane_id();
+        if (len == cap) {
+            if (x >= maximum()) return false;
+            len--;
+        }
+        int pos = 0;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int e = 64 * b + lane;
+            pos += __popcll(__ballot(e < len && d[b] <= x));
+        }
+#pragma unroll
+        for (int b = NB - 1; b >= 0; b--) {
+            const int e = 64 * b + lane;
+            float ud = __shfl_up(d[b], 1);
+            uint32_t ui = __shfl_up(id[b], 1);
+            uint32_t ua = 0, ub = 0;
+            if constexpr (PAY) {
+                ua = __shfl_up(pa[b], 1);
+                ub = __shfl_up(pb[b], 1);
+            }
+            if (b > 0) {
+                const float cd = rl_f(d[b > 0 ? b - 1 : 0], 63);
+                const uint32_t ci = rl_u(id[b > 0 ? b - 1 : 0], 63);
+                uint32_t ca = 0, cb = 0;
+                if constexpr (PAY) {
+                    ca = rl_u(pa[b > 0 ? b - 1 : 0], 63);
+                    cb = rl_u(pb[b > 0 ? b - 1 : 0], 63);
+                }
+                if (lane == 0) {
+                    ud = cd;
+                    ui = ci;
+                    ua = ca;
+                    ub = cb;
+                }
+            }
+            if (e > pos && e <= len) {
+                d[b] = ud;
+                id[b] = ui;
+                if constexpr (PAY) {
+                    pa[b] = ua;
+                    pb[b] = ub;
+                }
+            } else if (e == pos) {
+                d[b] = x;
+                id[b] = xid;
+                if constexpr (PAY) {
+                    pa[b] = xa;
+                    pb[b] = xb;
+                }
+            }
+        }
+        len++;
+        return true;
+    }
+
+    // pop_min!: returns slot 0 and shifts the rest down.
+    __device__ __forceinline__ void pop_min(float& x, uint32_t& xid, uint32_t& xa, uint32_t& xb)
+    {
+        const int lane = lane_id();
+        x = rl_f(d[0], 0);
+        xid = rl_u(id[0], 0);
+        if constexpr (PAY) {
+            xa = rl_u(pa[0], 0);
+            xb = rl_u(pb[0], 0);
+        } else {
+            xa = 0;
+            xb = 0;
+        }
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            float nd = __shfl_down(d[b], 1);
+            uint32_t ni = __shfl_down(id[b], 1);
+            uint32_t na = 0, nb = 0;
+            if constexpr (PAY) {
+                na = __shfl_down(pa[b], 1);
+                nb = __shfl_down(pb[b], 1);
+            }
+            if (b + 1 < NB) {
+                const int bn = b + 1 < NB ? b + 1 : b;
+                const float cd = rl_f(d[bn], 0);
+                const uint32_t ci = rl_u(id[bn], 0);
+                uint32_t ca = 0, cb = 0;
+                if constexpr (PAY) {
+                    ca = rl_u(pa[bn], 0);
+                    cb = rl_u(pb[bn], 0);
+                }
+                if (lane == 63) {
+                    nd = cd;
+                    ni = ci;
+                    na = ca;
+                    nb = cb;
+                }
+            }
+            d[b] = nd;
+            id[b] = ni;
+            if constexpr (PAY) {
+                pa[b] = na;
+                pb[b] = nb;
+            }
+        }
+        len--;
+    }
+};
+
+}  // namespace ssg

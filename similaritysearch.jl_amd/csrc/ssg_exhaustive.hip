@@ -1,0 +1,567 @@
+// ssg_exhaustive.hip -- exhaustive k-NN (ExhaustiveSearch / ParallelExhaustiveSearch)
+// and rerank! for gfx950.
+//
+// Result contract: the k objects that come first in (d, id) order, where d is the
+// canonical distance (ssg_device.h) -- exactly what the ascending scan of
+// src/exact/sequential-exhaustive.jl:68-80 leaves in a KnnSorted
+// (src/pqueue/knnsorted.jl:140-168), and what the batch merge of
+// src/exact/parallel-exhaustive.jl:79-98 reproduces.
+//
+// Stage 1 (MFMA): block = 64 queries x one DB slice.  Each wave owns 16 queries
+//   (the B operand of v_mfma_f32_16x16x4_f32, kept in registers for the whole
+//   slice); 64-row DB tiles are staged in LDS and shared by the 4 waves.  The dot
+//   products give approximate distances ||x||^2 + ||q||^2 - 2 x.q (or 1 - x.q)
+//   that are filtered against a per-query threshold tau into an LDS candidate
+//   buffer; a full buffer is compacted to its KC best (d, id) entries.  The slice
+//   emits its KC candidates and tau, a lower bound on the approximate distance of
+//   every row of the slice that it dropped.  The K dimension is permuted so each
+//   lane's A/B run is contiguous (dot products are order-free sums).
+// Stage 2: one wave per query merges the slices' lists, radix-selects the 64 best
+//   approximate candidates, evaluates their canonical distances, sorts by (d, id)
+//   and writes the top k.  A certificate checks that every dropped object is
+//   provably farther than the k-th exact distance, given an error bound on the
+//   approximate distances; a query that fails it is flagged.
+// Stage 3: flagged queries are rescanned exactly (canonical distance over every
+//   row, ascending id, KnnSorted pushes).
+#include "ssg_device.h"
+#include "ssg_kernels.h"
+
+namespace ssg {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int TR = 64;          // DB rows per LDS tile
+constexpr int CAP = 64;         // LDS candidate slots per query
+constexpr int SEL = 64;         // candidates rescored exactly in stage 2
+
+__device__ __forceinline__ uint32_t sort_key(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k)
+{
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
+
+// Keep the KC best (d, id) entries of query slot q; returns the new tau.
+template <int KC>
+__device__ __forceinline__ float compact(float* cd, uint32_t* ci, int* cnt, int lane)
+{
+    const int c = min((int)rfl_u((uint32_t)*cnt), CAP);
+    const float dv = lane < c ? cd[lane] : __builtin_inff();
+    const uint32_t iv = lane < c ? ci[lane] : 0xffffffffu;
+    int rank = 0;
+    for (int t = 0; t < c; t++) {
+        const float dt = rl_f(dv, t);
+        const uint32_t it = rl_u(iv, t);
+        rank += (dt < dv || (dt == dv && it < iv)) ? 1 : 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    if (lane < c && rank < KC) {
+        cd[rank] = dv;
+        ci[rank] = iv;
+    }
+    float tau = __builtin_inff();
+    if (c >= KC) {
+        const uint64_t b = __ballot(lane < c && rank == KC - 1);
+        tau = rl_f(dv, __ffsll((unsigned long long)b) - 1);
+    }
+    if (lane == 0) *cnt = c < KC ? c : KC;
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    return tau;
+}
+
+template <int DP, int KC>
+__global__ __launch_bounds__(256) void exh_stage1(const float* __restrict__ X,
+                                                  const float* __restrict__ xnorm, int64_t n,
+                                                  int metric, const float* __restrict__ qpad,
+                                                  const float* __restrict__ qnorm, int64_t rows_per_slice,
+                                                  int nslices, float* cand_d, uint32_t* cand_i,
+                                                  float* slice_tau)
+{
+    constexpr int LS = DP + 4;  // LDS row stride (floats)
+    constexpr int KS = DP / 4;  // MFMA k-steps
+    __shared__ float s_x[TR * LS];
+    __shared__ float s_xn[TR];
+    __shared__ float s_cd[EXH_QT][CAP];
+    __shared__ uint32_t s_ci[EXH_QT][CAP];
+    __shared__ int s_cnt[EXH_QT];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int col = lane & 15, gk = lane >> 4;
+    const int64_t qt = blockIdx.x;
+    const int sl = blockIdx.y;
+    const int64_t r0 = (int64_t)sl * rows_per_slice;
+    const int64_t r1 = min(n, r0 + rows_per_slice);
+    const int qloc = 16 * w + col;
+    const int64_t qj = qt * EXH_QT + qloc;
+
+    float bq[KS];
+    {
+        const float* qp = qpad + qj * DP + gk * KS;
+#pragma unroll
+        for (int s = 0; s < KS; s += 4) {
+            const float4 v = *reinterpret_cast<const float4*>(qp + s);
+            bq[s] = v.x;
+            bq[s + 1] = v.y;
+            bq[s + 2] = v.z;
+            bq[s + 3] = v.w;
+        }
+    }
+    const float qn = qnorm[qj];
+    float tau = __builtin_inff();
+    if (tid < EXH_QT) s_cnt[tid] = 0;
+
+    for (int64_t t0 = r0; t0 < r1; t0 += TR) {
+        __syncthreads();
+        for (int idx = tid; idx < TR * (DP / 4); idx += 256) {
+            const int row = idx / (DP / 4), c4 = idx % (DP / 4);
+            const int64_t gr = t0 + row;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (gr < r1) v = reinterpret_cast<const float4*>(X + gr * DP)[c4];
+            *reinterpret_cast<float4*>(&s_x[row * LS + 4 * c4]) = v;
+        }
+        if (tid < TR) s_xn[tid] = (t0 + tid < r1) ? xnorm[t0 + tid] : 0.f;
+        __syncthreads();
+#pragma unroll 1
+        for (int rb = 0; rb < TR / 16; rb++) {
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+            const float* ap = &s_x[(rb * 16 + col) * LS + gk * KS];
+#pragma unroll
+            for (int s = 0; s < KS; s++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[s], bq[s], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = rb * 16 + gk * 4 + r;
+                const int64_t gid = t0 + row;
+                const float dot = acc[r];
+                const float da = metric == M_NORMCOS ? 1.0f - dot : (s_xn[row] + qn) - 2.0f * dot;
+                if (gid < r1 && da < tau) {
+                    const int p = atomicAdd(&s_cnt[qloc], 1);
+                    if (p < CAP) {
+                        s_cd[qloc][p] = da;
+                        s_ci[qloc][p] = (uint32_t)gid;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            asm volatile("" ::: "memory");
+            const int cq = lane < 16 ? s_cnt[16 * w + lane] : 0;
+            uint64_t need = __ballot(lane < 16 && cq > CAP - 16);
+            while (need) {
+                const int qq = __ffsll((unsigned long long)need) - 1;
+                need &= need - 1;
+                const float nt = compact<KC>(s_cd[16 * w + qq], s_ci[16 * w + qq], &s_cnt[16 * w + qq], lane);
+                if (col == qq) tau = nt;
+            }
+        }
+    }
+    __syncthreads();
+    // final: every query of this wave down to KC entries, then out
+    for (int qq = 0; qq < 16; qq++) {
+        const int ql = 16 * w + qq;
+        const int c = (int)rfl_u((uint32_t)s_cnt[ql]);
+        float nt = rl_f(tau, qq);
+        if (c > KC) nt = compact<KC>(s_cd[ql], s_ci[ql], &s_cnt[ql], lane);
+        const int cc = min(c, KC);
+        const int64_t q = qt * EXH_QT + ql;
+        const int64_t base = (q * nslices + sl) * KC;
+        if (lane < KC) {
+            cand_d[base + lane] = lane < cc ? s_cd[ql][lane] : __builtin_inff();
+            cand_i[base + lane] = lane < cc ? s_ci[ql][lane] : 0xffffffffu;
+        }
+        if (lane == 0) slice_tau[q * nslices + sl] = nt;
+    }
+}
+
+// Stage 2: merge, exact rescoring of SEL candidates, certificate.
+template <int V>
+__global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, int64_t n, int metric,
+                                                  const float* __restrict__ qpad,
+                                                  const float* __restrict__ qnorm, float xnmax,
+                                                  int64_t m, int k, int nslices, int kc,
+                                                  const float* __restrict__ cand_d,
+                                                  const uint32_t* __restrict__ cand_i,
+                                                  const float* __restrict__ slice_tau,
+                                                  uint32_t* ids, float* dists, int kstride,
+                                                  unsigned int* flags)
+{
+    constexpr int PER = 32;  // entries per lane (nslices * kc <= 2048)
+    __shared__ uint32_t s_sel[4][SEL];
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int g = lane >> 4, j = lane & 15;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= m) return;
+    const int total = nslices * kc;
+    const int dp = 16 * V;
+
+    uint32_t key[PER];
+    uint32_t cid[PER];
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const int e = lane + 64 * t;
+        key[t] = 0xffffffffu;
+        cid[t] = 0xffffffffu;
+        if (e < total) {
+            const uint32_t id = cand_i[q * total + e];
+            if (id != 0xffffffffu) {
+                key[t] = sort_key(cand_d[q * total + e]);
+                cid[t] = id;
+            }
+        }
+    }
+    float tmin = __builtin_inff();
+    for (int s = lane; s < nslices; s += 64) tmin = fminf(tmin, slice_tau[q * nslices + s]);
+    for (int o = 32; o >= 1; o >>= 1) tmin = fminf(tmin, __shfl_xor(tmin, o));
+
+    int nvalid = 0;
+#pragma unroll
+    for (int t = 0; t < PER; t++) nvalid += key[t] != 0xffffffffu ? 1 : 0;
+    for (int o = 32; o >= 1; o >>= 1) nvalid += __shfl_xor(nvalid, o);
+
+    // radix select: T = SEL-th smallest key (or the largest valid one)
+    const int want0 = nvalid < SEL ? nvalid : SEL;
+    uint32_t T = 0xffffffffu;
+    if (want0 > 0) {
+        uint32_t prefix = 0;
+        int want = want0;
+        for (int bit = 31; bit >= 0; bit--) {
+            const uint32_t hi_mask = (bit == 31) ? 0u : (0xffffffffu << (bit + 1));
+            int c = 0;
+#pragma unroll
+            for (int t = 0; t < PER; t++)
+                c += (key[t] != 0xffffffffu && (key[t] & hi_mask) == prefix && !((key[t] >> bit) & 1u)) ? 1 : 0;
+            for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+            if (c < want) {
+                want -= c;
+                prefix |= (1u << bit);
+            }
+        }
+        T = prefix;
+    }
+    // gather the selection in index order: keys < T, then keys == T while room
+    int nsel = 0;
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const bool lt = key[t] < T;
+        const uint64_t bl = __ballot(lt);
+        if (lt) s_sel[w][nsel + __popcll(bl & ((1ull << lane) - 1ull))] = cid[t];
+        nsel += __popcll(bl);
+    }
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const bool eq = key[t] == T && key[t] != 0xffffffffu;
+        const uint64_t be = __ballot(eq);
+        const int p = nsel + __popcll(be & ((1ull << lane) - 1ull));
+        if (eq && p < want0) s_sel[w][p] = cid[t];
+        nsel = min(want0, nsel + __popcll(be));
+    }
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    const bool dropped = nvalid > nsel;
+
+    // exact canonical distances of the selection: 4 per round, 16 rounds
+    float qv[V];
+    {
+        const float* qp = qpad + q * dp + j * V;
+#pragma unroll
+        for (int t = 0; t < V; t++) qv[t] = qp[t];
+    }
+    float myd = __builtin_inff();
+    double mys = 0.0;
+    uint32_t myid = 0xffffffffu;
+    for (int r0 = 0; r0 < nsel; r0 += 16) {
+        float xv[4][V];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int u = r0 + 4 * r + g;
+            if (u < nsel) {
+                load_run<V>(X + (size_t)s_sel[w][u] * dp + j * V, xv[r]);
+            } else {
+#pragma unroll
+                for (int t = 0; t < V; t++) xv[r][t] = 0.f;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const double s = group16_sum(run_partial<V>(metric, xv[r], qv));
+            const float d = finish_dist(metric, s);
+            // candidate u = r0 + 4r + g goes to lane u
+            const float dt = __shfl(d, (lane & 3) * 16);
+            const double st = __shfl(s, (lane & 3) * 16);
+            if (lane >= r0 && lane < r0 + 16 && ((lane - r0) >> 2) == r && lane < nsel) {
+                myd = dt;
+                mys = st;
+                myid = s_sel[w][lane];
+            }
+        }
+    }
+    int rank = 0;
+    for (int t = 0; t < nsel; t++) {
+        const float dt = rl_f(myd, t);
+        const uint32_t it = rl_u(myid, t);
+        rank += (dt < myd || (dt == myd && it < myid)) ? 1 : 0;
+    }
+    if (lane < nsel && rank < k) {
+        ids[q * kstride + rank] = myid + 1u;
+        dists[q * kstride + rank] = myd;
+    }
+    if (lane >= nsel && lane < k) {
+        ids[q * kstride + lane] = 0u;
+        dists[q * kstride + lane] = __builtin_inff();
+    }
+    // certificate
+    float bound = tmin;
+    if (dropped) bound = fminf(bound, key_float(T));
+    bool ok = true;
+    if (bound < __builtin_inff()) {
+        if (nsel < k) {
+            ok = false;
+        } else {
+            const uint64_t bk = __ballot(lane < nsel && rank == k - 1);
+            const double sk = __shfl(mys, __ffsll((unsigned long long)bk) - 1);
+            const double dk = metric == M_NORMCOS ? 1.0 - sk : sk;
+            const double err = (double)(dp + 8) * 2.384185791015625e-07 * ((double)xnmax + (double)qnorm[q]);
+            ok = ((double)bound - err) > dk + fabs(dk) * 1e-6 + 1e-30;
+        }
+    }
+    if (!ok && lane == 0) {
+        const unsigned int p = atomicAdd(&flags[0], 1u);
+        flags[1 + p] = (unsigned int)q;
+    }
+}
+
+// Stage 3a: exact rescans.  Block (f, part): part of the rows for flagged query f;
+// each wave scans a contiguous range in ascending id order into a KnnSorted.
+template <int V>
+__global__ __launch_bounds__(256) void exh_rescan(const float* __restrict__ X, int64_t n, int metric,
+                                                  const float* __restrict__ qpad,
+                                                  const unsigned int* flags, int f0, int k,
+                                                  uint32_t* part_i, float* part_d, int nparts)
+{
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int g = lane >> 4, j = lane & 15;
+    const int f = f0 + blockIdx.x;
+    if (f >= (int)flags[0]) return;
+    const int64_t q = flags[1 + f];
+    const int dp = 16 * V;
+    const int wp = blockIdx.y * 4 + w;
+    const int64_t per = (n + (int64_t)nparts * 4 - 1) / ((int64_t)nparts * 4);
+    const int64_t a0 = (int64_t)wp * per, a1 = min(n, a0 + per);
+    float qv[V];
+#pragma unroll
+    for (int t = 0; t < V; t++) qv[t] = qpad[q * dp + j * V + t];
+    RegQueue<1, false> res;
+    res.init(k);
+    for (int64_t b = a0; b < a1; b += 4) {
+        float xv[V];
+        const int64_t row = b + g;
+        if (row < a1) {
+            load_run<V>(X + row * dp + j * V, xv);
+        } else {
+#pragma unroll
+            for (int t = 0; t < V; t++) xv[t] = 0.f;
+        }
+        const float d = finish_dist(metric, group16_sum(run_partial<V>(metric, xv, qv)));
+        for (int r = 0; r < 4 && b + r < a1; r++) res.push(rl_f(d, 16 * r), (uint32_t)(b + r), 0, 0);
+    }
+    const int64_t base = ((int64_t)blockIdx.x * nparts * 4 + wp) * k;
+    if (lane < k) {
+        part_i[base + lane] = lane < res.len ? res.id[0] : 0xffffffffu;
+        part_d[base + lane] = lane < res.len ? res.d[0] : __builtin_inff();
+    }
+}
+
+// Stage 3b: merge the partial lists in ascending row-range order.
+__global__ __launch_bounds__(64) void exh_rescan_merge(const unsigned int* flags, int f0, int k,
+                                                      const uint32_t* part_i, const float* part_d,
+                                                      int nparts, uint32_t* ids, float* dists,
+                                                      int kstride)
+{
+    const int lane = lane_id();
+    const int f = f0 + blockIdx.x;
+    if (f >= (int)flags[0]) return;
+    const int64_t q = flags[1 + f];
+    RegQueue<1, false> res;
+    res.init(k);
+    const int lists = nparts * 4;
+    for (int l = 0; l < lists; l++) {
+        const int64_t base = ((int64_t)blockIdx.x * lists + l) * k;
+        const uint32_t ci = lane < k ? part_i[base + lane] : 0xffffffffu;
+        const float cd = lane < k ? part_d[base + lane] : __builtin_inff();
+        for (int t = 0; t < k; t++) {
+            const uint32_t it = rl_u(ci, t);
+            if (it == 0xffffffffu) break;
+            res.push(rl_f(cd, t), it, 0, 0);
+        }
+    }
+    if (lane < k) {
+        ids[q * kstride + lane] = lane < res.len ? res.id[0] + 1u : 0u;
+        dists[q * kstride + lane] = lane < res.len ? res.d[0] : __builtin_inff();
+    }
+}
+
+// Pad rows to dp floats and compute FP32 squared norms.
+__global__ void pad_rows_kernel(const float* __restrict__ src, int64_t rows, int dim,
+                                float* __restrict__ dst, int dp, float* norms)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    float s = 0.f;
+    for (int i = 0; i < dp; i++) {
+        const float v = i < dim ? src[r * dim + i] : 0.f;
+        dst[r * dp + i] = v;
+        s = __builtin_fmaf(v, v, s);
+    }
+    if (norms) norms[r] = s;
+}
+
+hipError_t launch_pad_rows(const float* src, int64_t rows, int32_t dim, float* dst, int32_t dp,
+                           float* norms, hipStream_t s)
+{
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, src,
+                       rows, dim, dst, dp, norms);
+    return hipGetLastError();
+}
+
+// rerank! batch form: rescore the non-zero-id prefix, stable sort by distance.
+template <int V>
+__global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ X, int metric,
+                                                     const float* __restrict__ Q, int qdim,
+                                                     int64_t m, int k, uint32_t* ids, float* dists)
+{
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int g = lane >> 4, j = lane & 15;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= m) return;
+    const int dp = 16 * V;
+    const uint32_t myid = lane < k ? ids[q * k + lane] : 0u;
+    const uint64_t z = __ballot(lane < k && myid == 0u);
+    const int len = z ? (__ffsll((unsigned long long)z) - 1) : k;
+    float qv[V];
+#pragma unroll
+    for (int t = 0; t < V; t++) {
+        const int i = j * V + t;
+        qv[t] = i < qdim ? Q[q * qdim + i] : 0.f;
+    }
+    float myd = lane < k ? dists[q * k + lane] : 0.f;
+    for (int r0 = 0; r0 < len; r0 += 4) {
+        const int u = r0 + g;
+        float xv[V];
+        const uint32_t id = __shfl(myid, u < len ? u : 0);
+        if (u < len) {
+            load_run<V>(X + (size_t)(id - 1u) * dp + j * V, xv);
+        } else {
+#pragma unroll
+            for (int t = 0; t < V; t++) xv[t] = 0.f;
+        }
+        const float d = finish_dist(metric, group16_sum(run_partial<V>(metric, xv, qv)));
+        const float dt = __shfl(d, ((lane - r0) & 3) * 16);
+        if (lane >= r0 && lane < r0 + 4 && lane < len) myd = dt;
+    }
+    int rank = 0;
+    for (int t = 0; t < len; t++) {
+        const float dt = rl_f(myd, t);
+        rank += (dt < myd || (dt == myd && t < lane)) ? 1 : 0;
+    }
+    if (lane < len) {
+        ids[q * k + rank] = myid;
+        dists[q * k + rank] = myd;
+    }
+}
+
+int32_t exhaustive_slices(int64_t n, int64_t m, int num_cu)
+{
+    const int64_t qtiles = (m + EXH_QT - 1) / EXH_QT;
+    int64_t s = (4LL * num_cu + qtiles - 1) / qtiles;
+    const int64_t max_by_rows = (n + 4 * TR - 1) / (4 * TR);
+    if (s > max_by_rows) s = max_by_rows;
+    if (s < 1) s = 1;
+    if (s > 2048 / EXH_KC) s = 2048 / EXH_KC;
+    return (int32_t)s;
+}
+
+template <int DP>
+static void stage1_dp(dim3 grid, hipStream_t s, const ExhArgs& a, const ExhWork& w, int64_t rps,
+                      int ns)
+{
+    hipLaunchKernelGGL((exh_stage1<DP, EXH_KC>), grid, dim3(256), 0, s, a.X, a.xnorm, a.n,
+                       a.metric, w.qpad, w.qnorm, rps, ns, w.cand_d, w.cand_i, w.slice_tau);
+}
+
+hipError_t exh_launch_stage1(const ExhArgs& a, const ExhWork& w, int nslices, hipStream_t s)
+{
+    const int64_t qtiles = (a.m + EXH_QT - 1) / EXH_QT;
+    int64_t rps = (a.n + nslices - 1) / nslices;
+    rps = (rps + TR - 1) / TR * TR;
+    const dim3 grid((unsigned)qtiles, (unsigned)nslices);
+    switch (a.dp) {
+    case 16: stage1_dp<16>(grid, s, a, w, rps, nslices); break;
+    case 32: stage1_dp<32>(grid, s, a, w, rps, nslices); break;
+    case 48: stage1_dp<48>(grid, s, a, w, rps, nslices); break;
+    case 64: stage1_dp<64>(grid, s, a, w, rps, nslices); break;
+    case 80: stage1_dp<80>(grid, s, a, w, rps, nslices); break;
+    case 96: stage1_dp<96>(grid, s, a, w, rps, nslices); break;
+    case 112: stage1_dp<112>(grid, s, a, w, rps, nslices); break;
+    case 128: stage1_dp<128>(grid, s, a, w, rps, nslices); break;
+    case 192: stage1_dp<192>(grid, s, a, w, rps, nslices); break;
+    case 256: stage1_dp<256>(grid, s, a, w, rps, nslices); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t exh_launch_stage2(const ExhArgs& a, const ExhWork& w, float xnmax, int nslices,
+                             uint32_t* ids, float* dists, int kstride, hipStream_t s)
+{
+    const dim3 grid((unsigned)((a.m + 3) / 4)), block(256);
+    switch (a.dp / 16) {
+#define SSG_S2(VV) \
+    case VV: hipLaunchKernelGGL((exh_stage2<VV>), grid, block, 0, s, a.X, a.n, a.metric, w.qpad, \
+                                w.qnorm, xnmax, a.m, a.k, nslices, EXH_KC, w.cand_d, w.cand_i, \
+                                w.slice_tau, ids, dists, kstride, w.flags); break;
+        SSG_S2(1) SSG_S2(2) SSG_S2(3) SSG_S2(4) SSG_S2(5) SSG_S2(6) SSG_S2(7) SSG_S2(8) SSG_S2(12) SSG_S2(16)
+#undef SSG_S2
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t exh_launch_rescan(const ExhArgs& a, const ExhWork& w, int f0, int nf, uint32_t* part_i,
+                             float* part_d, int nparts, uint32_t* ids, float* dists, int kstride,
+                             hipStream_t s)
+{
+    const dim3 grid((unsigned)nf, (unsigned)nparts), block(256);
+    switch (a.dp / 16) {
+#define SSG_S3(VV) \
+    case VV: hipLaunchKernelGGL((exh_rescan<VV>), grid, block, 0, s, a.X, a.n, a.metric, w.qpad, \
+                                w.flags, f0, a.k, part_i, part_d, nparts); break;
+        SSG_S3(1) SSG_S3(2) SSG_S3(3) SSG_S3(4) SSG_S3(5) SSG_S3(6) SSG_S3(7) SSG_S3(8) SSG_S3(12) SSG_S3(16)
+#undef SSG_S3
+    default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(exh_rescan_merge, dim3((unsigned)nf), dim3(64), 0, s, w.flags, f0, a.k,
+                       part_i, part_d, nparts, ids, dists, kstride);
+    return hipGetLastError();
+}
+
+hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float* Q, int32_t qdim,
+                         int64_t m, int32_t k, uint32_t* ids, float* dists, hipStream_t s)
+{
+    const dim3 grid((unsigned)((m + 3) / 4)), block(256);
+    switch (dp / 16) {
+#define SSG_RR(VV) \
+    case VV: hipLaunchKernelGGL((rerank_kernel<VV>), grid, block, 0, s, X, metric, Q, qdim, m, k, ids, dists); break;
+        SSG_RR(1) SSG_RR(2) SSG_RR(3) SSG_RR(4) SSG_RR(5) SSG_RR(6) SSG_RR(7) SSG_RR(8) SSG_RR(12) SSG_RR(16)
+#undef SSG_RR
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ssg

@@ -1,0 +1,95 @@
+// ssg_kernels.h -- argument blocks and launchers of the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ssg {
+
+// Device layout of one index (one handle):
+//   X      n rows of dp floats (dp = roundup(dim,16), zero padded), 64-byte aligned rows
+//   xnorm  n floats: squared norms (SqL2/L2) used by the MFMA tiles
+//   offs   n+1 int64, exclusive prefix (offs[0] = 0) of the CSR lists
+//   adj    nnz uint32, 0-based neighbour ids
+//   hints  nhints uint32, 1-based (as SearchGraph.hints)
+struct BeamArgs {
+    const float* X;
+    int64_t n;
+    int32_t dp;
+    int32_t metric;
+    const int64_t* offs;
+    const uint32_t* adj;
+    const uint32_t* hints;  // 1-based
+    int32_t nhints;
+    int32_t log2n;          // ceil(log2(1+n)), beamsearch.jl:30
+    const float* Q;         // (m, qdim) rows, unpadded
+    int32_t qdim;
+    int64_t m;
+    int32_t k;
+    const int32_t* kq;
+    int32_t kstride;
+    int32_t bsize;
+    const int32_t* bq;
+    float delta;
+    int64_t maxvisits;
+    int32_t beamcap;
+    const uint32_t* qhints;  // 1-based, [m * qhstride]
+    const int32_t* qhcount;
+    int32_t qhstride;
+    uint32_t* ids;
+    float* dists;
+    int64_t* costd;
+    int64_t* costb;
+    uint8_t* vis;            // nslots * vis_stride epoch bytes
+    uint8_t* epochs;         // nslots
+    int64_t vis_stride;
+    unsigned int* counter;
+    int32_t has_dups;
+    int32_t allknn;
+};
+
+// Launch the persistent beam-search kernel over nslots waves (a multiple of 4).
+hipError_t launch_beam(const BeamArgs& a, int nslots, hipStream_t s);
+hipError_t launch_zero_u32(unsigned int* p, int64_t n, hipStream_t s);
+hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
+
+struct ExhArgs {
+    const float* X;
+    const float* xnorm;
+    int64_t n;
+    int32_t dp;
+    int32_t metric;
+    const float* Q;
+    int32_t qdim;
+    int64_t m;
+    int32_t k;
+};
+
+// Exhaustive k-NN: MFMA distance tiles + per-slice candidate selection, then an
+// exact rerank/merge with a certificate, then exact rescans of any query whose
+// certificate failed.  Work buffers are owned by the caller (the handle).
+struct ExhWork {
+    float* qpad;        // m_pad * dp
+    float* qnorm;       // m_pad
+    float* cand_d;      // m_pad * nslices * KC
+    uint32_t* cand_i;   // m_pad * nslices * KC
+    float* slice_tau;   // m_pad * nslices
+    unsigned int* flags;   // [0] = number of flagged queries, then the list
+    int64_t m_cap;
+    int32_t nslices_cap;
+};
+hipError_t exh_launch_stage1(const ExhArgs& a, const ExhWork& w, int nslices, hipStream_t s);
+hipError_t exh_launch_stage2(const ExhArgs& a, const ExhWork& w, float xnmax, int nslices,
+                             uint32_t* ids, float* dists, int kstride, hipStream_t s);
+hipError_t exh_launch_rescan(const ExhArgs& a, const ExhWork& w, int f0, int nf, uint32_t* part_i,
+                             float* part_d, int nparts, uint32_t* ids, float* dists, int kstride,
+                             hipStream_t s);
+int32_t exhaustive_slices(int64_t n, int64_t m, int num_cu);
+constexpr int EXH_KC = 32;        // candidates kept per (query, slice)
+constexpr int EXH_QT = 64;        // queries per block
+
+hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float* Q, int32_t qdim,
+                         int64_t m, int32_t k, uint32_t* ids, float* dists, hipStream_t s);
+hipError_t launch_pad_rows(const float* src, int64_t rows, int32_t dim, float* dst, int32_t dp,
+                           float* norms, hipStream_t s);
+
+}  // namespace ssg

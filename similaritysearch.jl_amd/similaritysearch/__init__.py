@@ -1,0 +1,383 @@
+"""similaritysearch -- host-side mirror of SimilaritySearch.jl's query API over the
+MI355X C ABI (libssg_hip.so, include/ssg.h).
+
+Names follow the reference (`searchbatch`, `searchbatch!` -> `searchbatch_`,
+`allknn`, `rerank!` -> `rerank_`, `macrorecall`, `ExhaustiveSearch`,
+`ParallelExhaustiveSearch`, `SearchGraph`, `BeamSearch`, `GenericContext`,
+`SearchGraphContext`, `MatrixDatabase`, `StaticAdjList`, `Dist.SqL2/L2/NormCosine`).
+
+Layouts are the reference's: a database or query set is a (dim, n) matrix whose
+columns are objects (a numpy array of shape (dim, n); its memory is taken as the
+(n, dim) row-major run of objects), results are (k, m) matrices with one column
+per query (numpy arrays of shape (k, m) in Fortran order), ids are 1-based
+uint32 with 0 for an empty slot and +Inf as its distance.  Every search runs on
+the GPU; there is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import SSGError, BeamParams, QueryOpts
+
+__all__ = ["Dist", "SqL2", "L2", "NormCosine", "MatrixDatabase", "StaticAdjList", "BeamSearch",
+           "GenericContext", "SearchGraphContext", "ExhaustiveSearch", "ParallelExhaustiveSearch",
+           "SearchGraph", "searchbatch", "searchbatch_", "searchbatch_knns", "allknn", "rerank_",
+           "macrorecall", "recallscore", "SSGError"]
+
+
+# --------------------------------------------------------------------------- distances
+class SqL2:
+    """Squared Euclidean distance (src/dist/vecs.jl:86-102)."""
+    metric = _lib.SSG_SQL2
+
+
+class L2:
+    """Euclidean distance (src/dist/vecs.jl:52-67)."""
+    metric = _lib.SSG_L2
+
+
+class NormCosine:
+    """1 - dot(u, v) for unit vectors (src/dist/cos.jl:39,76-85)."""
+    metric = _lib.SSG_NORMCOS
+
+
+class Dist:
+    SqL2 = SqL2
+    L2 = L2
+    NormCosine = NormCosine
+
+
+def _metric(dist):
+    m = getattr(dist, "metric", None)
+    if m is None:
+        raise ValueError(f"unsupported distance {dist!r}: use Dist.SqL2, Dist.L2 or Dist.NormCosine")
+    return m
+
+
+# --------------------------------------------------------------------------- data
+class MatrixDatabase:
+    """Column-major (dim, n) Float32 database (src/db/matrixdatabase.jl:27-59)."""
+
+    def __init__(self, X):
+        X = np.asarray(X, dtype=np.float32)
+        if X.ndim != 2:
+            raise ValueError("MatrixDatabase expects a (dim, n) matrix")
+        self.rows = np.ascontiguousarray(X.T)
+
+    @classmethod
+    def from_rows(cls, R):
+        """Wrap an (n, dim) row-major array of objects without reshaping."""
+        db = cls.__new__(cls)
+        db.rows = np.ascontiguousarray(R, dtype=np.float32)
+        return db
+
+    @property
+    def matrix(self):
+        return self.rows.T
+
+    @property
+    def dim(self):
+        return self.rows.shape[1]
+
+    def __len__(self):
+        return self.rows.shape[0]
+
+    def __getitem__(self, i):
+        """Object i (1-based, like the reference)."""
+        return self.rows[i - 1]
+
+
+def _as_db(Q):
+    return Q if isinstance(Q, MatrixDatabase) else MatrixDatabase(Q)
+
+
+class StaticAdjList:
+    """Frozen CSR adjacency (src/adj/adjstatic.jl:27-30): node i's neighbours are
+    end_point[offset[i-1]+1 : offset[i]] with 1-based ids (offset[0] implicit 0)."""
+
+    def __init__(self, offset, end_point):
+        self.offset = np.ascontiguousarray(offset, dtype=np.int64)
+        self.end_point = np.ascontiguousarray(end_point, dtype=np.uint32)
+
+    @classmethod
+    def from_csr(cls, offs, adj):
+        """From an (n+1)-entry exclusive prefix and 1-based ids."""
+        offs = np.asarray(offs, dtype=np.int64)
+        return cls(offs[1:], adj)
+
+    def __len__(self):
+        return len(self.offset)
+
+    def neighbors(self, i):
+        s = 0 if i == 1 else int(self.offset[i - 2])
+        return self.end_point[s:int(self.offset[i - 1])]
+
+    def neighbors_length(self, i):
+        return int(self.offset[i - 1]) - (0 if i == 1 else int(self.offset[i - 2]))
+
+
+# --------------------------------------------------------------------------- search params
+class BeamSearch:
+    """BeamSearch(; bsize=4, Δ=1.0, maxvisits=10^6) (src/searchgraph/SearchGraph.jl:94-100)."""
+
+    def __init__(self, bsize=4, delta=1.0, maxvisits=10**6, **kw):
+        if "Δ" in kw:
+            delta = kw.pop("Δ")
+        if kw:
+            raise TypeError(f"unknown arguments {sorted(kw)}")
+        self.bsize = int(bsize)
+        self.delta = float(np.float32(delta))
+        self.maxvisits = int(maxvisits)
+
+    def __repr__(self):
+        return f"BeamSearch(bsize={self.bsize}, Δ={self.delta}, maxvisits={self.maxvisits})"
+
+
+class GenericContext:
+    """Context of exhaustive indexes.  costdists holds the per-query distance
+    evaluations of the last batch call (src/SimilaritySearch.jl:187-258)."""
+
+    def __init__(self, verbose=False):
+        self.verbose = verbose
+        self.costdists = np.zeros(0, np.int64)
+        self.costblocks = np.zeros(0, np.int64)
+
+
+class SearchGraphContext(GenericContext):
+    """Context of SearchGraph; beamcap is the number of rows of its beam buffers,
+    the cap getbeam applies to bsize (src/searchgraph/context.jl:119-126,220-225)."""
+
+    def __init__(self, beamcap=32, verbose=False):
+        super().__init__(verbose)
+        if not 1 <= beamcap <= 256:
+            raise ValueError("beamcap must be in 1..256")
+        self.beamcap = int(beamcap)
+
+
+# --------------------------------------------------------------------------- indexes
+class _GpuIndex:
+    def __init__(self, dist, db, device=0):
+        self.dist = dist
+        self.db = _as_db(db)
+        self.device = device
+        self._h = None
+
+    def _handle(self):
+        if self._h is None:
+            L = _lib.load()
+            h = C.c_void_p()
+            _lib.check(L.ssg_create(C.byref(h), self.device), "ssg_create")
+            self._h = h
+            X = self.db.rows
+            _lib.check(L.ssg_upload_db(h, X.ctypes.data, X.shape[1], X.shape[0],
+                                       _metric(self.dist)), "ssg_upload_db")
+            self._after_upload()
+        return self._h
+
+    def _after_upload(self):
+        pass
+
+    def close(self):
+        if self._h is not None:
+            _lib.load().ssg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.db)
+
+    def database(self, i=None):
+        return self.db if i is None else self.db[i]
+
+    def distance(self):
+        return self.dist
+
+
+class ExhaustiveSearch(_GpuIndex):
+    """Exact k-NN index (src/exact/sequential-exhaustive.jl)."""
+
+    def __init__(self, dist=None, db=None, device=0):
+        super().__init__(dist if dist is not None else SqL2, db, device)
+
+
+class ParallelExhaustiveSearch(ExhaustiveSearch):
+    """Same results as ExhaustiveSearch (src/exact/parallel-exhaustive.jl:64-102);
+    on the GPU the DB split is the MFMA kernel's slice axis."""
+
+
+class SearchGraph(_GpuIndex):
+    """A frozen SearchGraph (src/searchgraph/SearchGraph.jl:147-158)."""
+
+    def __init__(self, dist, db, adj, hints=(), algo=None, device=0):
+        super().__init__(dist, db, device)
+        if not isinstance(adj, StaticAdjList):
+            raise TypeError("adj must be a StaticAdjList")
+        if len(adj) != len(self.db):
+            raise ValueError("graph and database sizes differ")
+        self.adj = adj
+        self.hints = np.ascontiguousarray(hints, dtype=np.uint32)
+        self.algo = algo if algo is not None else BeamSearch()
+
+    def _after_upload(self):
+        a = self.adj
+        _lib.check(_lib.load().ssg_upload_graph(self._h, a.offset.ctypes.data, len(a),
+                                               a.end_point.ctypes.data, len(a.end_point),
+                                               self.hints.ctypes.data, len(self.hints)),
+                   "ssg_upload_graph")
+
+    def beam_params(self, ctx, algo=None):
+        b = algo or self.algo
+        return BeamParams(b.bsize, b.delta, b.maxvisits, getattr(ctx, "beamcap", 32))
+
+
+# --------------------------------------------------------------------------- batch API
+def _out(m, k):
+    ids = np.zeros((m, k), np.uint32)
+    dists = np.full((m, k), np.inf, np.float32)
+    return ids, dists
+
+
+def searchbatch(index, ctx, Q, k, sorted=True):
+    """searchbatch(index, ctx, Q, k) -> (ids, dists), both (k, m)
+    (src/SimilaritySearch.jl:309-313)."""
+    Q = _as_db(Q)
+    m = len(Q)
+    if m == 0:
+        raise ValueError("empty set of queries")
+    ids, dists = _out(m, k)
+    searchbatch_(index, ctx, Q, ids.T, dists.T, sorted=sorted)
+    return ids.T, dists.T
+
+
+def searchbatch_(index, ctx, Q, ids, dists, sorted=False):
+    """searchbatch!(index, ctx, Q, ids, dists) in place on (k, m) Fortran-ordered
+    outputs (src/SimilaritySearch.jl:331-348)."""
+    Q = _as_db(Q)
+    m = len(Q)
+    if m == 0:
+        raise ValueError("empty set of queries")
+    if ids.shape[1] != m or dists.shape != ids.shape:
+        raise ValueError("the number of queries is different from the given output containers")
+    if not (ids.T.flags.c_contiguous and dists.T.flags.c_contiguous):
+        raise ValueError("outputs must be Fortran-ordered (k, m) arrays")
+    if ids.dtype != np.uint32 or dists.dtype != np.float32:
+        raise TypeError("ids must be UInt32 and dists Float32")
+    k = ids.shape[0]
+    L = _lib.load()
+    h = index._handle()
+    rows = Q.rows
+    if rows.shape[1] != index.db.dim:
+        raise ValueError("query dimension differs from the database")
+    if isinstance(index, SearchGraph):
+        cd = np.zeros(m, np.int64)
+        cb = np.zeros(m, np.int64)
+        bp = index.beam_params(ctx)
+        _lib.check(L.ssg_beamsearch(h, rows.ctypes.data, m, k, k, C.byref(bp), None,
+                                    ids.T.ctypes.data, dists.T.ctypes.data, cd.ctypes.data,
+                                    cb.ctypes.data), "searchbatch!")
+        ctx.costdists, ctx.costblocks = cd, cb
+    else:
+        _lib.check(L.ssg_exhaustive_knn(h, rows.ctypes.data, m, k, ids.T.ctypes.data,
+                                        dists.T.ctypes.data), "searchbatch!")
+        ctx.costdists = np.full(m, len(index), np.int64)
+        ctx.costblocks = np.zeros(m, np.int64)
+    return ids, dists
+
+
+def searchbatch_knns(index, ctx, Q, ks, bsizes=None, hints=None):
+    """Queue-vector form searchbatch!(index, ctx, Q, knns) (src/SimilaritySearch.jl:384-398)
+    with per-query k (ks), optional per-query beam sizes and per-query hint lists
+    (1-based; `hints` is a list of sequences).  Returns a list of (ids, dists) pairs."""
+    if not isinstance(index, SearchGraph):
+        raise TypeError("per-query queues are supported over SearchGraph")
+    Q = _as_db(Q)
+    m = len(Q)
+    if m == 0:
+        raise ValueError("empty set of queries")
+    ks = np.ascontiguousarray(ks, np.int32)
+    if len(ks) != m:
+        raise ValueError("the number of queries is different from the given output containers")
+    kstride = int(ks.max())
+    ids, dists = _out(m, kstride)
+    cd = np.zeros(m, np.int64)
+    cb = np.zeros(m, np.int64)
+    keep = [ks]
+    qo = QueryOpts(ks.ctypes.data, None, None, None, 0)
+    if bsizes is not None:
+        b = np.ascontiguousarray(bsizes, np.int32)
+        keep.append(b)
+        qo.bsize = b.ctypes.data
+    if hints is not None:
+        stride = max(1, max(len(x) for x in hints))
+        H = np.zeros((m, stride), np.uint32)
+        cnt = np.zeros(m, np.int32)
+        for j, x in enumerate(hints):
+            H[j, :len(x)] = x
+            cnt[j] = len(x)
+        keep += [H, cnt]
+        qo.hints, qo.nhints, qo.hint_stride = H.ctypes.data, cnt.ctypes.data, stride
+    bp = index.beam_params(ctx)
+    _lib.check(_lib.load().ssg_beamsearch(index._handle(), Q.rows.ctypes.data, m, kstride,
+                                          kstride, C.byref(bp), C.byref(qo), ids.ctypes.data,
+                                          dists.ctypes.data, cd.ctypes.data, cb.ctypes.data),
+               "searchbatch!")
+    ctx.costdists, ctx.costblocks = cd, cb
+    return [(ids[j, :ks[j]].copy(), dists[j, :ks[j]].copy()) for j in range(m)]
+
+
+def allknn(index, ctx, k, sort=True):
+    """allknn(index, ctx, k) -> (ids, dists), both (k, n) (src/allknn.jl:39-91)."""
+    n = len(index)
+    if not 0 < k <= n:
+        raise ValueError("allknn needs 0 < k <= n")
+    ids, dists = _out(n, k)
+    L = _lib.load()
+    h = index._handle()
+    if isinstance(index, SearchGraph):
+        bp = index.beam_params(ctx)
+        _lib.check(L.ssg_allknn_graph(h, k, C.byref(bp), ids.ctypes.data, dists.ctypes.data),
+                   "allknn")
+    else:
+        _lib.check(L.ssg_allknn_exhaustive(h, k, ids.ctypes.data, dists.ctypes.data), "allknn")
+    return ids.T, dists.T
+
+
+_rerank_cache = {}
+
+
+def rerank_(dist, db, Q, ids, dists):
+    """rerank!(dist, db, queries, ids, dists) batch form, in place (src/rerank.jl:77-86)."""
+    Q = _as_db(Q)
+    key = (id(db), _metric(dist))
+    idx = _rerank_cache.get(key)
+    if idx is None or idx.db is not (db if isinstance(db, MatrixDatabase) else idx.db):
+        idx = ExhaustiveSearch(dist, db)
+        _rerank_cache[key] = idx
+    m = len(Q)
+    k = ids.shape[0]
+    if not (ids.T.flags.c_contiguous and dists.T.flags.c_contiguous):
+        raise ValueError("outputs must be Fortran-ordered (k, m) arrays")
+    _lib.check(_lib.load().ssg_rerank(idx._handle(), Q.rows.ctypes.data, m, k, ids.T.ctypes.data,
+                                      dists.T.ctypes.data), "rerank!")
+    return ids, dists
+
+
+# --------------------------------------------------------------------------- metrics
+def recallscore(gold, res):
+    """|Set(gold) ∩ Set(res)| / length(gold) (src/perf.jl:31-33)."""
+    gold = np.asarray(gold)
+    return len(set(gold.tolist()) & set(np.asarray(res).tolist())) / len(gold)
+
+
+def macrorecall(gold, res, k=None):
+    """Mean per-column recallscore of (k, m) id matrices (src/perf.jl:71-79)."""
+    gold = np.asarray(gold)
+    res = np.asarray(res)
+    k = gold.shape[0] if k is None else k
+    return float(np.mean([recallscore(gold[:k, j], res[:k, j]) for j in range(gold.shape[1])]))

@@ -1,0 +1,76 @@
+"""Loader of libssg_hip.so (the C ABI declared in include/ssg.h).
+
+The product path has no CPU fallback: if the library is missing or cannot be
+loaded, every GPU index raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("SSG_HIP_LIB", os.path.join(_HERE, "..", "lib", "libssg_hip.so"))
+
+SSG_SQL2, SSG_L2, SSG_NORMCOS = 0, 1, 2
+
+_p = C.c_void_p
+_i32 = C.c_int32
+_i64 = C.c_int64
+
+
+class BeamParams(C.Structure):
+    _fields_ = [("bsize", C.c_int32), ("delta", C.c_float), ("maxvisits", C.c_int64),
+                ("beamcap", C.c_int32)]
+
+
+class QueryOpts(C.Structure):
+    _fields_ = [("k", _p), ("bsize", _p), ("hints", _p), ("nhints", _p),
+                ("hint_stride", C.c_int32)]
+
+
+# name -> (restype, argtypes); the exported surface of include/ssg.h
+SIGNATURES = {
+    "ssg_version": (C.c_char_p, []),
+    "ssg_last_error": (C.c_char_p, [_p]),
+    "ssg_create": (C.c_int, [C.POINTER(_p), C.c_int]),
+    "ssg_destroy": (C.c_int, [_p]),
+    "ssg_upload_db": (C.c_int, [_p, _p, _i32, _i64, _i32]),
+    "ssg_upload_graph": (C.c_int, [_p, _p, _i64, _p, _i64, _p, _i64]),
+    "ssg_exhaustive_knn": (C.c_int, [_p, _p, _i64, _i32, _p, _p]),
+    "ssg_exhaustive_knn_dev": (C.c_int, [_p, _p, _i64, _i32, _p, _p, _p]),
+    "ssg_beamsearch": (C.c_int, [_p, _p, _i64, _i32, _i32, C.POINTER(BeamParams),
+                                 C.POINTER(QueryOpts), _p, _p, _p, _p]),
+    "ssg_beamsearch_dev": (C.c_int, [_p, _p, _i64, _i32, _i32, C.POINTER(BeamParams),
+                                     C.POINTER(QueryOpts), _p, _p, _p, _p, _p]),
+    "ssg_allknn_graph": (C.c_int, [_p, _i32, C.POINTER(BeamParams), _p, _p]),
+    "ssg_allknn_exhaustive": (C.c_int, [_p, _i32, _p, _p]),
+    "ssg_rerank": (C.c_int, [_p, _p, _i64, _i32, _p, _p]),
+    "ssg_rerank_dev": (C.c_int, [_p, _p, _i64, _i32, _p, _p, _p]),
+}
+
+_lib = None
+
+
+class SSGError(RuntimeError):
+    """A failed libssg_hip call (the message comes from ssg_last_error)."""
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SSGError(f"libssg_hip.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = load().ssg_last_error(None)
+        msg = msg.decode() if msg else ""
+        if rc == -1:
+            raise ValueError(f"{what}: {msg}")
+        raise SSGError(f"{what} failed ({rc}): {msg}")
