@@ -69,14 +69,6 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
 int ssg_upload_graph(ssg_handle* h, const int64_t* offset, int64_t n, const uint32_t* end_point,
                      int64_t nnz, const uint32_t* hints, int64_t nhints);
 
-/* searchbatch!(::ExhaustiveSearch / ::ParallelExhaustiveSearch, ctx, Q, ids, dists)
- * (src/SimilaritySearch.jl:331-348 over src/exact/sequential-exhaustive.jl:68-80 and
- * src/exact/parallel-exhaustive.jl:64-102).  Q: column-major (dim, m). */
-int ssg_exhaustive_knn(ssg_handle* h, const float* Q, int64_t m, int32_t k, uint32_t* ids,
-                       float* dists);
-int ssg_exhaustive_knn_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, uint32_t* dids,
-                           float* ddists, void* stream);
-
 /* Beam-search parameters (BeamSearch, src/searchgraph/SearchGraph.jl:94-100) plus the
  * rows of the context's beam buffers (getbeam clamps bsize to them,
  * src/searchgraph/context.jl:119-126,220-225). */
@@ -86,6 +78,47 @@ typedef struct {
     int64_t maxvisits;
     int32_t beamcap; /* 32 in the default SearchGraphContext; at most 256 */
 } ssg_beam_params;
+
+/* Context knobs of index! (src/searchgraph/insertions.jl:94-113) with the defaults of
+ * SearchGraphContext (src/searchgraph/context.jl:105-138), Neighborhood
+ * (src/searchgraph/SearchGraph.jl:51-56), RandomHints (hints.jl:87-89) and
+ * OptimizeParameters (optbs.jl:62-127): 1.3, 2, 8*nthreads, 1.1, 1.5, 256, 0.97, 32, 10, 32. */
+typedef struct {
+    float nb_logbase;       /* Neighborhood.logbase */
+    int32_t nb_minsize;     /* Neighborhood.minsize */
+    int32_t parallel_block; /* insertion block; >= 2 (the parallel insertion loop) */
+    float hints_logbase;    /* RandomHints.logbase; <= 0 disables the callback */
+    float cb_logbase;       /* logbase_callback */
+    int32_t cb_starting;    /* starting_callback */
+    float opt_minrecall;    /* MinRecall of the OptimizeParameters callback; <= 0 disables it */
+    int32_t opt_numqueries;
+    int32_t opt_ksearch;
+    int32_t beamcap;        /* rows of the context's beam buffers */
+    uint64_t seed;          /* draws of RandomHints and of the optimiser's query sample */
+    int32_t verbose;        /* > 0: progress lines on stderr every `verbose` percent */
+} ssg_build_params;
+
+/* index!(G::SearchGraph, ctx) over the uploaded database: builds the graph on the device
+ * and installs it in the handle, together with its hints and the tuned BeamSearch. */
+int ssg_build_graph(ssg_handle* h, const ssg_build_params* p);
+/* Size of the installed graph and its current BeamSearch parameters. */
+int ssg_graph_info(ssg_handle* h, int64_t* nnz, int64_t* nhints, ssg_beam_params* algo);
+/* Copy the installed graph out in StaticAdjList form (offset[n] inclusive ends, 1-based
+ * end_point[nnz]) plus its 1-based hints (nullable). */
+int ssg_download_graph(ssg_handle* h, int64_t* offset, uint32_t* end_point, uint32_t* hints);
+/* optimize_index!(G, ctx, MinRecall(minrecall); queries=Q, ksearch) (src/opt.jl:220-317):
+ * algo_inout gives beamcap and the current maxvisits and receives the chosen
+ * (bsize, delta, maxvisits); *recall gets the sample recall.  Q: column-major (dim, m). */
+int ssg_optimize_minrecall(ssg_handle* h, const float* Q, int64_t m, int32_t ksearch,
+                           float minrecall, ssg_beam_params* algo_inout, double* recall);
+
+/* searchbatch!(::ExhaustiveSearch / ::ParallelExhaustiveSearch, ctx, Q, ids, dists)
+ * (src/SimilaritySearch.jl:331-348 over src/exact/sequential-exhaustive.jl:68-80 and
+ * src/exact/parallel-exhaustive.jl:64-102).  Q: column-major (dim, m). */
+int ssg_exhaustive_knn(ssg_handle* h, const float* Q, int64_t m, int32_t k, uint32_t* ids,
+                       float* dists);
+int ssg_exhaustive_knn_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, uint32_t* dids,
+                           float* ddists, void* stream);
 
 /* Per-query optional inputs of the queue-vector form searchbatch!(index, ctx, Q, knns)
  * (src/SimilaritySearch.jl:384-398): per-query k and bsize, and per-query hint lists

@@ -69,11 +69,8 @@ struct Searcher {
     {
         const int dp = 16 * V;
         cid = lane < cnt ? cidbuf[ub + lane] : 0u;
-        int64_t o0 = 0, o1 = 0;
-        if (need_deg && lane < cnt) {
-            o0 = a.offs[cid];
-            o1 = a.offs[cid + 1];
-        }
+        uint2 nd = make_uint2(0u, 0u);
+        if (need_deg && lane < cnt) nd = a.node[cid];
         float xv[ROUNDS][V];
 #pragma unroll
         for (int r = 0; r < ROUNDS; r++) {
@@ -102,14 +99,15 @@ struct Searcher {
                 if ((lane >> 2) == r) dd = t;
             }
         }
-        st = (uint32_t)o0;
-        dg = (uint32_t)(o1 - o0);
+        st = nd.x;
+        dg = nd.y;
     }
 
     // One chunk of up to 64 candidate ids (lane-indexed, `valid` lanes).  Returns
     // true when the maxvisits cut ends the search.
     __device__ __forceinline__ bool chunk(int mode, uint32_t cid, bool valid, bool dedupe)
     {
+        valid = valid && (int64_t)cid < a.n;
         bool unv = valid;
         if (mode != MODE_BRUTE) {
             vm_drain();
@@ -197,10 +195,11 @@ struct Searcher {
         const int cap = bsize < a.beamcap ? bsize : a.beamcap;
         beam.init(cap);
         {
+            if (res.len == 0) return;
             const uint32_t nid = res.id_at(0);
-            const float nd = res.dist_at(0);
-            const int64_t o0 = a.offs[nid], o1 = a.offs[nid + 1];
-            beam.push(nd, nid, (uint32_t)o0, (uint32_t)(o1 - o0));
+            const float ndist = res.dist_at(0);
+            const uint2 ni = a.node[nid];
+            beam.push(ndist, nid, ni.x, ni.y);
         }
         while (beam.len > 0) {
             float pd;
@@ -220,7 +219,7 @@ struct Searcher {
 };
 
 template <int V, int NB>
-__global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
+__device__ __forceinline__ void beam_body(const BeamArgs& a)
 {
     __shared__ uint32_t s_cid[WAVES][64];
     const int w = threadIdx.x >> 6;
@@ -265,8 +264,8 @@ __global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
         S.cd_inner = 0;
         S.cb = 0;
         if (a.allknn) {
-            const int64_t o0 = a.offs[qi], o1 = a.offs[qi + 1];
-            for (int64_t t = o0; t < o1; t++) {
+            const uint2 ni = a.node[qi];
+            for (int64_t t = ni.x; t < (int64_t)ni.x + ni.y; t++) {
                 const uint32_t h = a.adj[t];
                 vm_drain();
                 const uint32_t seen = rfl_u((uint32_t)S.vis[h]);
@@ -291,6 +290,21 @@ __global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
         }
     }
     if (lane == 0) a.epochs[slot] = (uint8_t)epoch;
+}
+
+// searchbatch! / allknn! launches
+template <int V, int NB>
+__global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
+{
+    beam_body<V, NB>(a);
+}
+
+// the same search issued by index! and optimize_index! (a separate symbol keeps
+// their launches apart from query traffic in kernel traces)
+template <int V, int NB>
+__global__ __launch_bounds__(256) void beam_build_kernel(BeamArgs a)
+{
+    beam_body<V, NB>(a);
 }
 
 __global__ void zero_u32_kernel(unsigned int* p, int64_t n)
@@ -324,10 +338,17 @@ template <int V>
 static hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
 {
     const dim3 grid((unsigned)(nslots / WAVES)), block(256);
-    if (a.beamcap <= 64)
-        hipLaunchKernelGGL((beam_kernel<V, 1>), grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL((beam_kernel<V, 4>), grid, block, 0, s, a);
+    if (a.internal) {
+        if (a.beamcap <= 64)
+            hipLaunchKernelGGL((beam_build_kernel<V, 1>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((beam_build_kernel<V, 4>), grid, block, 0, s, a);
+    } else {
+        if (a.beamcap <= 64)
+            hipLaunchKernelGGL((beam_kernel<V, 1>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((beam_kernel<V, 4>), grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -1,0 +1,107 @@
+// ssg_handle.h -- the handle behind the C ABI and the helpers shared by the ABI
+// entry points (ssg_abi.hip) and the graph builder (ssg_build.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ssg.h"
+#include "ssg_kernels.h"
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t b = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) cap = b;
+        return e;
+    }
+    // grow keeping the first `keep` bytes
+    hipError_t grow(size_t bytes, size_t keep, hipStream_t s)
+    {
+        if (bytes <= cap) return hipSuccess;
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) return e;
+        if (p && keep) e = hipMemcpyAsync(q, p, keep, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (p) (void)hipFree(p);
+        p = q;
+        cap = bytes;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct ssg_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    std::mutex mu;
+    // database
+    int32_t dim = 0, dp = 0, metric = 0;
+    int64_t n = 0, n_pad = 0;
+    DevBuf X, xnorm;
+    float xnmax = 0.f;
+    // graph: node[i] = {start, deg} into the adj pool (0-based ids), hints 1-based
+    bool has_graph = false;
+    DevBuf node, adj, hints;
+    int64_t nnz = 0, nhints = 0, pool_used = 0;
+    int has_dups = 0;
+    std::vector<std::vector<uint32_t>> lists;  // host copy of the lists (0-based)
+    std::vector<uint32_t> hints_host;          // 1-based
+    ssg_beam_params algo{4, 1.0f, 1000000, 32};
+    // beam scratch
+    DevBuf vis, epochs, counter;
+    int64_t vis_stride = 0;
+    int nslots = 0;
+    // exhaustive work
+    DevBuf qpad, qnorm, cand_d, cand_i, slice_tau, flags, part_i, part_d;
+    // host-call staging
+    DevBuf sQ, sIds, sD, sCd, sCb, sK, sB, sH, sHc;
+    // builder scratch
+    DevBuf bIds, bD, bOut, bCnt, bPairs, bNodeUpd;
+};
+
+int ssg_fail(int code, const std::string& msg);
+int ssg_hipfail(hipError_t e, const char* what);
+
+#define HIPCHK(call, what)                                      \
+    do {                                                        \
+        hipError_t _e = (call);                                 \
+        if (_e != hipSuccess) return ssg_hipfail(_e, what);     \
+    } while (0)
+
+// Beam search over the handle's graph with the first n_eff objects as the index
+// (n_eff = h->n outside graph construction).  Queries are rows of dQ with qdim
+// floats each.
+int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_t k,
+                 int32_t kstride, const ssg_beam_params* bp, const ssg_query_opts* qo,
+                 uint32_t* dids, float* ddists, int64_t* dcd, int64_t* dcb, hipStream_t s,
+                 bool allknn, int64_t n_eff, bool internal = false);
+
+// Exhaustive k-NN over the first n_eff objects.  Either dQ (m rows of h->dim
+// floats) or a prepared (qpad, qnorm) pair with m_pad = roundup(m, 64) rows of dp
+// floats is given.
+int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad, const float* qnorm, int64_t m,
+                int32_t k, int32_t kstride, uint32_t* dids, float* ddists, hipStream_t s,
+                int64_t n_eff);
+
+// Install a frozen CSR graph (0-based lists) on the device.
+int ssg_install_lists(ssg_handle* h);

@@ -8,15 +8,16 @@ namespace ssg {
 // Device layout of one index (one handle):
 //   X      n rows of dp floats (dp = roundup(dim,16), zero padded), 64-byte aligned rows
 //   xnorm  n floats: squared norms (SqL2/L2) used by the MFMA tiles
-//   offs   n+1 int64, exclusive prefix (offs[0] = 0) of the CSR lists
-//   adj    nnz uint32, 0-based neighbour ids
+//   node   n uint2 {start, deg}: node i's list is adj[start .. start+deg)
+//   adj    uint32 pool of 0-based neighbour ids (a frozen CSR, or the growable
+//          pool of the GPU graph builder)
 //   hints  nhints uint32, 1-based (as SearchGraph.hints)
 struct BeamArgs {
     const float* X;
     int64_t n;
     int32_t dp;
     int32_t metric;
-    const int64_t* offs;
+    const uint2* node;
     const uint32_t* adj;
     const uint32_t* hints;  // 1-based
     int32_t nhints;
@@ -45,6 +46,7 @@ struct BeamArgs {
     unsigned int* counter;
     int32_t has_dups;
     int32_t allknn;
+    int32_t internal;        // 1: issued by index! / optimize_index!
 };
 
 // Launch the persistent beam-search kernel over nslots waves (a multiple of 4).

@@ -226,6 +226,8 @@ class SearchGraph(_GpuIndex):
 
     def _after_upload(self):
         a = self.adj
+        if a is None:  # being built by index!
+            return
         _lib.check(_lib.load().ssg_upload_graph(self._h, a.offset.ctypes.data, len(a),
                                                a.end_point.ctypes.data, len(a.end_point),
                                                self.hints.ctypes.data, len(self.hints)),
@@ -234,6 +236,48 @@ class SearchGraph(_GpuIndex):
     def beam_params(self, ctx, algo=None):
         b = algo or self.algo
         return BeamParams(b.bsize, b.delta, b.maxvisits, getattr(ctx, "beamcap", 32))
+
+    @classmethod
+    def build(cls, dist, db, ctx=None, parallel_block=128, neighborhood_logbase=1.3,
+              neighborhood_minsize=2, hints_logbase=1.1, logbase_callback=1.5,
+              starting_callback=256, minrecall=0.97, numqueries=32, ksearch=10, seed=0,
+              device=0, verbose=0):
+        """SearchGraph(dist, db) followed by index!(G, ctx) (src/searchgraph/insertions.jl:94-113)
+        with the reference's default context knobs; the graph is built on the GPU."""
+        G = cls.__new__(cls)
+        _GpuIndex.__init__(G, dist, db, device)
+        G.adj, G.hints, G.algo = None, np.zeros(0, np.uint32), BeamSearch()
+        ctx = ctx or SearchGraphContext()
+        h = G._handle()
+        p = _lib.BuildParams(neighborhood_logbase, neighborhood_minsize, parallel_block,
+                             hints_logbase, logbase_callback, starting_callback, minrecall,
+                             numqueries, ksearch, ctx.beamcap, seed, verbose)
+        L = _lib.load()
+        _lib.check(L.ssg_build_graph(h, C.byref(p)), "index!")
+        nnz, nh, bp = C.c_int64(), C.c_int64(), BeamParams()
+        _lib.check(L.ssg_graph_info(h, C.byref(nnz), C.byref(nh), C.byref(bp)), "graph info")
+        n = len(G.db)
+        offset = np.zeros(n, np.int64)
+        end_point = np.zeros(max(nnz.value, 1), np.uint32)
+        hints = np.zeros(max(nh.value, 1), np.uint32)
+        _lib.check(L.ssg_download_graph(h, offset.ctypes.data, end_point.ctypes.data,
+                                        hints.ctypes.data), "graph download")
+        G.adj = StaticAdjList(offset, end_point[:nnz.value])
+        G.hints = hints[:nh.value].copy()
+        G.algo = BeamSearch(bp.bsize, bp.delta, bp.maxvisits)
+        return G
+
+    def optimize(self, ctx, minrecall, queries, ksearch=10):
+        """optimize_index!(G, ctx, MinRecall(r); queries, ksearch) (src/opt.jl:220-317);
+        installs and returns the chosen BeamSearch and the sample recall."""
+        Q = _as_db(queries)
+        bp = self.beam_params(ctx)
+        rec = C.c_double()
+        _lib.check(_lib.load().ssg_optimize_minrecall(self._handle(), Q.rows.ctypes.data, len(Q),
+                                                      ksearch, minrecall, C.byref(bp),
+                                                      C.byref(rec)), "optimize_index!")
+        self.algo = BeamSearch(bp.bsize, bp.delta, bp.maxvisits)
+        return self.algo, rec.value
 
 
 # --------------------------------------------------------------------------- batch API

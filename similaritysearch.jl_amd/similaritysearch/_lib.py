@@ -26,8 +26,22 @@ class QueryOpts(C.Structure):
                 ("hint_stride", C.c_int32)]
 
 
+class BuildParams(C.Structure):
+    _fields_ = [("nb_logbase", C.c_float), ("nb_minsize", C.c_int32),
+                ("parallel_block", C.c_int32), ("hints_logbase", C.c_float),
+                ("cb_logbase", C.c_float), ("cb_starting", C.c_int32),
+                ("opt_minrecall", C.c_float), ("opt_numqueries", C.c_int32),
+                ("opt_ksearch", C.c_int32), ("beamcap", C.c_int32), ("seed", C.c_uint64),
+                ("verbose", C.c_int32)]
+
+
 # name -> (restype, argtypes); the exported surface of include/ssg.h
 SIGNATURES = {
+    "ssg_build_graph": (C.c_int, [_p, C.POINTER(BuildParams)]),
+    "ssg_graph_info": (C.c_int, [_p, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(BeamParams)]),
+    "ssg_download_graph": (C.c_int, [_p, _p, _p, _p]),
+    "ssg_optimize_minrecall": (C.c_int, [_p, _p, _i64, _i32, C.c_float, C.POINTER(BeamParams),
+                                         C.POINTER(C.c_double)]),
     "ssg_version": (C.c_char_p, []),
     "ssg_last_error": (C.c_char_p, [_p]),
     "ssg_create": (C.c_int, [C.POINTER(_p), C.c_int]),

@@ -197,3 +197,30 @@ def test_errors_are_argument_errors():
         ss.searchbatch(E, ss.GenericContext(), ss.MatrixDatabase.from_rows(np.zeros((0, 8), np.float32)), 3)
     with pytest.raises(ValueError):
         ss.searchbatch(E, ss.GenericContext(), ss.MatrixDatabase.from_rows(_uniform(4, 7, 1)), 3)
+
+
+@pytest.mark.parametrize("n,dim,pb,seed", [(3000, 8, 64, 7), (1500, 16, 200, 2)])
+def test_gpu_index_build_matches_oracle(oracle, n, dim, pb, seed):
+    """index! on the GPU produces exactly the oracle's graph, hints and tuned BeamSearch."""
+    ss = _ss()
+    X = _uniform(n, dim, 100 + seed)
+    G = ss.SearchGraph.build(ss.SqL2, ss.MatrixDatabase.from_rows(X), parallel_block=pb, seed=seed)
+    g = oracle.build_graph("SqL2", X, parallel_block=pb, seed=seed)
+    np.testing.assert_array_equal(G.adj.offset, g["offs"][1:])
+    np.testing.assert_array_equal(G.adj.end_point, g["adj"])
+    np.testing.assert_array_equal(G.hints, g["hints"])
+    assert (G.algo.bsize, np.float32(G.algo.delta), G.algo.maxvisits) == \
+        (g["bsize"], np.float32(g["delta"]), g["maxvisits"])
+
+
+def test_gpu_optimize_matches_oracle(oracle, small_graph):
+    ss = _ss()
+    X, g = small_graph
+    Q = _uniform(64, 8, 61)
+    G = _graph_index(ss, X, g)
+    algo, rec = G.optimize(ss.SearchGraphContext(), 0.9, ss.MatrixDatabase.from_rows(Q))
+    o = oracle.optimize_minrecall("SqL2", X, g["offs"], g["adj"], g["hints"], Q, 10, 0.9,
+                                  cur_maxvisits=g["maxvisits"])
+    assert (algo.bsize, np.float32(algo.delta), algo.maxvisits) == \
+        (o["bsize"], np.float32(o["delta"]), o["maxvisits"])
+    assert rec == pytest.approx(o["recall"])

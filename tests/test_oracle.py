@@ -1,0 +1,279 @@
+"""Pins of the CPU oracle (oracle/ssg_oracle.c) against the reference's known answers,
+closed forms, invariants and brute force on small inputs.  None of these checks
+re-types the oracle's formulas: each compares against an independent formulation
+(FP64 numpy, sort-based golds, hand-traced searches, structural invariants)."""
+import numpy as np
+import pytest
+
+
+# ---------------------------------------------------------------- distances
+def test_distance_known_answers(oracle):
+    # test/testdistances.jl:34-76
+    u = np.array([1, 2, 3], np.float32)
+    v = np.array([4, 6, 8], np.float32)
+    assert oracle.dist("SqL2", u, v) == 50.0
+    assert oracle.dist("L2", u, v) == np.float32(np.sqrt(50.0))
+    assert oracle.dist("L2", u, u) == 0.0
+    p = np.array([3, 4], np.float32)
+    q = np.zeros(2, np.float32)
+    assert oracle.dist("L2", p, q) == 5.0
+    assert oracle.dist("SqL2", p, q) == 25.0
+    e = np.array([1, 0, 0], np.float32)
+    f = np.array([0, 1, 0], np.float32)
+    assert oracle.dist("SqL2", e, f) == 2.0
+    assert abs(oracle.dist("L2", e, f) - np.sqrt(2.0)) < 1e-6
+    assert oracle.dist("NormCosine", e, f) == 1.0
+    assert oracle.dist("NormCosine", e, e) == 0.0
+
+
+@pytest.mark.parametrize("dim", [1, 4, 15, 16, 17, 32, 96, 100, 128, 256])
+@pytest.mark.parametrize("metric", ["SqL2", "L2", "NormCosine"])
+def test_distance_is_rounded_fp64(oracle, dim, metric):
+    """canonical distance = float32 of the exact value (within one float32 ulp)."""
+    rng = np.random.default_rng(dim)
+    A = rng.standard_normal((200, dim)).astype(np.float32)
+    B = rng.standard_normal((200, dim)).astype(np.float32)
+    if metric == "NormCosine":
+        A /= np.linalg.norm(A, axis=1, keepdims=True)
+        B /= np.linalg.norm(B, axis=1, keepdims=True)
+    a64, b64 = A.astype(np.float64), B.astype(np.float64)
+    if metric == "NormCosine":
+        exact = 1.0 - np.sum(a64 * b64, axis=1)
+    else:
+        exact = np.sum((a64 - b64) ** 2, axis=1)
+        if metric == "L2":
+            exact = np.sqrt(exact)
+    got = np.array([oracle.dist(metric, A[i], B[i]) for i in range(200)], np.float64)
+    ulp = np.spacing(np.abs(exact).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(got - exact) <= ulp + 1e-12)
+
+
+def test_dist_f64_matches_numpy(oracle):
+    rng = np.random.default_rng(3)
+    a = rng.random(96, dtype=np.float32)
+    b = rng.random(96, dtype=np.float32)
+    assert oracle.dist_f64("SqL2", a, b) == pytest.approx(
+        float(np.sum((a.astype(np.float64) - b) ** 2)), rel=1e-14)
+
+
+# ---------------------------------------------------------------- KnnSorted
+@pytest.mark.parametrize("k", [1, 7, 8, 12, 15, 67])
+def test_knnsorted_against_sort_gold(oracle, k):
+    """test/testresults.jl:57-85: after every push the queue equals the k first items of a
+    stable sort of everything pushed so far."""
+    rng = np.random.default_rng(k)
+    n = 1000
+    d = rng.random(n, dtype=np.float32)
+    d[::7] = d[3]  # plenty of ties
+    ids = np.arange(1, n + 1, dtype=np.uint32)
+    for stop in (5, 50, 1000):
+        got_i, got_d, acc = oracle.knn_run(k, np.zeros(stop, np.int8), ids[:stop], d[:stop])
+        order = np.argsort(d[:stop], kind="stable")[:k]
+        np.testing.assert_array_equal(got_i, ids[:stop][order])
+        np.testing.assert_array_equal(got_d, d[:stop][order])
+
+
+def test_knnsorted_full_queue_rejects_equal_maximum(oracle):
+    ids = np.array([1, 2, 3, 4], np.uint32)
+    d = np.array([0.1, 0.5, 0.5, 0.3], np.float32)
+    gi, gd, acc = oracle.knn_run(2, np.zeros(4, np.int8), ids, d)
+    np.testing.assert_array_equal(acc, [True, True, False, True])
+    np.testing.assert_array_equal(gi, [1, 4])
+
+
+def test_knnsorted_pop_min_and_refill(oracle):
+    # pops shrink from the front; later pushes still land in order (compaction path)
+    ops = np.array([0, 0, 0, 1, 0, 1, 0, 0], np.int8)
+    ids = np.array([1, 2, 3, 0, 4, 0, 5, 6], np.uint32)
+    d = np.array([0.3, 0.1, 0.2, 0, 0.15, 0, 0.05, 0.25], np.float32)
+    gi, gd, acc = oracle.knn_run(3, ops, ids, d)
+    # after pushes 1,2,3: [2 .1, 3 .2, 1 .3]; pop -> [3, 1]; push 4 (.15) -> [4, 3, 1];
+    # pop -> [3, 1]; push 5 (.05) -> [5, 3, 1]; push 6 (.25): full, .25 < .3 -> [5, 3, 6]
+    np.testing.assert_array_equal(gi, [5, 3, 6])
+    np.testing.assert_array_equal(gd, np.array([0.05, 0.2, 0.25], np.float32))
+    with pytest.raises(ValueError):
+        oracle.knn_run(2, np.array([1], np.int8), np.zeros(1, np.uint32), np.zeros(1, np.float32))
+
+
+# ---------------------------------------------------------------- exhaustive
+def _exact_order(X, Q):
+    d = ((Q.astype(np.float64)[:, None, :] - X.astype(np.float64)[None, :, :]) ** 2).sum(-1)
+    return d, np.argsort(d, axis=1, kind="stable")
+
+
+def test_exhaustive_against_fp64_brute_force(oracle):
+    rng = np.random.default_rng(1)
+    X = rng.random((3000, 32), dtype=np.float32)
+    Q = rng.random((40, 32), dtype=np.float32)
+    ids, dists = oracle.exhaustive("SqL2", X, Q, 10)
+    d64, order = _exact_order(X, Q)
+    for j in range(len(Q)):
+        srt = np.sort(d64[j])
+        near_tie = np.any(np.diff(srt[:11]) < 4 * np.spacing(np.float32(srt[10])))
+        if not near_tie:
+            np.testing.assert_array_equal(ids[j], order[j, :10] + 1)
+        np.testing.assert_allclose(dists[j], d64[j, ids[j] - 1], rtol=1e-6)
+
+
+def test_exhaustive_self_query_and_empty_slots(oracle):
+    rng = np.random.default_rng(2)
+    X = rng.random((50, 8), dtype=np.float32)
+    ids, dists = oracle.exhaustive("SqL2", X, X[:10], 3)
+    np.testing.assert_array_equal(ids[:, 0], np.arange(1, 11))
+    assert (dists[:, 0] == 0).all()
+    ids, dists = oracle.exhaustive("SqL2", X[:2], X[:3], 4)
+    assert (ids[:, 2:] == 0).all() and np.isinf(dists[:, 2:]).all()
+
+
+@pytest.mark.parametrize("nb", [1, 2, 3, 7, 64])
+@pytest.mark.parametrize("k", [1, 4, 20])
+def test_parallel_equals_sequential(oracle, nb, k):
+    """test/testparallelexhaustive.jl:21-64 (exact equality: both use one distance)."""
+    rng = np.random.default_rng(nb * 100 + k)
+    X = rng.integers(0, 4, (500, 6)).astype(np.float32)  # many ties
+    Q = rng.integers(0, 4, (30, 6)).astype(np.float32)
+    a = oracle.exhaustive("SqL2", X, Q, k)
+    b = oracle.parallel_exhaustive("SqL2", X, Q, k, nb)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_ties_resolve_to_lower_id(oracle):
+    X = np.zeros((10, 4), np.float32)
+    Q = np.ones((1, 4), np.float32)
+    ids, _ = oracle.exhaustive("SqL2", X, Q, 3)
+    np.testing.assert_array_equal(ids[0], [1, 2, 3])
+
+
+# ---------------------------------------------------------------- beam search
+def _chain(n):
+    """points 1..n on a line, node i linked to i-1 and i+1 (1-based)."""
+    X = np.arange(1, n + 1, dtype=np.float32)[:, None] * np.ones((1, 4), np.float32) / 2.0
+    lists = []
+    for i in range(1, n + 1):
+        lists.append([j for j in (i - 1, i + 1) if 1 <= j <= n])
+    offs = np.concatenate([[0], np.cumsum([len(l) for l in lists])]).astype(np.int64)
+    adj = np.array([x for l in lists for x in l], np.uint32)
+    return X, offs, adj
+
+
+def test_beam_walk_along_a_chain(oracle):
+    """Hand trace: bsize 1, Δ 1, k 1, hint node 1, query at node 50 + 0.3: the walk
+    expands nodes 1..50, evaluates nodes 2..51 once each, and returns node 50."""
+    X, offs, adj = _chain(70)
+    q = (X[49] + 0.15)[None, :]
+    ids, d, cd, cb = oracle.beamsearch("SqL2", X, offs, adj, np.array([1], np.uint32), q, 1, 1,
+                                       1.0, 10**6)
+    assert ids[0, 0] == 50
+    assert cd[0] == 1 + 50 and cb[0] == 50
+
+
+def test_beam_maxvisits_cut(oracle):
+    """Same walk cut by maxvisits = 10: the 11th inner evaluation (node 12) is pushed and
+    ends the search (beamsearch.jl:61-63)."""
+    X, offs, adj = _chain(70)
+    q = (X[49] + 0.15)[None, :]
+    ids, d, cd, cb = oracle.beamsearch("SqL2", X, offs, adj, np.array([1], np.uint32), q, 1, 1,
+                                       1.0, 10)
+    assert ids[0, 0] == 12
+    assert cd[0] == 1 + 11 and cb[0] == 11
+
+
+def test_beam_log2_fallback_without_hints(oracle):
+    """No hints: ids 1:ceil(log2(1+n)) seed the result (beamsearch.jl:28-33)."""
+    X, offs, adj = _chain(70)
+    q = X[2:3]
+    ids, d, cd, cb = oracle.beamsearch("SqL2", X, offs, adj, np.zeros(0, np.uint32), q, 3, 2,
+                                       1.0, 10**6)
+    # 7 fallback evaluations; the nearest (node 3) has both neighbours already seen
+    assert cd[0] == 7
+    np.testing.assert_array_equal(ids[0], [3, 2, 4])
+
+
+def test_beam_small_n_is_brute_force(oracle):
+    rng = np.random.default_rng(9)
+    X = rng.random((40, 5), dtype=np.float32)
+    Q = rng.random((15, 5), dtype=np.float32)
+    offs = np.zeros(41, np.int64)
+    ids, d, cd, cb = oracle.beamsearch("SqL2", X, offs, np.zeros(0, np.uint32),
+                                       np.zeros(0, np.uint32), Q, 6, 4, 1.0, 10**6)
+    gi, gd = oracle.exhaustive("SqL2", X, Q, 6)
+    np.testing.assert_array_equal(ids, gi)
+    assert (cd == 40).all() and (cb == 0).all()
+
+
+@pytest.fixture(scope="module")
+def built(oracle):
+    rng = np.random.default_rng(11)
+    X = rng.random((2000, 4), dtype=np.float32)
+    g = oracle.build_graph("SqL2", X, parallel_block=32, seed=5)
+    return X, g
+
+
+def test_builder_graph_is_symmetric_without_loops(built):
+    X, g = built
+    offs, adj = g["offs"], g["adj"]
+    n = len(X)
+    edges = set()
+    for i in range(n):
+        for v in adj[offs[i]:offs[i + 1]]:
+            assert v != i + 1
+            edges.add((i + 1, int(v)))
+    assert all((v, u) in edges for (u, v) in edges)
+
+
+def test_builder_reaches_reference_recall_floor(oracle, built):
+    """test/testsearchgraph.jl:48-75: n=2000, dim 4, 30 queries, recall >= 0.8."""
+    X, g = built
+    Q = np.random.default_rng(12).random((30, 4), dtype=np.float32)
+    o = oracle.optimize_minrecall("SqL2", X, g["offs"], g["adj"], g["hints"], X[:64], 10, 0.9,
+                                  cur_maxvisits=g["maxvisits"])
+    ids, _, _, _ = oracle.beamsearch("SqL2", X, g["offs"], g["adj"], g["hints"], Q, 10,
+                                     o["bsize"], o["delta"], o["maxvisits"])
+    gi, _ = oracle.exhaustive("SqL2", X, Q, 10)
+    assert oracle.macrorecall(gi, ids) >= 0.8
+
+
+def test_builder_hints_are_well_connected(built):
+    X, g = built
+    deg = np.diff(g["offs"])
+    assert len(g["hints"]) > 0
+    assert (deg[g["hints"].astype(np.int64) - 1] > 2).all()
+    assert len(set(g["hints"].tolist())) == len(g["hints"])
+
+
+# ---------------------------------------------------------------- fft, rerank, recall
+def test_fft_invariants(oracle):
+    rng = np.random.default_rng(13)
+    X = rng.random((400, 3), dtype=np.float32)
+    centers, nn, nnd, eps = oracle.fft("SqL2", X, 12, start=1)
+    assert centers[0] == 1 and np.isinf(eps[0])
+    assert np.all(np.diff(eps[1:]) <= 0)
+    C = X[centers.astype(np.int64) - 1].astype(np.float64)
+    d = ((X.astype(np.float64)[:, None, :] - C[None]) ** 2).sum(-1)
+    np.testing.assert_allclose(nnd, d.min(1), rtol=1e-6, atol=1e-7)
+    for c in range(1, 12):  # each new center was a farthest point at its round
+        prev = d[:, :c].min(1)
+        assert prev[centers[c] - 1] == pytest.approx(prev.max(), rel=1e-6)
+
+
+def test_rerank_against_stable_sort(oracle):
+    rng = np.random.default_rng(14)
+    X = rng.random((300, 16), dtype=np.float32)
+    Q = rng.random((20, 16), dtype=np.float32)
+    ids = rng.integers(1, 301, (20, 12)).astype(np.uint32)
+    ids[:, 9:] = 0
+    gi, gd = oracle.rerank("SqL2", X, Q, ids, np.zeros((20, 12), np.float32))
+    for j in range(20):
+        d64 = ((X[ids[j, :9] - 1].astype(np.float64) - Q[j]) ** 2).sum(1)
+        order = np.argsort(d64.astype(np.float32), kind="stable")
+        np.testing.assert_array_equal(gi[j, :9], ids[j, :9][order])
+        assert (gi[j, 9:] == 0).all()
+
+
+def test_macrorecall(oracle):
+    g = np.array([[1, 2, 3], [4, 5, 6]], np.uint32)
+    assert oracle.macrorecall(g, g) == 1.0
+    assert oracle.macrorecall(g, g + 10) == 0.0
+    r = np.array([[3, 9, 1], [6, 7, 8]], np.uint32)
+    assert oracle.macrorecall(g, r) == pytest.approx((2 / 3 + 1 / 3) / 2)
