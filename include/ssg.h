@@ -76,7 +76,11 @@ typedef struct {
     int32_t bsize;
     float delta;
     int64_t maxvisits;
-    int32_t beamcap; /* 32 in the default SearchGraphContext; at most 256 */
+    int32_t beamcap;   /* 32 in the default SearchGraphContext; at most 256 */
+    int32_t precision; /* SSG_F32: canonical distances.  SSG_BF16 (no reference
+                          counterpart): BF16 vector gathers with FP32 arithmetic during the
+                          search, then rerank! of the result list with the canonical distance */
+    int32_t rerank;    /* SSG_BF16: result-list length reranked (max(k, rerank) <= 64) */
 } ssg_beam_params;
 
 /* Context knobs of index! (src/searchgraph/insertions.jl:94-113) with the defaults of

@@ -71,8 +71,20 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     if (!(bp->delta >= 0.f) || !std::isfinite(bp->delta))
         return ssg_fail(SSG_E_ARG, "delta must be finite and >= 0");
     if (bp->maxvisits < 0) return ssg_fail(SSG_E_ARG, "maxvisits must be >= 0");
+    if (bp->precision != SSG_F32 && bp->precision != SSG_BF16)
+        return ssg_fail(SSG_E_ARG, "precision must be SSG_F32 or SSG_BF16");
+    if (bp->precision == SSG_BF16 && (allknn || internal))
+        return ssg_fail(SSG_E_ARG, "the BF16 form serves searchbatch! only");
+    const int32_t krr = std::max(k, bp->rerank);
+    if (bp->precision == SSG_BF16 && (krr > 64 || kstride > 64))
+        return ssg_fail(SSG_E_LIMIT, "BF16 rerank list is limited to 64");
     int rc = ensure_vis(h);
     if (rc) return rc;
+    if (bp->precision == SSG_BF16 && !h->Xh.p) {
+        HIPCHK(h->Xh.ensure((size_t)std::max<int64_t>(h->n_pad, 64) * h->dp * 2), "alloc BF16 copy");
+        HIPCHK(launch_to_bf16(h->X.as<float>(), std::max<int64_t>(h->n_pad, 64) * h->dp,
+                              h->Xh.as<uint16_t>(), s), "BF16 copy");
+    }
     BeamArgs a{};
     a.X = h->X.as<float>();
     a.n = n_eff;
@@ -108,6 +120,9 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     a.has_dups = h->has_dups;
     a.allknn = allknn ? 1 : 0;
     a.internal = internal ? 1 : 0;
+    a.prec = bp->precision == SSG_BF16 ? 1 : 0;
+    a.Xh = h->Xh.as<uint16_t>();
+    a.krr = krr;
     const int slots = (int)std::min<int64_t>(h->nslots, (m + 3) / 4 * 4);
     HIPCHK(launch_zero_u32(a.counter, 1, s), "zero counter");
     HIPCHK(launch_beam(a, slots, s), "beam kernel launch");
@@ -263,7 +278,7 @@ int ssg_destroy(ssg_handle* h)
     if (!h) return SSG_OK;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
-    DevBuf* bufs[] = {&h->X, &h->xnorm, &h->node, &h->adj, &h->hints, &h->vis, &h->epochs,
+    DevBuf* bufs[] = {&h->X, &h->xnorm, &h->Xh, &h->node, &h->adj, &h->hints, &h->vis, &h->epochs,
                       &h->counter, &h->qpad, &h->qnorm, &h->cand_d, &h->cand_i, &h->slice_tau,
                       &h->flags, &h->part_i, &h->part_d, &h->sQ, &h->sIds, &h->sD, &h->sCd,
                       &h->sCb, &h->sK, &h->sB, &h->sH, &h->sHc, &h->bIds, &h->bD, &h->bOut,
@@ -296,6 +311,7 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     h->has_graph = false;
     h->lists.clear();
     h->hints_host.clear();
+    h->Xh.release();
     const int64_t np = std::max<int64_t>(h->n_pad, 64);
     HIPCHK(h->X.ensure((size_t)np * dp * sizeof(float)), "alloc X");
     HIPCHK(h->xnorm.ensure((size_t)np * sizeof(float)), "alloc xnorm");

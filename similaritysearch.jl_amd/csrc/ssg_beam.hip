@@ -46,7 +46,9 @@ __device__ __forceinline__ void vm_drain()
     asm volatile("" ::: "memory");
 }
 
-template <int V, int NB>
+// PREC 0: canonical distance on the FP32 vectors.  PREC 1: BF16 vectors with FP32
+// arithmetic during the search, then an exact (canonical) rerank of the result list.
+template <int V, int NB, int PREC>
 struct Searcher {
     const BeamArgs& a;
     float qv[V];
@@ -78,7 +80,10 @@ struct Searcher {
                 const int u = 4 * r + g;
                 if (u < cnt) {
                     const uint32_t c = cidbuf[ub + u];
-                    load_run<V>(a.X + (size_t)c * dp + j * V, xv[r]);
+                    if constexpr (PREC == 1)
+                        load_run_bf16<V>(a.Xh + (size_t)c * dp + j * V, xv[r]);
+                    else
+                        load_run<V>(a.X + (size_t)c * dp + j * V, xv[r]);
                 } else {
 #pragma unroll
                     for (int t = 0; t < V; t++) xv[r][t] = 0.f;
@@ -89,7 +94,12 @@ struct Searcher {
 #pragma unroll
         for (int r = 0; r < ROUNDS; r++) {
             dr[r] = __builtin_inff();
-            if (4 * r < cnt) dr[r] = finish_dist(a.metric, group16_sum(run_partial<V>(a.metric, xv[r], qv)));
+            if (4 * r < cnt) {
+                if constexpr (PREC == 1)
+                    dr[r] = finish_f32(a.metric, group16_sum_f32(run_partial_f32<V>(a.metric, xv[r], qv)));
+                else
+                    dr[r] = finish_dist(a.metric, group16_sum(run_partial<V>(a.metric, xv[r], qv)));
+            }
         }
         dd = __builtin_inff();
 #pragma unroll
@@ -216,15 +226,58 @@ struct Searcher {
             if (stop) break;
         }
     }
+
+    // Exact rerank of the result list (rerank!, src/rerank.jl:22-41): canonical
+    // distances from the FP32 vectors, stable sort, top kk written to row qi.
+    __device__ __forceinline__ void rerank_out(int64_t qi, int kk)
+    {
+        const int dp = 16 * V;
+        const int len = res.len;
+        float dex = __builtin_inff();
+        for (int r0 = 0; r0 < len; r0 += 16) {
+            float xv[4][V];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int u = r0 + 4 * r + g;
+                const uint32_t c = __shfl(res.id[0], u < len ? u : 0);
+                if (u < len) {
+                    load_run<V>(a.X + (size_t)c * dp + j * V, xv[r]);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < V; t++) xv[r][t] = 0.f;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float d = finish_dist(a.metric, group16_sum(run_partial<V>(a.metric, xv[r], qv)));
+                const float dt = __shfl(d, (lane & 3) * 16);
+                if (lane >= r0 && lane < r0 + 16 && ((lane - r0) >> 2) == r && lane < len) dex = dt;
+            }
+        }
+        int rank = 0;
+        for (int t = 0; t < len; t++) {
+            const float dt = rl_f(dex, t);
+            rank += (dt < dex || (dt == dex && t < lane)) ? 1 : 0;
+        }
+        if (lane < len && rank < kk) {
+            a.ids[(size_t)qi * a.kstride + rank] = res.id[0] + 1u;
+            a.dists[(size_t)qi * a.kstride + rank] = dex;
+        }
+        const int filled = len < kk ? len : kk;
+        if (lane >= filled && lane < a.kstride) {
+            a.ids[(size_t)qi * a.kstride + lane] = 0u;
+            a.dists[(size_t)qi * a.kstride + lane] = __builtin_inff();
+        }
+    }
 };
 
-template <int V, int NB>
+template <int V, int NB, int PREC>
 __device__ __forceinline__ void beam_body(const BeamArgs& a)
 {
     __shared__ uint32_t s_cid[WAVES][64];
     const int w = threadIdx.x >> 6;
     const int64_t slot = (int64_t)blockIdx.x * WAVES + w;
-    Searcher<V, NB> S(a);
+    Searcher<V, NB, PREC> S(a);
     S.lane = lane_id();
     S.g = S.lane >> 4;
     S.j = S.lane & 15;
@@ -259,7 +312,7 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
         }
         const int kk = a.kq ? a.kq[qi] : a.k;
         const int bs = a.bq ? a.bq[qi] : a.bsize;
-        S.res.init(kk);
+        S.res.init(PREC == 1 ? (kk > a.krr ? kk : a.krr) : kk);
         S.cd_init = 0;
         S.cd_inner = 0;
         S.cb = 0;
@@ -274,12 +327,14 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
                 S.search(&h1, 1, true, bs);
                 S.cd_inner = 0;
             }
-        } else if (a.qhints) {
+        } else if (a.qhints && a.qhcount[qi] >= 0) {
             S.search(a.qhints + (size_t)qi * a.qhstride, a.qhcount[qi], true, bs);
         } else {
             S.search(a.hints, a.nhints, true, bs);
         }
-        if (lane < a.kstride) {
+        if constexpr (PREC == 1) {
+            S.rerank_out(qi, kk);
+        } else if (lane < a.kstride) {
             const bool has = lane < S.res.len;
             a.ids[(size_t)qi * a.kstride + lane] = has ? S.res.id[0] + 1u : 0u;
             a.dists[(size_t)qi * a.kstride + lane] = has ? S.res.d[0] : __builtin_inff();
@@ -296,7 +351,14 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
 template <int V, int NB>
 __global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
 {
-    beam_body<V, NB>(a);
+    beam_body<V, NB, 0>(a);
+}
+
+// precision form: BF16 gathers, FP32 arithmetic, exact rerank of the result list
+template <int V, int NB>
+__global__ __launch_bounds__(256) void beam_bf16_kernel(BeamArgs a)
+{
+    beam_body<V, NB, 1>(a);
 }
 
 // the same search issued by index! and optimize_index! (a separate symbol keeps
@@ -304,7 +366,7 @@ __global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
 template <int V, int NB>
 __global__ __launch_bounds__(256) void beam_build_kernel(BeamArgs a)
 {
-    beam_body<V, NB>(a);
+    beam_body<V, NB, 0>(a);
 }
 
 __global__ void zero_u32_kernel(unsigned int* p, int64_t n)
@@ -334,6 +396,21 @@ hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s)
     return hipGetLastError();
 }
 
+__global__ void to_bf16_kernel(const float* __restrict__ src, int64_t n, uint16_t* __restrict__ dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t u = __float_as_uint(src[i]);
+    dst[i] = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+hipError_t launch_to_bf16(const float* src, int64_t n, uint16_t* dst, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, n, dst);
+    return hipGetLastError();
+}
+
 template <int V>
 static hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
 {
@@ -343,6 +420,11 @@ static hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
             hipLaunchKernelGGL((beam_build_kernel<V, 1>), grid, block, 0, s, a);
         else
             hipLaunchKernelGGL((beam_build_kernel<V, 4>), grid, block, 0, s, a);
+    } else if (a.prec == 1) {
+        if (a.beamcap <= 64)
+            hipLaunchKernelGGL((beam_bf16_kernel<V, 1>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((beam_bf16_kernel<V, 4>), grid, block, 0, s, a);
     } else {
         if (a.beamcap <= 64)
             hipLaunchKernelGGL((beam_kernel<V, 1>), grid, block, 0, s, a);

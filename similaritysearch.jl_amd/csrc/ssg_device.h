@@ -65,6 +65,68 @@ __device__ __forceinline__ void load_run(const float* __restrict__ p, float (&x)
     }
 }
 
+// BF16 run (upper halves of FP32 words) widened to FP32.
+template <int V>
+__device__ __forceinline__ void load_run_bf16(const uint16_t* __restrict__ p, float (&x)[V])
+{
+    if constexpr (V % 8 == 0) {
+#pragma unroll
+        for (int t = 0; t < V / 8; t++) {
+            const uint4 w = reinterpret_cast<const uint4*>(p)[t];
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                x[8 * t + 2 * u] = __uint_as_float(ws[u] << 16);
+                x[8 * t + 2 * u + 1] = __uint_as_float(ws[u] & 0xffff0000u);
+            }
+        }
+    } else if constexpr (V % 2 == 0) {
+#pragma unroll
+        for (int t = 0; t < V / 2; t++) {
+            const uint32_t w = reinterpret_cast<const uint32_t*>(p)[t];
+            x[2 * t] = __uint_as_float(w << 16);
+            x[2 * t + 1] = __uint_as_float(w & 0xffff0000u);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < V; t++) x[t] = __uint_as_float((uint32_t)p[t] << 16);
+    }
+}
+
+// FP32 partial sum (reduced-precision form; its result is never final).
+template <int V>
+__device__ __forceinline__ float run_partial_f32(int metric, const float (&x)[V], const float (&q)[V])
+{
+    float s = 0.f;
+    if (metric == M_NORMCOS) {
+#pragma unroll
+        for (int t = 0; t < V; t++) s = __builtin_fmaf(x[t], q[t], s);
+    } else {
+#pragma unroll
+        for (int t = 0; t < V; t++) {
+            const float m = x[t] - q[t];
+            s = __builtin_fmaf(m, m, s);
+        }
+    }
+    return s;
+}
+
+__device__ __forceinline__ float group16_sum_f32(float s)
+{
+    s = s + __shfl_xor(s, 8);
+    s = s + __shfl_xor(s, 4);
+    s = s + __shfl_xor(s, 2);
+    s = s + __shfl_xor(s, 1);
+    return s;
+}
+
+__device__ __forceinline__ float finish_f32(int metric, float s)
+{
+    if (metric == M_L2) return __builtin_sqrtf(s);
+    if (metric == M_NORMCOS) return 1.0f - s;
+    return s;
+}
+
 // Partial sum of one lane's run.
 template <int V>
 __device__ __forceinline__ double run_partial(int metric, const float (&x)[V], const float (&q)[V])

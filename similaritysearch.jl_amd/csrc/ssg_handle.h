@@ -58,6 +58,7 @@ struct ssg_handle {
     int32_t dim = 0, dp = 0, metric = 0;
     int64_t n = 0, n_pad = 0;
     DevBuf X, xnorm;
+    DevBuf Xh;  // BF16 copy of X, made on first use of the BF16 form
     float xnmax = 0.f;
     // graph: node[i] = {start, deg} into the adj pool (0-based ids), hints 1-based
     bool has_graph = false;

@@ -47,12 +47,17 @@ struct BeamArgs {
     int32_t has_dups;
     int32_t allknn;
     int32_t internal;        // 1: issued by index! / optimize_index!
+    int32_t prec;            // 0: canonical FP32 path, 1: BF16 gathers + exact rerank
+    const uint16_t* Xh;      // BF16 copy of X (n rows of dp), prec 1
+    int32_t krr;             // prec 1: result-list length reranked exactly (>= k)
 };
 
 // Launch the persistent beam-search kernel over nslots waves (a multiple of 4).
 hipError_t launch_beam(const BeamArgs& a, int nslots, hipStream_t s);
 hipError_t launch_zero_u32(unsigned int* p, int64_t n, hipStream_t s);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
+// FP32 -> BF16 with round-to-nearest-even
+hipError_t launch_to_bf16(const float* src, int64_t n, uint16_t* dst, hipStream_t s);
 
 struct ExhArgs {
     const float* X;

@@ -18,7 +18,7 @@ _i64 = C.c_int64
 
 class BeamParams(C.Structure):
     _fields_ = [("bsize", C.c_int32), ("delta", C.c_float), ("maxvisits", C.c_int64),
-                ("beamcap", C.c_int32)]
+                ("beamcap", C.c_int32), ("precision", C.c_int32), ("rerank", C.c_int32)]
 
 
 class QueryOpts(C.Structure):
