@@ -113,7 +113,9 @@ def main():
     dQ = torch.from_numpy(Q).to(dev)
     dI = torch.zeros((m, k), dtype=torch.int32, device=dev)
     dD = torch.zeros((m, k), dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream()
+    # a dedicated stream (default priority): the kernels and the timing events share it
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
     L = _lib.load()
     h = G._handle()
     bp = G.beam_params(ctx)

@@ -156,6 +156,59 @@ int ssg_allknn_graph(ssg_handle* h, int32_t k, const ssg_beam_params* bp, uint32
 /* allknn!(::ExhaustiveSearch, ...) (src/allknn.jl:66-91,108-110): exact, self included. */
 int ssg_allknn_exhaustive(ssg_handle* h, int32_t k, uint32_t* ids, float* dists);
 
+/* fft(dist, X, k; start) over the uploaded database (src/kcenters/fft.jl:53-87): farthest-first
+ * k-centers.  centers[k] (1-based), nn[n] (1-based nearest center, nullable), nndists[n]
+ * (nullable), eps[k] (the ε recorded before each center was added, nullable). */
+int ssg_fft(ssg_handle* h, int32_t k, int64_t start, uint32_t* centers, uint32_t* nn,
+            float* nndists, float* eps);
+
+/* ---------------------------------------------------------------------------------
+ * Learned entry points (no reference counterpart).  An MLP D -> H1 -> H2 -> (L + NB),
+ * ReLU, FP16 operands on MFMA with FP32 accumulation and FP32 Adam master weights,
+ * scores L landmarks (entry points, from ssg_fft) and NB beam-size buckets.  It is
+ * trained on traces of the plain search and feeds per-query hints and bsize into
+ * ssg_beamsearch_dev through ssg_learned_plan_dev; low-confidence queries fall back
+ * to the plain search (SearchGraph.hints and the plain bsize).
+ * --------------------------------------------------------------------------------- */
+typedef struct ssg_mlp ssg_mlp;
+int ssg_mlp_create(ssg_mlp** p, int device, int32_t D, int32_t H1, int32_t H2, int32_t L,
+                   int32_t NB, uint64_t seed);
+int ssg_mlp_destroy(ssg_mlp* p);
+/* FP32 parameters of layer 0..2: W row-major [out x in], b [out] (nullable). */
+int ssg_mlp_get(ssg_mlp* p, int32_t layer, float* W, float* b);
+int ssg_mlp_set(ssg_mlp* p, int32_t layer, const float* W, const float* b);
+/* Gradients of the last step (same layout as the parameters). */
+int ssg_mlp_grad(ssg_mlp* p, int32_t layer, float* gW, float* gb);
+/* One forward/backward on a host batch X [B x D]; update != 0 applies one Adam step.
+ * Loss = mean CE(landmark head) + bucket_weight * mean CE(bucket head). */
+int ssg_mlp_step(ssg_mlp* p, const float* X, const int32_t* ylandmark, const int32_t* ybucket,
+                 int32_t B, float bucket_weight, float lr, int32_t update, float* loss);
+/* Mini-batch Adam epochs over a device-resident training set dX [N x D]. */
+int ssg_mlp_train_dev(ssg_mlp* p, const float* dX, const int32_t* dyl, const int32_t* dyb,
+                      int64_t N, int32_t epochs, int32_t batch, float bucket_weight, float lr,
+                      uint64_t seed, float* losses);
+/* Inference on device rows dQ [m x D]: the h best landmarks (mapped through dlandmarks to
+ * 1-based object ids) into dhints [m x h], the best bucket, and the top-landmark
+ * softmax probability. */
+int ssg_mlp_infer_dev(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dlandmarks,
+                      int32_t h, uint32_t* dhints, int32_t* dbucket, float* dconf, void* stream);
+/* Per-query search inputs from the inference outputs: nhints[j] = h when conf[j] >=
+ * min_conf else -1 (the plain hints), bsize[j] = bucket_bsize[bucket[j]] or
+ * fallback_bsize.  bucket_bsize is a host array of NB entries. */
+int ssg_learned_plan_dev(ssg_handle* h, const int32_t* dbucket, const float* dconf, int64_t m,
+                         int32_t nh, float min_conf, const int32_t* bucket_bsize, int32_t nb,
+                         int32_t fallback_bsize, int32_t* dnhints, int32_t* dbsize, void* stream);
+
+/* The learned form end to end on `stream`: ssg_mlp_infer_dev (nh landmark hints per query),
+ * ssg_learned_plan_dev, then ssg_beamsearch_dev with those per-query hints and beam sizes
+ * (bp->bsize is the fallback beam size; bp->precision selects the BF16 form).  Scratch is
+ * owned by the handle, so repeated calls allocate nothing once sized. */
+int ssg_learned_search_dev(ssg_handle* h, ssg_mlp* mlp, const uint32_t* dlandmarks,
+                           const float* dQ, int64_t m, int32_t k, int32_t kstride,
+                           const ssg_beam_params* bp, int32_t nh, float min_conf,
+                           const int32_t* bucket_bsize, int32_t nb, uint32_t* dids,
+                           float* ddists, int64_t* dcd, int64_t* dcb, void* stream);
+
 /* rerank!(dist, db, queries, ids, dists) batch form (src/rerank.jl:77-86): recompute the
  * non-zero-id prefix of each column with the handle's distance and stable-sort it.
  * ids / dists are updated in place. */

@@ -282,7 +282,8 @@ int ssg_destroy(ssg_handle* h)
                       &h->counter, &h->qpad, &h->qnorm, &h->cand_d, &h->cand_i, &h->slice_tau,
                       &h->flags, &h->part_i, &h->part_d, &h->sQ, &h->sIds, &h->sD, &h->sCd,
                       &h->sCb, &h->sK, &h->sB, &h->sH, &h->sHc, &h->bIds, &h->bD, &h->bOut,
-                      &h->bCnt, &h->bPairs, &h->bNodeUpd};
+                      &h->bCnt, &h->bPairs, &h->bNodeUpd, &h->lH, &h->lB, &h->lC, &h->lN,
+                      &h->lS};
     for (DevBuf* b : bufs) b->release();
     (void)hipStreamDestroy(h->stream);
     delete h;

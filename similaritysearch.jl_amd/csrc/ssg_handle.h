@@ -78,6 +78,8 @@ struct ssg_handle {
     DevBuf sQ, sIds, sD, sCd, sCb, sK, sB, sH, sHc;
     // builder scratch
     DevBuf bIds, bD, bOut, bCnt, bPairs, bNodeUpd;
+    // learned-search scratch: hints, bucket, confidence, hint counts, beam sizes
+    DevBuf lH, lB, lC, lN, lS;
 };
 
 int ssg_fail(int code, const std::string& msg);

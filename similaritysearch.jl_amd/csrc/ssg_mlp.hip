@@ -1,0 +1,696 @@
+// ssg_mlp.hip -- the learned entry-point model: an MLP D -> H1 -> H2 -> (L + NB) with
+// ReLU, whose first L outputs score landmarks (entry points) and last NB outputs
+// score beam-size buckets.  FP16 operands on MFMA with FP32 accumulation, FP32
+// master weights updated by Adam.  No reference counterpart: this is the learned
+// form added beside the plain beam search.
+//
+// Kernels:
+//   gemm_f16: C[M x N] (FP32) = op(A)[M x K] . op(B)[K x N], FP16 operands given by
+//     element strides (so forward, backward-data and backward-weight products are
+//     one kernel).  A block of 4 waves owns a 64 x 64 tile; each wave a 32 x 32
+//     sub-tile of 2 x 2 v_mfma_f32_16x16x16_f16 accumulators.
+//   bias_relu, softmax_xent (two heads, loss + gradient in one pass per row),
+//   relu_grad, colsum (bias gradients), adam, and topk_heads (inference output).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "ssg_device.h"
+#include "ssg_handle.h"
+
+namespace ssg {
+
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef float floatx4v __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void gemm_f16_kernel(int M, int N, int K, const _Float16* A,
+                                                       int64_t sam, int64_t sak, const _Float16* B,
+                                                       int64_t sbk, int64_t sbn, float* C,
+                                                       int64_t ldc, int accumulate)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = blockIdx.y * 64 + (w >> 1) * 32;
+    const int n0 = blockIdx.x * 64 + (w & 1) * 32;
+    const int r16 = lane & 15, kq = (lane >> 4) * 4;
+    floatx4v acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = floatx4v{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        half4 fa[2], fb[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            const int m = m0 + 16 * a + r16;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k = k0 + kq + t;
+                fa[a][t] = (m < M && k < K) ? A[m * sam + k * sak] : (_Float16)0.f;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int n = n0 + 16 * b + r16;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k = k0 + kq + t;
+                fb[b][t] = (n < N && k < K) ? B[k * sbk + n * sbn] : (_Float16)0.f;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int n = n0 + 16 * b + r16;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int m = m0 + 16 * a + kq + t;
+                if (m < M && n < N) {
+                    float* p = C + (int64_t)m * ldc + n;
+                    *p = accumulate ? *p + acc[a][b][t] : acc[a][b][t];
+                }
+            }
+        }
+}
+
+// Y = C + bias (per column); relu when asked; FP32 pre-activation kept, FP16 copy out.
+__global__ void bias_relu_kernel(const float* C, const float* bias, int M, int N, int relu,
+                                 float* pre, _Float16* out16)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)M * N) return;
+    const float v = C[i] + bias[i % N];
+    if (pre) pre[i] = v;
+    if (out16) out16[i] = (_Float16)(relu ? fmaxf(v, 0.f) : v);
+}
+
+// Two softmax heads per row: [0, L) and [L, L+NB).  Writes the mean-loss contribution
+// and dlogits = (softmax - onehot) / B in FP16 (and FP32 when asked).
+__global__ __launch_bounds__(256) void softmax_xent_kernel(const float* logits, int B, int L,
+                                                           int NB, const int32_t* yl,
+                                                           const int32_t* yb, float wb,
+                                                           _Float16* dlog16, float* dlog32,
+                                                           float* loss)
+{
+    __shared__ float red[256];
+    const int row = blockIdx.x;
+    const int O = L + NB;
+    const float* z = logits + (int64_t)row * O;
+    const int tid = threadIdx.x;
+    for (int head = 0; head < 2; head++) {
+        const int s = head == 0 ? 0 : L, e = head == 0 ? L : L + NB;
+        const float wgt = head == 0 ? 1.f : wb;
+        float mx = -__builtin_inff();
+        for (int c = s + tid; c < e; c += 256) mx = fmaxf(mx, z[c]);
+        red[tid] = mx;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) red[tid] = fmaxf(red[tid], red[tid + o]);
+            __syncthreads();
+        }
+        mx = red[0];
+        __syncthreads();
+        float se = 0.f;
+        for (int c = s + tid; c < e; c += 256) se += __expf(z[c] - mx);
+        red[tid] = se;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) red[tid] += red[tid + o];
+            __syncthreads();
+        }
+        se = red[0];
+        __syncthreads();
+        const int y = s + (head == 0 ? yl[row] : yb[row]);
+        const float zy = z[y];
+        __syncthreads();  // every thread has read z[y] before any gradient overwrites it
+        for (int c = s + tid; c < e; c += 256) {
+            const float p = __expf(z[c] - mx) / se;
+            const float g = wgt * (p - (c == y ? 1.f : 0.f)) / (float)B;
+            dlog16[(int64_t)row * O + c] = (_Float16)g;
+            if (dlog32) dlog32[(int64_t)row * O + c] = g;
+        }
+        if (tid == 0 && loss) atomicAdd(loss, wgt * (logf(se) + mx - zy) / (float)B);
+        __syncthreads();
+    }
+}
+
+// dH = dH_raw * (pre > 0), FP16 copy for the next product
+__global__ void relu_grad_kernel(const float* draw, const float* pre, int64_t n, float* d32,
+                                 _Float16* d16)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = pre[i] > 0.f ? draw[i] : 0.f;
+    if (d32) d32[i] = g;
+    d16[i] = (_Float16)g;
+}
+
+// column sums of an M x N FP32 matrix (bias gradients)
+__global__ void colsum_kernel(const float* G, int M, int N, float* out)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.f;
+    for (int m = 0; m < M; m++) s += G[(int64_t)m * N + n];
+    out[n] = s;
+}
+
+__global__ void adam_kernel(float* w, const float* g, float* m1, float* m2, int64_t n, float lr,
+                            float b1, float b2, float eps, float c1, float c2, _Float16* w16)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    const float v = w[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+    w[i] = v;
+    if (w16) w16[i] = (_Float16)v;
+}
+
+__global__ void f32_to_f16_kernel(const float* s, int64_t n, _Float16* d)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (_Float16)s[i];
+}
+
+__global__ void gather_i32_kernel(const int32_t* src, const int32_t* idx, int B, int32_t* dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < B) dst[i] = src[idx[i]];
+}
+
+__global__ void gather_rows_f16_kernel(const float* X, int dim, const int32_t* idx, int B,
+                                       _Float16* out)
+{
+    const int r = blockIdx.x;
+    if (r >= B) return;
+    const int64_t src = idx ? idx[r] : r;
+    for (int c = threadIdx.x; c < dim; c += blockDim.x)
+        out[(int64_t)r * dim + c] = (_Float16)X[src * dim + c];
+}
+
+// Inference head: per row, the h best landmarks (ids into `landmarks`, 1-based),
+// the best bucket, and the top landmark's softmax probability.  One wave per row.
+__global__ __launch_bounds__(64) void topk_heads_kernel(const float* logits, int B, int L, int NB,
+                                                        int h, const uint32_t* landmarks,
+                                                        uint32_t* hints, int32_t* bucket,
+                                                        float* conf)
+{
+    const int row = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int O = L + NB;
+    const float* z = logits + (int64_t)row * O;
+    float mx = -__builtin_inff();
+    for (int c = lane; c < L; c += 64) mx = fmaxf(mx, z[c]);
+    for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    float se = 0.f;
+    for (int c = lane; c < L; c += 64) se += __expf(z[c] - mx);
+    for (int o = 32; o >= 1; o >>= 1) se += __shfl_xor(se, o);
+    if (lane == 0) conf[row] = 1.f / se;
+    // h rounds of argmax with exclusion (h is small)
+    float last = __builtin_inff();
+    int lastidx = -1;
+    for (int t = 0; t < h; t++) {
+        float bv = -__builtin_inff();
+        int bi = 0x7fffffff;
+        for (int c = lane; c < L; c += 64) {
+            const float v = z[c];
+            const bool allowed = v < last || (v == last && c > lastidx);
+            if (allowed && (v > bv || (v == bv && c < bi))) {
+                bv = v;
+                bi = c;
+            }
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            const float ov = __shfl_xor(bv, o);
+            const int oi = __shfl_xor(bi, o);
+            if (ov > bv || (ov == bv && oi < bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) hints[(int64_t)row * h + t] = landmarks[bi];
+        last = bv;
+        lastidx = bi;
+    }
+    if (lane == 0) {
+        float bb = -__builtin_inff();
+        int bix = 0;
+        for (int c = 0; c < NB; c++)
+            if (z[L + c] > bb) {
+                bb = z[L + c];
+                bix = c;
+            }
+        bucket[row] = bix;
+    }
+}
+
+struct BucketTable {
+    int32_t v[16];
+};
+
+__global__ void plan_kernel(const int32_t* bucket, const float* conf, int64_t m, int32_t nh,
+                            float min_conf, BucketTable table, int32_t nb, int32_t fallback,
+                            int32_t* nhints, int32_t* bsize)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const bool ok = conf[i] >= min_conf;
+    const int b = bucket[i];
+    nhints[i] = ok ? nh : -1;
+    bsize[i] = ok && b >= 0 && b < nb ? table.v[b] : fallback;
+}
+
+}  // namespace ssg
+
+using namespace ssg;
+
+extern "C" int ssg_learned_plan_dev(ssg_handle* h, const int32_t* dbucket, const float* dconf,
+                                    int64_t m, int32_t nh, float min_conf,
+                                    const int32_t* bucket_bsize, int32_t nb, int32_t fallback_bsize,
+                                    int32_t* dnhints, int32_t* dbsize, void* stream)
+{
+    if (!h || !dbucket || !dconf || !dnhints || !dbsize || !bucket_bsize || m < 1 || nb < 1 || nb > 16)
+        return ssg_fail(SSG_E_ARG, "bad plan arguments");
+    BucketTable t{};
+    for (int i = 0; i < nb; i++) {
+        if (bucket_bsize[i] < 1) return ssg_fail(SSG_E_ARG, "bucket beam sizes must be >= 1");
+        t.v[i] = bucket_bsize[i];
+    }
+    if (fallback_bsize < 1) return ssg_fail(SSG_E_ARG, "fallback bsize must be >= 1");
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    hipLaunchKernelGGL(plan_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, dbucket,
+                       dconf, m, nh, min_conf, t, nb, fallback_bsize, dnhints, dbsize);
+    HIPCHK(hipGetLastError(), "plan");
+    return SSG_OK;
+}
+
+struct ssg_mlp;
+extern "C" int ssg_mlp_infer_dev(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dlandmarks,
+                                 int32_t h, uint32_t* dhints, int32_t* dbucket, float* dconf,
+                                 void* stream);
+
+// Learned search: inference -> plan -> beam search, one stream, handle-owned scratch.
+extern "C" int ssg_learned_search_dev(ssg_handle* h, ssg_mlp* mlp, const uint32_t* dlandmarks,
+                                      const float* dQ, int64_t m, int32_t k, int32_t kstride,
+                                      const ssg_beam_params* bp, int32_t nh, float min_conf,
+                                      const int32_t* bucket_bsize, int32_t nb,
+                                      uint32_t* dids, float* ddists, int64_t* dcd, int64_t* dcb,
+                                      void* stream)
+{
+    if (!h || !mlp || !dlandmarks || !dQ || !bp || !bucket_bsize) return ssg_fail(SSG_E_ARG, "null argument");
+    if (m < 1 || nh < 1 || nh > 64) return ssg_fail(SSG_E_ARG, "bad learned-search arguments");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    {
+        std::lock_guard<std::mutex> lk(h->mu);
+        HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+        HIPCHK(h->lH.ensure((size_t)m * nh * 4), "alloc");
+        HIPCHK(h->lB.ensure((size_t)m * 4), "alloc");
+        HIPCHK(h->lC.ensure((size_t)m * 4), "alloc");
+        HIPCHK(h->lN.ensure((size_t)m * 4), "alloc");
+        HIPCHK(h->lS.ensure((size_t)m * 4), "alloc");
+    }
+    int rc = ssg_mlp_infer_dev(mlp, dQ, m, dlandmarks, nh, h->lH.as<uint32_t>(), h->lB.as<int32_t>(),
+                               h->lC.as<float>(), s);
+    if (rc) return rc;
+    rc = ssg_learned_plan_dev(h, h->lB.as<int32_t>(), h->lC.as<float>(), m, nh, min_conf,
+                              bucket_bsize, nb, bp->bsize, h->lN.as<int32_t>(), h->lS.as<int32_t>(), s);
+    if (rc) return rc;
+    ssg_query_opts qo{nullptr, h->lS.as<int32_t>(), h->lH.as<uint32_t>(), h->lN.as<int32_t>(), nh};
+    return ssg_beamsearch_dev(h, dQ, m, k, kstride, bp, &qo, dids, ddists, dcd, dcb, s);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct ssg_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int D = 0, H1 = 0, H2 = 0, L = 0, NB = 0;
+    // parameters: FP32 master, FP16 operand copy, Adam moments, gradients
+    DevBuf W[3], Wh[3], b[3], mW[3], vW[3], mb[3], vb[3], gW[3], gb[3];
+    // per-batch work
+    DevBuf X16, C1, P1, A1, C2, P2, A2, Z, dZ16, dA32, dA16, loss;
+    DevBuf xin, yl, yb, perm;  // training inputs of one batch / one epoch
+    int cap_b = 0;
+    int64_t step = 0;
+    std::vector<uint32_t> landmarks;
+    DevBuf dland;
+};
+
+namespace {
+
+int dims_out(const ssg_mlp* p, int layer) { return layer == 0 ? p->H1 : layer == 1 ? p->H2 : p->L + p->NB; }
+int dims_in(const ssg_mlp* p, int layer) { return layer == 0 ? p->D : layer == 1 ? p->H1 : p->H2; }
+
+hipError_t gemm(hipStream_t s, int M, int N, int K, const _Float16* A, int64_t sam, int64_t sak,
+                const _Float16* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc, int acc = 0)
+{
+    const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
+    hipLaunchKernelGGL(gemm_f16_kernel, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn,
+                       C, ldc, acc);
+    return hipGetLastError();
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+int ensure_batch(ssg_mlp* p, int B)
+{
+    if (B <= p->cap_b) return SSG_OK;
+    const int O = p->L + p->NB;
+    HIPCHK(p->X16.ensure((size_t)B * p->D * 2), "alloc");
+    HIPCHK(p->C1.ensure((size_t)B * p->H1 * 4), "alloc");
+    HIPCHK(p->P1.ensure((size_t)B * p->H1 * 4), "alloc");
+    HIPCHK(p->A1.ensure((size_t)B * p->H1 * 2), "alloc");
+    HIPCHK(p->C2.ensure((size_t)B * p->H2 * 4), "alloc");
+    HIPCHK(p->P2.ensure((size_t)B * p->H2 * 4), "alloc");
+    HIPCHK(p->A2.ensure((size_t)B * p->H2 * 2), "alloc");
+    HIPCHK(p->Z.ensure((size_t)B * O * 4), "alloc");
+    HIPCHK(p->dZ16.ensure((size_t)B * O * 2), "alloc");
+    const int hmax = std::max(p->H1, p->H2);
+    HIPCHK(p->dA32.ensure((size_t)B * hmax * 4), "alloc");
+    HIPCHK(p->dA16.ensure((size_t)B * hmax * 2), "alloc");
+    HIPCHK(p->loss.ensure(16), "alloc");
+    p->cap_b = B;
+    return SSG_OK;
+}
+
+// forward over X16 (B rows) -> logits Z
+int forward(ssg_mlp* p, int B)
+{
+    hipStream_t s = p->stream;
+    const int O = p->L + p->NB;
+    HIPCHK(gemm(s, B, p->H1, p->D, p->X16.as<_Float16>(), p->D, 1, p->Wh[0].as<_Float16>(), 1, p->D,
+                p->C1.as<float>(), p->H1), "gemm");
+    hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for((int64_t)B * p->H1)), dim3(256), 0, s,
+                       p->C1.as<float>(), p->b[0].as<float>(), B, p->H1, 1, p->P1.as<float>(),
+                       p->A1.as<_Float16>());
+    HIPCHK(gemm(s, B, p->H2, p->H1, p->A1.as<_Float16>(), p->H1, 1, p->Wh[1].as<_Float16>(), 1, p->H1,
+                p->C2.as<float>(), p->H2), "gemm");
+    hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for((int64_t)B * p->H2)), dim3(256), 0, s,
+                       p->C2.as<float>(), p->b[1].as<float>(), B, p->H2, 1, p->P2.as<float>(),
+                       p->A2.as<_Float16>());
+    HIPCHK(gemm(s, B, O, p->H2, p->A2.as<_Float16>(), p->H2, 1, p->Wh[2].as<_Float16>(), 1, p->H2,
+                p->Z.as<float>(), O), "gemm");
+    hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for((int64_t)B * O)), dim3(256), 0, s,
+                       p->Z.as<float>(), p->b[2].as<float>(), B, O, 0, p->Z.as<float>(),
+                       (_Float16*)nullptr);
+    HIPCHK(hipGetLastError(), "forward");
+    return SSG_OK;
+}
+
+// backward from dZ16 (B rows); fills gW / gb
+int backward(ssg_mlp* p, int B)
+{
+    hipStream_t s = p->stream;
+    const int O = p->L + p->NB;
+    // layer 3: gW3[O x H2] = dZ^T . A2 ; dA2 = dZ . W3
+    HIPCHK(gemm(s, O, p->H2, B, p->dZ16.as<_Float16>(), 1, O, p->A2.as<_Float16>(), p->H2, 1,
+                p->gW[2].as<float>(), p->H2), "gemm");
+    HIPCHK(gemm(s, B, p->H2, O, p->dZ16.as<_Float16>(), O, 1, p->Wh[2].as<_Float16>(), p->H2, 1,
+                p->dA32.as<float>(), p->H2), "gemm");
+    // gb3 = colsum(dZ): dZ is FP16 only, so sum the FP32 logit gradient via gemm with ones
+    // is avoided: softmax_xent also wrote it in FP32 into C1-sized scratch? no -> use Z reuse
+    hipLaunchKernelGGL(colsum_kernel, dim3(blocks_for(O)), dim3(256), 0, s, p->Z.as<float>(), B, O,
+                       p->gb[2].as<float>());
+    hipLaunchKernelGGL(relu_grad_kernel, dim3(blocks_for((int64_t)B * p->H2)), dim3(256), 0, s,
+                       p->dA32.as<float>(), p->P2.as<float>(), (int64_t)B * p->H2,
+                       p->C2.as<float>(), p->dA16.as<_Float16>());
+    hipLaunchKernelGGL(colsum_kernel, dim3(blocks_for(p->H2)), dim3(256), 0, s, p->C2.as<float>(), B,
+                       p->H2, p->gb[1].as<float>());
+    // layer 2: gW2[H2 x H1] = dH2^T . A1 ; dA1 = dH2 . W2
+    HIPCHK(gemm(s, p->H2, p->H1, B, p->dA16.as<_Float16>(), 1, p->H2, p->A1.as<_Float16>(), p->H1, 1,
+                p->gW[1].as<float>(), p->H1), "gemm");
+    HIPCHK(gemm(s, B, p->H1, p->H2, p->dA16.as<_Float16>(), p->H2, 1, p->Wh[1].as<_Float16>(), p->H1, 1,
+                p->dA32.as<float>(), p->H1), "gemm");
+    // reuse dA16 for dH1 after the product above consumed dH2 (stream order)
+    hipLaunchKernelGGL(relu_grad_kernel, dim3(blocks_for((int64_t)B * p->H1)), dim3(256), 0, s,
+                       p->dA32.as<float>(), p->P1.as<float>(), (int64_t)B * p->H1,
+                       p->C1.as<float>(), p->dA16.as<_Float16>());
+    hipLaunchKernelGGL(colsum_kernel, dim3(blocks_for(p->H1)), dim3(256), 0, s, p->C1.as<float>(), B,
+                       p->H1, p->gb[0].as<float>());
+    // layer 1: gW1[H1 x D] = dH1^T . X
+    HIPCHK(gemm(s, p->H1, p->D, B, p->dA16.as<_Float16>(), 1, p->H1, p->X16.as<_Float16>(), p->D, 1,
+                p->gW[0].as<float>(), p->D), "gemm");
+    return SSG_OK;
+}
+
+uint64_t splitmix(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssg_mlp_create(ssg_mlp** out, int device, int32_t D, int32_t H1, int32_t H2, int32_t L,
+                   int32_t NB, uint64_t seed)
+{
+    if (!out) return ssg_fail(SSG_E_ARG, "null output");
+    *out = nullptr;
+    if (D < 1 || H1 < 1 || H2 < 1 || L < 1 || NB < 1) return ssg_fail(SSG_E_ARG, "bad MLP shape");
+    HIPCHK(hipSetDevice(device), "hipSetDevice");
+    ssg_mlp* p = new ssg_mlp();
+    p->device = device;
+    p->D = D;
+    p->H1 = H1;
+    p->H2 = H2;
+    p->L = L;
+    p->NB = NB;
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete p;
+        return ssg_hipfail(e, "stream");
+    }
+    // He-uniform init from a splitmix stream
+    uint64_t st = seed;
+    for (int l = 0; l < 3; l++) {
+        const int fo = dims_out(p, l), fi = dims_in(p, l);
+        std::vector<float> w((size_t)fo * fi), z((size_t)fo * fi, 0.f), bz((size_t)fo, 0.f);
+        const float lim = sqrtf(6.f / (float)fi);
+        for (auto& x : w) x = lim * (2.f * (float)((splitmix(st) >> 11) * (1.0 / 9007199254740992.0)) - 1.f);
+        const size_t nw = w.size() * 4, nb = bz.size() * 4;
+        HIPCHK(p->W[l].ensure(nw), "alloc");
+        HIPCHK(p->Wh[l].ensure(w.size() * 2), "alloc");
+        HIPCHK(p->mW[l].ensure(nw), "alloc");
+        HIPCHK(p->vW[l].ensure(nw), "alloc");
+        HIPCHK(p->gW[l].ensure(nw), "alloc");
+        HIPCHK(p->b[l].ensure(nb), "alloc");
+        HIPCHK(p->mb[l].ensure(nb), "alloc");
+        HIPCHK(p->vb[l].ensure(nb), "alloc");
+        HIPCHK(p->gb[l].ensure(nb), "alloc");
+        HIPCHK(hipMemcpy(p->W[l].p, w.data(), nw, hipMemcpyHostToDevice), "copy");
+        HIPCHK(hipMemcpy(p->mW[l].p, z.data(), nw, hipMemcpyHostToDevice), "copy");
+        HIPCHK(hipMemcpy(p->vW[l].p, z.data(), nw, hipMemcpyHostToDevice), "copy");
+        HIPCHK(hipMemcpy(p->b[l].p, bz.data(), nb, hipMemcpyHostToDevice), "copy");
+        HIPCHK(hipMemcpy(p->mb[l].p, bz.data(), nb, hipMemcpyHostToDevice), "copy");
+        HIPCHK(hipMemcpy(p->vb[l].p, bz.data(), nb, hipMemcpyHostToDevice), "copy");
+        hipLaunchKernelGGL(f32_to_f16_kernel, dim3(blocks_for((int64_t)w.size())), dim3(256), 0,
+                           p->stream, p->W[l].as<float>(), (int64_t)w.size(), p->Wh[l].as<_Float16>());
+    }
+    HIPCHK(hipStreamSynchronize(p->stream), "sync");
+    *out = p;
+    return SSG_OK;
+}
+
+int ssg_mlp_destroy(ssg_mlp* p)
+{
+    if (!p) return SSG_OK;
+    (void)hipSetDevice(p->device);
+    (void)hipStreamSynchronize(p->stream);
+    for (int l = 0; l < 3; l++) {
+        DevBuf* bs[] = {&p->W[l], &p->Wh[l], &p->b[l], &p->mW[l], &p->vW[l], &p->mb[l], &p->vb[l],
+                        &p->gW[l], &p->gb[l]};
+        for (DevBuf* b : bs) b->release();
+    }
+    DevBuf* ws[] = {&p->X16, &p->C1, &p->P1, &p->A1, &p->C2, &p->P2, &p->A2, &p->Z, &p->dZ16,
+                    &p->dA32, &p->dA16, &p->loss, &p->dland, &p->xin, &p->yl, &p->yb, &p->perm};
+    for (DevBuf* b : ws) b->release();
+    (void)hipStreamDestroy(p->stream);
+    delete p;
+    return SSG_OK;
+}
+
+// Parameters in and out (FP32, row-major W[l] as [out x in], then b[l]); layer = 0..2.
+int ssg_mlp_get(ssg_mlp* p, int32_t layer, float* W, float* b)
+{
+    if (!p || layer < 0 || layer > 2) return ssg_fail(SSG_E_ARG, "bad layer");
+    const size_t nw = (size_t)dims_out(p, layer) * dims_in(p, layer);
+    if (W) HIPCHK(hipMemcpy(W, p->W[layer].p, nw * 4, hipMemcpyDeviceToHost), "copy");
+    if (b) HIPCHK(hipMemcpy(b, p->b[layer].p, (size_t)dims_out(p, layer) * 4, hipMemcpyDeviceToHost), "copy");
+    return SSG_OK;
+}
+
+int ssg_mlp_set(ssg_mlp* p, int32_t layer, const float* W, const float* b)
+{
+    if (!p || layer < 0 || layer > 2) return ssg_fail(SSG_E_ARG, "bad layer");
+    const size_t nw = (size_t)dims_out(p, layer) * dims_in(p, layer);
+    if (W) {
+        HIPCHK(hipMemcpy(p->W[layer].p, W, nw * 4, hipMemcpyHostToDevice), "copy");
+        hipLaunchKernelGGL(f32_to_f16_kernel, dim3(blocks_for((int64_t)nw)), dim3(256), 0, p->stream,
+                           p->W[layer].as<float>(), (int64_t)nw, p->Wh[layer].as<_Float16>());
+        HIPCHK(hipStreamSynchronize(p->stream), "sync");
+    }
+    if (b) HIPCHK(hipMemcpy(p->b[layer].p, b, (size_t)dims_out(p, layer) * 4, hipMemcpyHostToDevice), "copy");
+    return SSG_OK;
+}
+
+// One forward + backward over a batch (rows of X, FP32 [B x D] host array, labels);
+// returns the loss and leaves the gradients readable through ssg_mlp_grad.  `update`
+// applies one Adam step (lr, beta1 0.9, beta2 0.999, eps 1e-8).
+}  // extern "C"
+
+// forward, loss, backward and (optionally) an Adam step on the batch already in
+// X16 / yl / yb; the loss accumulates into p->loss.
+static int train_batch(ssg_mlp* p, int B, float bucket_weight, float lr, bool update)
+{
+    hipStream_t s = p->stream;
+    int rc = forward(p, B);
+    if (rc) return rc;
+    hipLaunchKernelGGL(softmax_xent_kernel, dim3(B), dim3(256), 0, s, p->Z.as<float>(), B, p->L,
+                       p->NB, p->yl.as<int32_t>(), p->yb.as<int32_t>(), bucket_weight,
+                       p->dZ16.as<_Float16>(), p->Z.as<float>(), p->loss.as<float>());
+    rc = backward(p, B);
+    if (rc) return rc;
+    if (update) {
+        p->step++;
+        const float c1 = 1.f - powf(0.9f, (float)p->step), c2 = 1.f - powf(0.999f, (float)p->step);
+        for (int l = 0; l < 3; l++) {
+            const int64_t nw = (int64_t)dims_out(p, l) * dims_in(p, l), nb = dims_out(p, l);
+            hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(nw)), dim3(256), 0, s, p->W[l].as<float>(),
+                               p->gW[l].as<float>(), p->mW[l].as<float>(), p->vW[l].as<float>(), nw, lr,
+                               0.9f, 0.999f, 1e-8f, c1, c2, p->Wh[l].as<_Float16>());
+            hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(nb)), dim3(256), 0, s, p->b[l].as<float>(),
+                               p->gb[l].as<float>(), p->mb[l].as<float>(), p->vb[l].as<float>(), nb, lr,
+                               0.9f, 0.999f, 1e-8f, c1, c2, (_Float16*)nullptr);
+        }
+    }
+    HIPCHK(hipGetLastError(), "train step");
+    return SSG_OK;
+}
+
+extern "C" {
+
+int ssg_mlp_step(ssg_mlp* p, const float* X, const int32_t* ylandmark, const int32_t* ybucket,
+                 int32_t B, float bucket_weight, float lr, int32_t update, float* loss_out)
+{
+    if (!p || !X || !ylandmark || !ybucket || B < 1) return ssg_fail(SSG_E_ARG, "bad batch");
+    HIPCHK(hipSetDevice(p->device), "hipSetDevice");
+    int rc = ensure_batch(p, B);
+    if (rc) return rc;
+    hipStream_t s = p->stream;
+    HIPCHK(p->xin.ensure((size_t)B * p->D * 4), "alloc");
+    HIPCHK(p->yl.ensure((size_t)B * 4), "alloc");
+    HIPCHK(p->yb.ensure((size_t)B * 4), "alloc");
+    for (int64_t i = 0; i < B; i++)
+        if (ylandmark[i] < 0 || ylandmark[i] >= p->L || ybucket[i] < 0 || ybucket[i] >= p->NB)
+            return ssg_fail(SSG_E_ARG, "label out of range");
+    HIPCHK(hipMemcpyAsync(p->xin.p, X, (size_t)B * p->D * 4, hipMemcpyHostToDevice, s), "copy");
+    HIPCHK(hipMemcpyAsync(p->yl.p, ylandmark, (size_t)B * 4, hipMemcpyHostToDevice, s), "copy");
+    HIPCHK(hipMemcpyAsync(p->yb.p, ybucket, (size_t)B * 4, hipMemcpyHostToDevice, s), "copy");
+    hipLaunchKernelGGL(gather_rows_f16_kernel, dim3(B), dim3(128), 0, s, p->xin.as<float>(), p->D,
+                       (const int32_t*)nullptr, B, p->X16.as<_Float16>());
+    HIPCHK(launch_fill_f32(p->loss.as<float>(), 1, 0.f, s), "fill");
+    rc = train_batch(p, B, bucket_weight, lr, update != 0);
+    if (rc) return rc;
+    if (loss_out) HIPCHK(hipMemcpyAsync(loss_out, p->loss.p, 4, hipMemcpyDeviceToHost, s), "copy");
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    return SSG_OK;
+}
+
+// Mini-batch Adam over a device-resident training set: dX [N x D] FP32 rows, dyl / dyb
+// labels.  Each epoch visits the rows in a fresh splitmix-shuffled order.
+// losses[epochs] (host, nullable) receives the mean batch loss per epoch.
+int ssg_mlp_train_dev(ssg_mlp* p, const float* dX, const int32_t* dyl, const int32_t* dyb, int64_t N,
+                      int32_t epochs, int32_t batch, float bucket_weight, float lr, uint64_t seed,
+                      float* losses)
+{
+    if (!p || !dX || !dyl || !dyb || N < 1 || epochs < 1 || batch < 1)
+        return ssg_fail(SSG_E_ARG, "bad training arguments");
+    HIPCHK(hipSetDevice(p->device), "hipSetDevice");
+    int rc = ensure_batch(p, batch);
+    if (rc) return rc;
+    hipStream_t s = p->stream;
+    HIPCHK(p->yl.ensure((size_t)batch * 4), "alloc");
+    HIPCHK(p->yb.ensure((size_t)batch * 4), "alloc");
+    HIPCHK(p->perm.ensure((size_t)N * 4), "alloc");
+    std::vector<int32_t> perm((size_t)N);
+    uint64_t st = seed;
+    for (int ep = 0; ep < epochs; ep++) {
+        for (int64_t i = 0; i < N; i++) perm[i] = (int32_t)i;
+        for (int64_t i = N - 1; i > 0; i--) std::swap(perm[i], perm[splitmix(st) % (uint64_t)(i + 1)]);
+        HIPCHK(hipMemcpyAsync(p->perm.p, perm.data(), (size_t)N * 4, hipMemcpyHostToDevice, s), "copy");
+        HIPCHK(launch_fill_f32(p->loss.as<float>(), 1, 0.f, s), "fill");
+        int64_t nbatch = 0;
+        for (int64_t b0 = 0; b0 + batch <= N; b0 += batch, nbatch++) {
+            const int32_t* idx = p->perm.as<int32_t>() + b0;
+            hipLaunchKernelGGL(gather_rows_f16_kernel, dim3(batch), dim3(128), 0, s, dX, p->D, idx,
+                               batch, p->X16.as<_Float16>());
+            hipLaunchKernelGGL(gather_i32_kernel, dim3(blocks_for(batch)), dim3(256), 0, s, dyl, idx,
+                               batch, p->yl.as<int32_t>());
+            hipLaunchKernelGGL(gather_i32_kernel, dim3(blocks_for(batch)), dim3(256), 0, s, dyb, idx,
+                               batch, p->yb.as<int32_t>());
+            rc = train_batch(p, batch, bucket_weight, lr, true);
+            if (rc) return rc;
+        }
+        float l = 0.f;
+        HIPCHK(hipMemcpyAsync(&l, p->loss.p, 4, hipMemcpyDeviceToHost, s), "copy");
+        HIPCHK(hipStreamSynchronize(s), "sync");
+        if (losses) losses[ep] = nbatch ? l / (float)nbatch : 0.f;
+    }
+    return SSG_OK;
+}
+
+int ssg_mlp_grad(ssg_mlp* p, int32_t layer, float* gW, float* gb)
+{
+    if (!p || layer < 0 || layer > 2) return ssg_fail(SSG_E_ARG, "bad layer");
+    const size_t nw = (size_t)dims_out(p, layer) * dims_in(p, layer);
+    if (gW) HIPCHK(hipMemcpy(gW, p->gW[layer].p, nw * 4, hipMemcpyDeviceToHost), "copy");
+    if (gb) HIPCHK(hipMemcpy(gb, p->gb[layer].p, (size_t)dims_out(p, layer) * 4, hipMemcpyDeviceToHost), "copy");
+    return SSG_OK;
+}
+
+// Inference on device query rows dQ [m x D] (FP32): logits, then per query the h
+// best landmarks (as 1-based object ids through `landmarks`), bucket and confidence.
+int ssg_mlp_infer_dev(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dlandmarks,
+                      int32_t h, uint32_t* dhints, int32_t* dbucket, float* dconf, void* stream)
+{
+    if (!p || !dQ || !dlandmarks || !dhints || !dbucket || !dconf || m < 1 || h < 1 || h > p->L)
+        return ssg_fail(SSG_E_ARG, "bad inference arguments");
+    HIPCHK(hipSetDevice(p->device), "hipSetDevice");
+    hipStream_t saved = p->stream;
+    if (stream) p->stream = (hipStream_t)stream;
+    const int chunk = 8192;
+    int rc = SSG_OK;
+    for (int64_t r0 = 0; r0 < m && rc == SSG_OK; r0 += chunk) {
+        const int B = (int)std::min<int64_t>(chunk, m - r0);
+        rc = ensure_batch(p, B);
+        if (rc) break;
+        hipLaunchKernelGGL(gather_rows_f16_kernel, dim3(B), dim3(128), 0, p->stream, dQ + r0 * p->D,
+                           p->D, (const int32_t*)nullptr, B, p->X16.as<_Float16>());
+        rc = forward(p, B);
+        if (rc) break;
+        hipLaunchKernelGGL(topk_heads_kernel, dim3(B), dim3(64), 0, p->stream, p->Z.as<float>(), B,
+                           p->L, p->NB, h, dlandmarks, dhints + r0 * h, dbucket + r0, dconf + r0);
+        if (hipGetLastError() != hipSuccess) rc = ssg_fail(SSG_E_HIP, "inference launch");
+    }
+    p->stream = saved;
+    return rc;
+}
+
+}  // extern "C"

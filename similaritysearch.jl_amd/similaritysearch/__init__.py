@@ -148,11 +148,17 @@ class SearchGraphContext(GenericContext):
     """Context of SearchGraph; beamcap is the number of rows of its beam buffers,
     the cap getbeam applies to bsize (src/searchgraph/context.jl:119-126,220-225)."""
 
-    def __init__(self, beamcap=32, verbose=False):
+    def __init__(self, beamcap=32, verbose=False, precision="f32", rerank=0):
         super().__init__(verbose)
         if not 1 <= beamcap <= 256:
             raise ValueError("beamcap must be in 1..256")
+        if precision not in ("f32", "bf16"):
+            raise ValueError("precision must be 'f32' or 'bf16'")
         self.beamcap = int(beamcap)
+        # precision form (no reference counterpart): BF16 gathers during the search and an
+        # exact rerank of the `rerank` best candidates
+        self.precision = precision
+        self.rerank = int(rerank)
 
 
 # --------------------------------------------------------------------------- indexes
@@ -235,7 +241,9 @@ class SearchGraph(_GpuIndex):
 
     def beam_params(self, ctx, algo=None):
         b = algo or self.algo
-        return BeamParams(b.bsize, b.delta, b.maxvisits, getattr(ctx, "beamcap", 32))
+        prec = 1 if getattr(ctx, "precision", "f32") == "bf16" else 0
+        return BeamParams(b.bsize, b.delta, b.maxvisits, getattr(ctx, "beamcap", 32), prec,
+                          getattr(ctx, "rerank", 0))
 
     @classmethod
     def build(cls, dist, db, ctx=None, parallel_block=128, neighborhood_logbase=1.3,
@@ -410,6 +418,21 @@ def rerank_(dist, db, Q, ids, dists):
     _lib.check(_lib.load().ssg_rerank(idx._handle(), Q.rows.ctypes.data, m, k, ids.T.ctypes.data,
                                       dists.T.ctypes.data), "rerank!")
     return ids, dists
+
+
+def fft(dist, X, k, start=1, device=0):
+    """fft(dist, X, k; start) farthest-first k-centers on the GPU (src/kcenters/fft.jl:53-87).
+    Returns (centers, nn, dists, eps) with 1-based ids; eps[i] is the ε recorded when
+    center i was added (Inf for the first)."""
+    E = X if isinstance(X, _GpuIndex) else ExhaustiveSearch(dist, X, device)
+    n = len(E)
+    centers = np.zeros(k, np.uint32)
+    nn = np.zeros(n, np.uint32)
+    nnd = np.zeros(n, np.float32)
+    eps = np.zeros(k, np.float32)
+    _lib.check(_lib.load().ssg_fft(E._handle(), k, start, centers.ctypes.data, nn.ctypes.data,
+                                   nnd.ctypes.data, eps.ctypes.data), "fft")
+    return centers, nn, nnd, eps
 
 
 # --------------------------------------------------------------------------- metrics

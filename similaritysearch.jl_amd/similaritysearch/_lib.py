@@ -37,6 +37,23 @@ class BuildParams(C.Structure):
 
 # name -> (restype, argtypes); the exported surface of include/ssg.h
 SIGNATURES = {
+    "ssg_mlp_create": (C.c_int, [C.POINTER(_p), C.c_int, _i32, _i32, _i32, _i32, _i32,
+                                 C.c_uint64]),
+    "ssg_mlp_destroy": (C.c_int, [_p]),
+    "ssg_mlp_get": (C.c_int, [_p, _i32, _p, _p]),
+    "ssg_mlp_set": (C.c_int, [_p, _i32, _p, _p]),
+    "ssg_mlp_grad": (C.c_int, [_p, _i32, _p, _p]),
+    "ssg_mlp_step": (C.c_int, [_p, _p, _p, _p, _i32, C.c_float, C.c_float, _i32,
+                               C.POINTER(C.c_float)]),
+    "ssg_mlp_train_dev": (C.c_int, [_p, _p, _p, _p, _i64, _i32, _i32, C.c_float, C.c_float,
+                                    C.c_uint64, _p]),
+    "ssg_mlp_infer_dev": (C.c_int, [_p, _p, _i64, _p, _i32, _p, _p, _p, _p]),
+    "ssg_learned_plan_dev": (C.c_int, [_p, _p, _p, _i64, _i32, C.c_float, _p, _i32, _i32, _p,
+                                       _p, _p]),
+    "ssg_learned_search_dev": (C.c_int, [_p, _p, _p, _p, _i64, _i32, _i32,
+                                         C.POINTER(BeamParams), _i32, C.c_float, _p, _i32, _p,
+                                         _p, _p, _p, _p]),
+    "ssg_fft": (C.c_int, [_p, _i32, _i64, _p, _p, _p, _p]),
     "ssg_build_graph": (C.c_int, [_p, C.POINTER(BuildParams)]),
     "ssg_graph_info": (C.c_int, [_p, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(BeamParams)]),
     "ssg_download_graph": (C.c_int, [_p, _p, _p, _p]),

@@ -1,0 +1,204 @@
+"""Learned entry points for SearchGraph beam search (no reference counterpart).
+
+Pipeline, all on the GPU:
+  1. landmarks: `fft` k-centers of the database (src/kcenters/fft.jl:53-87), start=1;
+     every object's nearest landmark comes with it.
+  2. traces: for each training query, the exact top-10 (exhaustive search) gives the
+     label landmark (the landmark nearest to the query's true nearest neighbour) and,
+     by running the search itself from that landmark with growing beam sizes, the
+     smallest beam-size bucket whose result equals the exact top-10.
+  3. an MLP (ssg_mlp, hand-written HIP forward/backward/Adam kernels) learns
+     query -> (landmark, bucket).
+  4. search: inference gives the top `nhints` landmarks as per-query hints and the
+     bucket's beam size; queries whose top-landmark probability is below `min_conf`
+     fall back to the plain search (SearchGraph.hints and the plain beam size).
+     The search itself runs in the BF16 precision form with an exact rerank.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import BeamParams, QueryOpts
+
+DEFAULT_BUCKETS = (2, 4, 8, 16, 32, 64, 128, 256)
+
+
+class Mlp:
+    """D -> H1 -> H2 -> (L + NB) MLP on the GPU (include/ssg.h, ssg_mlp_*)."""
+
+    def __init__(self, D, H1, H2, L, NB, seed=0, device=0):
+        self.shape = (D, H1, H2, L, NB)
+        self._p = C.c_void_p()
+        _lib.check(_lib.load().ssg_mlp_create(C.byref(self._p), device, D, H1, H2, L, NB, seed),
+                   "ssg_mlp_create")
+
+    def _dims(self, layer):
+        D, H1, H2, L, NB = self.shape
+        return [(H1, D), (H2, H1), (L + NB, H2)][layer]
+
+    def get(self, layer):
+        o, i = self._dims(layer)
+        W = np.zeros((o, i), np.float32)
+        b = np.zeros(o, np.float32)
+        _lib.check(_lib.load().ssg_mlp_get(self._p, layer, W.ctypes.data, b.ctypes.data), "get")
+        return W, b
+
+    def set(self, layer, W, b):
+        W = np.ascontiguousarray(W, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        _lib.check(_lib.load().ssg_mlp_set(self._p, layer, W.ctypes.data, b.ctypes.data), "set")
+
+    def grads(self, layer):
+        o, i = self._dims(layer)
+        gW = np.zeros((o, i), np.float32)
+        gb = np.zeros(o, np.float32)
+        _lib.check(_lib.load().ssg_mlp_grad(self._p, layer, gW.ctypes.data, gb.ctypes.data), "grad")
+        return gW, gb
+
+    def step(self, X, yl, yb, bucket_weight=1.0, lr=1e-3, update=True):
+        X = np.ascontiguousarray(X, np.float32)
+        yl = np.ascontiguousarray(yl, np.int32)
+        yb = np.ascontiguousarray(yb, np.int32)
+        loss = C.c_float()
+        _lib.check(_lib.load().ssg_mlp_step(self._p, X.ctypes.data, yl.ctypes.data, yb.ctypes.data,
+                                            len(X), bucket_weight, lr, int(update), C.byref(loss)),
+                   "mlp step")
+        return loss.value
+
+    def train_dev(self, dX, dyl, dyb, N, epochs, batch, bucket_weight=1.0, lr=1e-3, seed=0):
+        losses = np.zeros(epochs, np.float32)
+        _lib.check(_lib.load().ssg_mlp_train_dev(self._p, dX, dyl, dyb, N, epochs, batch,
+                                                 bucket_weight, lr, seed, losses.ctypes.data),
+                   "mlp train")
+        return losses
+
+    def close(self):
+        if self._p:
+            _lib.load().ssg_mlp_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LearnedSearch:
+    """Learned entry points + BF16 beam search over a SearchGraph `G`."""
+
+    def __init__(self, G, nlandmarks=4096, hidden=256, buckets=DEFAULT_BUCKETS, seed=9):
+        self.G = G
+        self.L = int(nlandmarks)
+        self.hidden = int(hidden)
+        self.buckets = tuple(int(b) for b in buckets)
+        self.seed = seed
+        self.mlp = None
+        self.landmarks = None
+
+    # ---------------------------------------------------------------- traces
+    def landmark_labels(self):
+        from . import fft
+        centers, nn, _, _ = fft(self.G.dist, self.G, self.L, start=1)
+        self.landmarks = centers
+        pos = np.full(len(self.G) + 1, -1, np.int64)
+        pos[centers.astype(np.int64)] = np.arange(self.L)
+        self.object_landmark = pos[nn.astype(np.int64)]  # landmark index of each object
+        return centers
+
+    def traces(self, Qtrain, ctx, k=10, rerank=20, batch=50_000):
+        """Labels (landmark index, bucket index) for each training query row."""
+        import torch
+        from . import MatrixDatabase
+        G = self.G
+        h = G._handle()
+        L = _lib.load()
+        m = len(Qtrain)
+        dev = torch.device("cuda:0")
+        yl = np.zeros(m, np.int32)
+        yb = np.full(m, len(self.buckets) - 1, np.int32)
+        found = np.zeros(m, bool)
+        for s in range(0, m, batch):
+            e = min(m, s + batch)
+            Qb = np.ascontiguousarray(Qtrain[s:e], np.float32)
+            gi = np.zeros((e - s, k), np.uint32)
+            gd = np.zeros((e - s, k), np.float32)
+            _lib.check(L.ssg_exhaustive_knn(h, Qb.ctypes.data, e - s, k, gi.ctypes.data,
+                                            gd.ctypes.data), "exhaustive")
+            yl[s:e] = self.object_landmark[gi[:, 0].astype(np.int64)]
+            hints = self.landmarks[yl[s:e]].astype(np.uint32).reshape(-1, 1).copy()
+            cnt = np.ones(e - s, np.int32)
+            gold = np.sort(gi, axis=1)
+            fnd = np.zeros(e - s, bool)
+            for bi, b in enumerate(self.buckets):
+                bq = np.full(e - s, b, np.int32)
+                qo = QueryOpts(None, bq.ctypes.data, hints.ctypes.data, cnt.ctypes.data, 1)
+                bp = BeamParams(b, G.algo.delta, 10**6, max(ctx.beamcap, b), 1, rerank)
+                ids = np.zeros((e - s, k), np.uint32)
+                ds = np.zeros((e - s, k), np.float32)
+                _lib.check(L.ssg_beamsearch(h, Qb.ctypes.data, e - s, k, k, C.byref(bp), C.byref(qo),
+                                            ids.ctypes.data, ds.ctypes.data, None, None), "trace search")
+                ok = np.all(np.sort(ids, axis=1) == gold, axis=1) & ~fnd
+                yb[s:e][ok] = bi
+                fnd |= ok
+            found[s:e] = fnd
+        del dev
+        return yl, yb, found
+
+    # ---------------------------------------------------------------- training
+    def fit(self, Qtrain, ctx, epochs=6, batch=1024, lr=2e-3, bucket_weight=0.5, verbose=None):
+        import torch
+        if self.landmarks is None:
+            self.landmark_labels()
+        yl, yb, found = self.traces(Qtrain, ctx)
+        if verbose:
+            verbose(f"traces: {len(Qtrain)} queries, exact top-10 reached for {found.mean():.3f}, "
+                    f"bucket histogram {np.bincount(yb, minlength=len(self.buckets)).tolist()}")
+        D = Qtrain.shape[1]
+        self.mlp = Mlp(D, self.hidden, self.hidden, self.L, len(self.buckets), seed=self.seed)
+        dev = torch.device("cuda:0")
+        dX = torch.from_numpy(np.ascontiguousarray(Qtrain, np.float32)).to(dev)
+        dyl = torch.from_numpy(yl).to(dev)
+        dyb = torch.from_numpy(yb).to(dev)
+        torch.cuda.synchronize()
+        losses = self.mlp.train_dev(dX.data_ptr(), dyl.data_ptr(), dyb.data_ptr(), len(Qtrain),
+                                    epochs, batch, bucket_weight, lr, self.seed)
+        self.dlandmarks = torch.from_numpy(self.landmarks.view(np.int32)).to(dev)
+        self.train_labels = (yl, yb)
+        return losses
+
+    # ---------------------------------------------------------------- search
+    def search_dev(self, ctx, dQ, m, k, dI, dD, nhints=1, min_conf=0.0, shift=0, rerank=20,
+                   fallback_bsize=None, stream=None, dcd=None, dcb=None):
+        """Device-pointer learned search; bucket b maps to beam size buckets[b + shift]."""
+        G = self.G
+        nb = len(self.buckets)
+        table = np.array([self.buckets[min(nb - 1, max(0, b + shift))] for b in range(nb)], np.int32)
+        fb = fallback_bsize or G.algo.bsize
+        bp = BeamParams(fb, G.algo.delta, 10**6, max(ctx.beamcap, int(table.max())), 1, rerank)
+        _lib.check(_lib.load().ssg_learned_search_dev(
+            G._handle(), self.mlp._p, self.dlandmarks.data_ptr(), dQ, m, k, k, C.byref(bp),
+            nhints, min_conf, table.ctypes.data, nb, dI, dD, dcd, dcb, stream), "learned search")
+
+    def search(self, ctx, Q, k, **kw):
+        import torch
+        from . import _as_db
+        Q = _as_db(Q)
+        m = len(Q)
+        dev = torch.device("cuda:0")
+        dQ = torch.from_numpy(Q.rows).to(dev)
+        dI = torch.zeros((m, k), dtype=torch.int32, device=dev)
+        dD = torch.zeros((m, k), dtype=torch.float32, device=dev)
+        dcd = torch.zeros(m, dtype=torch.int64, device=dev)
+        dcb = torch.zeros(m, dtype=torch.int64, device=dev)
+        s = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        self.search_dev(ctx, dQ.data_ptr(), m, k, dI.data_ptr(), dD.data_ptr(),
+                        stream=C.c_void_p(s.cuda_stream), dcd=dcd.data_ptr(), dcb=dcb.data_ptr(),
+                        **kw)
+        torch.cuda.synchronize()
+        ctx.costdists = dcd.cpu().numpy()
+        ctx.costblocks = dcb.cpu().numpy()
+        ids = dI.cpu().numpy().view(np.uint32)
+        return ids.T, dD.cpu().numpy().T
