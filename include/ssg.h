@@ -26,6 +26,7 @@
 #ifndef SSG_H
 #define SSG_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -48,6 +49,21 @@ enum { SSG_SQL2 = 0, SSG_L2 = 1, SSG_NORMCOS = 2 };
 
 /* Storage precision of the device copy of the database. */
 enum { SSG_F32 = 0, SSG_BF16 = 1 };
+
+/* Device memory, streams and events (thin HIP wrappers, no reference counterpart) for hosts
+ * that have no GPU runtime of their own; the _dev entry points take these pointers.
+ * ssg_memcpy kind: 0 host->device, 1 device->host, 2 device->device (synchronous). */
+int ssg_dev_malloc(void** p, size_t bytes);
+int ssg_dev_free(void* p);
+int ssg_memcpy(void* dst, const void* src, size_t bytes, int32_t kind);
+int ssg_device_synchronize(void);
+int ssg_stream_create(void** s);
+int ssg_stream_destroy(void* s);
+int ssg_stream_synchronize(void* s);
+int ssg_event_create(void** e);
+int ssg_event_destroy(void* e);
+int ssg_event_record(void* e, void* stream);
+int ssg_event_elapsed_ms(void* e0, void* e1, float* ms);
 
 /* Lifecycle.  Replaces constructing the Julia index object; device = HIP ordinal. */
 int ssg_create(ssg_handle** h, int device);

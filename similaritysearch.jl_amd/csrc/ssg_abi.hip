@@ -34,8 +34,10 @@ int ssg_hipfail(hipError_t e, const char* what)
                     std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Visited-byte slots: one per resident wave, n bytes each (rounded to 1 KiB).
-static int ensure_vis(ssg_handle* h)
+// Visited-byte slots: one per resident wave, n bytes each (rounded to 1 KiB).  They
+// are cleared on stream s, the stream of the launch that follows (a memset on the
+// null stream would not be ordered before work on a non-blocking stream).
+static int ensure_vis(ssg_handle* h, hipStream_t s)
 {
     const int64_t stride = (std::max<int64_t>(h->n, 1) + 1023) / 1024 * 1024;
     int want = h->num_cu * 32;
@@ -48,8 +50,9 @@ static int ensure_vis(ssg_handle* h)
     HIPCHK(h->vis.ensure((size_t)want * stride), "alloc visited slots");
     HIPCHK(h->epochs.ensure((size_t)want), "alloc epochs");
     HIPCHK(h->counter.ensure(sizeof(unsigned int) * 4), "alloc counter");
-    HIPCHK(hipMemset(h->vis.p, 0, (size_t)want * stride), "clear visited slots");
-    HIPCHK(hipMemset(h->epochs.p, 0, (size_t)want), "clear epochs");
+    HIPCHK(launch_zero_u32(h->vis.as<unsigned int>(), (int64_t)want * stride / 4, s),
+           "clear visited slots");
+    HIPCHK(launch_zero_u32(h->epochs.as<unsigned int>(), ((int64_t)want + 3) / 4, s), "clear epochs");
     h->vis_stride = stride;
     h->nslots = want;
     return SSG_OK;
@@ -78,7 +81,7 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     const int32_t krr = std::max(k, bp->rerank);
     if (bp->precision == SSG_BF16 && (krr > 64 || kstride > 64))
         return ssg_fail(SSG_E_LIMIT, "BF16 rerank list is limited to 64");
-    int rc = ensure_vis(h);
+    int rc = ensure_vis(h, s);
     if (rc) return rc;
     if (bp->precision == SSG_BF16 && !h->Xh.p) {
         HIPCHK(h->Xh.ensure((size_t)std::max<int64_t>(h->n_pad, 64) * h->dp * 2), "alloc BF16 copy");
@@ -200,6 +203,7 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
 // Frozen lists -> node table + packed pool on the device.
 int ssg_install_lists(ssg_handle* h)
 {
+    HIPCHK(hipStreamSynchronize(h->stream), "sync");  // no launch may still read the old graph
     const int64_t n = (int64_t)h->lists.size();
     std::vector<uint2> node((size_t)std::max<int64_t>(n, 1));
     int64_t nnz = 0;
@@ -242,6 +246,85 @@ static hipStream_t pick(ssg_handle* h, void* stream)
 }
 
 extern "C" {
+
+// device memory, streams and events for hosts without their own GPU runtime
+int ssg_dev_malloc(void** p, size_t bytes)
+{
+    if (!p) return ssg_fail(SSG_E_ARG, "null output");
+    *p = nullptr;
+    HIPCHK(hipMalloc(p, std::max<size_t>(bytes, 16)), "hipMalloc");
+    return SSG_OK;
+}
+
+int ssg_dev_free(void* p)
+{
+    if (p) HIPCHK(hipFree(p), "hipFree");
+    return SSG_OK;
+}
+
+int ssg_memcpy(void* dst, const void* src, size_t bytes, int32_t kind)
+{
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    if (bytes) HIPCHK(hipMemcpy(dst, src, bytes, k), "hipMemcpy");
+    return SSG_OK;
+}
+
+int ssg_device_synchronize(void)
+{
+    HIPCHK(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return SSG_OK;
+}
+
+int ssg_stream_create(void** s)
+{
+    if (!s) return ssg_fail(SSG_E_ARG, "null output");
+    hipStream_t st = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+    *s = st;
+    return SSG_OK;
+}
+
+int ssg_stream_destroy(void* s)
+{
+    if (s) HIPCHK(hipStreamDestroy((hipStream_t)s), "hipStreamDestroy");
+    return SSG_OK;
+}
+
+int ssg_stream_synchronize(void* s)
+{
+    HIPCHK(hipStreamSynchronize((hipStream_t)s), "hipStreamSynchronize");
+    return SSG_OK;
+}
+
+int ssg_event_create(void** e)
+{
+    if (!e) return ssg_fail(SSG_E_ARG, "null output");
+    hipEvent_t ev = nullptr;
+    HIPCHK(hipEventCreate(&ev), "hipEventCreate");
+    *e = ev;
+    return SSG_OK;
+}
+
+int ssg_event_destroy(void* e)
+{
+    if (e) HIPCHK(hipEventDestroy((hipEvent_t)e), "hipEventDestroy");
+    return SSG_OK;
+}
+
+int ssg_event_record(void* e, void* s)
+{
+    HIPCHK(hipEventRecord((hipEvent_t)e, (hipStream_t)s), "hipEventRecord");
+    return SSG_OK;
+}
+
+int ssg_event_elapsed_ms(void* e0, void* e1, float* ms)
+{
+    if (!ms) return ssg_fail(SSG_E_ARG, "null output");
+    HIPCHK(hipEventSynchronize((hipEvent_t)e1), "hipEventSynchronize");
+    HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)e0, (hipEvent_t)e1), "hipEventElapsedTime");
+    return SSG_OK;
+}
 
 const char* ssg_version(void) { return "ssg_hip 0.1 gfx950"; }
 
@@ -304,6 +387,7 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
         return ssg_fail(SSG_E_LIMIT, "ssg_upload_db: dim must pad to 16..128, 192 or 256");
     if (n >= (int64_t)0xffffffffLL) return ssg_fail(SSG_E_LIMIT, "ssg_upload_db: n must fit UInt32 ids");
     HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    HIPCHK(hipStreamSynchronize(h->stream), "sync");  // no launch may still read the old database
     h->dim = dim;
     h->dp = dp;
     h->metric = metric;

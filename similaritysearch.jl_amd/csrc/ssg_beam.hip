@@ -46,6 +46,18 @@ __device__ __forceinline__ void vm_drain()
     asm volatile("" ::: "memory");
 }
 
+// Visited bytes: all reads, marks, unmarks and the slot clear use the same
+// (workgroup-scope, relaxed) atomic access, so they share one cache policy and a
+// clear is never hidden behind a copy held at another cache level.
+__device__ __forceinline__ uint8_t vis_get(const uint8_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void vis_put(uint8_t* p, uint8_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // PREC 0: canonical distance on the FP32 vectors.  PREC 1: BF16 vectors with FP32
 // arithmetic during the search, then an exact (canonical) rerank of the result list.
 template <int V, int NB, int PREC>
@@ -58,15 +70,16 @@ struct Searcher {
     int64_t cd_inner;  // costdists of beamsearch_inner_beam
     int64_t cb;        // costblocks
     uint32_t epoch;
-    volatile uint8_t* vis;
+    uint8_t* vis;  // wave-private slot; every access is a workgroup-scope relaxed atomic
     uint32_t* cidbuf;  // 64 ids, wave-private LDS
     int lane, g, j;
 
     __device__ __forceinline__ Searcher(const BeamArgs& args) : a(args) {}
 
-    // Evaluate cidbuf[ub .. ub+cnt) (cnt <= BATCH); lane u receives child u's
-    // distance, id, list start and degree.
-    __device__ __forceinline__ void eval_batch(int ub, int cnt, float& dd, uint32_t& cid,
+    // Evaluate cidbuf[ub .. ub+cnt) (cnt <= BATCH).  Child u = 4r + q is evaluated by
+    // row q in round r and its distance is dr[r] at lane 16q + 15; lane u receives its
+    // id, list start and degree.
+    __device__ __forceinline__ void eval_batch(int ub, int cnt, float (&dr)[ROUNDS], uint32_t& cid,
                                                uint32_t& st, uint32_t& dg, bool need_deg)
     {
         const int dp = 16 * V;
@@ -90,27 +103,28 @@ struct Searcher {
                 }
             }
         }
-        float dr[ROUNDS];
 #pragma unroll
         for (int r = 0; r < ROUNDS; r++) {
             dr[r] = __builtin_inff();
             if (4 * r < cnt) {
                 if constexpr (PREC == 1)
-                    dr[r] = finish_f32(a.metric, group16_sum_f32(run_partial_f32<V>(a.metric, xv[r], qv)));
+                    dr[r] = finish_f32(a.metric, group16_tree_f32(run_partial_f32<V>(a.metric, xv[r], qv)));
                 else
-                    dr[r] = finish_dist(a.metric, group16_sum(run_partial<V>(a.metric, xv[r], qv)));
-            }
-        }
-        dd = __builtin_inff();
-#pragma unroll
-        for (int r = 0; r < ROUNDS; r++) {
-            if (4 * r < cnt) {
-                const float t = __shfl(dr[r], (lane & 3) * 16);
-                if ((lane >> 2) == r) dd = t;
+                    dr[r] = finish_dist(a.metric, group16_tree(run_partial<V>(a.metric, xv[r], qv)));
             }
         }
         st = nd.x;
         dg = nd.y;
+    }
+
+    // distance of child u (wave-uniform) of the last batch
+    __device__ __forceinline__ float child_dist(const float (&dr)[ROUNDS], int u) const
+    {
+        float d = 0.f;
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++)
+            if ((u >> 2) == r) d = rl_f(dr[r], 16 * (u & 3) + 15);
+        return d;
     }
 
     // One chunk of up to 64 candidate ids (lane-indexed, `valid` lanes).  Returns
@@ -121,7 +135,7 @@ struct Searcher {
         bool unv = valid;
         if (mode != MODE_BRUTE) {
             vm_drain();
-            if (valid) unv = vis[cid] != (uint8_t)epoch;
+            if (valid) unv = vis_get(vis + cid) != (uint8_t)epoch;
             if (dedupe) {
                 const uint64_t vm = __ballot(valid);
                 const int cnt = 64 - __clzll(vm);
@@ -130,7 +144,7 @@ struct Searcher {
                     if (lane > t && cid == c) unv = false;
                 }
             }
-            if (unv) vis[cid] = (uint8_t)epoch;
+            if (unv) vis_put(vis + cid, (uint8_t)epoch);
         }
         const uint64_t mask = __ballot(unv);
         const int U = __popcll(mask);
@@ -140,9 +154,9 @@ struct Searcher {
         bool stopped = false;
         for (int ub = 0; ub < U && !stopped; ub += BATCH) {
             const int cnt = U - ub < BATCH ? U - ub : BATCH;
-            float dd;
+            float dr[ROUNDS];
             uint32_t c, st, dg;
-            eval_batch(ub, cnt, dd, c, st, dg, mode == MODE_EXPAND);
+            eval_batch(ub, cnt, dr, c, st, dg, mode == MODE_EXPAND);
             int lim = cnt, stop = -1;
             if (mode == MODE_EXPAND && cd_inner + cnt > a.maxvisits) {
                 stop = (int)(a.maxvisits - cd_inner);
@@ -151,13 +165,24 @@ struct Searcher {
             const bool full = res.len == res.cap;
             const float mx0 = full ? res.maximum() : __builtin_inff();
             const float th0 = a.delta * mx0;
-            bool cand = lane < lim;
-            if (full) cand = cand && (dd < mx0 || (mode == MODE_EXPAND && dd <= th0));
-            uint64_t cm = __ballot(cand);
+            // candidates: children that may enter res or the beam (maximum(res) only
+            // decreases while res is full, so a child failing now fails later too)
+            uint64_t cm = 0;
+#pragma unroll
+            for (int r = 0; r < ROUNDS; r++) {
+                if (4 * r < cnt) {
+                    const float d = dr[r];
+                    bool cand = j == 15 && 4 * r + g < lim;
+                    if (full) cand = cand && (d < mx0 || (mode == MODE_EXPAND && d <= th0));
+                    const uint64_t b = __ballot(cand);
+                    cm |= (((b >> 15) & 1ull) | (((b >> 31) & 1ull) << 1) | (((b >> 47) & 1ull) << 2) |
+                           (((b >> 63) & 1ull) << 3)) << (4 * r);
+                }
+            }
             while (cm) {
                 const int u = __ffsll((unsigned long long)cm) - 1;
                 cm &= cm - 1;
-                const float du = rl_f(dd, u);
+                const float du = child_dist(dr, u);
                 const uint32_t cu = rl_u(c, u);
                 res.push(du, cu, 0, 0);
                 if (mode != MODE_EXPAND) continue;
@@ -170,7 +195,7 @@ struct Searcher {
                 cd_inner += lim;
                 if (stop >= 0) {
                     // children after the cut were marked ahead of time: unmark them
-                    if (lane > ub + stop && lane < U) vis[cidbuf[lane]] = 0;
+                    if (lane > ub + stop && lane < U) vis_put(vis + cidbuf[lane], 0);
                     stopped = true;
                 }
             } else {
@@ -294,9 +319,9 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
         if ((int64_t)qi >= a.m) break;
         epoch++;
         if (epoch > 255) {
-            uint8_t* vp = (uint8_t*)(a.vis + slot * a.vis_stride);
-            for (int64_t off = (int64_t)lane * 16; off < a.vis_stride; off += 64 * 16)
-                *reinterpret_cast<uint4*>(vp + off) = make_uint4(0, 0, 0, 0);
+            uint64_t* vp = reinterpret_cast<uint64_t*>(a.vis + slot * a.vis_stride);
+            for (int64_t w8 = lane; w8 < a.vis_stride / 8; w8 += 64)
+                __hip_atomic_store(vp + w8, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             vm_drain();
             epoch = 1;
         }
@@ -321,7 +346,7 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
             for (int64_t t = ni.x; t < (int64_t)ni.x + ni.y; t++) {
                 const uint32_t h = a.adj[t];
                 vm_drain();
-                const uint32_t seen = rfl_u((uint32_t)S.vis[h]);
+                const uint32_t seen = rfl_u((uint32_t)vis_get(S.vis + h));
                 if (seen == epoch) continue;
                 const uint32_t h1 = h + 1u;
                 S.search(&h1, 1, true, bs);

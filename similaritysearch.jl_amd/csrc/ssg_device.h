@@ -145,6 +145,52 @@ __device__ __forceinline__ double run_partial(int metric, const float (&x)[V], c
     return s;
 }
 
+// ---------------------------------------------------------------------------
+// DPP data movement (VALU, no LDS round trip).  Controls of the GFX9 family:
+// row_shr:n = 0x110 + n (within rows of 16 lanes), wave_shr:1 = 0x138,
+// wave_shl:1 = 0x130.  Lanes without a source read 0 (bound_ctrl).
+// ---------------------------------------------------------------------------
+enum { DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118,
+       DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138 };
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = dpp_u32<CTRL>((uint32_t)u), hi = dpp_u32<CTRL>((uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// The same 16-lane tree as group16_sum -- pairs (j, j+8), then (j, j+4), (j, j+2),
+// (j, j+1) of the partial results -- evaluated with row shifts; only lane 15 of each
+// 16-lane row holds the total (bit-identical to group16_sum's).
+__device__ __forceinline__ double group16_tree(double s)
+{
+    s = s + dpp_f64<DPP_ROW_SHR8>(s);
+    s = s + dpp_f64<DPP_ROW_SHR4>(s);
+    s = s + dpp_f64<DPP_ROW_SHR2>(s);
+    s = s + dpp_f64<DPP_ROW_SHR1>(s);
+    return s;
+}
+__device__ __forceinline__ float group16_tree_f32(float s)
+{
+    s = s + dpp_f32<DPP_ROW_SHR8>(s);
+    s = s + dpp_f32<DPP_ROW_SHR4>(s);
+    s = s + dpp_f32<DPP_ROW_SHR2>(s);
+    s = s + dpp_f32<DPP_ROW_SHR1>(s);
+    return s;
+}
+
 // xor-butterfly over the 16 lanes of a group; every lane ends with the total.
 __device__ __forceinline__ double group16_sum(double s)
 {
@@ -226,12 +272,12 @@ struct RegQueue {
 #pragma unroll
         for (int b = NB - 1; b >= 0; b--) {
             const int e = 64 * b + lane;
-            float ud = __shfl_up(d[b], 1);
-            uint32_t ui = __shfl_up(id[b], 1);
+            float ud = dpp_f32<DPP_WAVE_SHR1>(d[b]);
+            uint32_t ui = dpp_u32<DPP_WAVE_SHR1>(id[b]);
             uint32_t ua = 0, ub = 0;
             if constexpr (PAY) {
-                ua = __shfl_up(pa[b], 1);
-                ub = __shfl_up(pb[b], 1);
+                ua = dpp_u32<DPP_WAVE_SHR1>(pa[b]);
+                ub = dpp_u32<DPP_WAVE_SHR1>(pb[b]);
             }
             if (b > 0) {
                 const float cd = rl_f(d[b > 0 ? b - 1 : 0], 63);
@@ -283,12 +329,12 @@ struct RegQueue {
         }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
-            float nd = __shfl_down(d[b], 1);
-            uint32_t ni = __shfl_down(id[b], 1);
+            float nd = dpp_f32<DPP_WAVE_SHL1>(d[b]);
+            uint32_t ni = dpp_u32<DPP_WAVE_SHL1>(id[b]);
             uint32_t na = 0, nb = 0;
             if constexpr (PAY) {
-                na = __shfl_down(pa[b], 1);
-                nb = __shfl_down(pb[b], 1);
+                na = dpp_u32<DPP_WAVE_SHL1>(pa[b]);
+                nb = dpp_u32<DPP_WAVE_SHL1>(pb[b]);
             }
             if (b + 1 < NB) {
                 const int bn = b + 1 < NB ? b + 1 : b;

@@ -37,6 +37,17 @@ class BuildParams(C.Structure):
 
 # name -> (restype, argtypes); the exported surface of include/ssg.h
 SIGNATURES = {
+    "ssg_dev_malloc": (C.c_int, [C.POINTER(_p), C.c_size_t]),
+    "ssg_dev_free": (C.c_int, [_p]),
+    "ssg_memcpy": (C.c_int, [_p, _p, C.c_size_t, _i32]),
+    "ssg_device_synchronize": (C.c_int, []),
+    "ssg_stream_create": (C.c_int, [C.POINTER(_p)]),
+    "ssg_stream_destroy": (C.c_int, [_p]),
+    "ssg_stream_synchronize": (C.c_int, [_p]),
+    "ssg_event_create": (C.c_int, [C.POINTER(_p)]),
+    "ssg_event_destroy": (C.c_int, [_p]),
+    "ssg_event_record": (C.c_int, [_p, _p]),
+    "ssg_event_elapsed_ms": (C.c_int, [_p, _p, C.POINTER(C.c_float)]),
     "ssg_mlp_create": (C.c_int, [C.POINTER(_p), C.c_int, _i32, _i32, _i32, _i32, _i32,
                                  C.c_uint64]),
     "ssg_mlp_destroy": (C.c_int, [_p]),

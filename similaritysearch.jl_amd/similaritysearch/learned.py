@@ -20,6 +20,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import BeamParams, QueryOpts
+from .device import DeviceArray, Stream
 
 DEFAULT_BUCKETS = (2, 4, 8, 16, 32, 64, 128, 256)
 
@@ -109,13 +110,10 @@ class LearnedSearch:
 
     def traces(self, Qtrain, ctx, k=10, rerank=20, batch=50_000):
         """Labels (landmark index, bucket index) for each training query row."""
-        import torch
-        from . import MatrixDatabase
         G = self.G
         h = G._handle()
         L = _lib.load()
         m = len(Qtrain)
-        dev = torch.device("cuda:0")
         yl = np.zeros(m, np.int32)
         yb = np.full(m, len(self.buckets) - 1, np.int32)
         found = np.zeros(m, bool)
@@ -126,7 +124,7 @@ class LearnedSearch:
             gd = np.zeros((e - s, k), np.float32)
             _lib.check(L.ssg_exhaustive_knn(h, Qb.ctypes.data, e - s, k, gi.ctypes.data,
                                             gd.ctypes.data), "exhaustive")
-            yl[s:e] = self.object_landmark[gi[:, 0].astype(np.int64)]
+            yl[s:e] = self.object_landmark[gi[:, 0].astype(np.int64) - 1]
             hints = self.landmarks[yl[s:e]].astype(np.uint32).reshape(-1, 1).copy()
             cnt = np.ones(e - s, np.int32)
             gold = np.sort(gi, axis=1)
@@ -143,12 +141,10 @@ class LearnedSearch:
                 yb[s:e][ok] = bi
                 fnd |= ok
             found[s:e] = fnd
-        del dev
         return yl, yb, found
 
     # ---------------------------------------------------------------- training
     def fit(self, Qtrain, ctx, epochs=6, batch=1024, lr=2e-3, bucket_weight=0.5, verbose=None):
-        import torch
         if self.landmarks is None:
             self.landmark_labels()
         yl, yb, found = self.traces(Qtrain, ctx)
@@ -157,14 +153,12 @@ class LearnedSearch:
                     f"bucket histogram {np.bincount(yb, minlength=len(self.buckets)).tolist()}")
         D = Qtrain.shape[1]
         self.mlp = Mlp(D, self.hidden, self.hidden, self.L, len(self.buckets), seed=self.seed)
-        dev = torch.device("cuda:0")
-        dX = torch.from_numpy(np.ascontiguousarray(Qtrain, np.float32)).to(dev)
-        dyl = torch.from_numpy(yl).to(dev)
-        dyb = torch.from_numpy(yb).to(dev)
-        torch.cuda.synchronize()
-        losses = self.mlp.train_dev(dX.data_ptr(), dyl.data_ptr(), dyb.data_ptr(), len(Qtrain),
-                                    epochs, batch, bucket_weight, lr, self.seed)
-        self.dlandmarks = torch.from_numpy(self.landmarks.view(np.int32)).to(dev)
+        dX = DeviceArray.from_numpy(np.ascontiguousarray(Qtrain, np.float32))
+        dyl = DeviceArray.from_numpy(yl)
+        dyb = DeviceArray.from_numpy(yb)
+        losses = self.mlp.train_dev(dX.ptr, dyl.ptr, dyb.ptr, len(Qtrain), epochs, batch,
+                                    bucket_weight, lr, self.seed)
+        self.dlandmarks = DeviceArray.from_numpy(self.landmarks)
         self.train_labels = (yl, yb)
         return losses
 
@@ -178,27 +172,22 @@ class LearnedSearch:
         fb = fallback_bsize or G.algo.bsize
         bp = BeamParams(fb, G.algo.delta, 10**6, max(ctx.beamcap, int(table.max())), 1, rerank)
         _lib.check(_lib.load().ssg_learned_search_dev(
-            G._handle(), self.mlp._p, self.dlandmarks.data_ptr(), dQ, m, k, k, C.byref(bp),
+            G._handle(), self.mlp._p, self.dlandmarks.ptr, dQ, m, k, k, C.byref(bp),
             nhints, min_conf, table.ctypes.data, nb, dI, dD, dcd, dcb, stream), "learned search")
 
     def search(self, ctx, Q, k, **kw):
-        import torch
         from . import _as_db
         Q = _as_db(Q)
         m = len(Q)
-        dev = torch.device("cuda:0")
-        dQ = torch.from_numpy(Q.rows).to(dev)
-        dI = torch.zeros((m, k), dtype=torch.int32, device=dev)
-        dD = torch.zeros((m, k), dtype=torch.float32, device=dev)
-        dcd = torch.zeros(m, dtype=torch.int64, device=dev)
-        dcb = torch.zeros(m, dtype=torch.int64, device=dev)
-        s = torch.cuda.Stream()
-        torch.cuda.synchronize()
-        self.search_dev(ctx, dQ.data_ptr(), m, k, dI.data_ptr(), dD.data_ptr(),
-                        stream=C.c_void_p(s.cuda_stream), dcd=dcd.data_ptr(), dcb=dcb.data_ptr(),
-                        **kw)
-        torch.cuda.synchronize()
-        ctx.costdists = dcd.cpu().numpy()
-        ctx.costblocks = dcb.cpu().numpy()
-        ids = dI.cpu().numpy().view(np.uint32)
-        return ids.T, dD.cpu().numpy().T
+        dQ = DeviceArray.from_numpy(Q.rows)
+        dI = DeviceArray((m, k), np.uint32)
+        dD = DeviceArray((m, k), np.float32)
+        dcd = DeviceArray((m,), np.int64)
+        dcb = DeviceArray((m,), np.int64)
+        s = Stream()
+        self.search_dev(ctx, dQ.ptr, m, k, dI.ptr, dD.ptr, stream=s.handle, dcd=dcd.ptr,
+                        dcb=dcb.ptr, **kw)
+        s.synchronize()
+        ctx.costdists = dcd.numpy()
+        ctx.costblocks = dcb.numpy()
+        return dI.numpy().T, dD.numpy().T
