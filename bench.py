@@ -1,19 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark: QPS at recall@10 >= 0.95 of SearchGraph beam search on one MI355X.
+"""Benchmark: QPS at recall@10 >= 0.95 of SearchGraph search on one MI355X, learned+BF16
+form against the plain FP32 beam search.
 
 Workload (synthetic, seeded): 1,000,000 x 128 Float32 database drawn from a
 100-component isotropic Gaussian mixture (centers ~ U[-1,1]^128, sigma 0.15, equal
-weights) and 10,000 queries from the same mixture; squared L2; k = 10.  The graph
-is built on the GPU by index! with the reference's default context knobs
-(parallel_block = 8 x host threads = 128), autotuned with optimize_index!
-(MinRecall(0.9)), then tuned again until recall@10 against the exhaustive ground
-truth of the 10,000 queries is >= 0.95.  A step is one searchbatch! over the
-10,000 device-resident queries.
+weights), 10,000 test queries from the same mixture, squared L2, k = 10.  The graph is
+built on the GPU by index! with the reference's default context knobs
+(parallel_block = 8 x host threads = 128) and autotuned with optimize_index!
+(MinRecall(0.9)); recall is always measured against the exhaustive ground truth.
 
-Prints one JSON line.  `roofline` is HBM-bound: algorithmic bytes of the timed
-launches over their HIP-event duration.  `cpu_baseline` is the oracle's restated
-beam search (OpenMP over queries, all host threads) on the same graph and
-parameters over the full query set.
+Plain form: BeamSearch parameters raised until recall@10 >= 0.95 on the test set.
+Learned form: 4,096 fft landmarks, an MLP trained on the GPU on traces of 500,000
+training queries, per-query entry points and beam sizes, BF16 search with an exact
+FP32 rerank; its knobs are tuned on 2,000 held-out queries.  One step is one batch of
+10,000 device-resident queries; `value` is the learned form's QPS.
+
+`roofline`: HBM-bound; algorithmic bytes of the timed BF16 beam-search launches over
+their HIP-event duration.  `cpu_baseline`: the oracle's restated plain beam search
+(OpenMP over queries on every host thread) on the same graph and parameters.
 """
 import argparse
 import ctypes as C
@@ -39,10 +43,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--queries", type=int, default=10_000)
+    ap.add_argument("--train", type=int, default=500_000)
     ap.add_argument("--parallel-block", type=int, default=128)
     ap.add_argument("--target-recall", type=float, default=0.95)
-    ap.add_argument("--params", type=str, default="",
-                    help="bsize,delta,maxvisits: skip the recall calibration")
+    ap.add_argument("--plain", type=str, default="", help="bsize,delta,maxvisits (skip tuning)")
+    ap.add_argument("--learned", type=str, default="", help="nhints,shift,rerank (skip tuning)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -51,119 +56,188 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     import similaritysearch as ss
     from similaritysearch import datasets, _lib
+    from similaritysearch.device import DeviceArray, Event, Stream
+    from similaritysearch.learned import LearnedSearch
 
-    dim, k = 128, 10
+    L = _lib.load()
+    dim, k, m = 128, 10, args.queries
     t0 = time.time()
     X = datasets.gmm(args.n, dim, seed=4)
-    Q = datasets.gmm(args.queries, dim, seed=5)
-    Qtune = datasets.gmm(512, dim, seed=6)
-    log(f"data {args.n}x{dim} + {args.queries} queries in {time.time() - t0:.1f}s")
+    Q = datasets.gmm(m, dim, seed=5)
+    Qtune = datasets.gmm(2000, dim, seed=6)
+    Qtrain = datasets.gmm(args.train, dim, seed=9)
+    log(f"data in {time.time() - t0:.1f}s")
 
     t0 = time.time()
     ctx = ss.SearchGraphContext()
     G = ss.SearchGraph.build(ss.SqL2, ss.MatrixDatabase.from_rows(X), ctx,
-                             parallel_block=args.parallel_block, seed=1, verbose=10)
+                             parallel_block=args.parallel_block, seed=1, verbose=25)
     build_s = time.time() - t0
     log(f"index! {build_s:.1f}s nnz={len(G.adj.end_point)} hints={len(G.hints)} {G.algo}")
+    h = G._handle()
 
-    E = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(X))
-    gold, _ = ss.searchbatch(E, ss.GenericContext(), ss.MatrixDatabase.from_rows(Q), k)
-    E.close()
+    def exact(Qr):
+        gi = np.zeros((len(Qr), k), np.uint32)
+        gd = np.zeros((len(Qr), k), np.float32)
+        _lib.check(L.ssg_exhaustive_knn(h, np.ascontiguousarray(Qr).ctypes.data, len(Qr), k,
+                                        gi.ctypes.data, gd.ctypes.data), "exhaustive")
+        return gi
+    gold = exact(Q)
+    gold_tune = exact(Qtune)
 
+    def recall(g, ids):
+        return ss.macrorecall(g.T, ids.T)
+
+    # ------------------------------------------------------------------ plain form
     Qdb = ss.MatrixDatabase.from_rows(Q)
 
-    def run(algo):
+    def plain_recall(algo):
         G.algo = algo
         ids, _ = ss.searchbatch(G, ctx, Qdb, k)
-        return ss.macrorecall(gold, ids)
+        return recall(gold, ids.T)
 
-    if args.params:
-        b, d, mv = args.params.split(",")
-        algo = ss.BeamSearch(int(b), float(d), int(mv))
-        recall = run(algo)
+    if args.plain:
+        b, d, mv = args.plain.split(",")
+        palgo = ss.BeamSearch(int(b), float(d), int(mv))
+        prec = plain_recall(palgo)
     else:
-        # optimize_index!(MinRecall(0.9)) as the reference tunes the graph, then raise the
-        # target until the held-out recall reaches the benchmark's level
-        algo, _ = G.optimize(ctx, 0.9, ss.MatrixDatabase.from_rows(Qtune))
-        recall = run(algo)
-        log(f"MinRecall(0.9): {algo} recall@10={recall:.4f}")
+        palgo, _ = G.optimize(ctx, 0.9, ss.MatrixDatabase.from_rows(Qtune))
+        prec = plain_recall(palgo)
+        log(f"MinRecall(0.9): {palgo} recall@10={prec:.4f}")
         target = args.target_recall
-        while recall < args.target_recall and target < 1.0:
-            algo, _ = G.optimize(ctx, min(target, 0.999), ss.MatrixDatabase.from_rows(Qtune))
-            recall = run(algo)
-            log(f"MinRecall({target:.3f}): {algo} recall@10={recall:.4f}")
+        while prec < args.target_recall and target < 1.0:
+            palgo, _ = G.optimize(ctx, min(target, 0.999), ss.MatrixDatabase.from_rows(Qtune))
+            prec = plain_recall(palgo)
+            log(f"MinRecall({target:.3f}): {palgo} recall@10={prec:.4f}")
             target += 0.01
-        if recall < args.target_recall:
-            bs = algo.bsize
-            while recall < args.target_recall and bs < 256:
-                bs *= 2
-                algo = ss.BeamSearch(bs, algo.delta, 10**6)
-                recall = run(algo)
-                log(f"widened: {algo} recall@10={recall:.4f}")
-    G.algo = algo
-    costd = ctx.costdists.copy()
-    costb = ctx.costblocks.copy()
+        while prec < args.target_recall and palgo.bsize < 256:
+            palgo = ss.BeamSearch(palgo.bsize * 2, palgo.delta, 10**6)
+            prec = plain_recall(palgo)
+            log(f"widened: {palgo} recall@10={prec:.4f}")
+    G.algo = palgo
+    pcostd = ctx.costdists.copy()
+    pcostb = ctx.costblocks.copy()
 
-    # device-resident timed loop
-    dev = torch.device("cuda:0")
-    m = args.queries
-    dQ = torch.from_numpy(Q).to(dev)
-    dI = torch.zeros((m, k), dtype=torch.int32, device=dev)
-    dD = torch.zeros((m, k), dtype=torch.float32, device=dev)
-    # a dedicated stream (default priority): the kernels and the timing events share it
-    stream = torch.cuda.Stream()
-    torch.cuda.synchronize()
-    L = _lib.load()
-    h = G._handle()
-    bp = G.beam_params(ctx)
+    stream = Stream()
+    sp = stream.handle
+    dQ = DeviceArray.from_numpy(Q)
+    dI = DeviceArray((m, k), np.uint32)
+    dD = DeviceArray((m, k), np.float32)
+    bp_plain = G.beam_params(ctx)
 
-    def step():
-        _lib.check(L.ssg_beamsearch_dev(h, dQ.data_ptr(), m, k, k, C.byref(bp), None,
-                                        dI.data_ptr(), dD.data_ptr(), None, None,
-                                        C.c_void_p(stream.cuda_stream)), "searchbatch!")
+    def plain_step():
+        _lib.check(L.ssg_beamsearch_dev(h, dQ.ptr, m, k, k, C.byref(bp_plain), None, dI.ptr,
+                                        dD.ptr, None, None, sp), "searchbatch!")
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for e0, e1 in evs:
-        e0.record(stream)
-        step()
-        e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
-    ids_dev = dI.cpu().numpy().view(np.uint32)
-    rec_dev = ss.macrorecall(gold, ids_dev.T)
+    def timed(step_fns, steps, warmup):
+        """Runs the step (a list of launch functions) `warmup` + `steps` times; returns wall
+        seconds of the timed steps and the mean event time of each function."""
+        for _ in range(warmup):
+            for f in step_fns:
+                f()
+        stream.synchronize()
+        evs = [[Event() for _ in range(len(step_fns) + 1)] for _ in range(steps)]
+        t = time.perf_counter()
+        for row in evs:
+            row[0].record(stream)
+            for i, f in enumerate(step_fns):
+                f()
+                row[i + 1].record(stream)
+        stream.synchronize()
+        wall = time.perf_counter() - t
+        per = [float(np.mean([row[i].elapsed_ms(row[i + 1]) for row in evs]))
+               for i in range(len(step_fns))]
+        return wall, per
 
-    qps = m * args.steps / wall
-    n = args.n
-    deg = len(G.adj.end_point) / n
-    # algorithmic bytes per query: E vectors (dim*4) + E visited bytes + B node entries
-    # (8 B) + B adjacency lists (4 B x mean degree) + the query (dim*4) + the result (8k)
-    bytes_q = (costd * (dim * 4 + 1) + costb * (8 + 4 * deg)).astype(np.float64) + dim * 4 + 8 * k
-    step_bytes = float(bytes_q.sum())
-    achieved = step_bytes / (kern_ms * 1e-3) / 1e9
+    pwall, (pms,) = timed([plain_step], args.steps, args.warmup)
+    plain_qps = m * args.steps / pwall
+    prec_dev = recall(gold, dI.numpy())
+    log(f"plain: {plain_qps:.0f} QPS, recall@10 {prec_dev:.4f}, kernel {pms:.3f} ms")
+
+    # ------------------------------------------------------------------ learned form
+    t0 = time.time()
+    LS = LearnedSearch(G, nlandmarks=4096, hidden=256)
+    LS.landmark_labels()
+    losses = LS.fit(Qtrain, ctx, epochs=8, batch=2048, verbose=log)
+    fit_s = time.time() - t0
+    log(f"learned model in {fit_s:.1f}s, losses {np.round(losses, 3).tolist()}")
+
+    def learned_recall(Qr, g, nh, shift, rr):
+        ids, _ = LS.search(ctx, ss.MatrixDatabase.from_rows(Qr), k, nhints=nh, shift=shift,
+                           rerank=rr)
+        return recall(g, ids.T), float(ctx.costdists.mean())
+
+    if args.learned:
+        nh, shift, rr = (int(x) for x in args.learned.split(","))
+    else:
+        best = None
+        for rr in (10, 20):
+            for shift in range(0, 4):
+                r, cost = learned_recall(Qtune, gold_tune, 1, shift, rr)
+                log(f"tune nh=1 shift={shift} rerank={rr}: recall {r:.4f} mean costdists {cost:.0f}")
+                if r >= args.target_recall + 0.003:
+                    if best is None or cost < best[0]:
+                        best = (cost, 1, shift, rr)
+                    break
+        nh, shift, rr = best[1:] if best else (1, 3, 20)
+    lrec, _ = learned_recall(Q, gold, nh, shift, rr)
+    while lrec < args.target_recall and shift < 7:
+        shift += 1
+        lrec, _ = learned_recall(Q, gold, nh, shift, rr)
+    lcostd = ctx.costdists.copy()
+    lcostb = ctx.costblocks.copy()
+    log(f"learned: nhints={nh} shift={shift} rerank={rr} recall@10={lrec:.4f}")
+
+    nb = len(LS.buckets)
+    table = np.array([LS.buckets[min(nb - 1, max(0, b + shift))] for b in range(nb)], np.int32)
+    dH = DeviceArray((m, nh), np.uint32)
+    dB = DeviceArray((m,), np.int32)
+    dC = DeviceArray((m,), np.float32)
+    dN = DeviceArray((m,), np.int32)
+    dS = DeviceArray((m,), np.int32)
+    bp_l = ss.BeamParams(G.algo.bsize, G.algo.delta, 10**6, max(ctx.beamcap, int(table.max())), 1, rr)
+    qo = ss.QueryOpts(None, dS.ptr, dH.ptr, dN.ptr, nh)
+
+    def infer():
+        _lib.check(L.ssg_mlp_infer_dev(LS.mlp._p, dQ.ptr, m, LS.dlandmarks.ptr, nh, dH.ptr,
+                                       dB.ptr, dC.ptr, sp), "infer")
+        _lib.check(L.ssg_learned_plan_dev(h, dB.ptr, dC.ptr, m, nh, 0.0, table.ctypes.data, nb,
+                                          G.algo.bsize, dN.ptr, dS.ptr, sp), "plan")
+
+    def search():
+        _lib.check(L.ssg_beamsearch_dev(h, dQ.ptr, m, k, k, C.byref(bp_l), C.byref(qo), dI.ptr,
+                                        dD.ptr, None, None, sp), "learned search")
+
+    lwall, (ims, sms) = timed([infer, search], args.steps, args.warmup)
+    learned_qps = m * args.steps / lwall
+    lrec_dev = recall(gold, dI.numpy())
+    log(f"learned: {learned_qps:.0f} QPS, recall@10 {lrec_dev:.4f}, infer {ims:.3f} ms, "
+        f"search {sms:.3f} ms")
+
+    deg = len(G.adj.end_point) / args.n
+    # algorithmic bytes per query: E BF16 vectors (dim*2) + E visited bytes + B node entries
+    # (8 B) + B adjacency lists (4 B x mean degree) + the exact rerank of the result list
+    # (max(k, rerank) FP32 vectors) + the query (dim*4) + the result (8k)
+    lbytes = float((lcostd * (dim * 2 + 1) + lcostb * (8 + 4 * deg)).sum()
+                   + m * (max(k, rr) * dim * 4 + dim * 4 + 8 * k))
+    achieved = lbytes / (sms * 1e-3) / 1e9
+    pbytes = float((pcostd * (dim * 4 + 1) + pcostb * (8 + 4 * deg)).sum() + m * (dim * 4 + 8 * k))
 
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
-        threads = os.cpu_count() or 1
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         offs = np.concatenate([[0], G.adj.offset]).astype(np.int64)
-        pyoracle.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, Q[:200], k, algo.bsize,
-                            algo.delta, algo.maxvisits, nthreads=threads)
+        pyoracle.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, Q[:200], k, palgo.bsize,
+                            palgo.delta, palgo.maxvisits, nthreads=threads)
         t0 = time.perf_counter()
         reps = 0
         while True:
             ci, _, _, _ = pyoracle.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, Q, k,
-                                              algo.bsize, algo.delta, algo.maxvisits,
+                                              palgo.bsize, palgo.delta, palgo.maxvisits,
                                               nthreads=threads)
             reps += 1
             if time.perf_counter() - t0 > 10.0 or reps >= 20:
@@ -171,37 +245,49 @@ def main():
         cpu_s = time.perf_counter() - t0
         cpu = {"value": round(m * reps / cpu_s, 1), "unit": "queries/s", "cores": threads,
                "kind": "port",
-               "sample": f"restated searchbatch over all {m} queries x{reps}, same graph and "
-                         f"BeamSearch, OpenMP over queries; recall@10 "
-                         f"{ss.macrorecall(gold, ci.T):.4f}"}
+               "sample": f"restated plain searchbatch over all {m} test queries x{reps}, same "
+                         f"graph and BeamSearch as the plain GPU form, OpenMP over queries; "
+                         f"recall@10 {recall(gold, ci):.4f}"}
 
     out = {
         "metric": "QPS at recall@10=0.95, 1M×128 L2, 1 GPU: learned+BF16 vs plain beam search",
-        "value": round(qps, 1),
+        "value": round(learned_qps, 1),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "ms_per_step": round(lwall * 1e3 / args.steps, 3),
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic: seeded 100-component Gaussian mixture, sigma 0.15",
-        "config": {"workload": "plain SearchGraph beam search, 1,000,000x128 GMM, 10,000 queries, "
-                               "SqL2, k=10",
-                   "form": "plain FP32 beam search",
-                   "n": n, "queries": m, "k": k, "recall@10": round(rec_dev, 4),
-                   "bsize": algo.bsize, "delta": algo.delta, "maxvisits": algo.maxvisits,
-                   "beamcap": ctx.beamcap, "parallel_block": args.parallel_block,
-                   "graph_nnz": int(len(G.adj.end_point)), "hints": int(len(G.hints)),
-                   "build_s": round(build_s, 1),
-                   "mean_costdists": round(float(costd.mean()), 1),
-                   "mean_costblocks": round(float(costb.mean()), 1)},
+        "dtype": "bf16",
+        "data": "synthetic: seeded 100-component Gaussian mixture, sigma 0.15; 500,000 "
+                "training queries for the learned form",
+        "config": {
+            "workload": "SearchGraph search, 1,000,000x128 GMM database, 10,000 queries, SqL2, "
+                        "k=10, recall@10 >= 0.95 against exhaustive ground truth",
+            "form": "learned entry points + BF16 beam search + FP32 rerank",
+            "recall@10": round(lrec_dev, 4),
+            "learned": {"nhints": nh, "bucket_shift": shift, "rerank": rr,
+                        "buckets": list(LS.buckets), "landmarks": LS.L,
+                        "mean_costdists": round(float(lcostd.mean()), 1),
+                        "mean_costblocks": round(float(lcostb.mean()), 1),
+                        "infer_ms": round(ims, 3), "search_ms": round(sms, 3),
+                        "train_queries": args.train, "fit_s": round(fit_s, 1)},
+            "plain": {"qps": round(plain_qps, 1), "recall@10": round(prec_dev, 4),
+                      "bsize": palgo.bsize, "delta": palgo.delta, "maxvisits": palgo.maxvisits,
+                      "kernel_ms": round(pms, 3),
+                      "achieved_GBs": round(pbytes / (pms * 1e-3) / 1e9, 1),
+                      "mean_costdists": round(float(pcostd.mean()), 1),
+                      "mean_costblocks": round(float(pcostb.mean()), 1)},
+            "learned_over_plain": round(learned_qps / plain_qps, 3),
+            "n": args.n, "queries": m, "k": k, "beamcap": ctx.beamcap,
+            "parallel_block": args.parallel_block, "graph_nnz": int(len(G.adj.end_point)),
+            "hints": int(len(G.hints)), "build_s": round(build_s, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "beam_kernel<8,1>", "kernel_ms": round(kern_ms, 3),
-                     "algorithmic_bytes_per_launch": int(step_bytes)},
+                     "kernel": "beam_bf16_kernel<8,4>", "kernel_ms": round(sms, 3),
+                     "algorithmic_bytes_per_launch": int(lbytes)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -13,9 +13,13 @@
 //   * res (k <= 64) and the beam (<= 256 slots) live in registers (RegQueue).
 //   * Neighbour lists are expanded 64 ids at a time: one coalesced id load, one
 //     visited-byte load per id, then the unvisited ids are compacted in LDS and
-//     their vectors gathered 32 at a time (4 per 16-lane group, 8 rounds), with
-//     the child degrees loaded in the same round trip.
-//   * The order-dependent part (pushes into res and beam, the maxvisits cut) is
+//     their vectors gathered in batches (one child per 16-lane group per round;
+//     8 rounds = 32 children for FP32 rows, 16 rounds = 64 for BF16 rows), with
+//     the child degrees loaded in the same round trip.  Every row index of a
+//     batch is clamped into the batch, so all its loads are unconditional and in
+//     flight together before the first one is consumed.
+//   * Child distances are transposed through LDS to one lane per child; the
+//     order-dependent part (pushes into res and beam, the maxvisits cut) is then
 //     replayed serially over the children in list order with readlane, skipping
 //     children that provably change neither queue.
 //   * Visited set: one epoch byte per (wave slot, object).  A query takes the
@@ -26,8 +30,6 @@
 
 namespace ssg {
 
-constexpr int ROUNDS = 8;           // gather rounds per batch -> 32 children
-constexpr int BATCH = 4 * ROUNDS;
 constexpr int WAVES = 4;            // waves per block
 
 enum { MODE_HINT = 0, MODE_EXPAND = 1, MODE_BRUTE = 2 };
@@ -62,6 +64,8 @@ __device__ __forceinline__ void vis_put(uint8_t* p, uint8_t v)
 // arithmetic during the search, then an exact (canonical) rerank of the result list.
 template <int V, int NB, int PREC>
 struct Searcher {
+    static constexpr int ROUNDS = PREC == 1 ? 16 : 8;  // gather rounds per batch
+    static constexpr int BATCH = 4 * ROUNDS;           // children per batch
     const BeamArgs& a;
     float qv[V];
     RegQueue<1, false> res;
@@ -72,59 +76,56 @@ struct Searcher {
     uint32_t epoch;
     uint8_t* vis;  // wave-private slot; every access is a workgroup-scope relaxed atomic
     uint32_t* cidbuf;  // 64 ids, wave-private LDS
+    float* dbuf;       // 64 distances, wave-private LDS
     int lane, g, j;
 
     __device__ __forceinline__ Searcher(const BeamArgs& args) : a(args) {}
 
-    // Evaluate cidbuf[ub .. ub+cnt) (cnt <= BATCH).  Child u = 4r + q is evaluated by
-    // row q in round r and its distance is dr[r] at lane 16q + 15; lane u receives its
-    // id, list start and degree.
-    __device__ __forceinline__ void eval_batch(int ub, int cnt, float (&dr)[ROUNDS], uint32_t& cid,
-                                               uint32_t& st, uint32_t& dg, bool need_deg)
+    // Evaluate cidbuf[ub .. ub+cnt) (1 <= cnt <= BATCH).  Child u = 4r + q is gathered
+    // by row q in round r; rows past the batch repeat its last child, so every load is
+    // unconditional.  Returns the distance of child `lane` (meaningful for lane < cnt);
+    // lane u also receives the child's id, list start and degree.
+    __device__ __forceinline__ float eval_batch(int ub, int cnt, uint32_t& cid, uint32_t& st,
+                                                uint32_t& dg, bool need_deg)
     {
         const int dp = 16 * V;
-        cid = lane < cnt ? cidbuf[ub + lane] : 0u;
-        uint2 nd = make_uint2(0u, 0u);
-        if (need_deg && lane < cnt) nd = a.node[cid];
-        float xv[ROUNDS][V];
+        cid = cidbuf[ub + (lane < cnt ? lane : cnt - 1)];
+        uint32_t cr[ROUNDS];
 #pragma unroll
         for (int r = 0; r < ROUNDS; r++) {
-            if (4 * r < cnt) {
-                const int u = 4 * r + g;
-                if (u < cnt) {
-                    const uint32_t c = cidbuf[ub + u];
-                    if constexpr (PREC == 1)
-                        load_run_bf16<V>(a.Xh + (size_t)c * dp + j * V, xv[r]);
-                    else
-                        load_run<V>(a.X + (size_t)c * dp + j * V, xv[r]);
-                } else {
-#pragma unroll
-                    for (int t = 0; t < V; t++) xv[r][t] = 0.f;
-                }
-            }
+            const int u = 4 * r + g;
+            cr[r] = cidbuf[ub + (u < cnt ? u : cnt - 1)];
         }
+        uint2 nd = make_uint2(0u, 0u);
+        if (need_deg) nd = a.node[cid];
+        RunWords<V, PREC> xw[ROUNDS];
 #pragma unroll
         for (int r = 0; r < ROUNDS; r++) {
-            dr[r] = __builtin_inff();
             if (4 * r < cnt) {
                 if constexpr (PREC == 1)
-                    dr[r] = finish_f32(a.metric, group16_tree_f32(run_partial_f32<V>(a.metric, xv[r], qv)));
+                    load_words<V>(a.Xh + (size_t)cr[r] * dp + j * V, xw[r]);
                 else
-                    dr[r] = finish_dist(a.metric, group16_tree(run_partial<V>(a.metric, xv[r], qv)));
+                    load_words<V>(a.X + (size_t)cr[r] * dp + j * V, xw[r]);
             }
         }
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++) {
+            if (4 * r < cnt) {
+                float xv[V];
+                unpack_words<V>(xw[r], xv);
+                float d;
+                if constexpr (PREC == 1)
+                    d = finish_f32(a.metric, group16_tree_f32(run_partial_f32<V>(a.metric, xv, qv)));
+                else
+                    d = finish_dist(a.metric, group16_tree(run_partial<V>(a.metric, xv, qv)));
+                if (j == 15) dbuf[4 * r + g] = d;
+            }
+        }
+        lds_order();
+        const float dl = dbuf[lane];
         st = nd.x;
         dg = nd.y;
-    }
-
-    // distance of child u (wave-uniform) of the last batch
-    __device__ __forceinline__ float child_dist(const float (&dr)[ROUNDS], int u) const
-    {
-        float d = 0.f;
-#pragma unroll
-        for (int r = 0; r < ROUNDS; r++)
-            if ((u >> 2) == r) d = rl_f(dr[r], 16 * (u & 3) + 15);
-        return d;
+        return dl;
     }
 
     // One chunk of up to 64 candidate ids (lane-indexed, `valid` lanes).  Returns
@@ -154,9 +155,8 @@ struct Searcher {
         bool stopped = false;
         for (int ub = 0; ub < U && !stopped; ub += BATCH) {
             const int cnt = U - ub < BATCH ? U - ub : BATCH;
-            float dr[ROUNDS];
             uint32_t c, st, dg;
-            eval_batch(ub, cnt, dr, c, st, dg, mode == MODE_EXPAND);
+            const float dl = eval_batch(ub, cnt, c, st, dg, mode == MODE_EXPAND);
             int lim = cnt, stop = -1;
             if (mode == MODE_EXPAND && cd_inner + cnt > a.maxvisits) {
                 stop = (int)(a.maxvisits - cd_inner);
@@ -167,22 +167,13 @@ struct Searcher {
             const float th0 = a.delta * mx0;
             // candidates: children that may enter res or the beam (maximum(res) only
             // decreases while res is full, so a child failing now fails later too)
-            uint64_t cm = 0;
-#pragma unroll
-            for (int r = 0; r < ROUNDS; r++) {
-                if (4 * r < cnt) {
-                    const float d = dr[r];
-                    bool cand = j == 15 && 4 * r + g < lim;
-                    if (full) cand = cand && (d < mx0 || (mode == MODE_EXPAND && d <= th0));
-                    const uint64_t b = __ballot(cand);
-                    cm |= (((b >> 15) & 1ull) | (((b >> 31) & 1ull) << 1) | (((b >> 47) & 1ull) << 2) |
-                           (((b >> 63) & 1ull) << 3)) << (4 * r);
-                }
-            }
+            bool cand = lane < lim;
+            if (full) cand = cand && (dl < mx0 || (mode == MODE_EXPAND && dl <= th0));
+            uint64_t cm = __ballot(cand);
             while (cm) {
                 const int u = __ffsll((unsigned long long)cm) - 1;
                 cm &= cm - 1;
-                const float du = child_dist(dr, u);
+                const float du = rl_f(dl, u);
                 const uint32_t cu = rl_u(c, u);
                 res.push(du, cu, 0, 0);
                 if (mode != MODE_EXPAND) continue;
@@ -201,6 +192,7 @@ struct Searcher {
             } else {
                 cd_init += cnt;
             }
+            lds_order();
         }
         lds_order();
         return stopped;
@@ -260,21 +252,19 @@ struct Searcher {
         const int len = res.len;
         float dex = __builtin_inff();
         for (int r0 = 0; r0 < len; r0 += 16) {
-            float xv[4][V];
+            // rows past the list repeat its last entry: unconditional loads, in flight together
+            RunWords<V, 0> xw[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int u = r0 + 4 * r + g;
-                const uint32_t c = __shfl(res.id[0], u < len ? u : 0);
-                if (u < len) {
-                    load_run<V>(a.X + (size_t)c * dp + j * V, xv[r]);
-                } else {
-#pragma unroll
-                    for (int t = 0; t < V; t++) xv[r][t] = 0.f;
-                }
+                const uint32_t c = __shfl(res.id[0], u < len ? u : len - 1);
+                load_words<V>(a.X + (size_t)c * dp + j * V, xw[r]);
             }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const float d = finish_dist(a.metric, group16_sum(run_partial<V>(a.metric, xv[r], qv)));
+                float xv[V];
+                unpack_words<V>(xw[r], xv);
+                const float d = finish_dist(a.metric, group16_sum(run_partial<V>(a.metric, xv, qv)));
                 const float dt = __shfl(d, (lane & 3) * 16);
                 if (lane >= r0 && lane < r0 + 16 && ((lane - r0) >> 2) == r && lane < len) dex = dt;
             }
@@ -300,6 +290,7 @@ template <int V, int NB, int PREC>
 __device__ __forceinline__ void beam_body(const BeamArgs& a)
 {
     __shared__ uint32_t s_cid[WAVES][64];
+    __shared__ float s_dist[WAVES][64];
     const int w = threadIdx.x >> 6;
     const int64_t slot = (int64_t)blockIdx.x * WAVES + w;
     Searcher<V, NB, PREC> S(a);
@@ -307,6 +298,7 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
     S.g = S.lane >> 4;
     S.j = S.lane & 15;
     S.cidbuf = s_cid[w];
+    S.dbuf = s_dist[w];
     S.vis = a.vis + slot * a.vis_stride;
     uint32_t epoch = rfl_u(a.epochs[slot]);
     const int lane = S.lane;

@@ -93,6 +93,67 @@ __device__ __forceinline__ void load_run_bf16(const uint16_t* __restrict__ p, fl
     }
 }
 
+// Raw words of one lane's run, loaded without being consumed so that many rows can
+// be in flight at once: FP32 runs as V words; BF16 runs as V/2 packed words (even V)
+// or V zero-extended halves (odd V).
+template <int V, int PREC>
+struct RunWords {
+    static constexpr int N = PREC == 1 ? (V % 2 == 0 ? V / 2 : V) : V;
+    uint32_t w[N];
+};
+
+template <int V>
+__device__ __forceinline__ void load_words(const float* __restrict__ p, RunWords<V, 0>& r)
+{
+    float x[V];
+    load_run<V>(p, x);
+#pragma unroll
+    for (int t = 0; t < V; t++) r.w[t] = __float_as_uint(x[t]);
+}
+
+template <int V>
+__device__ __forceinline__ void load_words(const uint16_t* __restrict__ p, RunWords<V, 1>& r)
+{
+    if constexpr (V % 8 == 0) {
+#pragma unroll
+        for (int t = 0; t < V / 8; t++) {
+            const uint4 w = reinterpret_cast<const uint4*>(p)[t];
+            r.w[4 * t + 0] = w.x;
+            r.w[4 * t + 1] = w.y;
+            r.w[4 * t + 2] = w.z;
+            r.w[4 * t + 3] = w.w;
+        }
+    } else if constexpr (V % 2 == 0) {
+#pragma unroll
+        for (int t = 0; t < V / 2; t++) r.w[t] = reinterpret_cast<const uint32_t*>(p)[t];
+    } else {
+#pragma unroll
+        for (int t = 0; t < V; t++) r.w[t] = p[t];
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void unpack_words(const RunWords<V, 0>& r, float (&x)[V])
+{
+#pragma unroll
+    for (int t = 0; t < V; t++) x[t] = __uint_as_float(r.w[t]);
+}
+
+template <int V>
+__device__ __forceinline__ void unpack_words(const RunWords<V, 1>& r, float (&x)[V])
+{
+    if constexpr (V % 2 == 0) {
+#pragma unroll
+        for (int t = 0; t < V / 2; t++) {
+            x[2 * t] = __uint_as_float(r.w[t] << 16);
+            x[2 * t + 1] = __uint_as_float(r.w[t] & 0xffff0000u);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < V; t++) x[t] = __uint_as_float(r.w[t] << 16);
+    }
+}
+
 // FP32 partial sum (reduced-precision form; its result is never final).
 template <int V>
 __device__ __forceinline__ float run_partial_f32(int metric, const float (&x)[V], const float (&q)[V])
@@ -267,10 +328,12 @@ struct RegQueue {
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             const int e = 64 * b + lane;
-            pos += __popcll(__ballot(e < len && d[b] <= x));
+            if (64 * b < len) pos += __popcll(__ballot(e < len && d[b] <= x));
         }
+        // registers past the last occupied slot hold nothing to move
 #pragma unroll
         for (int b = NB - 1; b >= 0; b--) {
+            if (64 * b > len) continue;
             const int e = 64 * b + lane;
             float ud = dpp_f32<DPP_WAVE_SHR1>(d[b]);
             uint32_t ui = dpp_u32<DPP_WAVE_SHR1>(id[b]);
@@ -329,6 +392,7 @@ struct RegQueue {
         }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
+            if (64 * b >= len) continue;
             float nd = dpp_f32<DPP_WAVE_SHL1>(d[b]);
             uint32_t ni = dpp_u32<DPP_WAVE_SHL1>(id[b]);
             uint32_t na = 0, nb = 0;
