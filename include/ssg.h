@@ -97,6 +97,8 @@ typedef struct {
                           counterpart): BF16 vector gathers with FP32 arithmetic during the
                           search, then rerank! of the result list with the canonical distance */
     int32_t rerank;    /* SSG_BF16: result-list length reranked (max(k, rerank) <= 64) */
+    int32_t expand;    /* SSG_BF16 only (no reference counterpart): beam entries popped and
+                          expanded together per step, 0..4; 0 and 1 are the reference step */
 } ssg_beam_params;
 
 /* Context knobs of index! (src/searchgraph/insertions.jl:94-113) with the defaults of
@@ -150,6 +152,9 @@ typedef struct {
     const uint32_t* hints;   /* [m * hint_stride], 1-based */
     const int32_t* nhints;   /* [m] */
     int32_t hint_stride;
+    const uint32_t* order;   /* [m] nullable, no reference counterpart: the order in which
+                                queries are taken (a permutation of 0..m-1; results do not
+                                depend on it) */
 } ssg_query_opts;
 
 /* searchbatch!(::SearchGraph, ctx, Q, ids, dists) with search(::BeamSearch, ...)
@@ -164,6 +169,20 @@ int ssg_beamsearch(ssg_handle* h, const float* Q, int64_t m, int32_t k, int32_t 
 int ssg_beamsearch_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, int32_t kstride,
                        const ssg_beam_params* bp, const ssg_query_opts* qo, uint32_t* dids,
                        float* ddists, int64_t* dcostdists, int64_t* dcostblocks, void* stream);
+
+/* Handle options (no reference counterpart).
+ * SSG_OPT_QUERY_LAYOUT: 1 (default) lets searchbatch! run on a copy of the graph and the
+ * vectors renumbered breadth-first from the hints, each list entry carrying its target's
+ * {start, degree}; results, ids and cost counters are unchanged, memory traffic is
+ * lower.  0 searches the installed layout directly. */
+enum { SSG_OPT_QUERY_LAYOUT = 1 };
+int ssg_set_option(ssg_handle* h, int32_t option, int64_t value);
+
+/* Longest-first order for a heterogeneous batch (no reference counterpart): dorder[0..m)
+ * receives the query indices grouped by floor(log2(bsize[j])), largest first, for
+ * ssg_query_opts.order.  dbsize and dorder are device arrays. */
+int ssg_order_by_bsize_dev(ssg_handle* h, const int32_t* dbsize, int64_t m, uint32_t* dorder,
+                           void* stream);
 
 /* allknn!(::SearchGraph, ctx, ids, dists) (src/allknn.jl:66-106): every database object
  * is a query; the neighbours of object i seed one search each, sharing one visited set. */

@@ -58,6 +58,76 @@ static int ensure_vis(ssg_handle* h, hipStream_t s)
     return SSG_OK;
 }
 
+// Breadth-first renumbering of the installed graph from its hints (then any object not
+// reached, in id order), the renumbered CSR with per-entry target {start, degree}, and
+// the vectors in the new order (FP32, plus BF16 when asked).
+static int ensure_query_layout(ssg_handle* h, hipStream_t s, bool bf16)
+{
+    if (!h->q_ready) {
+        HIPCHK(hipDeviceSynchronize(), "sync");  // no launch may still read the old layout
+        const int64_t n = h->n;
+        const auto& lists = h->lists;
+        std::vector<uint32_t> order;
+        order.reserve((size_t)n);
+        std::vector<uint32_t> ipos((size_t)n, 0xffffffffu);
+        auto bfs = [&](uint32_t seed) {
+            if (ipos[seed] != 0xffffffffu) return;
+            size_t head = order.size();
+            ipos[seed] = (uint32_t)order.size();
+            order.push_back(seed);
+            while (head < order.size()) {
+                const uint32_t u = order[head++];
+                for (uint32_t v : lists[u])
+                    if (ipos[v] == 0xffffffffu) {
+                        ipos[v] = (uint32_t)order.size();
+                        order.push_back(v);
+                    }
+            }
+        };
+        for (uint32_t hh : h->hints_host) bfs(hh - 1u);
+        for (int64_t i = 0; i < n; i++) bfs((uint32_t)i);
+        std::vector<uint2> node((size_t)n);
+        std::vector<uint32_t> adj((size_t)std::max<int64_t>(h->nnz, 1));
+        std::vector<uint2> adjn((size_t)std::max<int64_t>(h->nnz, 1));
+        int64_t pos = 0;
+        for (int64_t i = 0; i < n; i++) {
+            const auto& L = lists[order[i]];
+            node[i] = make_uint2((uint32_t)pos, (uint32_t)L.size());
+            for (uint32_t v : L) adj[(size_t)pos++] = ipos[v];
+        }
+        if (pos != h->nnz) return ssg_fail(SSG_E_STATE, "query layout: lists and nnz disagree");
+        for (int64_t e = 0; e < pos; e++) adjn[(size_t)e] = node[adj[(size_t)e]];
+        std::vector<uint32_t> hints(h->hints_host.size());
+        for (size_t i = 0; i < hints.size(); i++) hints[i] = ipos[h->hints_host[i] - 1u] + 1u;
+        HIPCHK(h->qnode.ensure(node.size() * sizeof(uint2)), "alloc");
+        HIPCHK(h->qadj.ensure(adj.size() * sizeof(uint32_t)), "alloc");
+        HIPCHK(h->qadjn.ensure(adjn.size() * sizeof(uint2)), "alloc");
+        HIPCHK(h->qperm.ensure((size_t)n * sizeof(uint32_t)), "alloc");
+        HIPCHK(h->qipos.ensure((size_t)n * sizeof(uint32_t)), "alloc");
+        HIPCHK(h->qhints.ensure(std::max<size_t>(hints.size(), 1) * sizeof(uint32_t)), "alloc");
+        HIPCHK(hipMemcpy(h->qnode.p, node.data(), node.size() * sizeof(uint2), hipMemcpyHostToDevice), "copy");
+        HIPCHK(hipMemcpy(h->qadj.p, adj.data(), adj.size() * sizeof(uint32_t), hipMemcpyHostToDevice), "copy");
+        HIPCHK(hipMemcpy(h->qadjn.p, adjn.data(), adjn.size() * sizeof(uint2), hipMemcpyHostToDevice), "copy");
+        HIPCHK(hipMemcpy(h->qperm.p, order.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice), "copy");
+        HIPCHK(hipMemcpy(h->qipos.p, ipos.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice), "copy");
+        if (!hints.empty())
+            HIPCHK(hipMemcpy(h->qhints.p, hints.data(), hints.size() * sizeof(uint32_t), hipMemcpyHostToDevice), "copy");
+        const int64_t np = std::max<int64_t>(h->n_pad, 64);
+        HIPCHK(h->Xq.ensure((size_t)np * h->dp * sizeof(float)), "alloc permuted rows");
+        HIPCHK(launch_permute_rows(h->X.as<float>(), h->qperm.as<uint32_t>(), n, np, h->dp, h->Xq.as<float>(), s),
+               "permute rows");
+        h->q_ready = true;
+        h->q_bf16 = false;
+    }
+    if (bf16 && !h->q_bf16) {
+        const int64_t np = std::max<int64_t>(h->n_pad, 64);
+        HIPCHK(h->Xqh.ensure((size_t)np * h->dp * 2), "alloc BF16 rows");
+        HIPCHK(launch_to_bf16(h->Xq.as<float>(), np * h->dp, h->Xqh.as<uint16_t>(), s), "BF16 rows");
+        h->q_bf16 = true;
+    }
+    return SSG_OK;
+}
+
 int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_t k,
                  int32_t kstride, const ssg_beam_params* bp, const ssg_query_opts* qo,
                  uint32_t* dids, float* ddists, int64_t* dcd, int64_t* dcb, hipStream_t s,
@@ -78,12 +148,16 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
         return ssg_fail(SSG_E_ARG, "precision must be SSG_F32 or SSG_BF16");
     if (bp->precision == SSG_BF16 && (allknn || internal))
         return ssg_fail(SSG_E_ARG, "the BF16 form serves searchbatch! only");
+    if (bp->expand < 0 || bp->expand > 4) return ssg_fail(SSG_E_ARG, "expand must be in 0..4");
+    if (bp->expand > 1 && bp->precision != SSG_BF16)
+        return ssg_fail(SSG_E_ARG, "wide steps (expand > 1) belong to the BF16 form");
     const int32_t krr = std::max(k, bp->rerank);
     if (bp->precision == SSG_BF16 && (krr > 64 || kstride > 64))
         return ssg_fail(SSG_E_LIMIT, "BF16 rerank list is limited to 64");
     int rc = ensure_vis(h, s);
     if (rc) return rc;
-    if (bp->precision == SSG_BF16 && !h->Xh.p) {
+    const bool use_q = h->reorder && !allknn && !internal && n_eff == h->n && h->n >= 64;
+    if (bp->precision == SSG_BF16 && !use_q && !h->Xh.p) {
         HIPCHK(h->Xh.ensure((size_t)std::max<int64_t>(h->n_pad, 64) * h->dp * 2), "alloc BF16 copy");
         HIPCHK(launch_to_bf16(h->X.as<float>(), std::max<int64_t>(h->n_pad, 64) * h->dp,
                               h->Xh.as<uint16_t>(), s), "BF16 copy");
@@ -126,6 +200,20 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     a.prec = bp->precision == SSG_BF16 ? 1 : 0;
     a.Xh = h->Xh.as<uint16_t>();
     a.krr = krr;
+    a.expand = bp->expand > 1 ? bp->expand : 1;
+    a.order = qo ? qo->order : nullptr;
+    if (use_q) {
+        rc = ensure_query_layout(h, s, bp->precision == SSG_BF16);
+        if (rc) return rc;
+        a.X = h->Xq.as<float>();
+        a.Xh = h->Xqh.as<uint16_t>();
+        a.node = h->qnode.as<uint2>();
+        a.adj = h->qadj.as<uint32_t>();
+        a.adjn = h->qadjn.as<uint2>();
+        a.hints = h->qhints.as<uint32_t>();
+        a.perm = h->qperm.as<uint32_t>();
+        a.ipos = h->qipos.as<uint32_t>();
+    }
     const int slots = (int)std::min<int64_t>(h->nslots, (m + 3) / 4 * 4);
     HIPCHK(launch_zero_u32(a.counter, 1, s), "zero counter");
     HIPCHK(launch_beam(a, slots, s), "beam kernel launch");
@@ -237,6 +325,7 @@ int ssg_install_lists(ssg_handle* h)
     h->nhints = (int64_t)h->hints_host.size();
     h->has_dups = dups;
     h->has_graph = true;
+    ssg_query_layout_stale(h);
     return SSG_OK;
 }
 
@@ -397,6 +486,7 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     h->lists.clear();
     h->hints_host.clear();
     h->Xh.release();
+    ssg_query_layout_stale(h);
     const int64_t np = std::max<int64_t>(h->n_pad, 64);
     HIPCHK(h->X.ensure((size_t)np * dp * sizeof(float)), "alloc X");
     HIPCHK(h->xnorm.ensure((size_t)np * sizeof(float)), "alloc xnorm");
@@ -556,6 +646,32 @@ int ssg_beamsearch_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, int
                         pick(h, stream), false, h->n);
 }
 
+int ssg_set_option(ssg_handle* h, int32_t option, int64_t value)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (option == SSG_OPT_QUERY_LAYOUT) {
+        if (value != 0 && value != 1) return ssg_fail(SSG_E_ARG, "query layout option is 0 or 1");
+        HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+        HIPCHK(hipDeviceSynchronize(), "sync");
+        h->reorder = value == 1;
+        ssg_query_layout_stale(h);
+        return SSG_OK;
+    }
+    return ssg_fail(SSG_E_ARG, "unknown option");
+}
+
+int ssg_order_by_bsize_dev(ssg_handle* h, const int32_t* dbsize, int64_t m, uint32_t* dorder, void* stream)
+{
+    if (!h || !dbsize || !dorder) return ssg_fail(SSG_E_ARG, "null argument");
+    if (m < 1) return ssg_fail(SSG_E_ARG, "empty set of queries");
+    std::lock_guard<std::mutex> lk(h->mu);
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    HIPCHK(h->lCnt.ensure(64 * 4), "alloc");
+    HIPCHK(launch_order_desc(dbsize, m, dorder, h->lCnt.as<unsigned int>(), pick(h, stream)), "order");
+    return SSG_OK;
+}
+
 int ssg_beamsearch(ssg_handle* h, const float* Q, int64_t m, int32_t k, int32_t kstride,
                    const ssg_beam_params* bp, const ssg_query_opts* qo, uint32_t* ids,
                    float* dists, int64_t* costdists, int64_t* costblocks)
@@ -602,6 +718,17 @@ int ssg_beamsearch(ssg_handle* h, const float* Q, int64_t m, int32_t k, int32_t 
             dq.hints = h->sH.as<uint32_t>();
             dq.nhints = h->sHc.as<int32_t>();
             dq.hint_stride = qo->hint_stride;
+        }
+        if (qo->order) {
+            std::vector<uint8_t> seen((size_t)m, 0);
+            for (int64_t j = 0; j < m; j++) {
+                const uint32_t v = qo->order[j];
+                if ((int64_t)v >= m || seen[v]) return ssg_fail(SSG_E_ARG, "order must be a permutation of 0..m-1");
+                seen[v] = 1;
+            }
+            HIPCHK(h->sO.ensure((size_t)m * 4), "alloc");
+            HIPCHK(hipMemcpyAsync(h->sO.p, qo->order, (size_t)m * 4, hipMemcpyHostToDevice, s), "copy order");
+            dq.order = h->sO.as<uint32_t>();
         }
     }
     HIPCHK(h->sQ.ensure((size_t)m * h->dim * sizeof(float)), "alloc");

@@ -1,390 +1,13 @@
-// ssg_beam.hip -- wavefront-per-query graph beam search for gfx950.
-//
-// Restates search(bs::BeamSearch, ...) of src/searchgraph/beamsearch.jl:88-110:
-//   n == 0 -> nothing; n < 64 -> brute force over 1..n (no visited marks);
-//   otherwise beamsearch_init (:26-34, every hint through enqueue_item! :12-18,
-//   then ids 1:ceil(log2(1+n)) if the result is still empty) and
-//   beamsearch_inner_beam (:44-73): a KnnSorted beam of min(bsize, beamcap)
-//   slots seeded with nearest(res); pop_min, expand the CSR list, test-and-set
-//   visited, evaluate, push into res, stop once costdists > maxvisits, and admit
-//   the child into the beam when deg(child) > 1 and d <= Δ * maximum(res).
-//
-// GPU shape: one wave64 per query, persistent over an atomic work counter.
-//   * res (k <= 64) and the beam (<= 256 slots) live in registers (RegQueue).
-//   * Neighbour lists are expanded 64 ids at a time: one coalesced id load, one
-//     visited-byte load per id, then the unvisited ids are compacted in LDS and
-//     their vectors gathered in batches (one child per 16-lane group per round;
-//     8 rounds = 32 children for FP32 rows, 16 rounds = 64 for BF16 rows), with
-//     the child degrees loaded in the same round trip.  Every row index of a
-//     batch is clamped into the batch, so all its loads are unconditional and in
-//     flight together before the first one is consumed.
-//   * Child distances are transposed through LDS to one lane per child; the
-//     order-dependent part (pushes into res and beam, the maxvisits cut) is then
-//     replayed serially over the children in list order with readlane, skipping
-//     children that provably change neither queue.
-//   * Visited set: one epoch byte per (wave slot, object).  A query takes the
-//     next epoch of its slot; the slot is cleared once every 255 queries.  A
-//     byte equal to the current epoch means visited, so no per-query reset.
-#include "ssg_device.h"
+// ssg_beam.hip -- small device utilities of the beam-search path and the launch
+// dispatch over the per-V instantiations (ssg_beam_v*.hip).
+#include <algorithm>
+
 #include "ssg_kernels.h"
 
 namespace ssg {
 
-constexpr int WAVES = 4;            // waves per block
-
-enum { MODE_HINT = 0, MODE_EXPAND = 1, MODE_BRUTE = 2 };
-
-__device__ __forceinline__ void lds_order()
-{
-    __builtin_amdgcn_wave_barrier();
-    asm volatile("" ::: "memory");
-}
-
-// Wait for every outstanding vector memory operation of this wave (gfx9 vmcnt
-// covers stores), so visited marks are visible to the next visited loads.
-__device__ __forceinline__ void vm_drain()
-{
-    __builtin_amdgcn_s_waitcnt(0x0F70);
-    asm volatile("" ::: "memory");
-}
-
-// Visited bytes: all reads, marks, unmarks and the slot clear use the same
-// (workgroup-scope, relaxed) atomic access, so they share one cache policy and a
-// clear is never hidden behind a copy held at another cache level.
-__device__ __forceinline__ uint8_t vis_get(const uint8_t* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void vis_put(uint8_t* p, uint8_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// PREC 0: canonical distance on the FP32 vectors.  PREC 1: BF16 vectors with FP32
-// arithmetic during the search, then an exact (canonical) rerank of the result list.
-template <int V, int NB, int PREC>
-struct Searcher {
-    static constexpr int ROUNDS = PREC == 1 ? 16 : 8;  // gather rounds per batch
-    static constexpr int BATCH = 4 * ROUNDS;           // children per batch
-    const BeamArgs& a;
-    float qv[V];
-    RegQueue<1, false> res;
-    RegQueue<NB, true> beam;
-    int64_t cd_init;   // distance evaluations charged by enqueue_item!
-    int64_t cd_inner;  // costdists of beamsearch_inner_beam
-    int64_t cb;        // costblocks
-    uint32_t epoch;
-    uint8_t* vis;  // wave-private slot; every access is a workgroup-scope relaxed atomic
-    uint32_t* cidbuf;  // 64 ids, wave-private LDS
-    float* dbuf;       // 64 distances, wave-private LDS
-    int lane, g, j;
-
-    __device__ __forceinline__ Searcher(const BeamArgs& args) : a(args) {}
-
-    // Evaluate cidbuf[ub .. ub+cnt) (1 <= cnt <= BATCH).  Child u = 4r + q is gathered
-    // by row q in round r; rows past the batch repeat its last child, so every load is
-    // unconditional.  Returns the distance of child `lane` (meaningful for lane < cnt);
-    // lane u also receives the child's id, list start and degree.
-    __device__ __forceinline__ float eval_batch(int ub, int cnt, uint32_t& cid, uint32_t& st,
-                                                uint32_t& dg, bool need_deg)
-    {
-        const int dp = 16 * V;
-        cid = cidbuf[ub + (lane < cnt ? lane : cnt - 1)];
-        uint32_t cr[ROUNDS];
-#pragma unroll
-        for (int r = 0; r < ROUNDS; r++) {
-            const int u = 4 * r + g;
-            cr[r] = cidbuf[ub + (u < cnt ? u : cnt - 1)];
-        }
-        uint2 nd = make_uint2(0u, 0u);
-        if (need_deg) nd = a.node[cid];
-        RunWords<V, PREC> xw[ROUNDS];
-#pragma unroll
-        for (int r = 0; r < ROUNDS; r++) {
-            if (4 * r < cnt) {
-                if constexpr (PREC == 1)
-                    load_words<V>(a.Xh + (size_t)cr[r] * dp + j * V, xw[r]);
-                else
-                    load_words<V>(a.X + (size_t)cr[r] * dp + j * V, xw[r]);
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < ROUNDS; r++) {
-            if (4 * r < cnt) {
-                float xv[V];
-                unpack_words<V>(xw[r], xv);
-                float d;
-                if constexpr (PREC == 1)
-                    d = finish_f32(a.metric, group16_tree_f32(run_partial_f32<V>(a.metric, xv, qv)));
-                else
-                    d = finish_dist(a.metric, group16_tree(run_partial<V>(a.metric, xv, qv)));
-                if (j == 15) dbuf[4 * r + g] = d;
-            }
-        }
-        lds_order();
-        const float dl = dbuf[lane];
-        st = nd.x;
-        dg = nd.y;
-        return dl;
-    }
-
-    // One chunk of up to 64 candidate ids (lane-indexed, `valid` lanes).  Returns
-    // true when the maxvisits cut ends the search.
-    __device__ __forceinline__ bool chunk(int mode, uint32_t cid, bool valid, bool dedupe)
-    {
-        valid = valid && (int64_t)cid < a.n;
-        bool unv = valid;
-        if (mode != MODE_BRUTE) {
-            vm_drain();
-            if (valid) unv = vis_get(vis + cid) != (uint8_t)epoch;
-            if (dedupe) {
-                const uint64_t vm = __ballot(valid);
-                const int cnt = 64 - __clzll(vm);
-                for (int t = 0; t < cnt; t++) {
-                    const uint32_t c = rl_u(cid, t);
-                    if (lane > t && cid == c) unv = false;
-                }
-            }
-            if (unv) vis_put(vis + cid, (uint8_t)epoch);
-        }
-        const uint64_t mask = __ballot(unv);
-        const int U = __popcll(mask);
-        const int pos = __popcll(mask & ((1ull << lane) - 1ull));
-        if (unv) cidbuf[pos] = cid;
-        lds_order();
-        bool stopped = false;
-        for (int ub = 0; ub < U && !stopped; ub += BATCH) {
-            const int cnt = U - ub < BATCH ? U - ub : BATCH;
-            uint32_t c, st, dg;
-            const float dl = eval_batch(ub, cnt, c, st, dg, mode == MODE_EXPAND);
-            int lim = cnt, stop = -1;
-            if (mode == MODE_EXPAND && cd_inner + cnt > a.maxvisits) {
-                stop = (int)(a.maxvisits - cd_inner);
-                lim = stop + 1;
-            }
-            const bool full = res.len == res.cap;
-            const float mx0 = full ? res.maximum() : __builtin_inff();
-            const float th0 = a.delta * mx0;
-            // candidates: children that may enter res or the beam (maximum(res) only
-            // decreases while res is full, so a child failing now fails later too)
-            bool cand = lane < lim;
-            if (full) cand = cand && (dl < mx0 || (mode == MODE_EXPAND && dl <= th0));
-            uint64_t cm = __ballot(cand);
-            while (cm) {
-                const int u = __ffsll((unsigned long long)cm) - 1;
-                cm &= cm - 1;
-                const float du = rl_f(dl, u);
-                const uint32_t cu = rl_u(c, u);
-                res.push(du, cu, 0, 0);
-                if (mode != MODE_EXPAND) continue;
-                if (u == stop) break;
-                const uint32_t gu = rl_u(dg, u);
-                const float thr = a.delta * res.maximum();
-                if (gu > 1 && du <= thr) beam.push(du, cu, rl_u(st, u), gu);
-            }
-            if (mode == MODE_EXPAND) {
-                cd_inner += lim;
-                if (stop >= 0) {
-                    // children after the cut were marked ahead of time: unmark them
-                    if (lane > ub + stop && lane < U) vis_put(vis + cidbuf[lane], 0);
-                    stopped = true;
-                }
-            } else {
-                cd_init += cnt;
-            }
-            lds_order();
-        }
-        lds_order();
-        return stopped;
-    }
-
-    __device__ __forceinline__ void hint_list(const uint32_t* h, int nh, bool one_based)
-    {
-        for (int hb = 0; hb < nh; hb += 64) {
-            const bool valid = hb + lane < nh;
-            uint32_t c = valid ? h[hb + lane] : 1u;
-            if (one_based) c -= 1u;
-            chunk(MODE_HINT, c, valid, true);
-        }
-    }
-
-    // search(bs::BeamSearch, ...) for the current query with the given hints.
-    __device__ __forceinline__ void search(const uint32_t* h, int nh, bool one_based, int bsize)
-    {
-        const int64_t n = a.n;
-        if (n == 0) return;
-        if (n < 64) {
-            chunk(MODE_BRUTE, (uint32_t)lane, lane < n, false);
-            return;
-        }
-        hint_list(h, nh, one_based);
-        if (res.len == 0) chunk(MODE_HINT, (uint32_t)lane, lane < a.log2n, false);
-        const int cap = bsize < a.beamcap ? bsize : a.beamcap;
-        beam.init(cap);
-        {
-            if (res.len == 0) return;
-            const uint32_t nid = res.id_at(0);
-            const float ndist = res.dist_at(0);
-            const uint2 ni = a.node[nid];
-            beam.push(ndist, nid, ni.x, ni.y);
-        }
-        while (beam.len > 0) {
-            float pd;
-            uint32_t pid, pst, pdeg;
-            beam.pop_min(pd, pid, pst, pdeg);
-            cb++;
-            bool stop = false;
-            for (uint32_t base = 0; base < pdeg && !stop; base += 64) {
-                const uint32_t rem = pdeg - base;
-                const bool valid = (uint32_t)lane < rem;
-                const uint32_t c = valid ? a.adj[(size_t)pst + base + lane] : 0u;
-                stop = chunk(MODE_EXPAND, c, valid, a.has_dups != 0);
-            }
-            if (stop) break;
-        }
-    }
-
-    // Exact rerank of the result list (rerank!, src/rerank.jl:22-41): canonical
-    // distances from the FP32 vectors, stable sort, top kk written to row qi.
-    __device__ __forceinline__ void rerank_out(int64_t qi, int kk)
-    {
-        const int dp = 16 * V;
-        const int len = res.len;
-        float dex = __builtin_inff();
-        for (int r0 = 0; r0 < len; r0 += 16) {
-            // rows past the list repeat its last entry: unconditional loads, in flight together
-            RunWords<V, 0> xw[4];
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int u = r0 + 4 * r + g;
-                const uint32_t c = __shfl(res.id[0], u < len ? u : len - 1);
-                load_words<V>(a.X + (size_t)c * dp + j * V, xw[r]);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                float xv[V];
-                unpack_words<V>(xw[r], xv);
-                const float d = finish_dist(a.metric, group16_sum(run_partial<V>(a.metric, xv, qv)));
-                const float dt = __shfl(d, (lane & 3) * 16);
-                if (lane >= r0 && lane < r0 + 16 && ((lane - r0) >> 2) == r && lane < len) dex = dt;
-            }
-        }
-        int rank = 0;
-        for (int t = 0; t < len; t++) {
-            const float dt = rl_f(dex, t);
-            rank += (dt < dex || (dt == dex && t < lane)) ? 1 : 0;
-        }
-        if (lane < len && rank < kk) {
-            a.ids[(size_t)qi * a.kstride + rank] = res.id[0] + 1u;
-            a.dists[(size_t)qi * a.kstride + rank] = dex;
-        }
-        const int filled = len < kk ? len : kk;
-        if (lane >= filled && lane < a.kstride) {
-            a.ids[(size_t)qi * a.kstride + lane] = 0u;
-            a.dists[(size_t)qi * a.kstride + lane] = __builtin_inff();
-        }
-    }
-};
-
-template <int V, int NB, int PREC>
-__device__ __forceinline__ void beam_body(const BeamArgs& a)
-{
-    __shared__ uint32_t s_cid[WAVES][64];
-    __shared__ float s_dist[WAVES][64];
-    const int w = threadIdx.x >> 6;
-    const int64_t slot = (int64_t)blockIdx.x * WAVES + w;
-    Searcher<V, NB, PREC> S(a);
-    S.lane = lane_id();
-    S.g = S.lane >> 4;
-    S.j = S.lane & 15;
-    S.cidbuf = s_cid[w];
-    S.dbuf = s_dist[w];
-    S.vis = a.vis + slot * a.vis_stride;
-    uint32_t epoch = rfl_u(a.epochs[slot]);
-    const int lane = S.lane;
-    const int dp = 16 * V;
-
-    for (;;) {
-        uint32_t qi = 0;
-        if (lane == 0) qi = atomicAdd(a.counter, 1u);
-        qi = rl_u(qi, 0);
-        if ((int64_t)qi >= a.m) break;
-        epoch++;
-        if (epoch > 255) {
-            uint64_t* vp = reinterpret_cast<uint64_t*>(a.vis + slot * a.vis_stride);
-            for (int64_t w8 = lane; w8 < a.vis_stride / 8; w8 += 64)
-                __hip_atomic_store(vp + w8, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            vm_drain();
-            epoch = 1;
-        }
-        S.epoch = epoch;
-        if (a.allknn) {
-            load_run<V>(a.X + (size_t)qi * dp + S.j * V, S.qv);
-        } else {
-#pragma unroll
-            for (int t = 0; t < V; t++) {
-                const int i = S.j * V + t;
-                S.qv[t] = i < a.qdim ? a.Q[(size_t)qi * a.qdim + i] : 0.f;
-            }
-        }
-        const int kk = a.kq ? a.kq[qi] : a.k;
-        const int bs = a.bq ? a.bq[qi] : a.bsize;
-        S.res.init(PREC == 1 ? (kk > a.krr ? kk : a.krr) : kk);
-        S.cd_init = 0;
-        S.cd_inner = 0;
-        S.cb = 0;
-        if (a.allknn) {
-            const uint2 ni = a.node[qi];
-            for (int64_t t = ni.x; t < (int64_t)ni.x + ni.y; t++) {
-                const uint32_t h = a.adj[t];
-                vm_drain();
-                const uint32_t seen = rfl_u((uint32_t)vis_get(S.vis + h));
-                if (seen == epoch) continue;
-                const uint32_t h1 = h + 1u;
-                S.search(&h1, 1, true, bs);
-                S.cd_inner = 0;
-            }
-        } else if (a.qhints && a.qhcount[qi] >= 0) {
-            S.search(a.qhints + (size_t)qi * a.qhstride, a.qhcount[qi], true, bs);
-        } else {
-            S.search(a.hints, a.nhints, true, bs);
-        }
-        if constexpr (PREC == 1) {
-            S.rerank_out(qi, kk);
-        } else if (lane < a.kstride) {
-            const bool has = lane < S.res.len;
-            a.ids[(size_t)qi * a.kstride + lane] = has ? S.res.id[0] + 1u : 0u;
-            a.dists[(size_t)qi * a.kstride + lane] = has ? S.res.d[0] : __builtin_inff();
-        }
-        if (lane == 0) {
-            if (a.costd) a.costd[qi] = S.cd_init + S.cd_inner;
-            if (a.costb) a.costb[qi] = S.cb;
-        }
-    }
-    if (lane == 0) a.epochs[slot] = (uint8_t)epoch;
-}
-
-// searchbatch! / allknn! launches
-template <int V, int NB>
-__global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
-{
-    beam_body<V, NB, 0>(a);
-}
-
-// precision form: BF16 gathers, FP32 arithmetic, exact rerank of the result list
-template <int V, int NB>
-__global__ __launch_bounds__(256) void beam_bf16_kernel(BeamArgs a)
-{
-    beam_body<V, NB, 1>(a);
-}
-
-// the same search issued by index! and optimize_index! (a separate symbol keeps
-// their launches apart from query traffic in kernel traces)
-template <int V, int NB>
-__global__ __launch_bounds__(256) void beam_build_kernel(BeamArgs a)
-{
-    beam_body<V, NB, 0>(a);
-}
+template <int V>
+hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s);
 
 __global__ void zero_u32_kernel(unsigned int* p, int64_t n)
 {
@@ -428,26 +51,71 @@ hipError_t launch_to_bf16(const float* src, int64_t n, uint16_t* dst, hipStream_
     return hipGetLastError();
 }
 
-template <int V>
-static hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
+__global__ void permute_rows_kernel(const float* __restrict__ src, const uint32_t* __restrict__ perm,
+                                    int64_t n, int64_t n_pad, int32_t dp, float* __restrict__ dst)
 {
-    const dim3 grid((unsigned)(nslots / WAVES)), block(256);
-    if (a.internal) {
-        if (a.beamcap <= 64)
-            hipLaunchKernelGGL((beam_build_kernel<V, 1>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((beam_build_kernel<V, 4>), grid, block, 0, s, a);
-    } else if (a.prec == 1) {
-        if (a.beamcap <= 64)
-            hipLaunchKernelGGL((beam_bf16_kernel<V, 1>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((beam_bf16_kernel<V, 4>), grid, block, 0, s, a);
-    } else {
-        if (a.beamcap <= 64)
-            hipLaunchKernelGGL((beam_kernel<V, 1>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((beam_kernel<V, 4>), grid, block, 0, s, a);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = i / dp, col = i % dp;
+    if (row >= n_pad) return;
+    dst[i] = row < n ? src[(int64_t)perm[row] * dp + col] : 0.f;
+}
+
+hipError_t launch_permute_rows(const float* src, const uint32_t* perm, int64_t n, int64_t n_pad,
+                               int32_t dp, float* dst, hipStream_t s)
+{
+    const int64_t tot = n_pad * dp;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(permute_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, src,
+                       perm, n, n_pad, dp, dst);
+    return hipGetLastError();
+}
+
+// Longest-first processing order for heterogeneous batches: queries are grouped by
+// floor(log2(bsize)), larger classes first, so the longest searches start early and
+// do not form the tail of a persistent launch.  counts[0..32) is the histogram,
+// counts[32..64) the scatter cursors.
+__device__ __forceinline__ int order_class(int32_t key) { return 31 - __clz(key > 1 ? key : 1); }
+
+__global__ void order_hist_kernel(const int32_t* __restrict__ key, int64_t m, unsigned int* counts)
+{
+    __shared__ unsigned int h[32];
+    if (threadIdx.x < 32) h[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&h[order_class(key[i])], 1u);
+    __syncthreads();
+    if (threadIdx.x < 32 && h[threadIdx.x]) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void order_scatter_kernel(const int32_t* __restrict__ key, int64_t m, unsigned int* counts,
+                                     uint32_t* __restrict__ order)
+{
+    __shared__ unsigned int start[32];
+    if (threadIdx.x == 0) {
+        unsigned int acc = 0;
+        for (int c = 31; c >= 0; c--) {
+            start[c] = acc;
+            acc += counts[c];
+        }
     }
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = order_class(key[i]);
+        order[start[c] + atomicAdd(&counts[32 + c], 1u)] = (uint32_t)i;
+    }
+}
+
+hipError_t launch_order_desc(const int32_t* key, int64_t m, uint32_t* order, unsigned int* counts,
+                             hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipError_t e = launch_zero_u32(counts, 64, s);
+    if (e != hipSuccess) return e;
+    const unsigned blocks = (unsigned)std::min<int64_t>((m + 255) / 256, 1024);
+    hipLaunchKernelGGL(order_hist_kernel, dim3(blocks), dim3(256), 0, s, key, m, counts);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(order_scatter_kernel, dim3(blocks), dim3(256), 0, s, key, m, counts, order);
     return hipGetLastError();
 }
 

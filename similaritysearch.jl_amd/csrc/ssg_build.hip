@@ -416,6 +416,7 @@ int ssg_build_graph(ssg_handle* h, const ssg_build_params* p)
     Builder B{h, p, s, {}, {}, {}, {}, {}, {}, {}};
     h->lists.assign((size_t)n, {});
     h->hints_host.clear();
+    ssg_query_layout_stale(h);
     h->algo = ssg_beam_params{4, 1.0f, 1000000, p->beamcap};
     B.seg_start.assign((size_t)n, 0);
     B.seg_cap.assign((size_t)n, 0);
@@ -526,6 +527,7 @@ int ssg_build_graph(ssg_handle* h, const ssg_build_params* p)
     }
     h->nnz = nnz;
     h->has_dups = dups;
+    ssg_query_layout_stale(h);
     return SSG_OK;
 }
 

@@ -79,12 +79,27 @@ struct ssg_handle {
     // exhaustive work
     DevBuf qpad, qnorm, cand_d, cand_i, slice_tau, flags, part_i, part_d;
     // host-call staging
-    DevBuf sQ, sIds, sD, sCd, sCb, sK, sB, sH, sHc;
+    DevBuf sQ, sIds, sD, sCd, sCb, sK, sB, sH, sHc, sO;
     // builder scratch
     DevBuf bIds, bD, bOut, bCnt, bPairs, bNodeUpd;
     // learned-search scratch: hints, bucket, confidence, hint counts, beam sizes
-    DevBuf lH, lB, lC, lN, lS;
+    DevBuf lH, lB, lC, lN, lS, lO, lCnt;
+    // query layout: the installed graph and the vectors renumbered in breadth-first
+    // order from the hints (neighbours get nearby numbers, so their visited bytes,
+    // list entries and rows share cache lines and DRAM pages), with each list entry
+    // carrying its target's {start, degree}.  Built on the first query search after the
+    // graph or database changes; searchbatch! uses it, index!/allknn! do not.
+    bool reorder = true;
+    bool q_ready = false, q_bf16 = false;
+    DevBuf qnode, qadj, qadjn, qperm, qipos, qhints, Xq, Xqh;
 };
+
+// Invalidate the query layout (after any change of graph or database).
+inline void ssg_query_layout_stale(ssg_handle* h)
+{
+    h->q_ready = false;
+    h->q_bf16 = false;
+}
 
 int ssg_fail(int code, const std::string& msg);
 int ssg_hipfail(hipError_t e, const char* what);

@@ -50,6 +50,13 @@ struct BeamArgs {
     int32_t prec;            // 0: canonical FP32 path, 1: BF16 gathers + exact rerank
     const uint16_t* Xh;      // BF16 copy of X (n rows of dp), prec 1
     int32_t krr;             // prec 1: result-list length reranked exactly (>= k)
+    int32_t expand;          // prec 1: beam entries popped per step (1 = reference step)
+    const uint32_t* order;   // nullable: order[t] = the t-th query to search (0-based)
+    // Query layout (nullable): objects stored in another order.  adjn[e] = {start, degree}
+    // of the object adj[e] names; perm[stored] = original id (0-based); ipos = inverse.
+    const uint2* adjn;
+    const uint32_t* perm;
+    const uint32_t* ipos;
 };
 
 // Launch the persistent beam-search kernel over nslots waves (a multiple of 4).
@@ -58,6 +65,13 @@ hipError_t launch_zero_u32(unsigned int* p, int64_t n, hipStream_t s);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
 // FP32 -> BF16 with round-to-nearest-even
 hipError_t launch_to_bf16(const float* src, int64_t n, uint16_t* dst, hipStream_t s);
+// dst row i = src row perm[i] (rows of dp floats) for i < n; rows n..n_pad are zero
+hipError_t launch_permute_rows(const float* src, const uint32_t* perm, int64_t n, int64_t n_pad,
+                               int32_t dp, float* dst, hipStream_t s);
+// order[0..m) = query indices by descending floor(log2(key)) (key >= 1); counts is
+// scratch of 64 words.  Order within a class is unspecified.
+hipError_t launch_order_desc(const int32_t* key, int64_t m, uint32_t* order, unsigned int* counts,
+                             hipStream_t s);
 
 struct ExhArgs {
     const float* X;

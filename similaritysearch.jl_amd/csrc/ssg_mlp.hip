@@ -320,6 +320,8 @@ extern "C" int ssg_learned_search_dev(ssg_handle* h, ssg_mlp* mlp, const uint32_
         HIPCHK(h->lC.ensure((size_t)m * 4), "alloc");
         HIPCHK(h->lN.ensure((size_t)m * 4), "alloc");
         HIPCHK(h->lS.ensure((size_t)m * 4), "alloc");
+        HIPCHK(h->lO.ensure((size_t)m * 4), "alloc");
+        HIPCHK(h->lCnt.ensure(64 * 4), "alloc");
     }
     int rc = ssg_mlp_infer_dev(mlp, dQ, m, dlandmarks, nh, h->lH.as<uint32_t>(), h->lB.as<int32_t>(),
                                h->lC.as<float>(), s);
@@ -327,7 +329,10 @@ extern "C" int ssg_learned_search_dev(ssg_handle* h, ssg_mlp* mlp, const uint32_
     rc = ssg_learned_plan_dev(h, h->lB.as<int32_t>(), h->lC.as<float>(), m, nh, min_conf,
                               bucket_bsize, nb, bp->bsize, h->lN.as<int32_t>(), h->lS.as<int32_t>(), s);
     if (rc) return rc;
-    ssg_query_opts qo{nullptr, h->lS.as<int32_t>(), h->lH.as<uint32_t>(), h->lN.as<int32_t>(), nh};
+    HIPCHK(launch_order_desc(h->lS.as<int32_t>(), m, h->lO.as<uint32_t>(), h->lCnt.as<unsigned int>(), s),
+           "order");
+    ssg_query_opts qo{nullptr, h->lS.as<int32_t>(), h->lH.as<uint32_t>(), h->lN.as<int32_t>(), nh,
+                      h->lO.as<uint32_t>()};
     return ssg_beamsearch_dev(h, dQ, m, k, kstride, bp, &qo, dids, ddists, dcd, dcb, s);
 }
 

@@ -148,17 +148,21 @@ class SearchGraphContext(GenericContext):
     """Context of SearchGraph; beamcap is the number of rows of its beam buffers,
     the cap getbeam applies to bsize (src/searchgraph/context.jl:119-126,220-225)."""
 
-    def __init__(self, beamcap=32, verbose=False, precision="f32", rerank=0):
+    def __init__(self, beamcap=32, verbose=False, precision="f32", rerank=0, expand=1):
         super().__init__(verbose)
         if not 1 <= beamcap <= 256:
             raise ValueError("beamcap must be in 1..256")
         if precision not in ("f32", "bf16"):
             raise ValueError("precision must be 'f32' or 'bf16'")
+        if not 1 <= expand <= 4 or (expand > 1 and precision != "bf16"):
+            raise ValueError("expand must be 1..4, and above 1 only with precision 'bf16'")
         self.beamcap = int(beamcap)
         # precision form (no reference counterpart): BF16 gathers during the search and an
         # exact rerank of the `rerank` best candidates
         self.precision = precision
         self.rerank = int(rerank)
+        # wide steps of the BF16 form: beam entries popped and expanded together
+        self.expand = int(expand)
 
 
 # --------------------------------------------------------------------------- indexes
@@ -243,7 +247,7 @@ class SearchGraph(_GpuIndex):
         b = algo or self.algo
         prec = 1 if getattr(ctx, "precision", "f32") == "bf16" else 0
         return BeamParams(b.bsize, b.delta, b.maxvisits, getattr(ctx, "beamcap", 32), prec,
-                          getattr(ctx, "rerank", 0))
+                          getattr(ctx, "rerank", 0), getattr(ctx, "expand", 1))
 
     @classmethod
     def build(cls, dist, db, ctx=None, parallel_block=128, neighborhood_logbase=1.3,

@@ -16,14 +16,18 @@ _i32 = C.c_int32
 _i64 = C.c_int64
 
 
+SSG_OPT_QUERY_LAYOUT = 1
+
+
 class BeamParams(C.Structure):
     _fields_ = [("bsize", C.c_int32), ("delta", C.c_float), ("maxvisits", C.c_int64),
-                ("beamcap", C.c_int32), ("precision", C.c_int32), ("rerank", C.c_int32)]
+                ("beamcap", C.c_int32), ("precision", C.c_int32), ("rerank", C.c_int32),
+                ("expand", C.c_int32)]
 
 
 class QueryOpts(C.Structure):
     _fields_ = [("k", _p), ("bsize", _p), ("hints", _p), ("nhints", _p),
-                ("hint_stride", C.c_int32)]
+                ("hint_stride", C.c_int32), ("order", _p)]
 
 
 class BuildParams(C.Structure):
@@ -84,6 +88,8 @@ SIGNATURES = {
                                      C.POINTER(QueryOpts), _p, _p, _p, _p, _p]),
     "ssg_allknn_graph": (C.c_int, [_p, _i32, C.POINTER(BeamParams), _p, _p]),
     "ssg_allknn_exhaustive": (C.c_int, [_p, _i32, _p, _p]),
+    "ssg_order_by_bsize_dev": (C.c_int, [_p, _p, _i64, _p, _p]),
+    "ssg_set_option": (C.c_int, [_p, _i32, _i64]),
     "ssg_rerank": (C.c_int, [_p, _p, _i64, _i32, _p, _p]),
     "ssg_rerank_dev": (C.c_int, [_p, _p, _i64, _i32, _p, _p, _p]),
 }

@@ -164,13 +164,15 @@ class LearnedSearch:
 
     # ---------------------------------------------------------------- search
     def search_dev(self, ctx, dQ, m, k, dI, dD, nhints=1, min_conf=0.0, shift=0, rerank=20,
-                   fallback_bsize=None, stream=None, dcd=None, dcb=None):
-        """Device-pointer learned search; bucket b maps to beam size buckets[b + shift]."""
+                   fallback_bsize=None, stream=None, dcd=None, dcb=None, expand=1):
+        """Device-pointer learned search; bucket b maps to beam size buckets[b + shift];
+        `expand` beam entries are popped per step (1: the reference step)."""
         G = self.G
         nb = len(self.buckets)
         table = np.array([self.buckets[min(nb - 1, max(0, b + shift))] for b in range(nb)], np.int32)
         fb = fallback_bsize or G.algo.bsize
-        bp = BeamParams(fb, G.algo.delta, 10**6, max(ctx.beamcap, int(table.max())), 1, rerank)
+        bp = BeamParams(fb, G.algo.delta, 10**6, max(ctx.beamcap, int(table.max())), 1, rerank,
+                        expand)
         _lib.check(_lib.load().ssg_learned_search_dev(
             G._handle(), self.mlp._p, self.dlandmarks.ptr, dQ, m, k, k, C.byref(bp),
             nhints, min_conf, table.ctypes.data, nb, dI, dD, dcd, dcb, stream), "learned search")
