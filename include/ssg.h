@@ -170,6 +170,18 @@ int ssg_beamsearch_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, int
                        const ssg_beam_params* bp, const ssg_query_opts* qo, uint32_t* dids,
                        float* ddists, int64_t* dcostdists, int64_t* dcostblocks, void* stream);
 
+/* HIP graphs (no reference counterpart): launches issued by ssg_*_dev calls on `stream`
+ * between ssg_capture_begin and ssg_capture_end are recorded into a graph that
+ * ssg_graph_launch replays on any stream.  Every buffer those calls use must already be
+ * sized, so make each call once before capturing it; the calls record only kernels (no
+ * memsets, copies or synchronisation).  A graph replays with the arguments it was
+ * recorded with, so it is valid while those buffers and the handle's graph live. */
+typedef struct ssg_graph ssg_graph;
+int ssg_capture_begin(void* stream);
+int ssg_capture_end(void* stream, ssg_graph** graph);
+int ssg_graph_launch(ssg_graph* graph, void* stream);
+int ssg_graph_destroy(ssg_graph* graph);
+
 /* Handle options (no reference counterpart).
  * SSG_OPT_QUERY_LAYOUT: 1 (default) lets searchbatch! run on a copy of the graph and the
  * vectors renumbered breadth-first from the hints, each list entry carrying its target's
@@ -223,10 +235,12 @@ int ssg_mlp_train_dev(ssg_mlp* p, const float* dX, const int32_t* dyl, const int
                       int64_t N, int32_t epochs, int32_t batch, float bucket_weight, float lr,
                       uint64_t seed, float* losses);
 /* Inference on device rows dQ [m x D]: the h best landmarks (mapped through dlandmarks to
- * 1-based object ids) into dhints [m x h], the best bucket, and the top-landmark
- * softmax probability. */
+ * 1-based object ids) into dhints [m x h], the bucket, and the top-landmark softmax
+ * probability.  The bucket is the most probable one when bucket_quantile is 0, else the
+ * smallest bucket whose cumulative predicted probability reaches bucket_quantile. */
 int ssg_mlp_infer_dev(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dlandmarks,
-                      int32_t h, uint32_t* dhints, int32_t* dbucket, float* dconf, void* stream);
+                      int32_t h, float bucket_quantile, uint32_t* dhints, int32_t* dbucket,
+                      float* dconf, void* stream);
 /* Per-query search inputs from the inference outputs: nhints[j] = h when conf[j] >=
  * min_conf else -1 (the plain hints), bsize[j] = bucket_bsize[bucket[j]] or
  * fallback_bsize.  bucket_bsize is a host array of NB entries. */
@@ -235,14 +249,16 @@ int ssg_learned_plan_dev(ssg_handle* h, const int32_t* dbucket, const float* dco
                          int32_t fallback_bsize, int32_t* dnhints, int32_t* dbsize, void* stream);
 
 /* The learned form end to end on `stream`: ssg_mlp_infer_dev (nh landmark hints per query),
- * ssg_learned_plan_dev, then ssg_beamsearch_dev with those per-query hints and beam sizes
- * (bp->bsize is the fallback beam size; bp->precision selects the BF16 form).  Scratch is
- * owned by the handle, so repeated calls allocate nothing once sized. */
+ * ssg_learned_plan_dev, ssg_order_by_bsize_dev (largest beams first), then
+ * ssg_beamsearch_dev with those per-query hints, beam sizes and order (bp->bsize is the
+ * fallback beam size; bp->precision selects the BF16 form).  Scratch is owned by the
+ * handle, so repeated calls allocate nothing once sized and the sequence can be captured
+ * into a graph (ssg_capture_begin). */
 int ssg_learned_search_dev(ssg_handle* h, ssg_mlp* mlp, const uint32_t* dlandmarks,
                            const float* dQ, int64_t m, int32_t k, int32_t kstride,
                            const ssg_beam_params* bp, int32_t nh, float min_conf,
-                           const int32_t* bucket_bsize, int32_t nb, uint32_t* dids,
-                           float* ddists, int64_t* dcd, int64_t* dcb, void* stream);
+                           float bucket_quantile, const int32_t* bucket_bsize, int32_t nb,
+                           uint32_t* dids, float* ddists, int64_t* dcd, int64_t* dcb, void* stream);
 
 /* rerank!(dist, db, queries, ids, dists) batch form (src/rerank.jl:77-86): recompute the
  * non-zero-id prefix of each column with the handle's distance and stable-sort it.

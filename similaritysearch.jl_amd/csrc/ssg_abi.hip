@@ -58,34 +58,64 @@ static int ensure_vis(ssg_handle* h, hipStream_t s)
     return SSG_OK;
 }
 
-// Breadth-first renumbering of the installed graph from its hints (then any object not
-// reached, in id order), the renumbered CSR with per-entry target {start, degree}, and
-// the vectors in the new order (FP32, plus BF16 when asked).
+// Renumbering of the installed graph into cells of about 256 objects: a multi-source
+// breadth-first search from n/256 evenly spaced seeds assigns each object to the first
+// seed that reaches it (objects reached by none start cells of their own), and objects
+// are numbered cell by cell in discovery order.  A search visits a spatially compact
+// set of objects, so its visited bytes, list entries and rows fall into a few cells.
+// Then the renumbered CSR with per-entry target {start, degree} and the vectors in the
+// new order (FP32, plus BF16 when asked).
+static void cell_order(const std::vector<std::vector<uint32_t>>& lists, int64_t n,
+                       std::vector<uint32_t>& order, std::vector<uint32_t>& ipos)
+{
+    const int64_t S = std::max<int64_t>(1, n / 256);
+    std::vector<uint32_t> cell((size_t)n, 0xffffffffu), disc;
+    disc.reserve((size_t)n);
+    uint32_t ncell = 0;
+    auto run = [&](size_t head) {
+        while (head < disc.size()) {
+            const uint32_t u = disc[head++];
+            for (uint32_t v : lists[u])
+                if (cell[v] == 0xffffffffu) {
+                    cell[v] = cell[u];
+                    disc.push_back(v);
+                }
+        }
+    };
+    for (int64_t i = 0; i < S; i++) {
+        const uint32_t sd = (uint32_t)(i * n / S);
+        if (cell[sd] != 0xffffffffu) continue;
+        cell[sd] = ncell++;
+        disc.push_back(sd);
+    }
+    run(0);
+    for (int64_t i = 0; i < n; i++)
+        if (cell[i] == 0xffffffffu) {
+            const size_t head = disc.size();
+            cell[i] = ncell++;
+            disc.push_back((uint32_t)i);
+            run(head);
+        }
+    std::vector<uint32_t> start((size_t)ncell + 1, 0);
+    for (int64_t i = 0; i < n; i++) start[cell[i] + 1]++;
+    for (uint32_t c = 0; c < ncell; c++) start[c + 1] += start[c];
+    order.assign((size_t)n, 0);
+    ipos.assign((size_t)n, 0);
+    for (uint32_t u : disc) {
+        const uint32_t p = start[cell[u]]++;
+        order[p] = u;
+        ipos[u] = p;
+    }
+}
+
 static int ensure_query_layout(ssg_handle* h, hipStream_t s, bool bf16)
 {
     if (!h->q_ready) {
         HIPCHK(hipDeviceSynchronize(), "sync");  // no launch may still read the old layout
         const int64_t n = h->n;
         const auto& lists = h->lists;
-        std::vector<uint32_t> order;
-        order.reserve((size_t)n);
-        std::vector<uint32_t> ipos((size_t)n, 0xffffffffu);
-        auto bfs = [&](uint32_t seed) {
-            if (ipos[seed] != 0xffffffffu) return;
-            size_t head = order.size();
-            ipos[seed] = (uint32_t)order.size();
-            order.push_back(seed);
-            while (head < order.size()) {
-                const uint32_t u = order[head++];
-                for (uint32_t v : lists[u])
-                    if (ipos[v] == 0xffffffffu) {
-                        ipos[v] = (uint32_t)order.size();
-                        order.push_back(v);
-                    }
-            }
-        };
-        for (uint32_t hh : h->hints_host) bfs(hh - 1u);
-        for (int64_t i = 0; i < n; i++) bfs((uint32_t)i);
+        std::vector<uint32_t> order, ipos;
+        cell_order(lists, n, order, ipos);
         std::vector<uint2> node((size_t)n);
         std::vector<uint32_t> adj((size_t)std::max<int64_t>(h->nnz, 1));
         std::vector<uint2> adjn((size_t)std::max<int64_t>(h->nnz, 1));
@@ -644,6 +674,55 @@ int ssg_beamsearch_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, int
         return ssg_fail(SSG_E_ARG, "per-query hints need counts and a stride");
     return ssg_beam_run(h, dQ, h->dim, m, k, kstride, bp, qo, dids, ddists, dcd, dcb,
                         pick(h, stream), false, h->n);
+}
+
+// HIP graphs of device-pointer call sequences
+}  // extern "C"
+
+struct ssg_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int ssg_capture_begin(void* stream)
+{
+    if (!stream) return ssg_fail(SSG_E_ARG, "capture needs a created stream (not the null stream)");
+    HIPCHK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeRelaxed), "begin capture");
+    return SSG_OK;
+}
+
+int ssg_capture_end(void* stream, ssg_graph** g)
+{
+    if (!stream || !g) return ssg_fail(SSG_E_ARG, "null argument");
+    *g = nullptr;
+    ssg_graph* r = new ssg_graph();
+    hipError_t e = hipStreamEndCapture((hipStream_t)stream, &r->graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&r->exec, r->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (r->graph) (void)hipGraphDestroy(r->graph);
+        delete r;
+        return ssg_hipfail(e, "end capture");
+    }
+    *g = r;
+    return SSG_OK;
+}
+
+int ssg_graph_launch(ssg_graph* g, void* stream)
+{
+    if (!g || !stream) return ssg_fail(SSG_E_ARG, "null argument");
+    HIPCHK(hipGraphLaunch(g->exec, (hipStream_t)stream), "graph launch");
+    return SSG_OK;
+}
+
+int ssg_graph_destroy(ssg_graph* g)
+{
+    if (!g) return SSG_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return SSG_OK;
 }
 
 int ssg_set_option(ssg_handle* h, int32_t option, int64_t value)

@@ -493,6 +493,8 @@ hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
     } else if (a.prec == 1) {
         if (a.beamcap <= 64)
             hipLaunchKernelGGL((beam_bf16_kernel<V, 1>), grid, block, 0, s, a);
+        else if (a.beamcap <= 128)
+            hipLaunchKernelGGL((beam_bf16_kernel<V, 2>), grid, block, 0, s, a);
         else
             hipLaunchKernelGGL((beam_bf16_kernel<V, 4>), grid, block, 0, s, a);
     } else {

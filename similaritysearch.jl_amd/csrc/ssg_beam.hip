@@ -90,7 +90,7 @@ __global__ void order_hist_kernel(const int32_t* __restrict__ key, int64_t m, un
 __global__ void order_scatter_kernel(const int32_t* __restrict__ key, int64_t m, unsigned int* counts,
                                      uint32_t* __restrict__ order)
 {
-    __shared__ unsigned int start[32];
+    __shared__ unsigned int start[32], lcount[32], lbase[32];
     if (threadIdx.x == 0) {
         unsigned int acc = 0;
         for (int c = 31; c >= 0; c--) {
@@ -98,10 +98,23 @@ __global__ void order_scatter_kernel(const int32_t* __restrict__ key, int64_t m,
             acc += counts[c];
         }
     }
-    __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        const int c = order_class(key[i]);
-        order[start[c] + atomicAdd(&counts[32 + c], 1u)] = (uint32_t)i;
+    // per block step: rank within the block's class counts, then one cursor reservation
+    // per class and block
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < m; b0 += (int64_t)gridDim.x * blockDim.x) {
+        if (threadIdx.x < 32) lcount[threadIdx.x] = 0u;
+        __syncthreads();
+        const int64_t i = b0 + threadIdx.x;
+        int c = -1;
+        unsigned int lr = 0;
+        if (i < m) {
+            c = order_class(key[i]);
+            lr = atomicAdd(&lcount[c], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 32 && lcount[threadIdx.x]) lbase[threadIdx.x] = atomicAdd(&counts[32 + threadIdx.x], lcount[threadIdx.x]);
+        __syncthreads();
+        if (c >= 0) order[start[c] + lbase[c] + lr] = (uint32_t)i;
+        __syncthreads();
     }
 }
 

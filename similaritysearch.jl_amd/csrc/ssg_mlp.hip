@@ -7,9 +7,10 @@
 // Kernels:
 //   gemm_f16: C[M x N] (FP32) = op(A)[M x K] . op(B)[K x N], FP16 operands given by
 //     element strides (so forward, backward-data and backward-weight products are
-//     one kernel).  A block of 4 waves owns a 64 x 64 tile; each wave a 32 x 32
-//     sub-tile of 2 x 2 v_mfma_f32_16x16x16_f16 accumulators.
-//   bias_relu, softmax_xent (two heads, loss + gradient in one pass per row),
+//     one kernel), staged through LDS 32 k at a time.  A block of 4 waves owns a
+//     128 x 128 (or, for small products, 64 x 64) tile; each wave a quarter of it in
+//     v_mfma_f32_16x16x16_f16 accumulators.
+//   softmax_xent (two heads, loss + gradient in one pass per row),
 //   relu_grad, colsum (bias gradients), adam, and topk_heads (inference output).
 #include <hip/hip_runtime.h>
 
@@ -24,71 +25,124 @@ namespace ssg {
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float floatx4v __attribute__((ext_vector_type(4)));
 
+constexpr int GK = 32;       // k per LDS stage
+constexpr int GKP = GK + 4;  // padded LDS row (halves): fragment reads hit distinct banks
+
+// Stage a tile of GT rows x GK k of operand P (element (r, k) at P[r * sr + k * sk]) into
+// LDS as S[r * GKP + k].  Contiguous k (sk == 1) or contiguous rows (sr == 1) with 16-byte
+// alignment use 8-half vector loads; anything else, and the matrix edges, load per element.
+template <int GT>
+__device__ __forceinline__ void stage_tile(_Float16* __restrict__ S, const _Float16* __restrict__ P,
+                                           int64_t sr, int64_t sk, int r0, int R, int k0, int K,
+                                           int mode, int tid)
+{
+    constexpr int CH = GT * GK / 8;  // 8-half chunks per tile
+#pragma unroll
+    for (int i = 0; i < (CH + 255) / 256; i++) {
+        const int c = tid + 256 * i;
+        if (c >= CH) break;
+        if (mode == 1) {  // k contiguous
+            const int r = c / (GK / 8), kc = (c % (GK / 8)) * 8;
+            const int gr = r0 + r, gk = k0 + kc;
+            _Float16 v[8];
+            if (gr < R && gk + 7 < K) {
+                const uint4 w = *reinterpret_cast<const uint4*>(P + gr * sr + gk);
+                *reinterpret_cast<uint4*>(v) = w;
+            } else {
+#pragma unroll
+                for (int t = 0; t < 8; t++) v[t] = (gr < R && gk + t < K) ? P[gr * sr + (gk + t)] : (_Float16)0.f;
+            }
+            uint2* d = reinterpret_cast<uint2*>(S + r * GKP + kc);
+            d[0] = *reinterpret_cast<const uint2*>(v);
+            d[1] = *reinterpret_cast<const uint2*>(v + 4);
+        } else if (mode == 2) {  // rows contiguous
+            const int k = c / (GT / 8), rc = (c % (GT / 8)) * 8;
+            const int gk = k0 + k, gr = r0 + rc;
+            _Float16 v[8];
+            if (gk < K && gr + 7 < R) {
+                const uint4 w = *reinterpret_cast<const uint4*>(P + gk * sk + gr);
+                *reinterpret_cast<uint4*>(v) = w;
+            } else {
+#pragma unroll
+                for (int t = 0; t < 8; t++) v[t] = (gk < K && gr + t < R) ? P[(gr + t) * sr + gk * sk] : (_Float16)0.f;
+            }
+#pragma unroll
+            for (int t = 0; t < 8; t++) S[(rc + t) * GKP + k] = v[t];
+        } else {
+            const int r = c / (GK / 8), kc = (c % (GK / 8)) * 8;
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int gr = r0 + r, gk = k0 + kc + t;
+                S[r * GKP + kc + t] = (gr < R && gk < K) ? P[gr * sr + gk * sk] : (_Float16)0.f;
+            }
+        }
+    }
+}
+
+// C[M x N] (FP32) = A[M x K] . B[K x N] with A(m, k) = A[m * sam + k * sak] and
+// B(k, n) = B[k * sbk + n * sbn]; the epilogue adds bias[n] when given and writes an FP16
+// copy (ReLU'd when asked) with the same row stride when out16 is given.  A block of 4 waves owns a GT x GT tile, each wave a
+// (GT/2) x (GT/2) sub-tile of v_mfma_f32_16x16x16_f16 accumulators; k advances GK at a
+// time through LDS.  amode / bmode: 1 = k contiguous, 2 = rows contiguous, 0 = general.
+template <int GT>
 __global__ __launch_bounds__(256) void gemm_f16_kernel(int M, int N, int K, const _Float16* A,
                                                        int64_t sam, int64_t sak, const _Float16* B,
                                                        int64_t sbk, int64_t sbn, float* C,
-                                                       int64_t ldc, int accumulate)
+                                                       int64_t ldc, int accumulate, int amode, int bmode,
+                                                       const float* bias, int relu, _Float16* out16)
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int m0 = blockIdx.y * 64 + (w >> 1) * 32;
-    const int n0 = blockIdx.x * 64 + (w & 1) * 32;
+    constexpr int WT = GT / 2, NT = WT / 16;
+    __shared__ _Float16 As[GT * GKP];
+    __shared__ _Float16 Bs[GT * GKP];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int bm = blockIdx.y * GT, bn = blockIdx.x * GT;
+    const int wm = (w >> 1) * WT, wn = (w & 1) * WT;
     const int r16 = lane & 15, kq = (lane >> 4) * 4;
-    floatx4v acc[2][2];
+    floatx4v acc[NT][NT];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < NT; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = floatx4v{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        half4 fa[2], fb[2];
+        for (int b = 0; b < NT; b++) acc[a][b] = floatx4v{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += GK) {
+        stage_tile<GT>(As, A, sam, sak, bm, M, k0, K, amode, tid);
+        stage_tile<GT>(Bs, B, sbn, sbk, bn, N, k0, K, bmode, tid);
+        __syncthreads();
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
-            const int m = m0 + 16 * a + r16;
+        for (int kk = 0; kk < GK; kk += 16) {
+            half4 fa[NT], fb[NT];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int k = k0 + kq + t;
-                fa[a][t] = (m < M && k < K) ? A[m * sam + k * sak] : (_Float16)0.f;
-            }
+            for (int a = 0; a < NT; a++)
+                fa[a] = *reinterpret_cast<const half4*>(As + (wm + 16 * a + r16) * GKP + kk + kq);
+#pragma unroll
+            for (int b = 0; b < NT; b++)
+                fb[b] = *reinterpret_cast<const half4*>(Bs + (wn + 16 * b + r16) * GKP + kk + kq);
+#pragma unroll
+            for (int a = 0; a < NT; a++)
+#pragma unroll
+                for (int b = 0; b < NT; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a], fb[b], acc[a][b], 0, 0, 0);
         }
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-            const int n = n0 + 16 * b + r16;
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int k = k0 + kq + t;
-                fb[b][t] = (n < N && k < K) ? B[k * sbk + n * sbn] : (_Float16)0.f;
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int b = 0; b < 2; b++)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        __syncthreads();
     }
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < NT; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
-            const int n = n0 + 16 * b + r16;
+        for (int b = 0; b < NT; b++) {
+            const int n = bn + wn + 16 * b + r16;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int m = m0 + 16 * a + kq + t;
+                const int m = bm + wm + 16 * a + kq + t;
                 if (m < M && n < N) {
-                    float* p = C + (int64_t)m * ldc + n;
-                    *p = accumulate ? *p + acc[a][b][t] : acc[a][b][t];
+                    float v = acc[a][b][t];
+                    if (bias) v += bias[n];
+                    if (C) {
+                        float* p = C + (int64_t)m * ldc + n;
+                        *p = accumulate ? *p + v : v;
+                    }
+                    if (out16) out16[(int64_t)m * ldc + n] = (_Float16)(relu ? fmaxf(v, 0.f) : v);
                 }
             }
         }
-}
-
-// Y = C + bias (per column); relu when asked; FP32 pre-activation kept, FP16 copy out.
-__global__ void bias_relu_kernel(const float* C, const float* bias, int M, int N, int relu,
-                                 float* pre, _Float16* out16)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)M * N) return;
-    const float v = C[i] + bias[i % N];
-    if (pre) pre[i] = v;
-    if (out16) out16[i] = (_Float16)(relu ? fmaxf(v, 0.f) : v);
 }
 
 // Two softmax heads per row: [0, L) and [L, L+NB).  Writes the mean-loss contribution
@@ -152,13 +206,30 @@ __global__ void relu_grad_kernel(const float* draw, const float* pre, int64_t n,
     d16[i] = (_Float16)g;
 }
 
-// column sums of an M x N FP32 matrix (bias gradients)
-__global__ void colsum_kernel(const float* G, int M, int N, float* out)
+// column sums of an M x N FP32 matrix (bias gradients), in two deterministic passes:
+// block (x, y) sums rows y, y + CS_CHUNKS, ... of 64 columns with 4 interleaved row
+// phases, then colsum_finish adds the CS_CHUNKS partials of each column in order.
+constexpr int CS_CHUNKS = 16;
+
+__global__ __launch_bounds__(256) void colsum_kernel(const float* G, int M, int N, float* part)
+{
+    __shared__ float red[4][64];
+    const int c = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
+    float s = 0.f;
+    if (n < N)
+        for (int m = blockIdx.y + CS_CHUNKS * ph; m < M; m += 4 * CS_CHUNKS) s += G[(int64_t)m * N + n];
+    red[ph][c] = s;
+    __syncthreads();
+    if (ph == 0 && n < N) part[(int64_t)blockIdx.y * N + n] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+}
+
+__global__ void colsum_finish_kernel(const float* part, int N, float* out)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     float s = 0.f;
-    for (int m = 0; m < M; m++) s += G[(int64_t)m * N + n];
+    for (int y = 0; y < CS_CHUNKS; y++) s += part[(int64_t)y * N + n];
     out[n] = s;
 }
 
@@ -200,9 +271,10 @@ __global__ void gather_rows_f16_kernel(const float* X, int dim, const int32_t* i
 }
 
 // Inference head: per row, the h best landmarks (ids into `landmarks`, 1-based),
-// the best bucket, and the top landmark's softmax probability.  One wave per row.
+// the bucket (the most probable one, or with tau > 0 the smallest whose cumulative
+// probability reaches tau), and the top landmark's softmax probability.  One wave per row.
 __global__ __launch_bounds__(64) void topk_heads_kernel(const float* logits, int B, int L, int NB,
-                                                        int h, const uint32_t* landmarks,
+                                                        int h, float tau, const uint32_t* landmarks,
                                                         uint32_t* hints, int32_t* bucket,
                                                         float* conf)
 {
@@ -210,17 +282,35 @@ __global__ __launch_bounds__(64) void topk_heads_kernel(const float* logits, int
     const int lane = threadIdx.x;
     const int O = L + NB;
     const float* z = logits + (int64_t)row * O;
-    float mx = -__builtin_inff();
-    for (int c = lane; c < L; c += 64) mx = fmaxf(mx, z[c]);
-    for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    float se = 0.f;
-    for (int c = lane; c < L; c += 64) se += __expf(z[c] - mx);
-    for (int o = 32; o >= 1; o >>= 1) se += __shfl_xor(se, o);
-    if (lane == 0) conf[row] = 1.f / se;
-    // h rounds of argmax with exclusion (h is small)
-    float last = __builtin_inff();
-    int lastidx = -1;
-    for (int t = 0; t < h; t++) {
+    // one pass: running maximum with its first index and the softmax sum rescaled to it
+    float mx = -__builtin_inff(), se = 0.f;
+    int mi = 0x7fffffff;
+    for (int c = lane; c < L; c += 64) {
+        const float v = z[c];
+        if (v > mx) {
+            se = se * __expf(mx - v) + 1.f;
+            mx = v;
+            mi = c;
+        } else {
+            se += __expf(v - mx);
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float om = __shfl_xor(mx, o), os = __shfl_xor(se, o);
+        const int oi = __shfl_xor(mi, o);
+        const float nm = fmaxf(mx, om);
+        se = (mx == -__builtin_inff() ? 0.f : se * __expf(mx - nm)) + (om == -__builtin_inff() ? 0.f : os * __expf(om - nm));
+        if (om > mx || (om == mx && oi < mi)) mi = oi;
+        mx = nm;
+    }
+    if (lane == 0) {
+        conf[row] = 1.f / se;
+        hints[(int64_t)row * h] = landmarks[mi];
+    }
+    // further landmarks: rounds of argmax with exclusion (h is small)
+    float last = mx;
+    int lastidx = mi;
+    for (int t = 1; t < h; t++) {
         float bv = -__builtin_inff();
         int bi = 0x7fffffff;
         for (int c = lane; c < L; c += 64) {
@@ -251,6 +341,20 @@ __global__ __launch_bounds__(64) void topk_heads_kernel(const float* logits, int
                 bb = z[L + c];
                 bix = c;
             }
+        if (tau > 0.f) {
+            // smallest bucket whose cumulative predicted probability reaches tau
+            float tot = 0.f;
+            for (int c = 0; c < NB; c++) tot += __expf(z[L + c] - bb);
+            float acc = 0.f;
+            bix = NB - 1;
+            for (int c = 0; c < NB; c++) {
+                acc += __expf(z[L + c] - bb);
+                if (acc >= tau * tot) {
+                    bix = c;
+                    break;
+                }
+            }
+        }
         bucket[row] = bix;
     }
 }
@@ -298,14 +402,14 @@ extern "C" int ssg_learned_plan_dev(ssg_handle* h, const int32_t* dbucket, const
 
 struct ssg_mlp;
 extern "C" int ssg_mlp_infer_dev(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dlandmarks,
-                                 int32_t h, uint32_t* dhints, int32_t* dbucket, float* dconf,
-                                 void* stream);
+                                 int32_t h, float bucket_quantile, uint32_t* dhints, int32_t* dbucket,
+                                 float* dconf, void* stream);
 
 // Learned search: inference -> plan -> beam search, one stream, handle-owned scratch.
 extern "C" int ssg_learned_search_dev(ssg_handle* h, ssg_mlp* mlp, const uint32_t* dlandmarks,
                                       const float* dQ, int64_t m, int32_t k, int32_t kstride,
                                       const ssg_beam_params* bp, int32_t nh, float min_conf,
-                                      const int32_t* bucket_bsize, int32_t nb,
+                                      float bucket_quantile, const int32_t* bucket_bsize, int32_t nb,
                                       uint32_t* dids, float* ddists, int64_t* dcd, int64_t* dcb,
                                       void* stream)
 {
@@ -323,8 +427,8 @@ extern "C" int ssg_learned_search_dev(ssg_handle* h, ssg_mlp* mlp, const uint32_
         HIPCHK(h->lO.ensure((size_t)m * 4), "alloc");
         HIPCHK(h->lCnt.ensure(64 * 4), "alloc");
     }
-    int rc = ssg_mlp_infer_dev(mlp, dQ, m, dlandmarks, nh, h->lH.as<uint32_t>(), h->lB.as<int32_t>(),
-                               h->lC.as<float>(), s);
+    int rc = ssg_mlp_infer_dev(mlp, dQ, m, dlandmarks, nh, bucket_quantile, h->lH.as<uint32_t>(),
+                               h->lB.as<int32_t>(), h->lC.as<float>(), s);
     if (rc) return rc;
     rc = ssg_learned_plan_dev(h, h->lB.as<int32_t>(), h->lC.as<float>(), m, nh, min_conf,
                               bucket_bsize, nb, bp->bsize, h->lN.as<int32_t>(), h->lS.as<int32_t>(), s);
@@ -348,6 +452,7 @@ struct ssg_mlp {
     // per-batch work
     DevBuf X16, C1, P1, A1, C2, P2, A2, Z, dZ16, dA32, dA16, loss;
     DevBuf xin, yl, yb, perm;  // training inputs of one batch / one epoch
+    DevBuf colpart;            // column-sum partials
     int cap_b = 0;
     int64_t step = 0;
     std::vector<uint32_t> landmarks;
@@ -359,12 +464,42 @@ namespace {
 int dims_out(const ssg_mlp* p, int layer) { return layer == 0 ? p->H1 : layer == 1 ? p->H2 : p->L + p->NB; }
 int dims_in(const ssg_mlp* p, int layer) { return layer == 0 ? p->D : layer == 1 ? p->H1 : p->H2; }
 
-hipError_t gemm(hipStream_t s, int M, int N, int K, const _Float16* A, int64_t sam, int64_t sak,
-                const _Float16* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc, int acc = 0)
+hipError_t colsum(ssg_mlp* p, hipStream_t s, const float* G, int M, int N, float* out)
 {
-    const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
-    hipLaunchKernelGGL(gemm_f16_kernel, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn,
-                       C, ldc, acc);
+    hipError_t e = p->colpart.ensure((size_t)CS_CHUNKS * N * sizeof(float));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(colsum_kernel, dim3((unsigned)((N + 63) / 64), CS_CHUNKS), dim3(256), 0, s, G, M, N,
+                       p->colpart.as<float>());
+    hipLaunchKernelGGL(colsum_finish_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s,
+                       p->colpart.as<float>(), N, out);
+    return hipGetLastError();
+}
+
+// operand staging mode: 1 when k is contiguous, 2 when rows are, with 16-byte aligned
+// 8-half runs; 0 otherwise
+int stage_mode(const _Float16* P, int64_t sr, int64_t sk)
+{
+    const bool al = ((uintptr_t)P & 15) == 0;
+    if (al && sk == 1 && sr % 8 == 0) return 1;
+    if (al && sr == 1 && sk % 8 == 0) return 2;
+    return 0;
+}
+
+hipError_t gemm(hipStream_t s, int M, int N, int K, const _Float16* A, int64_t sam, int64_t sak,
+                const _Float16* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc, int acc = 0,
+                const float* bias = nullptr, int relu = 0, _Float16* out16 = nullptr)
+{
+    const int am = stage_mode(A, sam, sak), bmd = stage_mode(B, sbn, sbk);
+    const int64_t big = (int64_t)((N + 127) / 128) * ((M + 127) / 128);
+    if (big >= 256) {
+        const dim3 grid((unsigned)((N + 127) / 128), (unsigned)((M + 127) / 128));
+        hipLaunchKernelGGL(gemm_f16_kernel<128>, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk,
+                           sbn, C, ldc, acc, am, bmd, bias, relu, out16);
+    } else {
+        const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
+        hipLaunchKernelGGL(gemm_f16_kernel<64>, grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk,
+                           sbn, C, ldc, acc, am, bmd, bias, relu, out16);
+    }
     return hipGetLastError();
 }
 
@@ -396,22 +531,13 @@ int forward(ssg_mlp* p, int B)
 {
     hipStream_t s = p->stream;
     const int O = p->L + p->NB;
+    // pre-activations (bias added) into P1 / P2, ReLU'd FP16 activations into A1 / A2
     HIPCHK(gemm(s, B, p->H1, p->D, p->X16.as<_Float16>(), p->D, 1, p->Wh[0].as<_Float16>(), 1, p->D,
-                p->C1.as<float>(), p->H1), "gemm");
-    hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for((int64_t)B * p->H1)), dim3(256), 0, s,
-                       p->C1.as<float>(), p->b[0].as<float>(), B, p->H1, 1, p->P1.as<float>(),
-                       p->A1.as<_Float16>());
+                p->P1.as<float>(), p->H1, 0, p->b[0].as<float>(), 1, p->A1.as<_Float16>()), "gemm");
     HIPCHK(gemm(s, B, p->H2, p->H1, p->A1.as<_Float16>(), p->H1, 1, p->Wh[1].as<_Float16>(), 1, p->H1,
-                p->C2.as<float>(), p->H2), "gemm");
-    hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for((int64_t)B * p->H2)), dim3(256), 0, s,
-                       p->C2.as<float>(), p->b[1].as<float>(), B, p->H2, 1, p->P2.as<float>(),
-                       p->A2.as<_Float16>());
+                p->P2.as<float>(), p->H2, 0, p->b[1].as<float>(), 1, p->A2.as<_Float16>()), "gemm");
     HIPCHK(gemm(s, B, O, p->H2, p->A2.as<_Float16>(), p->H2, 1, p->Wh[2].as<_Float16>(), 1, p->H2,
-                p->Z.as<float>(), O), "gemm");
-    hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for((int64_t)B * O)), dim3(256), 0, s,
-                       p->Z.as<float>(), p->b[2].as<float>(), B, O, 0, p->Z.as<float>(),
-                       (_Float16*)nullptr);
-    HIPCHK(hipGetLastError(), "forward");
+                p->Z.as<float>(), O, 0, p->b[2].as<float>()), "gemm");
     return SSG_OK;
 }
 
@@ -425,15 +551,12 @@ int backward(ssg_mlp* p, int B)
                 p->gW[2].as<float>(), p->H2), "gemm");
     HIPCHK(gemm(s, B, p->H2, O, p->dZ16.as<_Float16>(), O, 1, p->Wh[2].as<_Float16>(), p->H2, 1,
                 p->dA32.as<float>(), p->H2), "gemm");
-    // gb3 = colsum(dZ): dZ is FP16 only, so sum the FP32 logit gradient via gemm with ones
-    // is avoided: softmax_xent also wrote it in FP32 into C1-sized scratch? no -> use Z reuse
-    hipLaunchKernelGGL(colsum_kernel, dim3(blocks_for(O)), dim3(256), 0, s, p->Z.as<float>(), B, O,
-                       p->gb[2].as<float>());
+    // gb3 = column sums of the FP32 logit gradient, which softmax_xent left in Z
+    HIPCHK(colsum(p, s, p->Z.as<float>(), B, O, p->gb[2].as<float>()), "colsum");
     hipLaunchKernelGGL(relu_grad_kernel, dim3(blocks_for((int64_t)B * p->H2)), dim3(256), 0, s,
                        p->dA32.as<float>(), p->P2.as<float>(), (int64_t)B * p->H2,
                        p->C2.as<float>(), p->dA16.as<_Float16>());
-    hipLaunchKernelGGL(colsum_kernel, dim3(blocks_for(p->H2)), dim3(256), 0, s, p->C2.as<float>(), B,
-                       p->H2, p->gb[1].as<float>());
+    HIPCHK(colsum(p, s, p->C2.as<float>(), B, p->H2, p->gb[1].as<float>()), "colsum");
     // layer 2: gW2[H2 x H1] = dH2^T . A1 ; dA1 = dH2 . W2
     HIPCHK(gemm(s, p->H2, p->H1, B, p->dA16.as<_Float16>(), 1, p->H2, p->A1.as<_Float16>(), p->H1, 1,
                 p->gW[1].as<float>(), p->H1), "gemm");
@@ -443,8 +566,7 @@ int backward(ssg_mlp* p, int B)
     hipLaunchKernelGGL(relu_grad_kernel, dim3(blocks_for((int64_t)B * p->H1)), dim3(256), 0, s,
                        p->dA32.as<float>(), p->P1.as<float>(), (int64_t)B * p->H1,
                        p->C1.as<float>(), p->dA16.as<_Float16>());
-    hipLaunchKernelGGL(colsum_kernel, dim3(blocks_for(p->H1)), dim3(256), 0, s, p->C1.as<float>(), B,
-                       p->H1, p->gb[0].as<float>());
+    HIPCHK(colsum(p, s, p->C1.as<float>(), B, p->H1, p->gb[0].as<float>()), "colsum");
     // layer 1: gW1[H1 x D] = dH1^T . X
     HIPCHK(gemm(s, p->H1, p->D, B, p->dA16.as<_Float16>(), 1, p->H1, p->X16.as<_Float16>(), p->D, 1,
                 p->gW[0].as<float>(), p->D), "gemm");
@@ -673,10 +795,13 @@ int ssg_mlp_grad(ssg_mlp* p, int32_t layer, float* gW, float* gb)
 // Inference on device query rows dQ [m x D] (FP32): logits, then per query the h
 // best landmarks (as 1-based object ids through `landmarks`), bucket and confidence.
 int ssg_mlp_infer_dev(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dlandmarks,
-                      int32_t h, uint32_t* dhints, int32_t* dbucket, float* dconf, void* stream)
+                      int32_t h, float bucket_quantile, uint32_t* dhints, int32_t* dbucket, float* dconf,
+                      void* stream)
 {
     if (!p || !dQ || !dlandmarks || !dhints || !dbucket || !dconf || m < 1 || h < 1 || h > p->L)
         return ssg_fail(SSG_E_ARG, "bad inference arguments");
+    if (!(bucket_quantile >= 0.f && bucket_quantile <= 1.f))
+        return ssg_fail(SSG_E_ARG, "bucket quantile must be in [0, 1]");
     HIPCHK(hipSetDevice(p->device), "hipSetDevice");
     hipStream_t saved = p->stream;
     if (stream) p->stream = (hipStream_t)stream;
@@ -691,7 +816,8 @@ int ssg_mlp_infer_dev(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dl
         rc = forward(p, B);
         if (rc) break;
         hipLaunchKernelGGL(topk_heads_kernel, dim3(B), dim3(64), 0, p->stream, p->Z.as<float>(), B,
-                           p->L, p->NB, h, dlandmarks, dhints + r0 * h, dbucket + r0, dconf + r0);
+                           p->L, p->NB, h, bucket_quantile, dlandmarks, dhints + r0 * h, dbucket + r0,
+                           dconf + r0);
         if (hipGetLastError() != hipSuccess) rc = ssg_fail(SSG_E_HIP, "inference launch");
     }
     p->stream = saved;

@@ -62,12 +62,12 @@ SIGNATURES = {
                                C.POINTER(C.c_float)]),
     "ssg_mlp_train_dev": (C.c_int, [_p, _p, _p, _p, _i64, _i32, _i32, C.c_float, C.c_float,
                                     C.c_uint64, _p]),
-    "ssg_mlp_infer_dev": (C.c_int, [_p, _p, _i64, _p, _i32, _p, _p, _p, _p]),
+    "ssg_mlp_infer_dev": (C.c_int, [_p, _p, _i64, _p, _i32, C.c_float, _p, _p, _p, _p]),
     "ssg_learned_plan_dev": (C.c_int, [_p, _p, _p, _i64, _i32, C.c_float, _p, _i32, _i32, _p,
                                        _p, _p]),
     "ssg_learned_search_dev": (C.c_int, [_p, _p, _p, _p, _i64, _i32, _i32,
-                                         C.POINTER(BeamParams), _i32, C.c_float, _p, _i32, _p,
-                                         _p, _p, _p, _p]),
+                                         C.POINTER(BeamParams), _i32, C.c_float, C.c_float, _p,
+                                         _i32, _p, _p, _p, _p, _p]),
     "ssg_fft": (C.c_int, [_p, _i32, _i64, _p, _p, _p, _p]),
     "ssg_build_graph": (C.c_int, [_p, C.POINTER(BuildParams)]),
     "ssg_graph_info": (C.c_int, [_p, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(BeamParams)]),
@@ -90,6 +90,10 @@ SIGNATURES = {
     "ssg_allknn_exhaustive": (C.c_int, [_p, _i32, _p, _p]),
     "ssg_order_by_bsize_dev": (C.c_int, [_p, _p, _i64, _p, _p]),
     "ssg_set_option": (C.c_int, [_p, _i32, _i64]),
+    "ssg_capture_begin": (C.c_int, [_p]),
+    "ssg_capture_end": (C.c_int, [_p, C.POINTER(_p)]),
+    "ssg_graph_launch": (C.c_int, [_p, _p]),
+    "ssg_graph_destroy": (C.c_int, [_p]),
     "ssg_rerank": (C.c_int, [_p, _p, _i64, _i32, _p, _p]),
     "ssg_rerank_dev": (C.c_int, [_p, _p, _i64, _i32, _p, _p, _p]),
 }

@@ -86,6 +86,27 @@ class Mlp:
             pass
 
 
+class LearnedGraph:
+    """A recorded learned-search sequence; launch() replays it on a stream."""
+
+    def __init__(self, g):
+        self._g = g
+
+    def launch(self, stream):
+        _lib.check(_lib.load().ssg_graph_launch(self._g, stream.handle), "graph launch")
+
+    def close(self):
+        if self._g:
+            _lib.load().ssg_graph_destroy(self._g)
+            self._g = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class LearnedSearch:
     """Learned entry points + BF16 beam search over a SearchGraph `G`."""
 
@@ -108,8 +129,11 @@ class LearnedSearch:
         self.object_landmark = pos[nn.astype(np.int64)]  # landmark index of each object
         return centers
 
-    def traces(self, Qtrain, ctx, k=10, rerank=20, batch=50_000):
-        """Labels (landmark index, bucket index) for each training query row."""
+    def traces(self, Qtrain, ctx, k=10, rerank=20, batch=50_000, min_hits=None):
+        """Labels (landmark index, bucket index) for each training query row: the bucket is
+        the smallest beam size whose search from the label landmark finds at least
+        `min_hits` (default k) of the exact k nearest neighbours."""
+        min_hits = k if min_hits is None else int(min_hits)
         G = self.G
         h = G._handle()
         L = _lib.load()
@@ -137,7 +161,8 @@ class LearnedSearch:
                 ds = np.zeros((e - s, k), np.float32)
                 _lib.check(L.ssg_beamsearch(h, Qb.ctypes.data, e - s, k, k, C.byref(bp), C.byref(qo),
                                             ids.ctypes.data, ds.ctypes.data, None, None), "trace search")
-                ok = np.all(np.sort(ids, axis=1) == gold, axis=1) & ~fnd
+                hits = (ids[:, :, None] == gold[:, None, :]).any(axis=2).sum(axis=1)
+                ok = (hits >= min_hits) & ~fnd
                 yb[s:e][ok] = bi
                 fnd |= ok
             found[s:e] = fnd
@@ -163,19 +188,43 @@ class LearnedSearch:
         return losses
 
     # ---------------------------------------------------------------- search
-    def search_dev(self, ctx, dQ, m, k, dI, dD, nhints=1, min_conf=0.0, shift=0, rerank=20,
-                   fallback_bsize=None, stream=None, dcd=None, dcb=None, expand=1):
-        """Device-pointer learned search; bucket b maps to beam size buckets[b + shift];
-        `expand` beam entries are popped per step (1: the reference step)."""
-        G = self.G
+    def table(self, shift=0):
+        """Beam size of each bucket: buckets[b + shift], clamped to the bucket range."""
         nb = len(self.buckets)
-        table = np.array([self.buckets[min(nb - 1, max(0, b + shift))] for b in range(nb)], np.int32)
+        return np.array([self.buckets[min(nb - 1, max(0, b + shift))] for b in range(nb)], np.int32)
+
+    def search_dev(self, ctx, dQ, m, k, dI, dD, nhints=1, min_conf=0.0, shift=0, rerank=20,
+                   fallback_bsize=None, stream=None, dcd=None, dcb=None, expand=1, quantile=0.0,
+                   delta=None):
+        """Device-pointer learned search.  Each query's bucket is the most probable one
+        (quantile 0) or the smallest whose cumulative probability reaches `quantile`;
+        bucket b maps to beam size buckets[b + shift]; `expand` beam entries are popped per
+        step (1: the reference step); `delta` defaults to the graph's tuned one."""
+        G = self.G
+        table = self.table(shift)
         fb = fallback_bsize or G.algo.bsize
-        bp = BeamParams(fb, G.algo.delta, 10**6, max(ctx.beamcap, int(table.max())), 1, rerank,
-                        expand)
+        d = G.algo.delta if delta is None else float(delta)
+        bp = BeamParams(fb, d, 10**6, max(ctx.beamcap, int(table.max())), 1, rerank, expand)
         _lib.check(_lib.load().ssg_learned_search_dev(
             G._handle(), self.mlp._p, self.dlandmarks.ptr, dQ, m, k, k, C.byref(bp),
-            nhints, min_conf, table.ctypes.data, nb, dI, dD, dcd, dcb, stream), "learned search")
+            nhints, min_conf, float(quantile), table.ctypes.data, len(table), dI, dD, dcd, dcb,
+            stream), "learned search")
+
+    def capture(self, ctx, dQ, m, k, dI, dD, stream, **kw):
+        """Records search_dev over the m device-resident queries into a HIP graph (one
+        persistent search launch behind the inference, plan and ordering kernels).  The
+        call is made once first so that every buffer is sized; returns a LearnedGraph."""
+        self.search_dev(ctx, dQ, m, k, dI, dD, stream=stream.handle, **kw)
+        stream.synchronize()
+        L = _lib.load()
+        _lib.check(L.ssg_capture_begin(stream.handle), "capture begin")
+        try:
+            self.search_dev(ctx, dQ, m, k, dI, dD, stream=stream.handle, **kw)
+        finally:
+            g = C.c_void_p()
+            rc = L.ssg_capture_end(stream.handle, C.byref(g))
+        _lib.check(rc, "capture end")
+        return LearnedGraph(g)
 
     def search(self, ctx, Q, k, **kw):
         from . import _as_db
