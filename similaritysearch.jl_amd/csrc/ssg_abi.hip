@@ -264,7 +264,7 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
     }
     if (n_eff == 0) return SSG_OK;
     const int64_t m_pad = (m + EXH_QT - 1) / EXH_QT * EXH_QT;
-    const int ns = exhaustive_slices(n_eff, m, h->num_cu);
+    const int ns = exhaustive_slices(n_eff, m, h->num_cu, h->dp);
     ExhWork w{};
     if (qpad_in) {
         w.qpad = const_cast<float*>(qpad_in);

@@ -7,9 +7,11 @@
 // (src/pqueue/knnsorted.jl:140-168), and what the batch merge of
 // src/exact/parallel-exhaustive.jl:79-98 reproduces.
 //
-// Stage 1 (MFMA): block = 64 queries x one DB slice.  Each wave owns 16 queries
-//   (the B operand of v_mfma_f32_16x16x4_f32, kept in registers for the whole
-//   slice); 64-row DB tiles are staged in LDS and shared by the 4 waves.  The dot
+// Stage 1 (MFMA): block = 64 queries x one DB slice, two blocks per CU.  Each wave owns
+//   16 queries (the B operand of v_mfma_f32_16x16x4_f32, kept in registers for the
+//   whole slice); 64-row DB tiles are staged in LDS (the next one prefetched into
+//   registers while the current one is computed) and shared by the 4 waves; row
+//   blocks are software-pipelined so the filter below overlaps the next block's MFMAs.  The dot
 //   products give approximate distances ||x||^2 + ||q||^2 - 2 x.q (or 1 - x.q)
 //   that are filtered against a per-query threshold tau into an LDS candidate
 //   buffer; a full buffer is compacted to its KC best (d, id) entries.  The slice
@@ -85,11 +87,13 @@ __global__ __launch_bounds__(256) void exh_stage1(const float* __restrict__ X,
 {
     constexpr int LS = DP + 4;  // LDS row stride (floats)
     constexpr int KS = DP / 4;  // MFMA k-steps
+    static_assert(EXH_QT == TR, "the query rows are staged through the row tile");
     __shared__ float s_x[TR * LS];
     __shared__ float s_xn[TR];
     __shared__ float s_cd[EXH_QT][CAP];
     __shared__ uint32_t s_ci[EXH_QT][CAP];
     __shared__ int s_cnt[EXH_QT];
+    __shared__ float s_qn[EXH_QT];
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int col = lane & 15, gk = lane >> 4;
@@ -98,11 +102,25 @@ __global__ __launch_bounds__(256) void exh_stage1(const float* __restrict__ X,
     const int64_t r0 = (int64_t)sl * rows_per_slice;
     const int64_t r1 = min(n, r0 + rows_per_slice);
     const int qloc = 16 * w + col;
-    const int64_t qj = qt * EXH_QT + qloc;
 
+    // The block's 64 query rows are staged through the tile buffer, so the B operands
+    // come from LDS reads and no register of the MFMA loop depends on a global load
+    // (the tile prefetch below stays in flight across the whole loop).  Query norms go
+    // through LDS for the same reason.
+    for (int idx = tid; idx < EXH_QT * (DP / 4); idx += 256) {
+        const int row = idx / (DP / 4), c4 = idx % (DP / 4);
+        *reinterpret_cast<float4*>(&s_x[row * LS + 4 * c4]) =
+            reinterpret_cast<const float4*>(qpad + (qt * EXH_QT + row) * DP)[c4];
+    }
+    if (tid < EXH_QT) {
+        s_cnt[tid] = 0;
+        s_qn[tid] = qnorm[qt * EXH_QT + tid];
+    }
+    __syncthreads();
+    const float qn = s_qn[qloc];
     float bq[KS];
     {
-        const float* qp = qpad + qj * DP + gk * KS;
+        const float* qp = &s_x[qloc * LS + gk * KS];
 #pragma unroll
         for (int s = 0; s < KS; s += 4) {
             const float4 v = *reinterpret_cast<const float4*>(qp + s);
@@ -112,52 +130,97 @@ __global__ __launch_bounds__(256) void exh_stage1(const float* __restrict__ X,
             bq[s + 3] = v.w;
         }
     }
-    const float qn = qnorm[qj];
     float tau = __builtin_inff();
-    if (tid < EXH_QT) s_cnt[tid] = 0;
 
+    // the next tile is loaded into registers while the current one is computed
+    constexpr int PF = (TR * (DP / 4) + 255) / 256;  // float4 per thread per tile
+    float4 pf[PF];
+    float4 pfn;  // the tile's 64 norms, 4 per thread of the first 16 (tiles start at multiples
+                 // of 64 rows and xnorm holds n_pad entries, so the load is always in bounds)
+    auto prefetch = [&](int64_t t) {
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+            const int idx = tid + 256 * i;
+            const int row = idx / (DP / 4), c4 = idx % (DP / 4);
+            const int64_t gr = t + row;
+            pf[i] = (idx < TR * (DP / 4) && gr < r1) ? reinterpret_cast<const float4*>(X + gr * DP)[c4]
+                                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        pfn = reinterpret_cast<const float4*>(xnorm + t)[tid & (TR / 4 - 1)];
+    };
+    if (r0 < r1) prefetch(r0);
+    const float* a_base = &s_x[col * LS + gk * KS];  // this lane's A run in row block 0
     for (int64_t t0 = r0; t0 < r1; t0 += TR) {
         __syncthreads();
-        for (int idx = tid; idx < TR * (DP / 4); idx += 256) {
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+            const int idx = tid + 256 * i;
             const int row = idx / (DP / 4), c4 = idx % (DP / 4);
-            const int64_t gr = t0 + row;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gr < r1) v = reinterpret_cast<const float4*>(X + gr * DP)[c4];
-            *reinterpret_cast<float4*>(&s_x[row * LS + 4 * c4]) = v;
+            if (idx < TR * (DP / 4)) *reinterpret_cast<float4*>(&s_x[row * LS + 4 * c4]) = pf[i];
         }
-        if (tid < TR) s_xn[tid] = (t0 + tid < r1) ? xnorm[t0 + tid] : 0.f;
+        if (tid < TR / 4) {
+            const float v[4] = {pfn.x, pfn.y, pfn.z, pfn.w};
+#pragma unroll
+            for (int c = 0; c < 4; c++) s_xn[4 * tid + c] = (t0 + 4 * tid + c < r1) ? v[c] : 0.f;
+        }
         __syncthreads();
-#pragma unroll 1
-        for (int rb = 0; rb < TR / 16; rb++) {
+        if (t0 + TR < r1) prefetch(t0 + TR);
+        // Row blocks are software-pipelined: the MFMA chain of block rb + 1 is issued
+        // before block rb's distances are filtered, so the filter's VALU and LDS work
+        // runs while the matrix pipe is busy.
+        auto chain = [&](int rb) {
             floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-            const float* ap = &s_x[(rb * 16 + col) * LS + gk * KS];
+            const float* ap = a_base + rb * (16 * LS);
 #pragma unroll
             for (int s = 0; s < KS; s++)
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ap[s], bq[s], acc, 0, 0, 0);
+            return acc;
+        };
+        floatx4 cur = chain(0);
+#pragma unroll
+        for (int rb = 0; rb < TR / 16; rb++) {
+            floatx4 nxt = {0.f, 0.f, 0.f, 0.f};
+            if (rb + 1 < TR / 16) nxt = chain(rb + 1);
+            // norms of this lane's 4 output rows of block rb
+            const float4 xn4 = *reinterpret_cast<const float4*>(&s_xn[rb * 16 + gk * 4]);
+            const float xn[4] = {xn4.x, xn4.y, xn4.z, xn4.w};
+            float da[4];
+            bool pass[4];
+            bool any = false;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = rb * 16 + gk * 4 + r;
-                const int64_t gid = t0 + row;
-                const float dot = acc[r];
-                const float da = metric == M_NORMCOS ? 1.0f - dot : (s_xn[row] + qn) - 2.0f * dot;
-                if (gid < r1 && da < tau) {
-                    const int p = atomicAdd(&s_cnt[qloc], 1);
-                    if (p < CAP) {
-                        s_cd[qloc][p] = da;
-                        s_ci[qloc][p] = (uint32_t)gid;
+                const int64_t gid = t0 + rb * 16 + gk * 4 + r;
+                const float dot = cur[r];
+                da[r] = metric == M_NORMCOS ? 1.0f - dot : (xn[r] + qn) - 2.0f * dot;
+                pass[r] = gid < r1 && da[r] < tau;
+                any = any || pass[r];
+            }
+            if (__ballot(any)) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    if (pass[r]) {
+                        const int p = atomicAdd(&s_cnt[qloc], 1);
+                        if (p < CAP) {
+                            s_cd[qloc][p] = da[r];
+                            s_ci[qloc][p] = (uint32_t)(t0 + rb * 16 + gk * 4 + r);
+                        }
                     }
                 }
+                // counts only change where a candidate was added; each query gains at
+                // most 16 per row block, so a count at or below CAP - 16 needs no
+                // compaction yet
+                __builtin_amdgcn_wave_barrier();
+                asm volatile("" ::: "memory");
+                const int cq = lane < 16 ? s_cnt[16 * w + lane] : 0;
+                uint64_t need = __ballot(lane < 16 && cq > CAP - 16);
+                while (need) {
+                    const int qq = __ffsll((unsigned long long)need) - 1;
+                    need &= need - 1;
+                    const float nt = compact<KC>(s_cd[16 * w + qq], s_ci[16 * w + qq], &s_cnt[16 * w + qq], lane);
+                    if (col == qq) tau = nt;
+                }
             }
-            __builtin_amdgcn_wave_barrier();
-            asm volatile("" ::: "memory");
-            const int cq = lane < 16 ? s_cnt[16 * w + lane] : 0;
-            uint64_t need = __ballot(lane < 16 && cq > CAP - 16);
-            while (need) {
-                const int qq = __ffsll((unsigned long long)need) - 1;
-                need &= need - 1;
-                const float nt = compact<KC>(s_cd[16 * w + qq], s_ci[16 * w + qq], &s_cnt[16 * w + qq], lane);
-                if (col == qq) tau = nt;
-            }
+            cur = nxt;
         }
     }
     __syncthreads();
@@ -475,15 +538,40 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ X
     }
 }
 
-int32_t exhaustive_slices(int64_t n, int64_t m, int num_cu)
+// Stage-1 blocks resident per CU: LDS-bound (row tile + candidate lists), at most two
+// by registers.
+static int stage1_blocks_per_cu(int dp)
+{
+    const int lds = TR * (dp + 4) * 4 + TR * 4 + EXH_QT * CAP * 8 + EXH_QT * 8;
+    const int by_lds = (160 * 1024) / lds;
+    return by_lds < 1 ? 1 : (by_lds > 2 ? 2 : by_lds);
+}
+
+// DB slices per query tile: enough blocks to fill the chip, and among the admissible
+// counts the one whose last round of resident blocks is fullest (every block of a launch
+// does the same work, so a partly filled last round idles the rest of the chip).
+int32_t exhaustive_slices(int64_t n, int64_t m, int num_cu, int dp)
 {
     const int64_t qtiles = (m + EXH_QT - 1) / EXH_QT;
-    int64_t s = (4LL * num_cu + qtiles - 1) / qtiles;
-    const int64_t max_by_rows = (n + 4 * TR - 1) / (4 * TR);
-    if (s > max_by_rows) s = max_by_rows;
-    if (s < 1) s = 1;
-    if (s > 2048 / EXH_KC) s = 2048 / EXH_KC;
-    return (int32_t)s;
+    const int64_t cap = (int64_t)num_cu * stage1_blocks_per_cu(dp);
+    int64_t smax = (n + 4 * TR - 1) / (4 * TR);  // at least four row tiles per slice
+    if (smax > 2048 / EXH_KC) smax = 2048 / EXH_KC;
+    if (smax < 1) smax = 1;
+    int64_t smin = (cap + qtiles - 1) / qtiles;
+    if (smin > smax) smin = smax;
+    if (smin < 1) smin = 1;
+    int64_t best = smin;
+    double best_fill = -1.0;
+    for (int64_t s = smin; s <= smax && s <= 4 * smin; s++) {
+        const int64_t blocks = qtiles * s;
+        const int64_t rounds = (blocks + cap - 1) / cap;
+        const double fill = (double)blocks / (double)(rounds * cap);
+        if (fill > best_fill + 1e-9) {
+            best_fill = fill;
+            best = s;
+        }
+    }
+    return (int32_t)best;
 }
 
 template <int DP>

@@ -104,9 +104,9 @@ hipError_t exh_launch_stage2(const ExhArgs& a, const ExhWork& w, float xnmax, in
 hipError_t exh_launch_rescan(const ExhArgs& a, const ExhWork& w, int f0, int nf, uint32_t* part_i,
                              float* part_d, int nparts, uint32_t* ids, float* dists, int kstride,
                              hipStream_t s);
-int32_t exhaustive_slices(int64_t n, int64_t m, int num_cu);
+int32_t exhaustive_slices(int64_t n, int64_t m, int num_cu, int dp);
 constexpr int EXH_KC = 32;        // candidates kept per (query, slice)
-constexpr int EXH_QT = 64;        // queries per block
+constexpr int EXH_QT = 64;        // queries per block of exhaustive stage 1
 
 hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float* Q, int32_t qdim,
                          int64_t m, int32_t k, uint32_t* ids, float* dists, hipStream_t s);
