@@ -14,8 +14,8 @@
 //   * res (k <= 64) and the beam (<= 256 slots) live in registers (RegQueue).
 //   * Neighbour lists are expanded 64 ids at a time: one coalesced id load, one
 //     visited-byte load per id, then the unvisited ids are compacted in LDS and
-//     their vectors gathered in batches (one child per 16-lane group per round;
-//     8 rounds = 32 children for FP32 rows, 16 rounds = 64 for BF16 rows), with
+//     their vectors gathered in batches (one child per 16-lane group per round,
+//     8 rounds = 32 children), with
 //     the child degrees loaded in the same round trip.  Every row index of a
 //     batch is clamped into the batch, so all its loads are unconditional and in
 //     flight together before the first one is consumed.
@@ -68,7 +68,9 @@ __device__ __forceinline__ void vis_put(uint8_t* p, uint8_t v)
 // arithmetic during the search, then an exact (canonical) rerank of the result list.
 template <int V, int NB, int PREC>
 struct Searcher {
-    static constexpr int ROUNDS = PREC == 1 ? 16 : 8;  // gather rounds per batch
+    // gather rounds per batch: 8 keeps the BF16 kernel at 4 waves per SIMD (more waves in
+    // flight beat fewer, larger batches for this latency-bound loop)
+    static constexpr int ROUNDS = 8;
     static constexpr int BATCH = 4 * ROUNDS;           // children per batch
     const BeamArgs& a;
     float qv[V];

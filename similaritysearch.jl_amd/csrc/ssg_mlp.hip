@@ -28,62 +28,86 @@ typedef float floatx4v __attribute__((ext_vector_type(4)));
 constexpr int GK = 32;       // k per LDS stage
 constexpr int GKP = GK + 4;  // padded LDS row (halves): fragment reads hit distinct banks
 
-// Stage a tile of GT rows x GK k of operand P (element (r, k) at P[r * sr + k * sk]) into
-// LDS as S[r * GKP + k].  Contiguous k (sk == 1) or contiguous rows (sr == 1) with 16-byte
-// alignment use 8-half vector loads; anything else, and the matrix edges, load per element.
+// A tile of GT rows x GK k of operand P (element (r, k) at P[r * sr + k * sk]) goes to
+// LDS as S[r * GKP + k] in two steps, so the next tile can be loaded into registers while
+// the current one is multiplied.  Each thread handles 8-half chunks: along k (mode 1,
+// sk == 1) or along rows (mode 2, sr == 1) with 16-byte aligned vector loads; any other
+// layout (mode 0) and the matrix edges load element by element.
 template <int GT>
-__device__ __forceinline__ void stage_tile(_Float16* __restrict__ S, const _Float16* __restrict__ P,
-                                           int64_t sr, int64_t sk, int r0, int R, int k0, int K,
-                                           int mode, int tid)
+struct TileRegs {
+    static constexpr int CH = GT * GK / 8;          // 8-half chunks per tile
+    static constexpr int N = (CH + 255) / 256;      // chunks per thread
+    uint4 v[N];
+};
+
+template <int GT>
+__device__ __forceinline__ void fetch_tile(TileRegs<GT>& t, const _Float16* __restrict__ P, int64_t sr,
+                                           int64_t sk, int r0, int R, int k0, int K, int mode, int tid)
 {
-    constexpr int CH = GT * GK / 8;  // 8-half chunks per tile
 #pragma unroll
-    for (int i = 0; i < (CH + 255) / 256; i++) {
+    for (int i = 0; i < TileRegs<GT>::N; i++) {
         const int c = tid + 256 * i;
-        if (c >= CH) break;
-        if (mode == 1) {  // k contiguous
+        _Float16 v[8];
+        if (c >= TileRegs<GT>::CH) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = (_Float16)0.f;
+        } else if (mode == 1) {
             const int r = c / (GK / 8), kc = (c % (GK / 8)) * 8;
             const int gr = r0 + r, gk = k0 + kc;
-            _Float16 v[8];
             if (gr < R && gk + 7 < K) {
-                const uint4 w = *reinterpret_cast<const uint4*>(P + gr * sr + gk);
-                *reinterpret_cast<uint4*>(v) = w;
-            } else {
-#pragma unroll
-                for (int t = 0; t < 8; t++) v[t] = (gr < R && gk + t < K) ? P[gr * sr + (gk + t)] : (_Float16)0.f;
+                t.v[i] = *reinterpret_cast<const uint4*>(P + gr * sr + gk);
+                continue;
             }
-            uint2* d = reinterpret_cast<uint2*>(S + r * GKP + kc);
-            d[0] = *reinterpret_cast<const uint2*>(v);
-            d[1] = *reinterpret_cast<const uint2*>(v + 4);
-        } else if (mode == 2) {  // rows contiguous
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = (gr < R && gk + e < K) ? P[gr * sr + (gk + e)] : (_Float16)0.f;
+        } else if (mode == 2) {
             const int k = c / (GT / 8), rc = (c % (GT / 8)) * 8;
             const int gk = k0 + k, gr = r0 + rc;
-            _Float16 v[8];
             if (gk < K && gr + 7 < R) {
-                const uint4 w = *reinterpret_cast<const uint4*>(P + gk * sk + gr);
-                *reinterpret_cast<uint4*>(v) = w;
-            } else {
-#pragma unroll
-                for (int t = 0; t < 8; t++) v[t] = (gk < K && gr + t < R) ? P[(gr + t) * sr + gk * sk] : (_Float16)0.f;
+                t.v[i] = *reinterpret_cast<const uint4*>(P + gk * sk + gr);
+                continue;
             }
 #pragma unroll
-            for (int t = 0; t < 8; t++) S[(rc + t) * GKP + k] = v[t];
+            for (int e = 0; e < 8; e++) v[e] = (gk < K && gr + e < R) ? P[(gr + e) * sr + gk * sk] : (_Float16)0.f;
         } else {
             const int r = c / (GK / 8), kc = (c % (GK / 8)) * 8;
 #pragma unroll
-            for (int t = 0; t < 8; t++) {
-                const int gr = r0 + r, gk = k0 + kc + t;
-                S[r * GKP + kc + t] = (gr < R && gk < K) ? P[gr * sr + gk * sk] : (_Float16)0.f;
+            for (int e = 0; e < 8; e++) {
+                const int gr = r0 + r, gk = k0 + kc + e;
+                v[e] = (gr < R && gk < K) ? P[gr * sr + gk * sk] : (_Float16)0.f;
             }
+        }
+        t.v[i] = *reinterpret_cast<const uint4*>(v);
+    }
+}
+
+template <int GT>
+__device__ __forceinline__ void store_tile(_Float16* __restrict__ S, const TileRegs<GT>& t, int mode, int tid)
+{
+#pragma unroll
+    for (int i = 0; i < TileRegs<GT>::N; i++) {
+        const int c = tid + 256 * i;
+        if (c >= TileRegs<GT>::CH) break;
+        if (mode == 2) {
+            const int k = c / (GT / 8), rc = (c % (GT / 8)) * 8;
+            const _Float16* v = reinterpret_cast<const _Float16*>(&t.v[i]);
+#pragma unroll
+            for (int e = 0; e < 8; e++) S[(rc + e) * GKP + k] = v[e];
+        } else {
+            const int r = c / (GK / 8), kc = (c % (GK / 8)) * 8;
+            uint2* d = reinterpret_cast<uint2*>(S + r * GKP + kc);
+            d[0] = make_uint2(t.v[i].x, t.v[i].y);
+            d[1] = make_uint2(t.v[i].z, t.v[i].w);
         }
     }
 }
 
 // C[M x N] (FP32) = A[M x K] . B[K x N] with A(m, k) = A[m * sam + k * sak] and
 // B(k, n) = B[k * sbk + n * sbn]; the epilogue adds bias[n] when given and writes an FP16
-// copy (ReLU'd when asked) with the same row stride when out16 is given.  A block of 4 waves owns a GT x GT tile, each wave a
-// (GT/2) x (GT/2) sub-tile of v_mfma_f32_16x16x16_f16 accumulators; k advances GK at a
-// time through LDS.  amode / bmode: 1 = k contiguous, 2 = rows contiguous, 0 = general.
+// copy (ReLU'd when asked) with the same row stride when out16 is given.  A block of 4
+// waves owns a GT x GT tile, each wave a (GT/2) x (GT/2) sub-tile of
+// v_mfma_f32_16x16x16_f16 accumulators; k advances GK at a time through LDS, the next
+// stage loaded into registers while the current one is multiplied.  amode / bmode: 1 = k contiguous, 2 = rows contiguous, 0 = general.
 template <int GT>
 __global__ __launch_bounds__(256) void gemm_f16_kernel(int M, int N, int K, const _Float16* A,
                                                        int64_t sam, int64_t sak, const _Float16* B,
@@ -103,10 +127,18 @@ __global__ __launch_bounds__(256) void gemm_f16_kernel(int M, int N, int K, cons
     for (int a = 0; a < NT; a++)
 #pragma unroll
         for (int b = 0; b < NT; b++) acc[a][b] = floatx4v{0.f, 0.f, 0.f, 0.f};
+    TileRegs<GT> ra, rb;
+    fetch_tile<GT>(ra, A, sam, sak, bm, M, 0, K, amode, tid);
+    fetch_tile<GT>(rb, B, sbn, sbk, bn, N, 0, K, bmode, tid);
     for (int k0 = 0; k0 < K; k0 += GK) {
-        stage_tile<GT>(As, A, sam, sak, bm, M, k0, K, amode, tid);
-        stage_tile<GT>(Bs, B, sbn, sbk, bn, N, k0, K, bmode, tid);
+        if (k0) __syncthreads();  // every wave is done with the previous stage
+        store_tile<GT>(As, ra, amode, tid);
+        store_tile<GT>(Bs, rb, bmode, tid);
         __syncthreads();
+        if (k0 + GK < K) {  // next stage in flight while this one is multiplied
+            fetch_tile<GT>(ra, A, sam, sak, bm, M, k0 + GK, K, amode, tid);
+            fetch_tile<GT>(rb, B, sbn, sbk, bn, N, k0 + GK, K, bmode, tid);
+        }
 #pragma unroll
         for (int kk = 0; kk < GK; kk += 16) {
             half4 fa[NT], fb[NT];
@@ -122,7 +154,6 @@ __global__ __launch_bounds__(256) void gemm_f16_kernel(int M, int N, int K, cons
                 for (int b = 0; b < NT; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16f16(fa[a], fb[b], acc[a][b], 0, 0, 0);
         }
-        __syncthreads();
     }
 #pragma unroll
     for (int a = 0; a < NT; a++)
@@ -805,7 +836,7 @@ int ssg_mlp_infer_dev(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dl
     HIPCHK(hipSetDevice(p->device), "hipSetDevice");
     hipStream_t saved = p->stream;
     if (stream) p->stream = (hipStream_t)stream;
-    const int chunk = 8192;
+    const int chunk = 32768;  // rows per inference pass
     int rc = SSG_OK;
     for (int64_t r0 = 0; r0 < m && rc == SSG_OK; r0 += chunk) {
         const int B = (int)std::min<int64_t>(chunk, m - r0);
