@@ -255,6 +255,13 @@ def main():
     st.synchronize()
     step_qps = m / (time.perf_counter() - t)
 
+    # HBM traffic of this launch from the committed counter measurement of the same kernel and
+    # workload (FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB copy), per query
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "beam_search_learned_bf16_1m128_traffic.json")
+    if os.path.exists(tfile) and args.n == 1_000_000:
+        with open(tfile) as fh:
+            traffic = int(json.load(fh)["hbm_bytes_per_query"] * m)
     deg = len(G.adj.end_point) / args.n
     # algorithmic bytes per query (BF16 form): E BF16 rows (dim*2) + E visited bytes +
     # B expansions x (list of deg ids and deg {start, degree} entries: 12 B per entry) +
@@ -327,7 +334,8 @@ def main():
             "parallel_block": args.parallel_block, "graph_nnz": int(len(G.adj.end_point)),
             "hints": int(len(G.hints)), "build_s": round(build_s, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/beam_search_learned_bf16_1m128_traffic.md" if traffic else None,
                      "kernel": f"beam_bf16_kernel<8,{1 if cap <= 64 else 2}>",
                      "kernel_ms": round(sms, 3), "algorithmic_bytes_per_launch": int(lbytes)},
         "cpu_baseline": cpu,
