@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--learned", type=str, default="",
                     help="quantile,rerank,cap,delta (skip tuning)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("search", "allknn-bf16"), default="search",
+                    help="search: the headline graph-search metric (default); allknn-bf16: "
+                         "exhaustive allknn with the BF16 candidate pass and FP32 rerank")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the query-sharded "
@@ -65,6 +68,8 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
+    if args.workload == "allknn-bf16":
+        return allknn_bf16(args)
 
     import numpy as np
     import similaritysearch as ss
@@ -338,6 +343,122 @@ def main():
                      "traffic_source": "profiles/beam_search_learned_bf16_1m128_traffic.md" if traffic else None,
                      "kernel": f"beam_bf16_kernel<8,{1 if cap <= 64 else 2}>",
                      "kernel_ms": round(sms, 3), "algorithmic_bytes_per_launch": int(lbytes)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def allknn_bf16(args):
+    """allknn (k = 32) of 1,000,000 unit vectors (96 dims, N(0, I) seed 6, L2-normalised),
+    NormCosine, with the precision form: BF16 candidate pass on the matrix cores keeping
+    128 candidates per object, then rerank! with the FP32 canonical distance.  A step is
+    one whole allknn; `value` is objects (queries) per second.  `roofline`: the candidate
+    pass is MFMA-bound at 2 * 96 * n^2 flops per allknn, priced against the BF16 dense
+    peak over the time of the whole call (rerank included, so the fraction is a floor).
+    Recall and errors are measured on 10,000 evenly spaced objects against the exact FP32
+    exhaustive search (recall) and FP64 (errors)."""
+    import numpy as np
+    import similaritysearch as ss
+    from similaritysearch import datasets, _lib
+    from similaritysearch.device import DeviceArray, Event, Stream
+
+    L = _lib.load()
+    n = args.n
+    dim, k, ncand = 96, 32, 128
+    t0 = time.time()
+    X = datasets.unit(n, dim, seed=6)
+    log(f"data in {time.time() - t0:.1f}s")
+    E = ss.ExhaustiveSearch(ss.NormCosine, ss.MatrixDatabase.from_rows(X))
+    h = E._handle()
+    st = Stream()
+    sp = st.handle
+    dI = DeviceArray((n, k), np.uint32)
+    dD = DeviceArray((n, k), np.float32)
+    dCI = DeviceArray((n, ncand), np.uint32)
+    dCD = DeviceArray((n, ncand), np.float32)
+
+    def run(cands):
+        _lib.check(L.ssg_allknn_exhaustive_prec_dev(h, k, 1, ncand, dI.ptr, dD.ptr,
+                                                     dCI.ptr if cands else None,
+                                                     dCD.ptr if cands else None, sp), "allknn")
+
+    for _ in range(max(1, args.warmup)):
+        run(False)
+    st.synchronize()
+    e0, e1 = Event(), Event()
+    e0.record(st)
+    for _ in range(args.steps):
+        run(False)
+    e1.record(st)
+    st.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    log(f"allknn-bf16: {ms:.2f} ms per allknn of {n}")
+    run(True)
+    st.synchronize()
+    ids, dists = dI.numpy(), dD.numpy()
+    cids, cdists = dCI.numpy(), dCD.numpy()
+
+    # recall and errors on 10,000 evenly spaced objects
+    sub = np.linspace(0, n - 1, min(n, 10_000)).astype(np.int64)
+    gi = np.zeros((len(sub), k), np.uint32)
+    gd = np.zeros((len(sub), k), np.float32)
+    Qs = np.ascontiguousarray(X[sub])
+    _lib.check(L.ssg_exhaustive_knn(h, Qs.ctypes.data, len(sub), k, gi.ctypes.data, gd.ctypes.data),
+               "exhaustive")
+    rec = float((np.sort(ids[sub], 1)[:, :, None] == np.sort(gi, 1)[:, None, :]).any(2).sum() / gi.size)
+    X64 = X.astype(np.float64)
+    d64 = 1.0 - np.einsum("qd,qkd->qk", X64[sub], X64[ids[sub].astype(np.int64) - 1])
+    err_final = np.abs(dists[sub].astype(np.float64) - d64)
+    c64 = 1.0 - np.einsum("qd,qkd->qk", X64[sub], X64[cids[sub].astype(np.int64) - 1])
+    err_cand = np.abs(cdists[sub].astype(np.float64) - c64)
+    log(f"recall@{k} {rec:.5f}; final |err| max {err_final.max():.3e} mean {err_final.mean():.3e}; "
+        f"candidate |err| max {err_cand.max():.3e} mean {err_cand.mean():.3e}")
+
+    flops = 2.0 * dim * float(n) * float(n)
+    tflops = flops / (ms * 1e-3) / 1e12
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        pyoracle.exhaustive("NormCosine", X, Qs[:16], k, nthreads=threads)
+        nq, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and nq < len(sub):
+            pyoracle.exhaustive("NormCosine", X, Qs[nq:nq + 64], k, nthreads=threads)
+            nq += 64
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(nq / cpu_s, 2), "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": f"restated ExhaustiveSearch (FP32 canonical distance, ascending scan, "
+                         f"KnnSorted) of {nq} of the objects against all {n}, OpenMP over queries; "
+                         f"a whole allknn would take {n / (nq / cpu_s) / 60:.0f} min"}
+
+    out = {
+        "metric": "allknn objects/s, 1M×96 NormCosine k=32, BF16 candidate pass (128 per object) + FP32 rerank",
+        "value": round(n / (ms * 1e-3), 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic: N(0, I) rows, seed 6, L2-normalised",
+        "config": {"workload": f"allknn over ExhaustiveSearch, {n}x{dim} NormCosine, k={k}, "
+                               f"{ncand} BF16 candidates per object reranked in FP32",
+                   f"recall@{k}_vs_fp32_exact": round(rec, 5),
+                   "final_abs_err_vs_fp64": {"max": float(err_final.max()), "mean": float(err_final.mean())},
+                   "candidate_abs_err_vs_fp64": {"max": float(err_cand.max()), "mean": float(err_cand.mean())},
+                   "error_sample": len(sub)},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
+                     "traffic": None, "kernel": "prec_stage1<3> + prec_stage2<6> (whole call)",
+                     "kernel_ms": round(ms, 3), "flops_per_launch": flops},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

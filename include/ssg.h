@@ -187,7 +187,9 @@ int ssg_graph_destroy(ssg_graph* graph);
  * vectors renumbered breadth-first from the hints, each list entry carrying its target's
  * {start, degree}; results, ids and cost counters are unchanged, memory traffic is
  * lower.  0 searches the installed layout directly. */
-enum { SSG_OPT_QUERY_LAYOUT = 1 };
+/* SSG_OPT_PREC_SLICES: number of DB slices of the precision form's candidate pass (0, the
+ * default, chooses it from the device size); results do not depend on it. */
+enum { SSG_OPT_QUERY_LAYOUT = 1, SSG_OPT_PREC_SLICES = 2 };
 int ssg_set_option(ssg_handle* h, int32_t option, int64_t value);
 
 /* Longest-first order for a heterogeneous batch (no reference counterpart): dorder[0..m)
@@ -202,6 +204,31 @@ int ssg_allknn_graph(ssg_handle* h, int32_t k, const ssg_beam_params* bp, uint32
                      float* dists);
 /* allknn!(::ExhaustiveSearch, ...) (src/allknn.jl:66-91,108-110): exact, self included. */
 int ssg_allknn_exhaustive(ssg_handle* h, int32_t k, uint32_t* ids, float* dists);
+
+/* Precision form of searchbatch!/allknn! over ExhaustiveSearch (no reference counterpart;
+ * the reference computes Float32 distances only).  It is allknn/searchbatch with a
+ * candidate queue of ncand entries followed by rerank! (src/rerank.jl:22-98):
+ *   1. a candidate pass with SSG_BF16 operands on the matrix cores (FP32 accumulation)
+ *      keeps, per query, the ncand objects with the best approximate distance (ties to
+ *      the lower id);
+ *   2. those are rescored with the canonical FP32 distance and the k first in
+ *      (distance, id) order are returned, (k, m) column-major, 1-based ids.
+ * The returned distances are exact; an object outside the ncand best approximate
+ * candidates can be missed (recall < 1).  k <= ncand <= 256, dim <= 128.  cand_ids /
+ * cand_dists (nullable together, (ncand, m)) receive every candidate in (distance, id)
+ * order with its approximate distance (BF16 operands, FP32 accumulation).  The allknn
+ * forms take every database object as a query, self included (src/allknn.jl:66-91). */
+int ssg_exhaustive_knn_prec(ssg_handle* h, const float* Q, int64_t m, int32_t k, int32_t precision,
+                            int32_t ncand, uint32_t* ids, float* dists, uint32_t* cand_ids,
+                            float* cand_dists);
+int ssg_exhaustive_knn_prec_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k,
+                                int32_t precision, int32_t ncand, uint32_t* dids, float* ddists,
+                                uint32_t* dcand_ids, float* dcand_dists, void* stream);
+int ssg_allknn_exhaustive_prec(ssg_handle* h, int32_t k, int32_t precision, int32_t ncand,
+                               uint32_t* ids, float* dists, uint32_t* cand_ids, float* cand_dists);
+int ssg_allknn_exhaustive_prec_dev(ssg_handle* h, int32_t k, int32_t precision, int32_t ncand,
+                                   uint32_t* dids, float* ddists, uint32_t* dcand_ids,
+                                   float* dcand_dists, void* stream);
 
 /* fft(dist, X, k; start) over the uploaded database (src/kcenters/fft.jl:53-87): farthest-first
  * k-centers.  centers[k] (1-based), nn[n] (1-based nearest center, nullable), nndists[n]

@@ -262,3 +262,54 @@ def optimize_minrecall(metric, X, offs, adj, hints, Q, ksearch, minrecall, cur_m
                                     Q.shape[0], ksearch, minrecall, cur_maxvisits, beamcap,
                                     C.byref(bs), C.byref(dl), C.byref(mv), nthreads)
     return dict(bsize=bs.value, delta=dl.value, maxvisits=mv.value, recall=r)
+
+
+# --------------------------------------------------------------------------- precision form
+# The precision form of exhaustive k-NN (a candidate pass on BF16 operands, then rerank!
+# with the canonical distance; src/rerank.jl:22-98 for the second step) has no reference
+# counterpart for its first step.  Its approximate distance is modelled here exactly in
+# FP64 from the BF16-rounded operands; the GPU accumulates the same products in FP32,
+# which the tests bound with prec_error_bound.
+
+def bf16_round(x):
+    """Float32 -> nearest BF16 value (round to nearest, ties to even), returned as float32.
+    Pinned against torch's bfloat16 conversion and hand cases in tests/test_oracle.py."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return (r & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+
+
+def prec_approx(metric, X, Q):
+    """(m, n) FP64 approximate distances of the BF16 candidate pass: s = sum of the exact
+    products of the BF16-rounded coordinates; NormCosine 1 - s, SqL2 |q|^2 + |x|^2 - 2 s
+    (norms of the FP32 vectors), L2 its square root."""
+    mt = METRICS.get(metric, metric)
+    Xb = bf16_round(X).astype(np.float64)
+    Qb = bf16_round(Q).astype(np.float64)
+    s = Qb @ Xb.T
+    if mt == NORMCOS:
+        return 1.0 - s
+    xn = (np.asarray(X, np.float64) ** 2).sum(1)
+    qn = (np.asarray(Q, np.float64) ** 2).sum(1)
+    d = qn[:, None] + xn[None, :] - 2.0 * s
+    return np.sqrt(np.maximum(d, 0.0)) if mt == L2 else d
+
+
+def prec_error_bound(metric, X, Q):
+    """(m, n) bound on |GPU approximate distance - prec_approx| for the SqL2/NormCosine
+    forms: FP32 accumulation of the dim exact BF16 products plus the -|x|^2/2 start value
+    in any order ((dim + 2) units of 2^-24 of the absolute sum), FP32 norms of dim terms,
+    and the final FP32 roundings."""
+    mt = METRICS.get(metric, metric)
+    u = 2.0 ** -24
+    dim = np.asarray(X).shape[1]
+    Xb = np.abs(bf16_round(X).astype(np.float64))
+    Qb = np.abs(bf16_round(Q).astype(np.float64))
+    a = Qb @ Xb.T
+    if mt == NORMCOS:
+        return (dim + 2) * u * a + 2 * u
+    xn = (np.asarray(X, np.float64) ** 2).sum(1)
+    qn = (np.asarray(Q, np.float64) ** 2).sum(1)
+    acc = (dim + 2) * u * (2.0 * a + xn[None, :])
+    norms = (dim + 1) * u * (xn[None, :] + qn[:, None])
+    return acc + norms + 4 * u * (qn[:, None] + xn[None, :] + 2.0 * a)

@@ -318,6 +318,95 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
     return SSG_OK;
 }
 
+// Precision form of exhaustive k-NN over the whole database: queries are dQ (m rows of
+// h->dim floats), or the database itself (allknn, dQ == nullptr).
+static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, int32_t kstride,
+                        int32_t precision, int32_t ncand, uint32_t* dids, float* ddists,
+                        uint32_t* dcand_i, float* dcand_d, hipStream_t s)
+{
+    if (!h->X.p) return ssg_fail(SSG_E_STATE, "no database uploaded");
+    if (m <= 0) return ssg_fail(SSG_E_ARG, "empty set of queries");
+    if (precision != SSG_BF16) return ssg_fail(SSG_E_ARG, "the precision form takes SSG_BF16");
+    if (k < 1 || kstride < k) return ssg_fail(SSG_E_ARG, "k must satisfy 1 <= k <= kstride");
+    if (ncand < k || ncand > PREC_NCAND_MAX)
+        return ssg_fail(SSG_E_LIMIT, "ncand must satisfy k <= ncand <= 256");
+    if (h->dp > 128) return ssg_fail(SSG_E_LIMIT, "the precision form needs dim <= 128");
+    if ((dcand_i == nullptr) != (dcand_d == nullptr))
+        return ssg_fail(SSG_E_ARG, "candidate ids and distances go together");
+    const int64_t n = h->n;
+    if (kstride > k || n < k) {
+        HIPCHK(launch_fill_f32(ddists, m * kstride, __builtin_inff(), s), "fill");
+        HIPCHK(launch_zero_u32(dids, m * kstride, s), "fill");
+    }
+    if (n == 0) {
+        if (dcand_i) {
+            HIPCHK(launch_fill_f32(dcand_d, m * ncand, __builtin_inff(), s), "fill");
+            HIPCHK(launch_zero_u32(dcand_i, m * ncand, s), "fill");
+        }
+        return SSG_OK;
+    }
+    const int32_t dpb = (h->dp + 31) / 32 * 32;
+    const int64_t m_pad = (m + PREC_QB - 1) / PREC_QB * PREC_QB;
+    const int64_t xe_rows = std::max<int64_t>(h->n_pad, (n + PREC_QB - 1) / PREC_QB * PREC_QB);
+    if (!h->Xe.p) {
+        // rows past n_pad are zero (the FP32 matrix holds n_pad rows)
+        HIPCHK(h->Xe.ensure((size_t)xe_rows * dpb * 2), "alloc BF16 rows");
+        HIPCHK(launch_zero_u32(h->Xe.as<unsigned int>(), xe_rows * dpb / 2, s), "clear BF16 rows");
+        HIPCHK(launch_rows_to_bf16(h->X.as<float>(), n, h->dp, h->Xe.as<uint16_t>(), dpb, s), "BF16 rows");
+    }
+    PrecOut o{};
+    PrecArgs a{};
+    if (dQ) {
+        HIPCHK(h->qpad.ensure((size_t)m_pad * h->dp * sizeof(float)), "alloc qpad");
+        HIPCHK(h->qnorm.ensure((size_t)m_pad * sizeof(float)), "alloc qnorm");
+        HIPCHK(h->pQe.ensure((size_t)m_pad * dpb * 2), "alloc BF16 queries");
+        float* qpad = h->qpad.as<float>();
+        HIPCHK(launch_fill_f32(qpad + m * h->dp, (m_pad - m) * h->dp, 0.f, s), "fill qpad");
+        HIPCHK(launch_fill_f32(h->qnorm.as<float>() + m, m_pad - m, 0.f, s), "fill qnorm");
+        HIPCHK(launch_pad_rows(dQ, m, h->dim, qpad, h->dp, h->qnorm.as<float>(), s), "pad queries");
+        HIPCHK(launch_rows_to_bf16(qpad, m_pad, h->dp, h->pQe.as<uint16_t>(), dpb, s), "BF16 queries");
+        a.Qe = h->pQe.as<uint16_t>();
+        o.qpad = qpad;
+        o.qnorm = h->qnorm.as<float>();
+    } else {
+        a.Qe = h->Xe.as<uint16_t>();  // xe_rows >= m_pad
+        o.qpad = h->X.as<float>();
+        o.qnorm = h->xnorm.as<float>();
+    }
+    const int32_t kc = ncand, cap = 2 * ncand + 128;
+    int32_t ns = h->prec_slices > 0 ? h->prec_slices : prec_slices(n, m, h->num_cu, kc);
+    ns = std::max(1, std::min<int32_t>(ns, 2048 / kc));
+    int64_t rps = (n + ns - 1) / ns;
+    rps = (rps + 127) / 128 * 128;
+    ns = (int32_t)((n + rps - 1) / rps);
+    HIPCHK(h->pList.ensure((size_t)m_pad * ns * cap * sizeof(uint2)), "alloc candidate lists");
+    HIPCHK(h->pListN.ensure((size_t)m_pad * ns * sizeof(int32_t)), "alloc candidate counts");
+    a.Xe = h->Xe.as<uint16_t>();
+    a.xnorm = h->xnorm.as<float>();
+    a.X = h->X.as<float>();
+    a.n = n;
+    a.dpb = dpb;
+    a.metric = h->metric;
+    a.m_pad = m_pad;
+    a.kc = kc;
+    a.cap = cap;
+    a.rows_per_slice = rps;
+    a.nslices = ns;
+    a.list = h->pList.as<uint2>();
+    a.list_n = h->pListN.as<int32_t>();
+    o.m = m;
+    o.k = k;
+    o.kstride = kstride;
+    o.ncand = ncand;
+    o.ids = dids;
+    o.dists = ddists;
+    o.cand_ids = dcand_i;
+    o.cand_dists = dcand_d;
+    HIPCHK(prec_launch_stage1(a, s), "precision candidate pass");
+    HIPCHK(prec_launch_stage2(a, o, h->dp, s), "precision rerank");
+    return SSG_OK;
+}
+
 // Frozen lists -> node table + packed pool on the device.
 int ssg_install_lists(ssg_handle* h)
 {
@@ -516,6 +605,7 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     h->lists.clear();
     h->hints_host.clear();
     h->Xh.release();
+    h->Xe.release();
     ssg_query_layout_stale(h);
     const int64_t np = std::max<int64_t>(h->n_pad, 64);
     HIPCHK(h->X.ensure((size_t)np * dp * sizeof(float)), "alloc X");
@@ -737,6 +827,11 @@ int ssg_set_option(ssg_handle* h, int32_t option, int64_t value)
         ssg_query_layout_stale(h);
         return SSG_OK;
     }
+    if (option == SSG_OPT_PREC_SLICES) {
+        if (value < 0 || value > 2048) return ssg_fail(SSG_E_ARG, "slice count must be in 0..2048");
+        h->prec_slices = (int32_t)value;
+        return SSG_OK;
+    }
     return ssg_fail(SSG_E_ARG, "unknown option");
 }
 
@@ -847,6 +942,96 @@ int ssg_allknn_graph(ssg_handle* h, int32_t k, const ssg_beam_params* bp, uint32
     HIPCHK(hipMemcpyAsync(dists, h->sD.p, (size_t)n * k * 4, hipMemcpyDeviceToHost, s), "copy dists");
     HIPCHK(hipStreamSynchronize(s), "sync");
     return SSG_OK;
+}
+
+int ssg_exhaustive_knn_prec_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k,
+                                int32_t precision, int32_t ncand, uint32_t* dids, float* ddists,
+                                uint32_t* dcand_ids, float* dcand_dists, void* stream)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    if (!dQ || !dids || !ddists) return ssg_fail(SSG_E_ARG, "null buffer");
+    return ssg_prec_run(h, dQ, m, k, k, precision, ncand, dids, ddists, dcand_ids, dcand_dists,
+                        pick(h, stream));
+}
+
+int ssg_allknn_exhaustive_prec_dev(ssg_handle* h, int32_t k, int32_t precision, int32_t ncand,
+                                   uint32_t* dids, float* ddists, uint32_t* dcand_ids,
+                                   float* dcand_dists, void* stream)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    if (!dids || !ddists) return ssg_fail(SSG_E_ARG, "null buffer");
+    if (h->n <= 0) return ssg_fail(SSG_E_ARG, "allknn needs a non-empty database");
+    if (k < 1 || k > h->n) return ssg_fail(SSG_E_ARG, "allknn needs 0 < k <= n");
+    return ssg_prec_run(h, nullptr, h->n, k, k, precision, ncand, dids, ddists, dcand_ids,
+                        dcand_dists, pick(h, stream));
+}
+
+// Host-buffer forms: Q (m rows) or the database itself when Q is null.
+static int prec_host(ssg_handle* h, const float* Q, int64_t m, int32_t k, int32_t precision,
+                     int32_t ncand, uint32_t* ids, float* dists, uint32_t* cand_ids,
+                     float* cand_dists)
+{
+    if (!ids || !dists) return ssg_fail(SSG_E_ARG, "null buffer");
+    if ((cand_ids == nullptr) != (cand_dists == nullptr))
+        return ssg_fail(SSG_E_ARG, "candidate ids and distances go together");
+    if (m <= 0) return ssg_fail(SSG_E_ARG, "empty set of queries");
+    if (k < 1) return ssg_fail(SSG_E_ARG, "k must be >= 1");
+    if (ncand < k || ncand > PREC_NCAND_MAX)
+        return ssg_fail(SSG_E_LIMIT, "ncand must satisfy k <= ncand <= 256");
+    hipStream_t s = h->stream;
+    const float* dQ = nullptr;
+    if (Q) {
+        HIPCHK(h->sQ.ensure((size_t)m * h->dim * sizeof(float)), "alloc");
+        HIPCHK(hipMemcpyAsync(h->sQ.p, Q, (size_t)m * h->dim * sizeof(float), hipMemcpyHostToDevice, s),
+               "copy Q");
+        dQ = h->sQ.as<float>();
+    }
+    HIPCHK(h->sIds.ensure((size_t)m * k * sizeof(uint32_t)), "alloc");
+    HIPCHK(h->sD.ensure((size_t)m * k * sizeof(float)), "alloc");
+    if (cand_ids) {
+        HIPCHK(h->sCd.ensure((size_t)m * ncand * sizeof(uint32_t)), "alloc");
+        HIPCHK(h->sCb.ensure((size_t)m * ncand * sizeof(float)), "alloc");
+    }
+    int rc = ssg_prec_run(h, dQ, m, k, k, precision, ncand, h->sIds.as<uint32_t>(), h->sD.as<float>(),
+                          cand_ids ? h->sCd.as<uint32_t>() : nullptr,
+                          cand_ids ? h->sCb.as<float>() : nullptr, s);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ids, h->sIds.p, (size_t)m * k * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "copy ids");
+    HIPCHK(hipMemcpyAsync(dists, h->sD.p, (size_t)m * k * sizeof(float), hipMemcpyDeviceToHost, s), "copy dists");
+    if (cand_ids) {
+        HIPCHK(hipMemcpyAsync(cand_ids, h->sCd.p, (size_t)m * ncand * sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+               "copy candidates");
+        HIPCHK(hipMemcpyAsync(cand_dists, h->sCb.p, (size_t)m * ncand * sizeof(float), hipMemcpyDeviceToHost, s),
+               "copy candidates");
+    }
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    return SSG_OK;
+}
+
+int ssg_exhaustive_knn_prec(ssg_handle* h, const float* Q, int64_t m, int32_t k, int32_t precision,
+                            int32_t ncand, uint32_t* ids, float* dists, uint32_t* cand_ids,
+                            float* cand_dists)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    if (!Q) return ssg_fail(SSG_E_ARG, "null buffer");
+    return prec_host(h, Q, m, k, precision, ncand, ids, dists, cand_ids, cand_dists);
+}
+
+int ssg_allknn_exhaustive_prec(ssg_handle* h, int32_t k, int32_t precision, int32_t ncand,
+                               uint32_t* ids, float* dists, uint32_t* cand_ids, float* cand_dists)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    if (h->n <= 0) return ssg_fail(SSG_E_ARG, "allknn needs a non-empty database");
+    if (k < 1 || k > h->n) return ssg_fail(SSG_E_ARG, "allknn needs 0 < k <= n");
+    return prec_host(h, nullptr, h->n, k, precision, ncand, ids, dists, cand_ids, cand_dists);
 }
 
 int ssg_rerank_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, uint32_t* dids,

@@ -89,6 +89,11 @@ struct ssg_handle {
     // list entries and rows share cache lines and DRAM pages), with each list entry
     // carrying its target's {start, degree}.  Built on the first query search after the
     // graph or database changes; searchbatch! uses it, index!/allknn! do not.
+    // precision form of exhaustive k-NN: BF16 rows of 32 * ceil(dim / 32) elements (made
+    // on first use), BF16 queries, per-(query, slice) candidate lists and their lengths;
+    // prec_slices > 0 forces the number of DB slices
+    DevBuf Xe, pQe, pList, pListN;
+    int32_t prec_slices = 0;
     bool reorder = true;
     bool q_ready = false, q_bf16 = false;
     DevBuf qnode, qadj, qadjn, qperm, qipos, qhints, Xq, Xqh;

@@ -108,6 +108,42 @@ int32_t exhaustive_slices(int64_t n, int64_t m, int num_cu, int dp);
 constexpr int EXH_KC = 32;        // candidates kept per (query, slice)
 constexpr int EXH_QT = 64;        // queries per block of exhaustive stage 1
 
+// Precision form of exhaustive k-NN: BF16 MFMA candidate pass + exact rerank.
+constexpr int PREC_QB = 256;          // queries per candidate-pass block
+constexpr int PREC_NCAND_MAX = 256;   // candidates per query
+constexpr int PREC_CAP_MAX = 640;     // list capacity per (query, slice): 2 * ncand + 128
+struct PrecArgs {
+    const uint16_t* Xe;     // n_pad rows of dpb BF16 (zero padded)
+    const float* xnorm;     // FP32 squared norms of the rows
+    const float* X;         // n_pad rows of dp FP32 (the rerank reads them)
+    int64_t n;
+    int32_t dpb;            // 32 * KB
+    int32_t metric;
+    const uint16_t* Qe;     // m_pad rows of dpb BF16 queries
+    int64_t m_pad;          // multiple of PREC_QB
+    int32_t kc;             // candidates kept per (query, slice)
+    int32_t cap;            // list capacity per (query, slice)
+    int64_t rows_per_slice; // multiple of the 128-row tile
+    int32_t nslices;
+    uint2* list;            // [m_pad][nslices][cap] {score bits, 0-based id}
+    int32_t* list_n;        // [m_pad][nslices]
+};
+struct PrecOut {
+    int64_t m;
+    int32_t k, kstride, ncand;
+    const float* qpad;      // m rows of dp FP32 queries (exact rerank)
+    const float* qnorm;     // FP32 squared query norms (approximate distances)
+    uint32_t* ids;          // (k, m), 1-based
+    float* dists;
+    uint32_t* cand_ids;     // nullable: (ncand, m) candidates in (distance, id) order
+    float* cand_dists;      // their approximate distances
+};
+int32_t prec_slices(int64_t n, int64_t m, int num_cu, int kc);
+hipError_t prec_launch_stage1(const PrecArgs& a, hipStream_t s);
+hipError_t prec_launch_stage2(const PrecArgs& a, const PrecOut& o, int32_t dp, hipStream_t s);
+hipError_t launch_rows_to_bf16(const float* src, int64_t rows, int32_t dp, uint16_t* dst, int32_t dpb,
+                               hipStream_t s);
+
 hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float* Q, int32_t qdim,
                          int64_t m, int32_t k, uint32_t* ids, float* dists, hipStream_t s);
 hipError_t launch_pad_rows(const float* src, int64_t rows, int32_t dim, float* dst, int32_t dp,

@@ -138,10 +138,19 @@ class GenericContext:
     """Context of exhaustive indexes.  costdists holds the per-query distance
     evaluations of the last batch call (src/SimilaritySearch.jl:187-258)."""
 
-    def __init__(self, verbose=False):
+    def __init__(self, verbose=False, precision="f32", ncand=0):
         self.verbose = verbose
         self.costdists = np.zeros(0, np.int64)
         self.costblocks = np.zeros(0, np.int64)
+        if precision not in ("f32", "bf16"):
+            raise ValueError("precision must be 'f32' or 'bf16'")
+        # precision form of the exhaustive indexes (no reference counterpart): a BF16
+        # candidate pass keeping max(ncand, 4k) candidates per query, then rerank!
+        self.precision = precision
+        self.ncand = int(ncand)
+
+    def exhaustive_ncand(self, k):
+        return min(256, max(k, self.ncand if self.ncand > 0 else 4 * k))
 
 
 class SearchGraphContext(GenericContext):
@@ -338,6 +347,12 @@ def searchbatch_(index, ctx, Q, ids, dists, sorted=False):
                                     ids.T.ctypes.data, dists.T.ctypes.data, cd.ctypes.data,
                                     cb.ctypes.data), "searchbatch!")
         ctx.costdists, ctx.costblocks = cd, cb
+    elif getattr(ctx, "precision", "f32") == "bf16":
+        _lib.check(L.ssg_exhaustive_knn_prec(h, rows.ctypes.data, m, k, 1, ctx.exhaustive_ncand(k),
+                                             ids.T.ctypes.data, dists.T.ctypes.data, None, None),
+                   "searchbatch!")
+        ctx.costdists = np.full(m, len(index), np.int64)
+        ctx.costblocks = np.zeros(m, np.int64)
     else:
         _lib.check(L.ssg_exhaustive_knn(h, rows.ctypes.data, m, k, ids.T.ctypes.data,
                                         dists.T.ctypes.data), "searchbatch!")
@@ -399,9 +414,45 @@ def allknn(index, ctx, k, sort=True):
         bp = index.beam_params(ctx)
         _lib.check(L.ssg_allknn_graph(h, k, C.byref(bp), ids.ctypes.data, dists.ctypes.data),
                    "allknn")
+    elif getattr(ctx, "precision", "f32") == "bf16":
+        _lib.check(L.ssg_allknn_exhaustive_prec(h, k, 1, ctx.exhaustive_ncand(k), ids.ctypes.data,
+                                                dists.ctypes.data, None, None), "allknn")
     else:
         _lib.check(L.ssg_allknn_exhaustive(h, k, ids.ctypes.data, dists.ctypes.data), "allknn")
     return ids.T, dists.T
+
+
+def exhaustive_candidates(index, k, ncand, Q=None):
+    """Precision form of an exhaustive index with its candidate queue exposed (no reference
+    counterpart): returns (ids, dists, cand_ids, cand_dists), all (rows, m) with m the
+    number of queries (every database object when Q is None, as allknn).  cand_* hold
+    the ncand BF16-pass candidates in exact (distance, id) order and their approximate
+    distances; ids/dists are the exact top k among them."""
+    if not isinstance(index, ExhaustiveSearch):
+        raise TypeError("the precision form serves exhaustive indexes")
+    L = _lib.load()
+    h = index._handle()
+    if Q is None:
+        m = len(index)
+        if not 0 < k <= m:
+            raise ValueError("allknn needs 0 < k <= n")
+    else:
+        Q = _as_db(Q)
+        m = len(Q)
+        if m == 0:
+            raise ValueError("empty set of queries")
+    ids, dists = _out(m, k)
+    ci, cd = _out(m, ncand)
+    if Q is None:
+        _lib.check(L.ssg_allknn_exhaustive_prec(h, k, 1, ncand, ids.ctypes.data, dists.ctypes.data,
+                                                ci.ctypes.data, cd.ctypes.data), "allknn")
+    else:
+        if Q.dim != index.db.dim:
+            raise ValueError("query dimension differs from the database")
+        _lib.check(L.ssg_exhaustive_knn_prec(h, Q.rows.ctypes.data, m, k, 1, ncand, ids.ctypes.data,
+                                             dists.ctypes.data, ci.ctypes.data, cd.ctypes.data),
+                   "searchbatch!")
+    return ids.T, dists.T, ci.T, cd.T
 
 
 _rerank_cache = {}

@@ -17,6 +17,7 @@ _i64 = C.c_int64
 
 
 SSG_OPT_QUERY_LAYOUT = 1
+SSG_OPT_PREC_SLICES = 2
 
 
 class BeamParams(C.Structure):
@@ -96,6 +97,10 @@ SIGNATURES = {
     "ssg_graph_destroy": (C.c_int, [_p]),
     "ssg_rerank": (C.c_int, [_p, _p, _i64, _i32, _p, _p]),
     "ssg_rerank_dev": (C.c_int, [_p, _p, _i64, _i32, _p, _p, _p]),
+    "ssg_exhaustive_knn_prec": (C.c_int, [_p, _p, _i64, _i32, _i32, _i32, _p, _p, _p, _p]),
+    "ssg_exhaustive_knn_prec_dev": (C.c_int, [_p, _p, _i64, _i32, _i32, _i32, _p, _p, _p, _p, _p]),
+    "ssg_allknn_exhaustive_prec": (C.c_int, [_p, _i32, _i32, _i32, _p, _p, _p, _p]),
+    "ssg_allknn_exhaustive_prec_dev": (C.c_int, [_p, _i32, _i32, _i32, _p, _p, _p, _p, _p]),
 }
 
 _lib = None

@@ -1,0 +1,206 @@
+"""GPU parity of the precision form of exhaustive k-NN and allknn: a candidate pass with
+BF16 operands on the matrix cores keeps the ncand best approximate candidates per query,
+then rerank! (src/rerank.jl:22-98) rescores them with the canonical distance.
+
+Pinned against the oracle:
+- returned distances are the canonical FP32 distances of the returned ids (bit-exact)
+  and results are in (distance, id) order;
+- every approximate distance is within the FP32-accumulation bound of the FP64 model of
+  BF16 operands (oracle.prec_approx / prec_error_bound), which fixes the operand
+  rounding and the matrix-core data layout;
+- the candidate set is the ncand best under those approximate distances: no object
+  outside it is provably better than one inside it;
+- the returned ids are the exact top k among the candidates;
+- with ncand >= n every object is a candidate and the result equals the exact
+  exhaustive search bit for bit;
+- recall against the exact exhaustive search on random data.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _ss():
+    import similaritysearch as ss
+    return ss
+
+
+def _uniform(n, dim, seed):
+    return np.random.default_rng(seed).random((n, dim), dtype=np.float32)
+
+
+def _unit(n, dim, seed):
+    x = np.random.default_rng(seed).standard_normal((n, dim)).astype(np.float32)
+    return (x / np.linalg.norm(x, axis=1, keepdims=True)).astype(np.float32)
+
+
+def _dist(ss, metric):
+    return {"SqL2": ss.SqL2, "L2": ss.L2, "NormCosine": ss.NormCosine}[metric]
+
+
+def _check(oracle, metric, X, Q, k, ncand, ids, d, ci, cd):
+    """The pinned properties for one call; ids/d are (m, k), ci/cd (m, ncand)."""
+    n = len(X)
+    m = len(Q)
+    D = oracle.dist_batch(metric, X, Q)            # canonical FP32 distances, (m, n)
+    A = oracle.prec_approx(metric, X, Q)           # FP64 model of the approximate distances
+    B = oracle.prec_error_bound(metric, X, Q)
+    nc = min(ncand, n)
+    for j in range(m):
+        c = ci[j, :nc].astype(np.int64)
+        assert (c >= 1).all() and (c <= n).all() and len(set(c.tolist())) == nc
+        assert (ci[j, nc:] == 0).all() and np.isinf(cd[j, nc:]).all()
+        c0 = c - 1
+        # approximate distances within the accumulation bound of the model
+        assert (np.abs(cd[j, :nc].astype(np.float64) - A[j, c0]) <= B[j, c0]).all(), j
+        # candidates in exact (distance, id) order
+        key = np.lexsort((c0, D[j, c0]))
+        np.testing.assert_array_equal(key, np.arange(nc))
+        # no object outside the candidates is provably nearer than one inside
+        if nc < n:
+            out = np.ones(n, bool)
+            out[c0] = False
+            worst_in = (A[j, c0] - B[j, c0]).max()
+            assert ((A[j, out] + B[j, out]) >= worst_in).all(), j
+        # the result is the exact top k among the candidates, with canonical distances
+        kk = min(k, nc)
+        np.testing.assert_array_equal(ids[j, :kk], ci[j, :kk])
+        np.testing.assert_array_equal(d[j, :kk], D[j, c0[:kk]])
+        assert (ids[j, kk:] == 0).all() and np.isinf(d[j, kk:]).all()
+
+
+@pytest.mark.parametrize("metric,dim,n,m,k,ncand", [
+    ("SqL2", 32, 200, 300, 10, 256),
+    ("NormCosine", 96, 150, 70, 32, 128),
+    ("SqL2", 16, 9, 5, 12, 16),
+])
+def test_every_object_a_candidate_equals_exact(oracle, metric, dim, n, m, k, ncand):
+    ss = _ss()
+    X = _unit(n, dim, 1) if metric == "NormCosine" else _uniform(n, dim, 1)
+    Q = _unit(m, dim, 2) if metric == "NormCosine" else _uniform(m, dim, 2)
+    E = ss.ExhaustiveSearch(_dist(ss, metric), ss.MatrixDatabase.from_rows(X))
+    ids, d, ci, cd = ss.exhaustive_candidates(E, k, ncand, ss.MatrixDatabase.from_rows(Q))
+    gi, gd = oracle.exhaustive(metric, X, Q, k)
+    np.testing.assert_array_equal(ids.T, gi)
+    np.testing.assert_array_equal(d.T, gd)
+    _check(oracle, metric, X, Q, k, ncand, ids.T, d.T, ci.T, cd.T)
+
+
+@pytest.mark.parametrize("metric,dim,n,m,k,ncand,slices", [
+    ("NormCosine", 96, 6000, 300, 32, 128, 0),
+    ("NormCosine", 96, 6000, 300, 32, 128, 3),
+    ("SqL2", 64, 5000, 257, 10, 64, 0),
+    ("SqL2", 128, 4000, 100, 10, 40, 5),
+    ("SqL2", 20, 3000, 64, 1, 32, 2),
+])
+def test_candidates_and_rerank(oracle, metric, dim, n, m, k, ncand, slices):
+    ss = _ss()
+    from similaritysearch import _lib
+    X = _unit(n, dim, 3) if metric == "NormCosine" else _uniform(n, dim, 3)
+    Q = _unit(m, dim, 4) if metric == "NormCosine" else _uniform(m, dim, 4)
+    E = ss.ExhaustiveSearch(_dist(ss, metric), ss.MatrixDatabase.from_rows(X))
+    _lib.check(_lib.load().ssg_set_option(E._handle(), _lib.SSG_OPT_PREC_SLICES, slices), "option")
+    ids, d, ci, cd = ss.exhaustive_candidates(E, k, ncand, ss.MatrixDatabase.from_rows(Q))
+    _check(oracle, metric, X, Q, k, ncand, ids.T, d.T, ci.T, cd.T)
+    gi, _ = oracle.exhaustive(metric, X, Q, k)
+    rec = oracle.macrorecall(gi, ids.T)
+    assert rec >= 0.99, rec
+
+
+def test_slice_count_does_not_change_results():
+    ss = _ss()
+    from similaritysearch import _lib
+    X = _unit(7000, 96, 5)
+    Q = _unit(260, 96, 6)
+    E = ss.ExhaustiveSearch(ss.NormCosine, ss.MatrixDatabase.from_rows(X))
+    outs = []
+    for s in (1, 2, 7, 0):
+        _lib.check(_lib.load().ssg_set_option(E._handle(), _lib.SSG_OPT_PREC_SLICES, s), "option")
+        outs.append(ss.exhaustive_candidates(E, 16, 64, ss.MatrixDatabase.from_rows(Q)))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_ties_prefer_lower_ids(oracle):
+    """300 copies of one vector among random rows: all have the same approximate and
+    exact distance to it, so the candidate pass must keep the lowest ids (a compaction
+    with a tie at its threshold) and the result is the k lowest copies."""
+    ss = _ss()
+    rng = np.random.default_rng(7)
+    X = rng.random((5000, 32), dtype=np.float32)
+    v = rng.random(32, dtype=np.float32)
+    pos = np.sort(rng.choice(5000, 300, replace=False))
+    X[pos] = v
+    Q = np.stack([v, v + np.float32(1e-3)]).astype(np.float32)
+    E = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(X))
+    from similaritysearch import _lib
+    # one slice: the ties are resolved by the compactions of one list
+    _lib.check(_lib.load().ssg_set_option(E._handle(), _lib.SSG_OPT_PREC_SLICES, 1), "option")
+    ids, d, ci, cd = ss.exhaustive_candidates(E, 10, 64, ss.MatrixDatabase.from_rows(Q))
+    np.testing.assert_array_equal(ci.T[0], pos[:64] + 1)
+    np.testing.assert_array_equal(ids.T[0], pos[:10] + 1)
+    assert (d.T[0] == 0).all()
+    gi, gd = oracle.exhaustive("SqL2", X, Q, 10)
+    np.testing.assert_array_equal(ids.T, gi)
+    np.testing.assert_array_equal(d.T, gd)
+
+
+def test_allknn_precision_form(oracle):
+    """allknn with the BF16 candidate pass (the C3 shape at reduced n): self first,
+    exact distances, recall against the exact allknn."""
+    ss = _ss()
+    X = _unit(8000, 96, 9)
+    E = ss.ExhaustiveSearch(ss.NormCosine, ss.MatrixDatabase.from_rows(X))
+    ctx = ss.GenericContext(precision="bf16")
+    ids, d = ss.allknn(E, ctx, 32)
+    ids, d = ids.T, d.T
+    assert (ids[:, 0] == np.arange(1, 8001)).all()
+    sub = np.arange(0, 8000, 40)
+    gi, gd = oracle.exhaustive("NormCosine", X, X[sub], 32)
+    D = oracle.dist_batch("NormCosine", X, X[sub])
+    for r, j in enumerate(sub):
+        np.testing.assert_array_equal(d[j], D[r, ids[j].astype(np.int64) - 1])
+    assert oracle.macrorecall(gi, ids[sub]) >= 0.999
+    # the candidate form of allknn agrees with the plain one
+    i2, d2, ci, cd = ss.exhaustive_candidates(E, 32, 128)
+    np.testing.assert_array_equal(i2.T, ids)
+    np.testing.assert_array_equal(d2.T, d)
+
+
+def test_searchbatch_precision_context(oracle):
+    ss = _ss()
+    X = _uniform(3000, 48, 10)
+    Q = _uniform(90, 48, 11)
+    E = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(X))
+    ids, d = ss.searchbatch(E, ss.GenericContext(precision="bf16", ncand=64),
+                            ss.MatrixDatabase.from_rows(Q), 8)
+    gi, gd = oracle.exhaustive("SqL2", X, Q, 8)
+    assert oracle.macrorecall(gi, ids.T) >= 0.99
+    D = oracle.dist_batch("SqL2", X, Q)
+    for j in range(90):
+        np.testing.assert_array_equal(d.T[j], D[j, ids.T[j].astype(np.int64) - 1])
+
+
+def test_precision_form_limits():
+    ss = _ss()
+    from similaritysearch import _lib
+    L = _lib.load()
+    X = _uniform(500, 16, 12)
+    E = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(X))
+    h = E._handle()
+    Q = _uniform(4, 16, 13)
+    ids = np.zeros((4, 300), np.uint32)
+    d = np.zeros((4, 300), np.float32)
+    assert L.ssg_exhaustive_knn_prec(h, Q.ctypes.data, 4, 10, 1, 300, ids.ctypes.data,
+                                     d.ctypes.data, None, None) == -5
+    assert L.ssg_exhaustive_knn_prec(h, Q.ctypes.data, 4, 10, 1, 8, ids.ctypes.data,
+                                     d.ctypes.data, None, None) == -5
+    assert L.ssg_exhaustive_knn_prec(h, Q.ctypes.data, 4, 10, 0, 64, ids.ctypes.data,
+                                     d.ctypes.data, None, None) == -1
+    Xw = _uniform(100, 190, 14)
+    W = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(Xw))
+    Qw = _uniform(2, 190, 15)
+    assert L.ssg_exhaustive_knn_prec(W._handle(), Qw.ctypes.data, 2, 4, 1, 64, ids.ctypes.data,
+                                     d.ctypes.data, None, None) == -5

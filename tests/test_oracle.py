@@ -271,6 +271,66 @@ def test_rerank_against_stable_sort(oracle):
         assert (gi[j, 9:] == 0).all()
 
 
+def test_bf16_round_hand_cases(oracle):
+    # exact values stay; halfway cases go to the even neighbour; overflow of the mantissa
+    # carries into the exponent
+    x = np.array([1.0, -2.5, 0.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8, 1 + 2 ** -8 + 2 ** -20,
+                  2 - 2 ** -9, 3.0e38], np.float32)
+    want = np.array([1.0, -2.5, 0.0, 1.0, 1 + 2 ** -6, 1 + 2 ** -7, 2.0, 2.996e38], np.float32)
+    got = oracle.bf16_round(x)
+    np.testing.assert_array_equal(got[:7], want[:7])
+    assert abs(got[7] / 3.0e38 - 1) < 2 ** -8
+
+
+def test_bf16_round_matches_torch(oracle):
+    """torch's float32 -> bfloat16 conversion is round to nearest even; compared in a
+    child process so that torch's own GPU runtime never shares a process with the HIP
+    library."""
+    import subprocess
+    import sys
+    rng = np.random.default_rng(21)
+    x = np.concatenate([rng.standard_normal(20000).astype(np.float32),
+                        (rng.random(20000) * 1e-30).astype(np.float32),
+                        rng.integers(-2 ** 20, 2 ** 20, 20000).astype(np.float32)])
+    code = ("import sys, numpy as np, torch\n"
+            "x = np.frombuffer(sys.stdin.buffer.read(), np.float32)\n"
+            "y = torch.from_numpy(x.copy()).to(torch.bfloat16).to(torch.float32).numpy()\n"
+            "sys.stdout.buffer.write(y.tobytes())\n")
+    out = subprocess.run([sys.executable, "-c", code], input=x.tobytes(), capture_output=True,
+                         check=True, timeout=300).stdout
+    np.testing.assert_array_equal(oracle.bf16_round(x), np.frombuffer(out, np.float32))
+
+
+def test_prec_model_bounds_hold_in_fp32(oracle):
+    """The precision form's error bound covers an FP32 evaluation of the same products
+    in two different orders (a plausible accumulation of the matrix cores)."""
+    rng = np.random.default_rng(22)
+    for metric, X, Q in [
+        ("NormCosine", *[(lambda a: a / np.linalg.norm(a, axis=1, keepdims=True))(
+            rng.standard_normal((n, 96)).astype(np.float32)).astype(np.float32) for n in (300, 40)]),
+        ("SqL2", rng.random((300, 64), dtype=np.float32), rng.random((40, 64), dtype=np.float32)),
+    ]:
+        A = oracle.prec_approx(metric, X, Q)
+        B = oracle.prec_error_bound(metric, X, Q)
+        Xb, Qb = oracle.bf16_round(X), oracle.bf16_round(Q)
+        for order in (slice(None), slice(None, None, -1)):
+            s = np.zeros((len(Q), len(X)), np.float32)
+            for i in range(X.shape[1])[order]:
+                s = (s + np.outer(Qb[:, i], Xb[:, i]).astype(np.float32)).astype(np.float32)
+            if metric == "NormCosine":
+                g = (np.float32(1) - s).astype(np.float64)
+            else:
+                xn = (X.astype(np.float32) ** 2).sum(1, dtype=np.float32)
+                qn = (Q.astype(np.float32) ** 2).sum(1, dtype=np.float32)
+                g = (qn[:, None] + xn[None, :] - np.float32(2) * s).astype(np.float64)
+            assert (np.abs(g - A) <= B).all()
+        # a dropped rounding step (FP32 operands instead of BF16) breaks the model
+        s32 = Q.astype(np.float64) @ X.astype(np.float64).T
+        g32 = 1 - s32 if metric == "NormCosine" else None
+        if g32 is not None:
+            assert (np.abs(g32 - A) > B).any()
+
+
 def test_macrorecall(oracle):
     g = np.array([[1, 2, 3], [4, 5, 6]], np.uint32)
     assert oracle.macrorecall(g, g) == 1.0
