@@ -373,11 +373,16 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
         o.qpad = h->X.as<float>();
         o.qnorm = h->xnorm.as<float>();
     }
-    const int32_t kc = ncand, cap = 2 * ncand + 128;
+    const int32_t kc = ncand, cap = prec_list_cap(ncand);
     int32_t ns = h->prec_slices > 0 ? h->prec_slices : prec_slices(n, m, h->num_cu, kc);
     ns = std::max(1, std::min<int32_t>(ns, 2048 / kc));
-    int64_t rps = (n + ns - 1) / ns;
-    rps = (rps + 127) / 128 * 128;
+    // Sampled thresholds (below) on long slices; their slices hold whole groups of STEP tiles
+    // so that each slice's sample is a contiguous range of the packed sample rows.
+    constexpr int32_t STEP = 16;
+    const int64_t rps0 = (n + ns - 1) / ns;
+    const bool sample = h->prec_sample > 0 || (h->prec_sample == 0 && rps0 >= (int64_t)STEP * 128 * 64);
+    const int64_t unit = sample ? (int64_t)STEP * 128 : 128;
+    const int64_t rps = (rps0 + unit - 1) / unit * unit;
     ns = (int32_t)((n + rps - 1) / rps);
     HIPCHK(h->pList.ensure((size_t)m_pad * ns * cap * sizeof(uint2)), "alloc candidate lists");
     HIPCHK(h->pListN.ensure((size_t)m_pad * ns * sizeof(int32_t)), "alloc candidate counts");
@@ -394,6 +399,7 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     a.nslices = ns;
     a.list = h->pList.as<uint2>();
     a.list_n = h->pListN.as<int32_t>();
+    a.tile_step = 1;
     o.m = m;
     o.k = k;
     o.kstride = kstride;
@@ -402,8 +408,72 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     o.dists = ddists;
     o.cand_ids = dcand_i;
     o.cand_dists = dcand_d;
+    // Sampled thresholds: a pass over every 16th tile of each slice finds, per query, the
+    // T-th best sampled score (about rank 16 T of the slice); the main pass drops every row
+    // at or below it, which removes most appends, and the rerank stage certifies each
+    // query (the ncand-th best candidate must score above the threshold).  Queries it
+    // cannot certify are searched again from no threshold.
+    const int32_t T = h->prec_sample > 0 ? h->prec_sample : std::max(16, 3 * kc / 16);
+    if (!sample) {
+        HIPCHK(prec_launch_stage1(a, s), "precision candidate pass");
+        HIPCHK(prec_launch_stage2(a, o, h->dp, s), "precision rerank");
+        return SSG_OK;
+    }
+    HIPCHK(h->pTau0.ensure((size_t)m_pad * ns * sizeof(float)), "alloc thresholds");
+    HIPCHK(h->pFlags.ensure((size_t)(m + 1) * sizeof(unsigned int)), "alloc flags");
+    unsigned int* flags = h->pFlags.as<unsigned int>();
+    const int64_t nsr = prec_sample_rows(n, STEP);
+    if (!h->Xs.p) {
+        HIPCHK(h->Xs.ensure((size_t)std::max<int64_t>(nsr, 1) * dpb * 2), "alloc sample rows");
+        HIPCHK(h->xns.ensure((size_t)std::max<int64_t>(nsr, 1) * sizeof(float)), "alloc sample norms");
+        HIPCHK(launch_sample_rows(h->Xe.as<uint16_t>(), h->xnorm.as<float>(), nsr, STEP, dpb,
+                                  h->Xs.as<uint16_t>(), h->xns.as<float>(), s), "sample rows");
+    }
+    PrecArgs as = a;
+    as.Xe = h->Xs.as<uint16_t>();
+    as.xnorm = h->xns.as<float>();
+    as.n = nsr;
+    as.rows_per_slice = rps / STEP;
+    as.kc = std::min(T, PREC_NCAND_MAX);
+    as.cap = prec_list_cap(as.kc);
+    as.tile_step = 1;
+    as.sample = 1;
+    as.tau0 = h->pTau0.as<float>();
+    HIPCHK(prec_launch_stage1(as, s), "precision sample pass");
+    a.tau0 = as.tau0;
+    o.flags = flags;
+    HIPCHK(launch_zero_u32(flags, 1, s), "zero flags");
     HIPCHK(prec_launch_stage1(a, s), "precision candidate pass");
     HIPCHK(prec_launch_stage2(a, o, h->dp, s), "precision rerank");
+    unsigned int nf = 0;
+    HIPCHK(hipMemcpyAsync(&nf, flags, sizeof(unsigned int), hipMemcpyDeviceToHost, s), "read flags");
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    if (nf == 0) return SSG_OK;
+    // redo of the flagged queries, gathered, over enough slices to fill the device
+    const int64_t mf_pad = ((int64_t)nf + PREC_QB - 1) / PREC_QB * PREC_QB;
+    int32_t nsf = (int32_t)std::max<int64_t>(1, (2 * (int64_t)h->num_cu + mf_pad / PREC_QB - 1) / (mf_pad / PREC_QB));
+    nsf = std::max(1, std::min<int32_t>(nsf, 2048 / kc));
+    int64_t rpsf = (n + nsf - 1) / nsf;
+    rpsf = (rpsf + 127) / 128 * 128;
+    nsf = (int32_t)((n + rpsf - 1) / rpsf);
+    HIPCHK(h->pQf.ensure((size_t)mf_pad * dpb * 2), "alloc redo queries");
+    HIPCHK(h->pList.grow((size_t)mf_pad * nsf * cap * sizeof(uint2), 0, s), "alloc candidate lists");
+    HIPCHK(h->pListN.grow((size_t)mf_pad * nsf * sizeof(int32_t), 0, s), "alloc candidate counts");
+    HIPCHK(launch_gather_rows_u16(a.Qe, flags + 1, flags, mf_pad, dpb, h->pQf.as<uint16_t>(), s), "gather");
+    PrecArgs af = a;
+    af.Qe = h->pQf.as<uint16_t>();
+    af.m_pad = mf_pad;
+    af.rows_per_slice = rpsf;
+    af.nslices = nsf;
+    af.list = h->pList.as<uint2>();
+    af.list_n = h->pListN.as<int32_t>();
+    af.tau0 = nullptr;
+    PrecOut of = o;
+    of.m = nf;
+    of.qmap = flags + 1;
+    of.flags = nullptr;
+    HIPCHK(prec_launch_stage1(af, s), "precision candidate pass (redo)");
+    HIPCHK(prec_launch_stage2(af, of, h->dp, s), "precision rerank (redo)");
     return SSG_OK;
 }
 
@@ -606,6 +676,7 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     h->hints_host.clear();
     h->Xh.release();
     h->Xe.release();
+    h->Xs.release();
     ssg_query_layout_stale(h);
     const int64_t np = std::max<int64_t>(h->n_pad, 64);
     HIPCHK(h->X.ensure((size_t)np * dp * sizeof(float)), "alloc X");
@@ -825,6 +896,11 @@ int ssg_set_option(ssg_handle* h, int32_t option, int64_t value)
         HIPCHK(hipDeviceSynchronize(), "sync");
         h->reorder = value == 1;
         ssg_query_layout_stale(h);
+        return SSG_OK;
+    }
+    if (option == SSG_OPT_PREC_SAMPLE) {
+        if (value < -1 || value > PREC_NCAND_MAX) return ssg_fail(SSG_E_ARG, "sample option must be in -1..256");
+        h->prec_sample = (int32_t)value;
         return SSG_OK;
     }
     if (option == SSG_OPT_PREC_SLICES) {

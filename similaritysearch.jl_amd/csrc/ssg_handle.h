@@ -92,8 +92,10 @@ struct ssg_handle {
     // precision form of exhaustive k-NN: BF16 rows of 32 * ceil(dim / 32) elements (made
     // on first use), BF16 queries, per-(query, slice) candidate lists and their lengths;
     // prec_slices > 0 forces the number of DB slices
-    DevBuf Xe, pQe, pList, pListN;
+    DevBuf Xe, pQe, pList, pListN, pTau0, pFlags, pQf;
+    DevBuf Xs, xns;  // rows of every 16th tile of Xe, packed, and their norms (sample pass)
     int32_t prec_slices = 0;
+    int32_t prec_sample = 0;  // 0: sampled thresholds when slices are long, -1: never, T > 0: always, rank T
     bool reorder = true;
     bool q_ready = false, q_bf16 = false;
     DevBuf qnode, qadj, qadjn, qperm, qipos, qhints, Xq, Xqh;

@@ -111,7 +111,13 @@ constexpr int EXH_QT = 64;        // queries per block of exhaustive stage 1
 // Precision form of exhaustive k-NN: BF16 MFMA candidate pass + exact rerank.
 constexpr int PREC_QB = 256;          // queries per candidate-pass block
 constexpr int PREC_NCAND_MAX = 256;   // candidates per query
-constexpr int PREC_CAP_MAX = 640;     // list capacity per (query, slice): 2 * ncand + 128
+constexpr int PREC_CAP_MAX = 1024;    // list capacity per (query, slice): prec_list_cap(ncand)
+// 4 sub-lists of a power-of-two number of entries each, room for the kept entries of a
+// compaction (at most ncand + 64) plus at least 250 appends
+inline int32_t prec_list_cap(int32_t ncand)
+{
+    return ncand <= 128 ? 512 : 1024;
+}
 struct PrecArgs {
     const uint16_t* Xe;     // n_pad rows of dpb BF16 (zero padded)
     const float* xnorm;     // FP32 squared norms of the rows
@@ -127,6 +133,11 @@ struct PrecArgs {
     int32_t nslices;
     uint2* list;            // [m_pad][nslices][cap] {score bits, 0-based id}
     int32_t* list_n;        // [m_pad][nslices]
+    int32_t tile_step;      // 1: every 128-row tile of the slice; f: every f-th (sample pass)
+    int32_t sample;         // 1: sample pass, writes tau0 (the kc-th best sampled score)
+    float* tau0;            // nullable, [m_pad][nslices]: the sample pass writes it; the main
+                            // pass starts each query's threshold there (rows at or below it
+                            // are dropped) and the rerank stage certifies the result
 };
 struct PrecOut {
     int64_t m;
@@ -137,7 +148,21 @@ struct PrecOut {
     float* dists;
     uint32_t* cand_ids;     // nullable: (ncand, m) candidates in (distance, id) order
     float* cand_dists;      // their approximate distances
+    const uint32_t* qmap;   // nullable: query j of this launch is row qmap[j] of qpad,
+                            // qnorm and the outputs (the redo of flagged queries)
+    unsigned int* flags;    // nullable with tau0: [0] = number of queries whose result is
+                            // not certified, then their indices
 };
+// Number of rows of the 1-in-step-tiles sample of n rows, and the packing kernel.
+inline int64_t prec_sample_rows(int64_t n, int32_t step)
+{
+    const int64_t g = (int64_t)128 * step, groups = n / g, rest = n % g;
+    return groups * 128 + (rest < 128 ? rest : 128);
+}
+hipError_t launch_sample_rows(const uint16_t* Xe, const float* xnorm, int64_t ns_rows, int32_t step,
+                              int32_t dpb, uint16_t* Xs, float* xns, hipStream_t s);
+hipError_t launch_gather_rows_u16(const uint16_t* src, const uint32_t* idx, const unsigned int* count,
+                                  int64_t max_rows, int32_t width, uint16_t* dst, hipStream_t s);
 int32_t prec_slices(int64_t n, int64_t m, int num_cu, int kc);
 hipError_t prec_launch_stage1(const PrecArgs& a, hipStream_t s);
 hipError_t prec_launch_stage2(const PrecArgs& a, const PrecOut& o, int32_t dp, hipStream_t s);

@@ -32,10 +32,10 @@
 namespace ssg {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x2 __attribute__((__vector_size__(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int PQ_TR = 128;          // DB rows per LDS tile
-constexpr int PQ_J = PREC_CAP_MAX / 64;  // list entries per lane in a compaction
 constexpr int PQ_JS = 32;           // merged entries per lane in the rerank stage
 constexpr int PQ_NS = PREC_NCAND_MAX / 64;  // candidate slots per lane in the rerank stage
 
@@ -50,25 +50,41 @@ __device__ __forceinline__ float key_score(uint32_t k)
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-// The want-th largest key among the active entries of the wave (want >= 1 and at most
-// the number of active entries), by an 8-bit radix select with a 256-bin histogram in
-// LDS (hist: 256 words owned by this wave).  *above = active entries with a larger key.
+// Orders a wave's LDS accesses for the compiler (a wave's LDS operations complete in
+// issue order, so no wait is needed between them).
+__device__ __forceinline__ void lds_order()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Radix select over the active entries of the wave: the top 8 * npass bits of the
+// want-th largest key (want >= 1, at most the number of active entries), one 8-bit digit
+// per pass with a 256-bin histogram in LDS (hist: 256 words owned by this wave).
+// *above = active entries whose top 8 * npass bits are larger.
+// kand / diff (wave-uniform): the AND of the active keys and the bits in which they
+// differ; a byte in which no active key differs needs no histogram (all of its increments
+// would go to one bin).  diff = ~0 skips nothing.
 template <int J>
-__device__ uint32_t kth_largest(const uint32_t (&key)[J], const bool (&act)[J], int want,
-                                uint32_t* hist, int lane, int* above)
+__device__ uint32_t kth_largest(const uint32_t (&key)[J], const bool (&act)[J], int want, int npass,
+                                uint32_t* hist, int lane, int* above, uint32_t kand, uint32_t diff)
 {
     uint32_t prefix = 0;
     int rem = want, abv = 0;
-    for (int pass = 0; pass < 4; pass++) {
+    for (int pass = 0; pass < npass; pass++) {
         const int sh = 24 - 8 * pass;
+        if (((diff >> sh) & 255u) == 0u) {
+            prefix = (prefix << 8) | ((kand >> sh) & 255u);
+            continue;
+        }
         reinterpret_cast<uint4*>(hist)[lane] = make_uint4(0u, 0u, 0u, 0u);
-        wave_sync();
+        lds_order();
 #pragma unroll
         for (int j = 0; j < J; j++) {
             const bool in = act[j] && (pass == 0 || (key[j] >> (sh + 8)) == prefix);
             if (in) atomicAdd(&hist[(key[j] >> sh) & 255u], 1u);
         }
-        wave_sync();
+        lds_order();
         const uint4 h4 = reinterpret_cast<const uint4*>(hist)[lane];
         const uint32_t tot = h4.x + h4.y + h4.z + h4.w;
         uint32_t suf = tot;  // entries in bins 4*lane .. 255
@@ -81,7 +97,7 @@ __device__ uint32_t kth_largest(const uint32_t (&key)[J], const bool (&act)[J], 
         const uint64_t bl = __ballot(suf >= (uint32_t)rem);
         const int L = 63 - __builtin_clzll(bl);
         const int t3 = (int)rl_u(s3, L), t2 = (int)rl_u(s2, L), t1 = (int)rl_u(s1, L);
-        const int t0 = (int)rl_u(suf, L), hw = (int)rl_u(h4.w, L);
+        const int hw = (int)rl_u(h4.w, L);
         int d, sa;
         if (t3 >= rem) {
             d = 3;
@@ -96,24 +112,42 @@ __device__ uint32_t kth_largest(const uint32_t (&key)[J], const bool (&act)[J], 
             d = 0;
             sa = t1;
         }
-        (void)t0;
         rem -= sa;
         abv += sa;
         prefix = (prefix << 8) | (uint32_t)(4 * L + d);
-        wave_sync();
+        lds_order();
     }
     *above = abv;
     return prefix;
+}
+
+// Wave-uniform AND of the active keys and the bits in which they differ.
+template <int J>
+__device__ __forceinline__ void key_bits(const uint32_t (&key)[J], const bool (&act)[J], uint32_t* kand,
+                                         uint32_t* diff)
+{
+    uint32_t a = 0xffffffffu, o = 0u;
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        a &= act[j] ? key[j] : 0xffffffffu;
+        o |= act[j] ? key[j] : 0u;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        a &= (uint32_t)__shfl_xor((int)a, off);
+        o |= (uint32_t)__shfl_xor((int)o, off);
+    }
+    *kand = rfl_u(a);
+    *diff = rfl_u(a ^ o);
 }
 
 // Marks the kc best active entries by (key descending, id ascending); returns the kc-th
 // best key.  Requires more than kc active entries.
 template <int J>
 __device__ uint32_t select_best(const uint32_t (&key)[J], const uint32_t (&id)[J], const bool (&act)[J],
-                                int kc, uint32_t* hist, int lane, bool (&keep)[J])
+                                int kc, uint32_t* hist, int lane, bool (&keep)[J], uint32_t kand, uint32_t diff)
 {
     int above = 0;
-    const uint32_t T = kth_largest<J>(key, act, kc, hist, lane, &above);
+    const uint32_t T = kth_largest<J>(key, act, kc, 4, hist, lane, &above, kand, diff);
     const int need = kc - above;  // entries with key == T to keep, >= 1
     int eq = 0;
     bool tie[J];
@@ -127,45 +161,99 @@ __device__ uint32_t select_best(const uint32_t (&key)[J], const uint32_t (&id)[J
     uint32_t TI = 0;
     if (eq > need) {
         int a2 = 0;
-        TI = kth_largest<J>(nid, tie, need, hist, lane, &a2);  // ids are distinct
+        TI = kth_largest<J>(nid, tie, need, 4, hist, lane, &a2, 0u, 0xffffffffu);  // ids are distinct
     }
 #pragma unroll
     for (int j = 0; j < J; j++) keep[j] = act[j] && (key[j] > T || (tie[j] && nid[j] >= TI));
     return T;
 }
 
-// Compaction of one (query, slice) list to its kc best entries; returns the new
-// threshold score.  The whole wave calls it with the same list.
-__device__ float compact_list(uint2* L, int* cntp, int kc, int cap, uint32_t* hist, int lane)
+// A (query, slice) list of cap entries is split into 4 sub-lists of sc = cap / 4
+// entries; sub-list g is appended to by the lanes 16g .. 16g + 15 of the owning wave
+// (the lanes that hold rows 4g .. 4g + 3 of each 16-row block), each with its own count
+// in a register, so appends need no atomics.
+//
+// Compaction of one list with sub-list counts c0..c3.  final == false (during the
+// scan): keeps every entry whose key reaches the 16-bit prefix of the kc-th best key
+// (at least kc entries, at most kc + 64, else the exact selection below), spread
+// round-robin over the sub-lists; returns the new threshold (a row passes if its score
+// is larger) and *kept.  final == true: keeps exactly the kc best by (score descending,
+// id ascending), or every entry if there are at most kc, written contiguously at the
+// start of the list.
+// J = cap / 64 entries per lane (8 for lists of up to 512 entries, 16 for larger ones).
+template <int J>
+__device__ __attribute__((noinline)) float compact_list_j(uint2* L, int sc, int c0, int c1, int c2, int c3, int kc,
+                                                          bool final, uint32_t* hist, int lane, int* kept)
 {
-    const int c = min((int)rfl_u((uint32_t)*cntp), cap);
-    uint32_t key[PQ_J], id[PQ_J];
-    bool act[PQ_J], keep[PQ_J];
+    uint32_t key[J], id[J];
+    bool act[J], keep[J];
+    int total = 0;
 #pragma unroll
-    for (int j = 0; j < PQ_J; j++) {
+    for (int j = 0; j < J; j++) {
         const int e = lane + 64 * j;
-        act[j] = e < c;
+        const int g = e / sc, pos = e - g * sc;
+        const int c = g == 0 ? c0 : g == 1 ? c1 : g == 2 ? c2 : g == 3 ? c3 : 0;
+        act[j] = pos < c;
         const uint2 v = act[j] ? L[e] : make_uint2(0u, 0u);
         key[j] = score_key(__uint_as_float(v.x));
         id[j] = v.y;
+        total += __popcll(__ballot(act[j]));
     }
-    const uint32_t T = select_best<PQ_J>(key, id, act, kc, hist, lane, keep);
+    // fewer than kc entries: no threshold yet (and none was set before, since a
+    // compaction leaves at least kc); the entries are only spread over the sub-lists
+    float tau = -__builtin_inff();
+    bool exact = final;
+    if (!final && total < kc) {
+#pragma unroll
+        for (int j = 0; j < J; j++) keep[j] = act[j];
+    } else if (!final) {
+        int above = 0;
+        uint32_t kand, diff;
+        key_bits<J>(key, act, &kand, &diff);
+        const uint32_t K = kth_largest<J>(key, act, kc, 2, hist, lane, &above, kand, diff) << 16;
+        int nk = 0;
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            keep[j] = act[j] && key[j] >= K;
+            nk += __popcll(__ballot(keep[j]));
+        }
+        if (nk <= kc + 64) tau = key_score(K - 1u);
+        else exact = true;
+    }
+    if (exact) {
+        if (total > kc) {
+            uint32_t kand, diff;
+            key_bits<J>(key, act, &kand, &diff);
+            tau = key_score(select_best<J>(key, id, act, kc, hist, lane, keep, kand, diff));
+        } else {
+#pragma unroll
+            for (int j = 0; j < J; j++) keep[j] = act[j];
+        }
+    }
+    // at most kc + 64 entries stay (ids are distinct, so the selections above keep at most
+    // that many; the bound also holds the list in its space for any input)
+    const int lim = kc + 64;
     int base = 0;
 #pragma unroll
-    for (int j = 0; j < PQ_J; j++) {
+    for (int j = 0; j < J; j++) {
         const uint64_t b = __ballot(keep[j]);
         if (keep[j]) {
-            const int p = base + __popcll(b & ((1ull << lane) - 1ull));
-            L[p] = make_uint2(__float_as_uint(key_score(key[j])), id[j]);
+            const int i = base + __popcll(b & ((1ull << lane) - 1ull));
+            const int p = final ? i : (i & 3) * sc + (i >> 2);
+            if (i < lim) L[p] = make_uint2(__float_as_uint(key_score(key[j])), id[j]);
         }
         base += __popcll(b);
     }
-    if (lane == 0) *cntp = kc;
-    wave_sync();
-    return key_score(T);
+    *kept = base < lim ? base : lim;
+    wave_sync();  // the rewritten entries are visible to later appends and compactions
+    return tau;
 }
 
-template <int KB>
+
+// J = cap / 64 list entries per lane in a compaction (8: lists of up to 512 entries, 16:
+// of 1024).  Each kernel calls one compaction instance only: the registers that instance
+// uses are what the kernel has to keep clear around the call.
+template <int KB, int J>
 __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
 {
     constexpr int DPB = 32 * KB;             // BF16 row length
@@ -176,8 +264,9 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
     static_assert(PQ_TR * U4 % 256 == 0, "tile must split evenly over the block");
     __shared__ __attribute__((aligned(16))) uint8_t s_x[2][PQ_TR * LSB];
     __shared__ __attribute__((aligned(16))) float s_h[2][PQ_TR];
-    __shared__ int s_cnt[PREC_QB];
     __shared__ __attribute__((aligned(16))) uint32_t s_hist[4][256];
+    __shared__ int s_cnt[4][NCT * 64];   // sub-list counts of each wave's queries
+    __shared__ float s_tau[4][NCT * 16]; // their thresholds
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int col = lane & 15, g = lane >> 4;
@@ -191,7 +280,6 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
         return a.list + ((qb0 + ql) * a.nslices + sl) * (int64_t)cap;
     };
 
-    s_cnt[tid] = 0;
     bf16x8 bq[NCT][KB];
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
@@ -199,15 +287,37 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
 #pragma unroll
         for (int s = 0; s < KB; s++) bq[ct][s] = *reinterpret_cast<const bf16x8*>(qp + 8 * s);
     }
+    // Appends go through a buffer resource over the block's lists (stores past it are
+    // dropped by the hardware).  Each lane keeps, per query, the byte offset of the next
+    // free entry of its sub-list; sub-lists are sc = cap / 4 entries, sc a power of two,
+    // so the fill level is the offset's low bits.
+    const int sc = cap / 4;
+    const uint32_t smask = (uint32_t)sc * 8u - 1u;
+    const uint32_t full_at = (uint32_t)(sc - 5) * 8u;  // fill level that needs a compaction
+    const int64_t qstride = (int64_t)a.nslices * cap;  // entries between consecutive queries
+    const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
+        a.list + (qb0 * a.nslices + sl) * (int64_t)cap, 0,
+        (int)(((PREC_QB - 1) * qstride + cap) * (int64_t)sizeof(uint2)), 0x00020000);
     float tau[NCT];
+    uint32_t wp[NCT];
 #pragma unroll
-    for (int ct = 0; ct < NCT; ct++) tau[ct] = -__builtin_inff();
+    for (int ct = 0; ct < NCT; ct++) {
+        tau[ct] = (a.tau0 && !a.sample) ? a.tau0[(qb0 + 64 * w + 16 * ct + col) * a.nslices + sl]
+                                        : -__builtin_inff();
+        wp[ct] = (uint32_t)(((64 * w + 16 * ct + col) * qstride + g * sc) * (int64_t)sizeof(uint2));
+    }
 
     if (r0 < r1) {
         uint4 pf0, pf1, pf2, pf3, pf4, pf5, pf6, pf7;  // the next tile (PF <= 8 units per thread)
-        float pfh = 0.f;
+        float pfx = 0.f;                                // the norm of its row tid % 128
+        int64_t pft = r0;                               // its first row
         static_assert(PF <= 8, "prefetch registers");
+        // Every load is unconditional (clamped indices) and nothing consumes it before
+        // stage(): a conditional load consumed in its branch would make the wave wait for
+        // all of its loads in flight, the tile prefetch included.
         auto prefetch = [&](int64_t t) {
+            pft = t;
+            pfx = a.xnorm[min(t + (tid & (PQ_TR - 1)), r1 - 1)];
 #define SSG_PF(I, R)                                                                        \
             if (I < PF) {                                                                   \
                 const int idx = tid + 256 * I;                                              \
@@ -218,11 +328,6 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
             SSG_PF(0, pf0) SSG_PF(1, pf1) SSG_PF(2, pf2) SSG_PF(3, pf3)
             SSG_PF(4, pf4) SSG_PF(5, pf5) SSG_PF(6, pf6) SSG_PF(7, pf7)
 #undef SSG_PF
-            if (tid < PQ_TR) {
-                const int64_t gr = t + tid;
-                const float xn = a.xnorm[min(gr, r1 - 1)];
-                pfh = gr < r1 ? (a.metric == M_NORMCOS ? 0.f : -0.5f * xn) : -__builtin_inff();
-            }
         };
         auto stage = [&](int b) {
 #define SSG_ST(I, R)                                                                        \
@@ -234,15 +339,17 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
             SSG_ST(0, pf0) SSG_ST(1, pf1) SSG_ST(2, pf2) SSG_ST(3, pf3)
             SSG_ST(4, pf4) SSG_ST(5, pf5) SSG_ST(6, pf6) SSG_ST(7, pf7)
 #undef SSG_ST
-            if (tid < PQ_TR) s_h[b][tid] = pfh;
+            if (tid < PQ_TR)
+                s_h[b][tid] = pft + tid < r1 ? (a.metric == M_NORMCOS ? 0.f : -0.5f * pfx) : -__builtin_inff();
         };
         prefetch(r0);
         stage(0);
         __syncthreads();
         int b = 0;
-        for (int64_t t0 = r0; t0 < r1; t0 += PQ_TR) {
-            const bool more = t0 + PQ_TR < r1;
-            if (more) prefetch(t0 + PQ_TR);
+        const int64_t step = (int64_t)PQ_TR * a.tile_step;
+        for (int64_t t0 = r0; t0 < r1; t0 += step) {
+            const bool more = t0 + step < r1;
+            if (more) prefetch(t0 + step);
             const uint8_t* xb = s_x[b];
             const float* hb = s_h[b];
             // MFMA chains of one 16-row block for the wave's 4 query tiles
@@ -266,38 +373,59 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
 #pragma unroll
             for (int rb = 0; rb < PQ_TR / 16; rb++) {
                 if (rb + 1 < PQ_TR / 16) chain(rb + 1, nxt);
-                bool any = false;
+                // Each 16-query tile is tested on its own (the maximum of its 4 x 16 scores
+                // against each query's threshold; 256 scores per decision), and the append
+                // path of a tile runs only when one of its scores passes.
+                const uint32_t rowb = (uint32_t)(t0 + rb * 16 + 4 * g);
+                uint64_t qfull = 0;  // bit 16 * ct + col: that query's list is compacted now
 #pragma unroll
                 for (int ct = 0; ct < NCT; ct++) {
-                    const float mx = fmaxf(fmaxf(cur[ct][0], cur[ct][1]), fmaxf(cur[ct][2], cur[ct][3]));
-                    any = any || mx > tau[ct];
-                }
-                if (__ballot(any)) {
-                    const uint32_t rowb = (uint32_t)(t0 + rb * 16 + 4 * g);
-#pragma unroll
-                    for (int ct = 0; ct < NCT; ct++) {
-                        const int ql = 64 * w + 16 * ct + col;
+                    const float mx = __builtin_elementwise_maximum(
+                        __builtin_elementwise_maximum(cur[ct][0], cur[ct][1]),
+                        __builtin_elementwise_maximum(cur[ct][2], cur[ct][3]));
+                    if (__builtin_expect(__ballot(mx > tau[ct]) != 0, 0)) {
+                        const float t = tau[ct];
+                        uint32_t o = wp[ct];
 #pragma unroll
                         for (int r = 0; r < 4; r++) {
-                            if (cur[ct][r] > tau[ct]) {
-                                const int p = atomicAdd(&s_cnt[ql], 1);
-                                if (p < cap) list_of(ql)[p] = make_uint2(__float_as_uint(cur[ct][r]), rowb + r);
+                            if (cur[ct][r] > t) {
+                                const u32x2 e = {__float_as_uint(cur[ct][r]), rowb + r};
+                                __builtin_amdgcn_raw_buffer_store_b64(e, lrs, (int)o, 0, 0);
+                                o += 8u;
                             }
                         }
+                        wp[ct] = o;
+                        // A lane adds at most 4 entries per query and row block, so a
+                        // sub-list with at most sc - 5 entries cannot fill up on the next
+                        // (and the offset's low bits never wrap to 0).
+                        const uint64_t f = __ballot((o & smask) > full_at);
+                        qfull |= ((f | (f >> 16) | (f >> 32) | (f >> 48)) & 0xffffull) << (16 * ct);
                     }
-                    // A query gains at most 16 entries per row block, so a list with at
-                    // most cap - 16 entries cannot overflow on the next one.
-                    wave_sync();
-                    const int cq = s_cnt[64 * w + lane];
-                    uint64_t need = __ballot(cq > cap - 16);
-                    while (need) {
-                        const int qq = __builtin_ctzll(need);
-                        need &= need - 1;
-                        const float nt = compact_list(list_of(64 * w + qq), &s_cnt[64 * w + qq], kc, cap,
-                                                      s_hist[w], lane);
+                }
+                if (__builtin_expect(qfull != 0, 0)) {
+                    // counts and thresholds go through LDS so that the loop indexes no
+                    // register array
 #pragma unroll
-                        for (int ct = 0; ct < NCT; ct++)
-                            if (ct == (qq >> 4) && col == (qq & 15)) tau[ct] = nt;
+                    for (int ct = 0; ct < NCT; ct++) {
+                        s_cnt[w][64 * ct + lane] = (int)((wp[ct] & smask) >> 3);
+                        if (g == 0) s_tau[w][16 * ct + col] = tau[ct];
+                    }
+                    wave_sync();  // also makes this wave's appends visible to the reads
+                    while (qfull) {
+                        const int ql = __builtin_ctzll(qfull);
+                        qfull &= qfull - 1;
+                        int* cq = &s_cnt[w][64 * (ql >> 4) + (ql & 15)];
+                        int kept = 0;
+                        const float nt = compact_list_j<J>(list_of(64 * w + ql), sc, cq[0], cq[16], cq[32], cq[48],
+                                                      kc, false, s_hist[w], lane, &kept);
+                        if (lane < 4) cq[16 * lane] = (kept + 3 - lane) >> 2;
+                        if (lane == 0) s_tau[w][ql] = nt;
+                        lds_order();
+                    }
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) {
+                        wp[ct] = (wp[ct] & ~smask) + (uint32_t)s_cnt[w][64 * ct + lane] * 8u;
+                        tau[ct] = s_tau[w][16 * ct + col];
                     }
                 }
                 if (rb + 1 < PQ_TR / 16) {
@@ -310,18 +438,24 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
             b ^= 1;
         }
     }
-    // final lists: exactly the slice's kc best (or all of its rows)
+    // final lists: exactly the slice's kc best (or all of its rows), contiguous
     wave_sync();
-    {
-        const int cq = s_cnt[64 * w + lane];
-        uint64_t need = __ballot(cq > kc);
-        while (need) {
-            const int qq = __builtin_ctzll(need);
-            need &= need - 1;
-            (void)compact_list(list_of(64 * w + qq), &s_cnt[64 * w + qq], kc, cap, s_hist[w], lane);
+    // the sub-list counts of the wave's 64 queries go through LDS so that the loop below
+    // indexes no register array
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) s_cnt[w][64 * ct + lane] = (int)((wp[ct] & smask) >> 3);
+    lds_order();
+    for (int ql = 0; ql < 64; ql++) {
+        const int ct = ql >> 4, qc = ql & 15;
+        const int* cq = &s_cnt[w][64 * ct + qc];
+        int kept = 0;
+        const float tk = compact_list_j<J>(list_of(64 * w + ql), sc, cq[0], cq[16], cq[32], cq[48], kc, true,
+                                      s_hist[w], lane, &kept);
+        if (lane == 0) {
+            a.list_n[(qb0 + 64 * w + ql) * a.nslices + sl] = kept;
+            // sample pass: the kc-th best sampled score (-Inf with at most kc sampled rows)
+            if (a.sample) a.tau0[(qb0 + 64 * w + ql) * a.nslices + sl] = tk;
         }
-        const int cf = s_cnt[64 * w + lane];
-        a.list_n[(qb0 + 64 * w + lane) * a.nslices + sl] = cf < kc ? cf : kc;
     }
 }
 
@@ -336,6 +470,7 @@ __global__ __launch_bounds__(256) void prec_stage2(PrecArgs a, PrecOut o)
     const int g = lane >> 4, j16 = lane & 15;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
     if (q >= o.m) return;
+    const int64_t qo = o.qmap ? (int64_t)o.qmap[q] : q;  // row of qpad, qnorm and the outputs
     const int dp = 16 * V;
     const int ns = a.nslices, kc = a.kc;
 
@@ -358,11 +493,32 @@ __global__ __launch_bounds__(256) void prec_stage2(PrecArgs a, PrecOut o)
         total += __popcll(__ballot(act[j]));
     }
     const int nsel = total < o.ncand ? total : o.ncand;
+    uint32_t kth = 0u;  // key of the ncand-th best candidate (0 with fewer)
     if (total > o.ncand) {
-        (void)select_best<PQ_JS>(key, cid, act, o.ncand, s_hist[w], lane, keep);
+        uint32_t kand, diff;
+        key_bits<PQ_JS>(key, act, &kand, &diff);
+        kth = select_best<PQ_JS>(key, cid, act, o.ncand, s_hist[w], lane, keep, kand, diff);
     } else {
+        uint32_t mn = 0xffffffffu;
 #pragma unroll
-        for (int j = 0; j < PQ_JS; j++) keep[j] = act[j];
+        for (int j = 0; j < PQ_JS; j++) {
+            keep[j] = act[j];
+            if (act[j]) mn = min(mn, key[j]);
+        }
+        for (int off = 32; off >= 1; off >>= 1) mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+        if (total == o.ncand) kth = mn;
+    }
+    // Certificate of a pass that started from sampled thresholds: every row it dropped
+    // scored at most tau0 of its slice, so the candidates are exact if the ncand-th best
+    // of them scores above every slice's tau0.
+    if (a.tau0 && o.flags) {
+        float tmax = -__builtin_inff();
+        for (int s2 = 0; s2 < ns; s2++) tmax = fmaxf(tmax, a.tau0[q * ns + s2]);
+        const bool ok = tmax == -__builtin_inff() || (total >= o.ncand && key_score(kth) > tmax);
+        if (!ok && lane == 0) {
+            const unsigned int p = atomicAdd(&o.flags[0], 1u);
+            o.flags[1 + p] = (unsigned int)q;
+        }
     }
     int base = 0;
 #pragma unroll
@@ -380,7 +536,7 @@ __global__ __launch_bounds__(256) void prec_stage2(PrecArgs a, PrecOut o)
     // canonical distances: candidate u = 64*b + lane lives in slot b of its lane
     float qv[V];
     {
-        const float* qp = o.qpad + q * dp + j16 * V;
+        const float* qp = o.qpad + qo * dp + j16 * V;
 #pragma unroll
         for (int t = 0; t < V; t++) qv[t] = qp[t];
     }
@@ -429,14 +585,14 @@ __global__ __launch_bounds__(256) void prec_stage2(PrecArgs a, PrecOut o)
                 rank[b] += (dt < myd[b] || (dt == myd[b] && it < myid[b])) ? 1 : 0;
         }
     }
-    const float qn = o.qnorm ? o.qnorm[q] : 0.f;
+    const float qn = o.qnorm ? o.qnorm[qo] : 0.f;
 #pragma unroll
     for (int b = 0; b < PQ_NS; b++) {
         const int e = 64 * b + lane;
         if (e < nsel) {
             if (rank[b] < o.k) {
-                o.ids[q * o.kstride + rank[b]] = myid[b] + 1u;
-                o.dists[q * o.kstride + rank[b]] = myd[b];
+                o.ids[qo * o.kstride + rank[b]] = myid[b] + 1u;
+                o.dists[qo * o.kstride + rank[b]] = myd[b];
             }
             if (o.cand_ids) {
                 float ad;
@@ -445,20 +601,67 @@ __global__ __launch_bounds__(256) void prec_stage2(PrecArgs a, PrecOut o)
                     ad = qn - 2.0f * mysc[b];
                     if (a.metric == M_L2) ad = __builtin_sqrtf(fmaxf(ad, 0.f));
                 }
-                o.cand_ids[q * o.ncand + rank[b]] = myid[b] + 1u;
-                o.cand_dists[q * o.ncand + rank[b]] = ad;
+                o.cand_ids[qo * o.ncand + rank[b]] = myid[b] + 1u;
+                o.cand_dists[qo * o.ncand + rank[b]] = ad;
             }
         } else {
             if (e < o.k) {
-                o.ids[q * o.kstride + e] = 0u;
-                o.dists[q * o.kstride + e] = __builtin_inff();
+                o.ids[qo * o.kstride + e] = 0u;
+                o.dists[qo * o.kstride + e] = __builtin_inff();
             }
             if (o.cand_ids && e < o.ncand) {
-                o.cand_ids[q * o.ncand + e] = 0u;
-                o.cand_dists[q * o.ncand + e] = __builtin_inff();
+                o.cand_ids[qo * o.ncand + e] = 0u;
+                o.cand_dists[qo * o.ncand + e] = __builtin_inff();
             }
         }
     }
+}
+
+// dst row r = src row idx[r] for r < *count (rows of `width` 16-bit words); rows from *count
+// to max_rows are zero.
+__global__ void gather_rows_u16_kernel(const uint16_t* __restrict__ src, const uint32_t* __restrict__ idx,
+                                       const unsigned int* __restrict__ count, int64_t max_rows,
+                                       int32_t width, uint16_t* __restrict__ dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = i / width, c = i % width;
+    if (r >= max_rows) return;
+    dst[i] = r < (int64_t)*count ? src[(int64_t)idx[r] * width + c] : (uint16_t)0;
+}
+
+hipError_t launch_gather_rows_u16(const uint16_t* src, const uint32_t* idx, const unsigned int* count,
+                                  int64_t max_rows, int32_t width, uint16_t* dst, hipStream_t s)
+{
+    const int64_t tot = max_rows * width;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_u16_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, src, idx,
+                       count, max_rows, width, dst);
+    return hipGetLastError();
+}
+
+// The rows of every step-th 128-row tile (tiles 0, step, 2 step, ...) of the BF16 rows
+// and their norms, packed: sample row i is row (i / 128) * 128 * step + i % 128.  Rows at
+// or past n are not copied (the caller passes the number of sample rows below n).
+__global__ void sample_rows_kernel(const uint16_t* __restrict__ Xe, const float* __restrict__ xnorm,
+                                   int64_t ns_rows, int32_t step, int32_t dpb, uint16_t* __restrict__ Xs,
+                                   float* __restrict__ xns)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = i / dpb, c = i % dpb;
+    if (r >= ns_rows) return;
+    const int64_t orig = (r / 128) * 128 * step + r % 128;
+    Xs[i] = Xe[orig * dpb + c];
+    if (c == 0) xns[r] = xnorm[orig];
+}
+
+hipError_t launch_sample_rows(const uint16_t* Xe, const float* xnorm, int64_t ns_rows, int32_t step,
+                              int32_t dpb, uint16_t* Xs, float* xns, hipStream_t s)
+{
+    const int64_t tot = ns_rows * dpb;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sample_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, Xe, xnorm,
+                       ns_rows, step, dpb, Xs, xns);
+    return hipGetLastError();
 }
 
 // Rows of dp floats -> rows of dpb BF16 (round to nearest even), zero padded.
@@ -517,11 +720,16 @@ int32_t prec_slices(int64_t n, int64_t m, int num_cu, int kc)
 hipError_t prec_launch_stage1(const PrecArgs& a, hipStream_t s)
 {
     const dim3 grid((unsigned)(a.m_pad / PREC_QB), (unsigned)a.nslices), block(256);
+    if (a.cap != 512 && a.cap != 1024) return hipErrorInvalidValue;
+    const bool big = a.cap == 1024;
     switch (a.dpb / 32) {
-    case 1: hipLaunchKernelGGL(prec_stage1<1>, grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(prec_stage1<2>, grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL(prec_stage1<3>, grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL(prec_stage1<4>, grid, block, 0, s, a); break;
+#define SSG_P1(KK)                                                                                  \
+    case KK:                                                                                        \
+        if (big) hipLaunchKernelGGL((prec_stage1<KK, 16>), grid, block, 0, s, a);                  \
+        else hipLaunchKernelGGL((prec_stage1<KK, 8>), grid, block, 0, s, a);                       \
+        break;
+        SSG_P1(1) SSG_P1(2) SSG_P1(3) SSG_P1(4)
+#undef SSG_P1
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

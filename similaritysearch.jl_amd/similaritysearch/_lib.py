@@ -18,6 +18,7 @@ _i64 = C.c_int64
 
 SSG_OPT_QUERY_LAYOUT = 1
 SSG_OPT_PREC_SLICES = 2
+SSG_OPT_PREC_SAMPLE = 3
 
 
 class BeamParams(C.Structure):

@@ -123,6 +123,38 @@ def test_slice_count_does_not_change_results():
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("slices", [1, 3])
+def test_sampled_thresholds_do_not_change_results(oracle, slices):
+    """The sampled starting thresholds only drop rows that cannot be candidates, and
+    queries the rerank stage cannot certify are searched again: results are identical
+    with the sample pass off, on (T = 24) and on with a threshold so high (T = 2) that
+    most queries fail the certificate and go through the redo."""
+    ss = _ss()
+    from similaritysearch import _lib
+    L = _lib.load()
+    X = _unit(20000, 96, 15)
+    Q = _unit(600, 96, 16)
+    E = ss.ExhaustiveSearch(ss.NormCosine, ss.MatrixDatabase.from_rows(X))
+    h = E._handle()
+    _lib.check(L.ssg_set_option(h, _lib.SSG_OPT_PREC_SLICES, slices), "option")
+    outs = []
+    for t in (-1, 24, 2):
+        _lib.check(L.ssg_set_option(h, _lib.SSG_OPT_PREC_SAMPLE, t), "option")
+        outs.append(ss.exhaustive_candidates(E, 32, 128, ss.MatrixDatabase.from_rows(Q)))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            np.testing.assert_array_equal(a, b)
+    ids, d, ci, cd = outs[2]
+    _check(oracle, "NormCosine", X, Q[:40], 32, 128, ids.T[:40], d.T[:40], ci.T[:40], cd.T[:40])
+    # allknn form through the same path
+    _lib.check(L.ssg_set_option(h, _lib.SSG_OPT_PREC_SAMPLE, 2), "option")
+    a2 = ss.exhaustive_candidates(E, 16, 64)
+    _lib.check(L.ssg_set_option(h, _lib.SSG_OPT_PREC_SAMPLE, -1), "option")
+    a1 = ss.exhaustive_candidates(E, 16, 64)
+    for a, b in zip(a1, a2):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_ties_prefer_lower_ids(oracle):
     """300 copies of one vector among random rows: all have the same approximate and
     exact distance to it, so the candidate pass must keep the lowest ids (a compaction
