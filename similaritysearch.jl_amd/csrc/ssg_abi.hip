@@ -278,6 +278,11 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
         HIPCHK(launch_fill_f32(w.qnorm + m, m_pad - m, 0.f, s), "fill qnorm");
         HIPCHK(launch_pad_rows(dQ, m, h->dim, w.qpad, h->dp, w.qnorm, s), "pad queries");
     }
+    if (h->dp <= 128) {
+        HIPCHK(h->xList.ensure((size_t)m_pad * ns * 512 * sizeof(uint2)), "alloc candidate lists");
+        w.list = h->xList.as<uint2>();
+    }
+    w.m_valid = m;
     HIPCHK(h->cand_d.ensure((size_t)m_pad * ns * EXH_KC * sizeof(float)), "alloc cand");
     HIPCHK(h->cand_i.ensure((size_t)m_pad * ns * EXH_KC * sizeof(uint32_t)), "alloc cand");
     HIPCHK(h->slice_tau.ensure((size_t)m_pad * ns * sizeof(float)), "alloc tau");

@@ -27,6 +27,7 @@
 //   row, ascending id, KnnSorted pushes).
 #include "ssg_device.h"
 #include "ssg_kernels.h"
+#include "ssg_select.h"
 
 namespace ssg {
 
@@ -78,7 +79,7 @@ __device__ __forceinline__ float compact(float* cd, uint32_t* ci, int* cnt, int 
 }
 
 template <int DP, int KC>
-__global__ __launch_bounds__(256) void exh_stage1(const float* __restrict__ X,
+__global__ __launch_bounds__(256) void exh_stage1_lds(const float* __restrict__ X,
                                                   const float* __restrict__ xnorm, int64_t n,
                                                   int metric, const float* __restrict__ qpad,
                                                   const float* __restrict__ qnorm, int64_t rows_per_slice,
@@ -87,13 +88,13 @@ __global__ __launch_bounds__(256) void exh_stage1(const float* __restrict__ X,
 {
     constexpr int LS = DP + 4;  // LDS row stride (floats)
     constexpr int KS = DP / 4;  // MFMA k-steps
-    static_assert(EXH_QT == TR, "the query rows are staged through the row tile");
+    constexpr int QT = TR;  // 64 queries per block, staged through the row tile
     __shared__ float s_x[TR * LS];
     __shared__ float s_xn[TR];
-    __shared__ float s_cd[EXH_QT][CAP];
-    __shared__ uint32_t s_ci[EXH_QT][CAP];
-    __shared__ int s_cnt[EXH_QT];
-    __shared__ float s_qn[EXH_QT];
+    __shared__ float s_cd[QT][CAP];
+    __shared__ uint32_t s_ci[QT][CAP];
+    __shared__ int s_cnt[QT];
+    __shared__ float s_qn[QT];
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int col = lane & 15, gk = lane >> 4;
@@ -107,14 +108,14 @@ __global__ __launch_bounds__(256) void exh_stage1(const float* __restrict__ X,
     // come from LDS reads and no register of the MFMA loop depends on a global load
     // (the tile prefetch below stays in flight across the whole loop).  Query norms go
     // through LDS for the same reason.
-    for (int idx = tid; idx < EXH_QT * (DP / 4); idx += 256) {
+    for (int idx = tid; idx < QT * (DP / 4); idx += 256) {
         const int row = idx / (DP / 4), c4 = idx % (DP / 4);
         *reinterpret_cast<float4*>(&s_x[row * LS + 4 * c4]) =
-            reinterpret_cast<const float4*>(qpad + (qt * EXH_QT + row) * DP)[c4];
+            reinterpret_cast<const float4*>(qpad + (qt * QT + row) * DP)[c4];
     }
-    if (tid < EXH_QT) {
+    if (tid < QT) {
         s_cnt[tid] = 0;
-        s_qn[tid] = qnorm[qt * EXH_QT + tid];
+        s_qn[tid] = qnorm[qt * QT + tid];
     }
     __syncthreads();
     const float qn = s_qn[qloc];
@@ -231,13 +232,244 @@ __global__ __launch_bounds__(256) void exh_stage1(const float* __restrict__ X,
         float nt = rl_f(tau, qq);
         if (c > KC) nt = compact<KC>(s_cd[ql], s_ci[ql], &s_cnt[ql], lane);
         const int cc = min(c, KC);
-        const int64_t q = qt * EXH_QT + ql;
+        const int64_t q = qt * QT + ql;
         const int64_t base = (q * nslices + sl) * KC;
         if (lane < KC) {
             cand_d[base + lane] = lane < cc ? s_cd[ql][lane] : __builtin_inff();
             cand_i[base + lane] = lane < cc ? s_ci[ql][lane] : 0xffffffffu;
         }
         if (lane == 0) slice_tau[q * nslices + sl] = nt;
+    }
+}
+
+// Stage 1 for dp <= 128: block = 128 queries x one DB slice, 4 waves of 32 queries (two
+// independent accumulator chains of v_mfma_f32_16x16x4f32 per wave, the B operands in
+// registers for the whole slice); 64-row DB tiles double-buffered in LDS, one barrier per
+// tile, the next tile prefetched into registers with unconditional (clamped) loads.  The
+// accumulator starts at -||x||^2/2 (SqL2, L2) or 0 (NormCosine), so the MFMA result is a
+// score s = x.q - ||x||^2/2 (or x.q): larger is nearer, a row past the slice starts at
+// -Inf, and the approximate distance is ||q||^2 - 2 s (or 1 - s).  Each 16-query tile is
+// tested against its queries' thresholds; a tile with a passing score appends to the
+// queries' candidate lists in global memory (4 sub-lists per list, one per 16-lane group,
+// written through a buffer resource at a running offset); a full sub-list triggers a
+// compaction to about EXH_KC entries and raises the threshold (ssg_select.h).  At the end
+// each list keeps exactly the slice's EXH_KC best rows by (score, id); they are written
+// as approximate distances with slice_tau, a lower bound on the approximate distance of
+// every row of the slice that was dropped.
+constexpr int X_TR = 64;  // DB rows per LDS tile
+
+template <int DP>
+__global__ __launch_bounds__(256, 2) void exh_stage1_l(const float* __restrict__ X,
+                                                       const float* __restrict__ xnorm, int64_t n,
+                                                       int metric, const float* __restrict__ qpad,
+                                                       const float* __restrict__ qnorm, int64_t m,
+                                                       int64_t rows_per_slice, int nslices, uint2* list,
+                                                       float* cand_d, uint32_t* cand_i, float* slice_tau)
+{
+    constexpr int KS = DP / 4;            // MFMA k-steps
+    constexpr int LS = DP + 4;            // LDS row stride in floats (staggers banks)
+    constexpr int NCT = 2;                // 16-query column tiles per wave
+    constexpr int U4 = DP / 4;            // float4 per row
+    constexpr int PF = X_TR * U4 / 256;   // float4 per thread per tile
+    constexpr int CAPL = 512;             // list entries per (query, slice)
+    constexpr int SC = CAPL / 4;          // sub-list entries
+    static_assert(X_TR * U4 % 256 == 0 && PF <= 8, "tile must split evenly over the block");
+    static_assert(EXH_QT == 4 * 16 * NCT, "queries per block");
+    __shared__ __attribute__((aligned(16))) float s_x[2][X_TR * LS];
+    __shared__ __attribute__((aligned(16))) float s_h[2][X_TR];
+    __shared__ __attribute__((aligned(16))) uint32_t s_hist[4][256];
+    __shared__ int s_cnt[4][NCT * 64];
+    __shared__ float s_tau[4][NCT * 16];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int col = lane & 15, g = lane >> 4;
+    const int64_t qb0 = (int64_t)blockIdx.x * EXH_QT;
+    const int sl = blockIdx.y;
+    const int64_t r0 = (int64_t)sl * rows_per_slice;
+    const int64_t r1 = min(n, r0 + rows_per_slice);
+    constexpr int kc = EXH_KC;
+    const int64_t qstride = (int64_t)nslices * CAPL;
+    uint2* lbase = list + (qb0 * nslices + sl) * (int64_t)CAPL;
+    auto list_of = [&](int ql) -> uint2* { return lbase + ql * qstride; };
+
+    // B operands: the lane's run of KS elements of query 32w + 16ct + col (query rows past
+    // m read row m - 1; their results are never used)
+    float bq[NCT][KS];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+        const int64_t qr = min(qb0 + 32 * w + 16 * ct + col, m - 1);
+        const float4* qp = reinterpret_cast<const float4*>(qpad + qr * DP + g * KS);
+#pragma unroll
+        for (int s = 0; s < KS / 4; s++) {
+            const float4 v = qp[s];
+            bq[ct][4 * s] = v.x;
+            bq[ct][4 * s + 1] = v.y;
+            bq[ct][4 * s + 2] = v.z;
+            bq[ct][4 * s + 3] = v.w;
+        }
+    }
+    const uint32_t smask = (uint32_t)SC * 8u - 1u;
+    const uint32_t full_at = (uint32_t)(SC - 5) * 8u;
+    const __amdgpu_buffer_rsrc_t lrs = __builtin_amdgcn_make_buffer_rsrc(
+        lbase, 0, (int)(((EXH_QT - 1) * qstride + CAPL) * (int64_t)sizeof(uint2)), 0x00020000);
+    float tau[NCT];
+    uint32_t wp[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+        tau[ct] = -__builtin_inff();
+        wp[ct] = (uint32_t)(((32 * w + 16 * ct + col) * qstride + g * SC) * (int64_t)sizeof(uint2));
+    }
+
+    if (r0 < r1) {
+        float4 pf0, pf1, pf2, pf3, pf4, pf5, pf6, pf7;  // the next tile
+        float pfx = 0.f;                                  // the norm of its row tid % 64
+        int64_t pft = r0;
+        // unconditional loads, consumed only by stage()
+        auto prefetch = [&](int64_t t) {
+            pft = t;
+            pfx = xnorm[min(t + (tid & (X_TR - 1)), r1 - 1)];
+#define SSG_PF(I, R)                                                                 \
+            if (I < PF) {                                                            \
+                const int idx = tid + 256 * I;                                       \
+                const int row = idx / U4, c = idx % U4;                              \
+                const int64_t gr = min(t + row, r1 - 1);                             \
+                R = reinterpret_cast<const float4*>(X + gr * DP)[c];                 \
+            }
+            SSG_PF(0, pf0) SSG_PF(1, pf1) SSG_PF(2, pf2) SSG_PF(3, pf3)
+            SSG_PF(4, pf4) SSG_PF(5, pf5) SSG_PF(6, pf6) SSG_PF(7, pf7)
+#undef SSG_PF
+        };
+        auto stage = [&](int b) {
+#define SSG_ST(I, R)                                                                 \
+            if (I < PF) {                                                            \
+                const int idx = tid + 256 * I;                                       \
+                const int row = idx / U4, c = idx % U4;                              \
+                *reinterpret_cast<float4*>(&s_x[b][row * LS + 4 * c]) = R;           \
+            }
+            SSG_ST(0, pf0) SSG_ST(1, pf1) SSG_ST(2, pf2) SSG_ST(3, pf3)
+            SSG_ST(4, pf4) SSG_ST(5, pf5) SSG_ST(6, pf6) SSG_ST(7, pf7)
+#undef SSG_ST
+            if (tid < X_TR)
+                s_h[b][tid] = pft + tid < r1 ? (metric == M_NORMCOS ? 0.f : -0.5f * pfx) : -__builtin_inff();
+        };
+        prefetch(r0);
+        stage(0);
+        __syncthreads();
+        int b = 0;
+        for (int64_t t0 = r0; t0 < r1; t0 += X_TR) {
+            const bool more = t0 + X_TR < r1;
+            if (more) prefetch(t0 + X_TR);
+            const float* xb = s_x[b];
+            const float* hb = s_h[b];
+            auto chain = [&](int rb, floatx4 (&acc)[NCT]) {
+                const floatx4 c0 = *reinterpret_cast<const floatx4*>(&hb[rb * 16 + 4 * g]);
+                const float* ap = xb + (rb * 16 + col) * LS + g * KS;
+                floatx4 c[NCT];
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) c[ct] = c0;
+#pragma unroll
+                for (int s4 = 0; s4 < KS / 4; s4++) {
+                    const float4 av = *reinterpret_cast<const float4*>(ap + 4 * s4);
+                    const float a4[4] = {av.x, av.y, av.z, av.w};
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ct++)
+                            c[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[u], bq[ct][4 * s4 + u], c[ct], 0, 0, 0);
+                }
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) acc[ct] = c[ct];
+            };
+            floatx4 cur[NCT], nxt[NCT];
+            chain(0, cur);
+#pragma unroll
+            for (int rb = 0; rb < X_TR / 16; rb++) {
+                if (rb + 1 < X_TR / 16) chain(rb + 1, nxt);
+                const uint32_t rowb = (uint32_t)(t0 + rb * 16 + 4 * g);
+                uint64_t qfull = 0;  // bit 16 * ct + col: that query's list is compacted now
+#pragma unroll
+                for (int ct = 0; ct < NCT; ct++) {
+                    const float mx = __builtin_elementwise_maximum(
+                        __builtin_elementwise_maximum(cur[ct][0], cur[ct][1]),
+                        __builtin_elementwise_maximum(cur[ct][2], cur[ct][3]));
+                    if (__builtin_expect(__ballot(mx > tau[ct]) != 0, 0)) {
+                        const float t = tau[ct];
+                        uint32_t o = wp[ct];
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            if (cur[ct][r] > t) {
+                                const u32x2 e = {__float_as_uint(cur[ct][r]), rowb + r};
+                                __builtin_amdgcn_raw_buffer_store_b64(e, lrs, (int)o, 0, 0);
+                                o += 8u;
+                            }
+                        }
+                        wp[ct] = o;
+                        // at most 4 appends per lane, query and row block: a sub-list with
+                        // at most SC - 5 entries cannot fill up on the next one
+                        const uint64_t f = __ballot((o & smask) > full_at);
+                        qfull |= ((f | (f >> 16) | (f >> 32) | (f >> 48)) & 0xffffull) << (16 * ct);
+                    }
+                }
+                if (__builtin_expect(qfull != 0, 0)) {
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) {
+                        s_cnt[w][64 * ct + lane] = (int)((wp[ct] & smask) >> 3);
+                        if (g == 0) s_tau[w][16 * ct + col] = tau[ct];
+                    }
+                    wave_sync();  // this wave's appends are visible to the reads
+                    while (qfull) {
+                        const int ql = __builtin_ctzll(qfull);
+                        qfull &= qfull - 1;
+                        int* cq = &s_cnt[w][64 * (ql >> 4) + (ql & 15)];
+                        int kept = 0;
+                        const float nt = compact_list_j<8>(list_of(32 * w + ql), SC, cq[0], cq[16], cq[32], cq[48],
+                                                           kc, false, s_hist[w], lane, &kept);
+                        if (lane < 4) cq[16 * lane] = (kept + 3 - lane) >> 2;
+                        if (lane == 0) s_tau[w][ql] = nt;
+                        lds_order();
+                    }
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) {
+                        wp[ct] = (wp[ct] & ~smask) + (uint32_t)s_cnt[w][64 * ct + lane] * 8u;
+                        tau[ct] = s_tau[w][16 * ct + col];
+                    }
+                }
+                if (rb + 1 < X_TR / 16) {
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ct++) cur[ct] = nxt[ct];
+                }
+            }
+            if (more) stage(b ^ 1);
+            __syncthreads();
+            b ^= 1;
+        }
+    }
+    // final lists: the slice's EXH_KC best rows by (score, id), written as approximate
+    // distances; slice_tau bounds every dropped row (its score is at most the larger of
+    // the running threshold and the EXH_KC-th best score)
+    wave_sync();
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+        s_cnt[w][64 * ct + lane] = (int)((wp[ct] & smask) >> 3);
+        if (g == 0) s_tau[w][16 * ct + col] = tau[ct];
+    }
+    lds_order();
+    for (int ql = 0; ql < 32; ql++) {
+        const int* cq = &s_cnt[w][64 * (ql >> 4) + (ql & 15)];
+        int kept = 0;
+        const float tk = compact_list_j<8>(list_of(32 * w + ql), SC, cq[0], cq[16], cq[32], cq[48], kc, true,
+                                           s_hist[w], lane, &kept);
+        const float drop = fmaxf(tk, s_tau[w][ql]);
+        const int64_t q = qb0 + 32 * w + ql;
+        const float qn = q < m ? qnorm[q] : 0.f;
+        auto dist_of = [&](float sc) { return metric == M_NORMCOS ? 1.0f - sc : qn - 2.0f * sc; };
+        const int64_t base = (q * nslices + sl) * EXH_KC;
+        if (lane < EXH_KC) {
+            const uint2 v = lane < kept ? list_of(32 * w + ql)[lane] : make_uint2(0u, 0xffffffffu);
+            cand_d[base + lane] = lane < kept ? dist_of(__uint_as_float(v.x)) : __builtin_inff();
+            cand_i[base + lane] = lane < kept ? v.y : 0xffffffffu;
+        }
+        if (lane == 0) slice_tau[q * nslices + sl] = drop == -__builtin_inff() ? __builtin_inff() : dist_of(drop);
     }
 }
 
@@ -540,9 +772,12 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ X
 
 // Stage-1 blocks resident per CU: LDS-bound (row tile + candidate lists), at most two
 // by registers.
+// Stage-1 blocks resident per CU: two for the list kernel (dp <= 128); LDS-bound (row
+// tile + candidate lists), at most two, for the dp > 128 kernel.
 static int stage1_blocks_per_cu(int dp)
 {
-    const int lds = TR * (dp + 4) * 4 + TR * 4 + EXH_QT * CAP * 8 + EXH_QT * 8;
+    if (dp <= 128) return 2;
+    const int lds = TR * (dp + 4) * 4 + TR * 4 + TR * CAP * 8 + TR * 8;
     const int by_lds = (160 * 1024) / lds;
     return by_lds < 1 ? 1 : (by_lds > 2 ? 2 : by_lds);
 }
@@ -578,8 +813,16 @@ template <int DP>
 static void stage1_dp(dim3 grid, hipStream_t s, const ExhArgs& a, const ExhWork& w, int64_t rps,
                       int ns)
 {
-    hipLaunchKernelGGL((exh_stage1<DP, EXH_KC>), grid, dim3(256), 0, s, a.X, a.xnorm, a.n,
-                       a.metric, w.qpad, w.qnorm, rps, ns, w.cand_d, w.cand_i, w.slice_tau);
+    if constexpr (DP <= 128) {
+        hipLaunchKernelGGL((exh_stage1_l<DP>), grid, dim3(256), 0, s, a.X, a.xnorm, a.n, a.metric, w.qpad,
+                           w.qnorm, w.m_valid, rps, ns, w.list, w.cand_d, w.cand_i, w.slice_tau);
+    } else {
+        // 64-query blocks; query rows up to roundup(m_valid, 64) (the padded rows of X when
+        // the queries are the database itself)
+        const dim3 g64((unsigned)((w.m_valid + TR - 1) / TR), grid.y);
+        hipLaunchKernelGGL((exh_stage1_lds<DP, EXH_KC>), g64, dim3(256), 0, s, a.X, a.xnorm, a.n, a.metric,
+                           w.qpad, w.qnorm, rps, ns, w.cand_d, w.cand_i, w.slice_tau);
+    }
 }
 
 hipError_t exh_launch_stage1(const ExhArgs& a, const ExhWork& w, int nslices, hipStream_t s)
@@ -587,6 +830,7 @@ hipError_t exh_launch_stage1(const ExhArgs& a, const ExhWork& w, int nslices, hi
     const int64_t qtiles = (a.m + EXH_QT - 1) / EXH_QT;
     int64_t rps = (a.n + nslices - 1) / nslices;
     rps = (rps + TR - 1) / TR * TR;
+    if (a.dp <= 128 && !w.list) return hipErrorInvalidValue;
     const dim3 grid((unsigned)qtiles, (unsigned)nslices);
     switch (a.dp) {
     case 16: stage1_dp<16>(grid, s, a, w, rps, nslices); break;

@@ -77,7 +77,7 @@ struct ssg_handle {
     int64_t vis_stride = 0;
     int nslots = 0;
     // exhaustive work
-    DevBuf qpad, qnorm, cand_d, cand_i, slice_tau, flags, part_i, part_d;
+    DevBuf qpad, qnorm, cand_d, cand_i, slice_tau, flags, part_i, part_d, xList;
     // host-call staging
     DevBuf sQ, sIds, sD, sCd, sCb, sK, sB, sH, sHc, sO;
     // builder scratch

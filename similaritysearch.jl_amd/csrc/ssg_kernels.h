@@ -97,6 +97,8 @@ struct ExhWork {
     unsigned int* flags;   // [0] = number of flagged queries, then the list
     int64_t m_cap;
     int32_t nslices_cap;
+    int64_t m_valid;       // queries actually present (qpad may hold fewer than m_pad rows)
+    uint2* list;           // dp <= 128: m_pad * nslices * 512 candidate-list entries
 };
 hipError_t exh_launch_stage1(const ExhArgs& a, const ExhWork& w, int nslices, hipStream_t s);
 hipError_t exh_launch_stage2(const ExhArgs& a, const ExhWork& w, float xnmax, int nslices,
@@ -106,7 +108,7 @@ hipError_t exh_launch_rescan(const ExhArgs& a, const ExhWork& w, int f0, int nf,
                              hipStream_t s);
 int32_t exhaustive_slices(int64_t n, int64_t m, int num_cu, int dp);
 constexpr int EXH_KC = 32;        // candidates kept per (query, slice)
-constexpr int EXH_QT = 64;        // queries per block of exhaustive stage 1
+constexpr int EXH_QT = 128;       // queries per block of exhaustive stage 1 (query padding)
 
 // Precision form of exhaustive k-NN: BF16 MFMA candidate pass + exact rerank.
 constexpr int PREC_QB = 256;          // queries per candidate-pass block
