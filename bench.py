@@ -58,9 +58,10 @@ def main():
     ap.add_argument("--learned", type=str, default="",
                     help="quantile,rerank,cap,delta (skip tuning)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("search", "allknn-bf16"), default="search",
+    ap.add_argument("--workload", choices=("search", "allknn-bf16", "exhaustive"), default="search",
                     help="search: the headline graph-search metric (default); allknn-bf16: "
-                         "exhaustive allknn with the BF16 candidate pass and FP32 rerank")
+                         "exhaustive allknn with the BF16 candidate pass and FP32 rerank; "
+                         "exhaustive: exact FP32 exhaustive k-NN")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the query-sharded "
@@ -70,6 +71,8 @@ def main():
         sys.exit("--steps must be >= 1")
     if args.workload == "allknn-bf16":
         return allknn_bf16(args)
+    if args.workload == "exhaustive":
+        return exhaustive_fp32(args)
 
     import numpy as np
     import similaritysearch as ss
@@ -349,6 +352,82 @@ def main():
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+MFMA_FP32_PEAK_TFLOPS = 157.3
+
+
+def exhaustive_fp32(args):
+    """Exact exhaustive k-NN (ExhaustiveSearch + searchbatch, k = 10, SqL2): the 1,000,000 x
+    128 GMM database (seed 4) with 10,000 queries (seed 5) per step, and the 100,000 x 32
+    U[0,1) database (seed 1) with 1,000 queries (seed 2).  `value` is queries/s on the
+    first; `roofline` prices its 2 * dim * n * m MFMA flops over the whole call against the
+    FP32 dense peak.  Results are checked against the oracle on 200 queries of each."""
+    import numpy as np
+    import similaritysearch as ss
+    from similaritysearch import datasets, _lib
+    from similaritysearch.device import DeviceArray, Event, Stream
+
+    L = _lib.load()
+    k = 10
+    st = Stream()
+    sp = st.handle
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    res = {}
+    for name, n, m, dim, X, Q in [
+            ("gmm_1m128", args.n, 10_000, 128, datasets.gmm(args.n, 128, seed=4), datasets.gmm(10_000, 128, seed=5)),
+            ("uniform_100k32", 100_000, 1_000, 32, datasets.uniform(100_000, 32, 1), datasets.uniform(1_000, 32, 2))]:
+        E = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(X))
+        h = E._handle()
+        dQ = DeviceArray.from_numpy(Q)
+        dI = DeviceArray((m, k), np.uint32)
+        dD = DeviceArray((m, k), np.float32)
+
+        def run():
+            _lib.check(L.ssg_exhaustive_knn_dev(h, dQ.ptr, m, k, dI.ptr, dD.ptr, sp), "searchbatch!")
+        for _ in range(max(1, args.warmup)):
+            run()
+        st.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(st)
+        for _ in range(args.steps):
+            run()
+        e1.record(st)
+        st.synchronize()
+        ms = e0.elapsed_ms(e1) / args.steps
+        gi, gd = pyoracle.exhaustive("SqL2", X, Q[:200], k, nthreads=int(os.environ.get("OMP_NUM_THREADS", 8)))
+        ok = bool((dI.numpy()[:200] == gi).all() and (dD.numpy()[:200] == gd).all())
+        tf = 2.0 * dim * n * m / (ms * 1e-3) / 1e12
+        log(f"exhaustive {name}: {ms:.3f} ms per {m} queries, {tf:.1f} TFLOP/s, oracle match on 200 queries: {ok}")
+        res[name] = {"ms": round(ms, 3), "qps": round(m / (ms * 1e-3), 1), "tflops": round(tf, 1),
+                     "oracle_match_200": ok, "n": n, "m": m, "dim": dim}
+        cpu = None
+        if name == "gmm_1m128" and not args.no_cpu:
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            nq, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < 10.0 and nq < m:
+                pyoracle.exhaustive("SqL2", X, Q[nq:nq + 64], k, nthreads=threads)
+                nq += 64
+            cpu_s = time.perf_counter() - t0
+            cpu = {"value": round(nq / cpu_s, 2), "unit": "queries/s", "cores": threads, "kind": "port",
+                   "sample": f"restated ExhaustiveSearch (FP32 canonical distance, ascending scan, KnnSorted) "
+                             f"of {nq} queries against the 1,000,000 x 128 database, OpenMP over queries"}
+            res["cpu"] = cpu
+        E.close()
+    main = res["gmm_1m128"]
+    out = {
+        "metric": "exact exhaustive k-NN queries/s, 1M×128 SqL2 k=10 (10,000 queries per step)",
+        "value": main["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": main["ms"], "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic: seeded GMM (1M x 128) and U[0,1) (100k x 32) rows",
+        "config": {"workload": "ExhaustiveSearch searchbatch, exact FP32 canonical distances, k=10",
+                   "gmm_1m128": main, "uniform_100k32": res["uniform_100k32"]},
+        "roofline": {"bound": "mfma", "achieved": main["tflops"], "peak": MFMA_FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(main["tflops"] / MFMA_FP32_PEAK_TFLOPS, 4),
+                     "traffic": None, "kernel": "exh_stage1_l<128> + exh_stage2<8> (whole call)",
+                     "kernel_ms": main["ms"], "flops_per_launch": 2.0 * 128 * main["n"] * main["m"]},
+        "cpu_baseline": res.get("cpu"),
+    }
+    print(json.dumps(out), flush=True)
 
 
 def allknn_bf16(args):
