@@ -58,10 +58,11 @@ def main():
     ap.add_argument("--learned", type=str, default="",
                     help="quantile,rerank,cap,delta (skip tuning)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("search", "allknn-bf16", "exhaustive"), default="search",
-                    help="search: the headline graph-search metric (default); allknn-bf16: "
-                         "exhaustive allknn with the BF16 candidate pass and FP32 rerank; "
-                         "exhaustive: exact FP32 exhaustive k-NN")
+    ap.add_argument("--workload", choices=("search", "allknn-bf16", "allknn-fp8", "exhaustive"),
+                    default="search",
+                    help="search: the headline graph-search metric (default); allknn-bf16 / "
+                         "allknn-fp8: exhaustive allknn with the BF16 / FP8 candidate pass and "
+                         "FP32 rerank; exhaustive: exact FP32 exhaustive k-NN")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the query-sharded "
@@ -69,8 +70,8 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
-    if args.workload == "allknn-bf16":
-        return allknn_bf16(args)
+    if args.workload in ("allknn-bf16", "allknn-fp8"):
+        return allknn_prec(args, args.workload[len("allknn-"):])
     if args.workload == "exhaustive":
         return exhaustive_fp32(args)
 
@@ -352,6 +353,7 @@ def main():
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+MFMA_FP8_PEAK_TFLOPS = 5000.0
 MFMA_FP32_PEAK_TFLOPS = 157.3
 
 
@@ -430,15 +432,16 @@ def exhaustive_fp32(args):
     print(json.dumps(out), flush=True)
 
 
-def allknn_bf16(args):
+def allknn_prec(args, fmt):
     """allknn (k = 32) of 1,000,000 unit vectors (96 dims, N(0, I) seed 6, L2-normalised),
-    NormCosine, with the precision form: BF16 candidate pass on the matrix cores keeping
-    128 candidates per object, then rerank! with the FP32 canonical distance.  A step is
-    one whole allknn; `value` is objects (queries) per second.  `roofline`: the candidate
-    pass is MFMA-bound at 2 * 96 * n^2 flops per allknn, priced against the BF16 dense
-    peak over the time of the whole call (rerank included, so the fraction is a floor).
-    Recall and errors are measured on 10,000 evenly spaced objects against the exact FP32
-    exhaustive search (recall) and FP64 (errors)."""
+    NormCosine, with the precision form: BF16 or FP8 (fmt) candidate pass on the matrix
+    cores keeping 128 candidates per object, then rerank! with the FP32 canonical
+    distance.  A step is one whole allknn; `value` is objects (queries) per second.
+    `roofline`: the candidate pass is MFMA-bound at 2 * 96 * n^2 flops per allknn (the
+    FP8 pass pads rows to 128 values; the padding is not counted), priced against the
+    BF16 or FP8 dense peak over the time of the whole call (rerank included, so the
+    fraction is a floor).  Recall and errors are measured on 10,000 evenly spaced objects
+    against the exact FP32 exhaustive search (recall) and FP64 (errors)."""
     import numpy as np
     import similaritysearch as ss
     from similaritysearch import datasets, _lib
@@ -459,8 +462,10 @@ def allknn_bf16(args):
     dCI = DeviceArray((n, ncand), np.uint32)
     dCD = DeviceArray((n, ncand), np.float32)
 
+    pcode = {"bf16": 1, "fp8": 2}[fmt]  # SSG_BF16, SSG_FP8
+
     def run(cands):
-        _lib.check(L.ssg_allknn_exhaustive_prec_dev(h, k, 1, ncand, dI.ptr, dD.ptr,
+        _lib.check(L.ssg_allknn_exhaustive_prec_dev(h, k, pcode, ncand, dI.ptr, dD.ptr,
                                                      dCI.ptr if cands else None,
                                                      dCD.ptr if cands else None, sp), "allknn")
 
@@ -474,7 +479,7 @@ def allknn_bf16(args):
     e1.record(st)
     st.synchronize()
     ms = e0.elapsed_ms(e1) / args.steps
-    log(f"allknn-bf16: {ms:.2f} ms per allknn of {n}")
+    log(f"allknn-{fmt}: {ms:.2f} ms per allknn of {n}")
     run(True)
     st.synchronize()
     ids, dists = dI.numpy(), dD.numpy()
@@ -515,8 +520,10 @@ def allknn_bf16(args):
                          f"KnnSorted) of {nq} of the objects against all {n}, OpenMP over queries; "
                          f"a whole allknn would take {n / (nq / cpu_s) / 60:.0f} min"}
 
+    FMT = fmt.upper()
+    peak = MFMA_BF16_PEAK_TFLOPS if fmt == "bf16" else MFMA_FP8_PEAK_TFLOPS
     out = {
-        "metric": "allknn objects/s, 1M×96 NormCosine k=32, BF16 candidate pass (128 per object) + FP32 rerank",
+        "metric": f"allknn objects/s, 1M×96 NormCosine k=32, {FMT} candidate pass (128 per object) + FP32 rerank",
         "value": round(n / (ms * 1e-3), 1),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -526,17 +533,19 @@ def allknn_bf16(args):
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "bf16",
+        "dtype": fmt,
         "data": "synthetic: N(0, I) rows, seed 6, L2-normalised",
         "config": {"workload": f"allknn over ExhaustiveSearch, {n}x{dim} NormCosine, k={k}, "
-                               f"{ncand} BF16 candidates per object reranked in FP32",
+                               f"{ncand} {FMT} candidates per object reranked in FP32",
                    f"recall@{k}_vs_fp32_exact": round(rec, 5),
                    "final_abs_err_vs_fp64": {"max": float(err_final.max()), "mean": float(err_final.mean())},
                    "candidate_abs_err_vs_fp64": {"max": float(err_cand.max()), "mean": float(err_cand.mean())},
                    "error_sample": len(sub)},
-        "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
-                     "traffic": None, "kernel": "prec_stage1<3> + prec_stage2<6> (whole call)",
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": peak,
+                     "unit": "TFLOP/s", "frac": round(tflops / peak, 4),
+                     "traffic": None,
+                     "kernel": f"prec_stage1<{'3, 8, false' if fmt == 'bf16' else '1, 8, true'}> + "
+                               "prec_stage2<6> (whole call)",
                      "kernel_ms": round(ms, 3), "flops_per_launch": flops},
         "cpu_baseline": cpu,
     }

@@ -47,8 +47,9 @@ enum {
 /* Distance functions (src/dist/vecs.jl:52-102, src/dist/cos.jl:39,76-85). */
 enum { SSG_SQL2 = 0, SSG_L2 = 1, SSG_NORMCOS = 2 };
 
-/* Storage precision of the device copy of the database. */
-enum { SSG_F32 = 0, SSG_BF16 = 1 };
+/* Storage precision of the device copy of the database.  SSG_FP8 (OCP e4m3fn) is taken by
+ * the precision form of exhaustive search only. */
+enum { SSG_F32 = 0, SSG_BF16 = 1, SSG_FP8 = 2 };
 
 /* Device memory, streams and events (thin HIP wrappers, no reference counterpart) for hosts
  * that have no GPU runtime of their own; the _dev entry points take these pointers.
@@ -213,16 +214,19 @@ int ssg_allknn_exhaustive(ssg_handle* h, int32_t k, uint32_t* ids, float* dists)
 /* Precision form of searchbatch!/allknn! over ExhaustiveSearch (no reference counterpart;
  * the reference computes Float32 distances only).  It is allknn/searchbatch with a
  * candidate queue of ncand entries followed by rerank! (src/rerank.jl:22-98):
- *   1. a candidate pass with SSG_BF16 operands on the matrix cores (FP32 accumulation)
- *      keeps, per query, the ncand objects with the best approximate distance (ties to
- *      the lower id);
+ *   1. a candidate pass on the matrix cores (FP32 accumulation) with precision SSG_BF16
+ *      operands (round to nearest even) or SSG_FP8 operands (OCP e4m3fn of the values
+ *      times 2^e, round to nearest even; e is the largest integer with
+ *      max |value| * 2^e <= 448, one e for the database and one for the queries, clamped
+ *      to [-120, 120]) keeps, per query, the ncand objects with the best approximate
+ *      distance (ties to the lower id);
  *   2. those are rescored with the canonical FP32 distance and the k first in
  *      (distance, id) order are returned, (k, m) column-major, 1-based ids.
  * The returned distances are exact; an object outside the ncand best approximate
  * candidates can be missed (recall < 1).  k <= ncand <= 256, dim <= 128.  cand_ids /
  * cand_dists (nullable together, (ncand, m)) receive every candidate in (distance, id)
- * order with its approximate distance (BF16 operands, FP32 accumulation).  The allknn
- * forms take every database object as a query, self included (src/allknn.jl:66-91). */
+ * order with its approximate distance.  The allknn forms take every database object as
+ * a query, self included (src/allknn.jl:66-91). */
 int ssg_exhaustive_knn_prec(ssg_handle* h, const float* Q, int64_t m, int32_t k, int32_t precision,
                             int32_t ncand, uint32_t* ids, float* dists, uint32_t* cand_ids,
                             float* cand_dists);

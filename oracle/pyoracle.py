@@ -77,6 +77,8 @@ def lib():
         L.or_build_graph.argtypes = [C.c_int, _p, _i64, C.c_int, C.POINTER(BuildParams),
                                      C.POINTER(Graph)]
         L.or_free.argtypes = [_p]
+        L.or_fp8_mfma.argtypes = [_p, _i64, _p, _i64, C.c_int, _p, C.c_int, C.c_int, C.c_int, _p, _p]
+        L.or_fma_norms.argtypes = [_p, _i64, C.c_int, _p]
         L.or_optimize_minrecall.restype = C.c_double
         L.or_optimize_minrecall.argtypes = [C.c_int, _p, _i64, C.c_int, _p, _p, _p, _i64, _p,
                                             _i64, _i32, C.c_float, _i64, _i32, _p, _p, _p,
@@ -279,13 +281,48 @@ def bf16_round(x):
     return (r & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
 
 
-def prec_approx(metric, X, Q):
-    """(m, n) FP64 approximate distances of the BF16 candidate pass: s = sum of the exact
-    products of the BF16-rounded coordinates; NormCosine 1 - s, SqL2 |q|^2 + |x|^2 - 2 s
-    (norms of the FP32 vectors), L2 its square root."""
+def fp8_scale_exp(a):
+    """The FP8 operand scale of a set of values: the largest integer e with
+    max |a| * 2^e <= 448 (the largest finite e4m3fn value), clamped to [-120, 120]; 0 when
+    every value is 0 or one is not finite.  Pinned by hand cases in tests/test_oracle.py."""
+    a = np.asarray(a, np.float32)
+    mx = float(np.max(np.abs(a))) if a.size else 0.0
+    if mx == 0.0 or not np.isfinite(mx):
+        return 0
+    f, E = np.frexp(mx)          # mx = f * 2^E, f in [0.5, 1); 448 = 0.875 * 2^9
+    e = (9 if f <= 0.875 else 8) - int(E)
+    return min(120, max(-120, e))
+
+
+def fp8_round(x, e):
+    """float32 values -> the OCP FP8 e4m3fn value nearest x * 2^e (round to nearest, ties
+    to even; 3 mantissa bits, exponent bias 7, subnormal quantum 2^-9), times 2^-e, as
+    float64.  Needs |x * 2^e| <= 448.  Pinned against torch's float8_e4m3fn conversion and
+    hand cases in tests/test_oracle.py."""
+    y = np.ldexp(np.asarray(x, np.float32).astype(np.float64), e)
+    a = np.abs(y)
+    if (a > 448.0).any():
+        raise ValueError("value beyond the e4m3fn range")
+    _, E = np.frexp(np.where(a > 0, a, 1.0))   # a in [2^(E-1), 2^E)
+    q = np.ldexp(1.0, np.maximum(E - 1, -6) - 3)  # spacing of the binade (subnormals: 2^-9)
+    return np.ldexp(np.round(y / q) * q, -e)     # np.round: ties to even
+
+
+def _prec_operands(X, Q, fmt):
+    if fmt == "bf16":
+        return bf16_round(X).astype(np.float64), bf16_round(Q).astype(np.float64)
+    if fmt == "fp8":
+        return fp8_round(X, fp8_scale_exp(X)), fp8_round(Q, fp8_scale_exp(Q))
+    raise ValueError(fmt)
+
+
+def prec_approx(metric, X, Q, fmt="bf16"):
+    """(m, n) FP64 approximate distances of the candidate pass: s = sum of the exact
+    products of the rounded coordinates (fmt "bf16": BF16; "fp8": e4m3fn of X * 2^ex and
+    Q * 2^eq, ex and eq from fp8_scale_exp of X and of Q, scaled back); NormCosine 1 - s,
+    SqL2 |q|^2 + |x|^2 - 2 s (norms of the FP32 vectors), L2 its square root."""
     mt = METRICS.get(metric, metric)
-    Xb = bf16_round(X).astype(np.float64)
-    Qb = bf16_round(Q).astype(np.float64)
+    Xb, Qb = _prec_operands(X, Q, fmt)
     s = Qb @ Xb.T
     if mt == NORMCOS:
         return 1.0 - s
@@ -296,15 +333,16 @@ def prec_approx(metric, X, Q):
 
 
 def prec_error_bound(metric, X, Q):
-    """(m, n) bound on |GPU approximate distance - prec_approx| for the SqL2/NormCosine
-    forms: FP32 accumulation of the dim exact BF16 products plus the -|x|^2/2 start value
-    in any order ((dim + 2) units of 2^-24 of the absolute sum), FP32 norms of dim terms,
-    and the final FP32 roundings."""
+    """(m, n) bound on |GPU approximate distance - prec_approx| of the BF16 pass for the
+    SqL2/NormCosine forms: FP32 accumulation of the dim exact BF16 products plus the
+    -|x|^2/2 start value in any order ((dim + 2) units of 2^-24 of the absolute sum),
+    FP32 norms of dim terms, and the final FP32 roundings.  (The FP8 pass does not
+    accumulate in FP32 order: fp8_scores restates it exactly.)"""
     mt = METRICS.get(metric, metric)
     u = 2.0 ** -24
     dim = np.asarray(X).shape[1]
-    Xb = np.abs(bf16_round(X).astype(np.float64))
-    Qb = np.abs(bf16_round(Q).astype(np.float64))
+    Xb, Qb = _prec_operands(X, Q, "bf16")
+    Xb, Qb = np.abs(Xb), np.abs(Qb)
     a = Qb @ Xb.T
     if mt == NORMCOS:
         return (dim + 2) * u * a + 2 * u
@@ -313,3 +351,71 @@ def prec_error_bound(metric, X, Q):
     acc = (dim + 2) * u * (2.0 * a + xn[None, :])
     norms = (dim + 1) * u * (xn[None, :] + qn[:, None])
     return acc + norms + 4 * u * (qn[:, None] + xn[None, :] + 2.0 * a)
+
+
+def fma_norms(X):
+    """FP32 squared norms of the rows as the device computes them (fma chain in
+    increasing index)."""
+    X = _f32c(X)
+    out = np.zeros(len(X), np.float32)
+    lib().or_fma_norms(_ptr(X), len(X), X.shape[1], _ptr(out))
+    return out
+
+
+def fp8_operands(X, e, K=128):
+    """Rows of X as the FP8 pass holds them: the e4m3fn values of X * 2^e (fp8_round, in
+    scaled units, exact in float32), zero padded to K values."""
+    X = np.asarray(X, np.float32)
+    out = np.zeros((len(X), K), np.float32)
+    out[:, :X.shape[1]] = np.ldexp(fp8_round(X, e), e)
+    return out
+
+
+FP8_ACC_UNITS = 18  # accumulation bound of the FP8 product, units of 2^-24 of the magnitude
+
+
+def fp8_mfma(P, R, C=None, c_mode=2, s=0, variant=0):
+    """The e4m3fn matrix-core product of P rows and R rows (exact FP8 values, K a
+    multiple of 8 each) started at C (c_mode 0: (m, n); 1: per R row; 2: zero), operand
+    scale 2^s: returns (value, magnitude), both (m, n) float64, with the device's FP32
+    result within FP8_ACC_UNITS * 2^-24 * magnitude of value; see or_fp8_mfma."""
+    P, R = _f32c(P), _f32c(R)
+    m, n, K = len(P), len(R), P.shape[1]
+    Cc = None if C is None else _f32c(C)
+    D = np.zeros((m, n), np.float64)
+    M = np.zeros((m, n), np.float64)
+    lib().or_fp8_mfma(_ptr(P), m, _ptr(R), n, K, _ptr(Cc), c_mode, s, variant, _ptr(D), _ptr(M))
+    return D, M
+
+
+def fp8_scores(metric, X, Q, Qscale=None):
+    """Scores of the FP8 candidate pass as (value, magnitude), (m, n) float64 (see
+    fp8_mfma): the e4m3fn product of the rows scaled by 2^ex (ex = fp8_scale_exp(X)) and
+    the queries scaled by 2^eq (eq = fp8_scale_exp of Qscale, every query of the call, or
+    of Q), operand scale 2^-(ex + eq), started at -|x|^2/2 (fma_norms) for SqL2/L2 and at
+    0 for NormCosine; larger is nearer."""
+    mt = METRICS.get(metric, metric)
+    ex = fp8_scale_exp(X)
+    eq = fp8_scale_exp(Q if Qscale is None else Qscale)
+    Px, Pq = fp8_operands(X, ex), fp8_operands(Q, eq)
+    if mt == NORMCOS:
+        return fp8_mfma(Pq, Px, None, 2, -(ex + eq))
+    c = (np.float32(-0.5) * fma_norms(X)).astype(np.float32)
+    return fp8_mfma(Pq, Px, c, 1, -(ex + eq))
+
+
+def fp8_approx(metric, X, Q, Qscale=None):
+    """(m, n) float64 approximate distances of the FP8 pass and their bound: NormCosine
+    1 - s, SqL2 |q|^2 - 2 s (fma_norms of the query; the device evaluates both in FP32
+    from its FP32 score s), with |device - value| <= bound (the score's accumulation bound
+    and the final FP32 rounding)."""
+    mt = METRICS.get(metric, metric)
+    if mt == L2:
+        raise ValueError("the FP8 approximate-distance bound covers SqL2 and NormCosine")
+    S, M = fp8_scores(metric, X, Q, Qscale)
+    u = 2.0 ** -24
+    eb = FP8_ACC_UNITS * u * M
+    if mt == NORMCOS:
+        return 1.0 - S, eb + u * (1.0 + np.abs(S) + eb)
+    qn = fma_norms(Q).astype(np.float64)[:, None]
+    return qn - 2.0 * S, 2.0 * eb + u * (qn + 2.0 * (np.abs(S) + eb))

@@ -110,6 +110,15 @@ int or_build_graph(int metric, const float* X, int64_t n, int dim, const or_buil
                    or_graph* out);
 void or_free(void* p);
 
+/* FP8 matrix-core products of the precision form: D[i n + j] is the restated value (before
+ * the FP32 accumulation) of the e4m3fn dot product of P row i and R row j (K values each,
+ * K a multiple of 8, given as their exact values) started at C[i n + j] (c_mode 0), C[j]
+ * (1) or 0 (2), operand scale 2^s; Dmag the magnitude its accumulation bound scales with.
+ * or_fma_norms: FP32 fma-chain squared norms of n rows of dim floats. */
+void or_fp8_mfma(const float* P, int64_t m, const float* R, int64_t n, int K, const float* C, int c_mode,
+                 int s, int variant, double* D, double* Dmag);
+void or_fma_norms(const float* X, int64_t n, int dim, float* out);
+
 /* optimize_index!(MinRecall) restated over the BeamSearchSpace grid; queries are given
  * explicitly (1-based ids of database objects).  Returns the chosen (bsize, delta,
  * maxvisits) through the out pointers and the achieved recall. */

@@ -331,7 +331,8 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
 {
     if (!h->X.p) return ssg_fail(SSG_E_STATE, "no database uploaded");
     if (m <= 0) return ssg_fail(SSG_E_ARG, "empty set of queries");
-    if (precision != SSG_BF16) return ssg_fail(SSG_E_ARG, "the precision form takes SSG_BF16");
+    if (precision != SSG_BF16 && precision != SSG_FP8)
+        return ssg_fail(SSG_E_ARG, "the precision form takes SSG_BF16 or SSG_FP8");
     if (k < 1 || kstride < k) return ssg_fail(SSG_E_ARG, "k must satisfy 1 <= k <= kstride");
     if (ncand < k || ncand > PREC_NCAND_MAX)
         return ssg_fail(SSG_E_LIMIT, "ncand must satisfy k <= ncand <= 256");
@@ -350,31 +351,58 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
         }
         return SSG_OK;
     }
-    const int32_t dpb = (h->dp + 31) / 32 * 32;
+    // encoded row bytes: BF16 rows of 32 * ceil(dp / 32) values, FP8 rows of 128 values
+    const bool f8 = precision == SSG_FP8;
+    const int32_t rowb = f8 ? 128 : 2 * ((h->dp + 31) / 32 * 32);
     const int64_t m_pad = (m + PREC_QB - 1) / PREC_QB * PREC_QB;
     const int64_t xe_rows = std::max<int64_t>(h->n_pad, (n + PREC_QB - 1) / PREC_QB * PREC_QB);
+    if (h->xe_fmt != precision) {
+        h->Xe.release();
+        h->Xs.release();
+        h->xe_fmt = 0;
+    }
+    if (f8) HIPCHK(h->xAmax.ensure(4 * sizeof(uint32_t)), "alloc scales");
     if (!h->Xe.p) {
         // rows past n_pad are zero (the FP32 matrix holds n_pad rows)
-        HIPCHK(h->Xe.ensure((size_t)xe_rows * dpb * 2), "alloc BF16 rows");
-        HIPCHK(launch_zero_u32(h->Xe.as<unsigned int>(), xe_rows * dpb / 2, s), "clear BF16 rows");
-        HIPCHK(launch_rows_to_bf16(h->X.as<float>(), n, h->dp, h->Xe.as<uint16_t>(), dpb, s), "BF16 rows");
+        HIPCHK(h->Xe.ensure((size_t)xe_rows * rowb), "alloc encoded rows");
+        HIPCHK(launch_zero_u32(h->Xe.as<unsigned int>(), xe_rows * rowb / 4, s), "clear encoded rows");
+        if (f8) {
+            uint32_t* am = h->xAmax.as<uint32_t>();
+            HIPCHK(launch_zero_u32(am, 2, s), "clear scale");
+            HIPCHK(launch_absmax(h->X.as<float>(), n * h->dp, am, 2, s), "row scale");
+            HIPCHK(launch_rows_to_fp8(h->X.as<float>(), n, h->dp, am, h->Xe.as<uint8_t>(), rowb, s), "FP8 rows");
+        } else {
+            HIPCHK(launch_rows_to_bf16(h->X.as<float>(), n, h->dp, h->Xe.as<uint16_t>(), rowb / 2, s),
+                   "BF16 rows");
+        }
+        h->xe_fmt = precision;
     }
     PrecOut o{};
     PrecArgs a{};
+    a.fp8 = f8 ? 1 : 0;
     if (dQ) {
         HIPCHK(h->qpad.ensure((size_t)m_pad * h->dp * sizeof(float)), "alloc qpad");
         HIPCHK(h->qnorm.ensure((size_t)m_pad * sizeof(float)), "alloc qnorm");
-        HIPCHK(h->pQe.ensure((size_t)m_pad * dpb * 2), "alloc BF16 queries");
+        HIPCHK(h->pQe.ensure((size_t)m_pad * rowb), "alloc encoded queries");
         float* qpad = h->qpad.as<float>();
         HIPCHK(launch_fill_f32(qpad + m * h->dp, (m_pad - m) * h->dp, 0.f, s), "fill qpad");
         HIPCHK(launch_fill_f32(h->qnorm.as<float>() + m, m_pad - m, 0.f, s), "fill qnorm");
         HIPCHK(launch_pad_rows(dQ, m, h->dim, qpad, h->dp, h->qnorm.as<float>(), s), "pad queries");
-        HIPCHK(launch_rows_to_bf16(qpad, m_pad, h->dp, h->pQe.as<uint16_t>(), dpb, s), "BF16 queries");
-        a.Qe = h->pQe.as<uint16_t>();
+        if (f8) {
+            uint32_t* am = h->xAmax.as<uint32_t>();
+            HIPCHK(launch_zero_u32(am + 2, 1, s), "clear scale");
+            HIPCHK(launch_absmax(qpad, m * h->dp, am + 2, 1, s), "query scale");
+            HIPCHK(launch_rows_to_fp8(qpad, m_pad, h->dp, am + 2, h->pQe.as<uint8_t>(), rowb, s), "FP8 queries");
+            a.f8max = am + 1;  // {rows, queries}
+        } else {
+            HIPCHK(launch_rows_to_bf16(qpad, m_pad, h->dp, h->pQe.as<uint16_t>(), rowb / 2, s), "BF16 queries");
+        }
+        a.Qe = h->pQe.as<uint8_t>();
         o.qpad = qpad;
         o.qnorm = h->qnorm.as<float>();
     } else {
-        a.Qe = h->Xe.as<uint16_t>();  // xe_rows >= m_pad
+        a.Qe = h->Xe.as<uint8_t>();  // xe_rows >= m_pad
+        if (f8) a.f8max = h->xAmax.as<uint32_t>();  // {rows, rows}
         o.qpad = h->X.as<float>();
         o.qnorm = h->xnorm.as<float>();
     }
@@ -391,11 +419,11 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     ns = (int32_t)((n + rps - 1) / rps);
     HIPCHK(h->pList.ensure((size_t)m_pad * ns * cap * sizeof(uint2)), "alloc candidate lists");
     HIPCHK(h->pListN.ensure((size_t)m_pad * ns * sizeof(int32_t)), "alloc candidate counts");
-    a.Xe = h->Xe.as<uint16_t>();
+    a.Xe = h->Xe.as<uint8_t>();
     a.xnorm = h->xnorm.as<float>();
     a.X = h->X.as<float>();
     a.n = n;
-    a.dpb = dpb;
+    a.rowb = rowb;
     a.metric = h->metric;
     a.m_pad = m_pad;
     a.kc = kc;
@@ -429,13 +457,13 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     unsigned int* flags = h->pFlags.as<unsigned int>();
     const int64_t nsr = prec_sample_rows(n, STEP);
     if (!h->Xs.p) {
-        HIPCHK(h->Xs.ensure((size_t)std::max<int64_t>(nsr, 1) * dpb * 2), "alloc sample rows");
+        HIPCHK(h->Xs.ensure((size_t)std::max<int64_t>(nsr, 1) * rowb), "alloc sample rows");
         HIPCHK(h->xns.ensure((size_t)std::max<int64_t>(nsr, 1) * sizeof(float)), "alloc sample norms");
-        HIPCHK(launch_sample_rows(h->Xe.as<uint16_t>(), h->xnorm.as<float>(), nsr, STEP, dpb,
+        HIPCHK(launch_sample_rows(h->Xe.as<uint16_t>(), h->xnorm.as<float>(), nsr, STEP, rowb / 2,
                                   h->Xs.as<uint16_t>(), h->xns.as<float>(), s), "sample rows");
     }
     PrecArgs as = a;
-    as.Xe = h->Xs.as<uint16_t>();
+    as.Xe = h->Xs.as<uint8_t>();
     as.xnorm = h->xns.as<float>();
     as.n = nsr;
     as.rows_per_slice = rps / STEP;
@@ -461,12 +489,13 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     int64_t rpsf = (n + nsf - 1) / nsf;
     rpsf = (rpsf + 127) / 128 * 128;
     nsf = (int32_t)((n + rpsf - 1) / rpsf);
-    HIPCHK(h->pQf.ensure((size_t)mf_pad * dpb * 2), "alloc redo queries");
+    HIPCHK(h->pQf.ensure((size_t)mf_pad * rowb), "alloc redo queries");
     HIPCHK(h->pList.grow((size_t)mf_pad * nsf * cap * sizeof(uint2), 0, s), "alloc candidate lists");
     HIPCHK(h->pListN.grow((size_t)mf_pad * nsf * sizeof(int32_t), 0, s), "alloc candidate counts");
-    HIPCHK(launch_gather_rows_u16(a.Qe, flags + 1, flags, mf_pad, dpb, h->pQf.as<uint16_t>(), s), "gather");
+    HIPCHK(launch_gather_rows_u16(reinterpret_cast<const uint16_t*>(a.Qe), flags + 1, flags, mf_pad, rowb / 2,
+                                  h->pQf.as<uint16_t>(), s), "gather");
     PrecArgs af = a;
-    af.Qe = h->pQf.as<uint16_t>();
+    af.Qe = h->pQf.as<uint8_t>();
     af.m_pad = mf_pad;
     af.rows_per_slice = rpsf;
     af.nslices = nsf;
@@ -682,6 +711,7 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     h->Xh.release();
     h->Xe.release();
     h->Xs.release();
+    h->xe_fmt = 0;
     ssg_query_layout_stale(h);
     const int64_t np = std::max<int64_t>(h->n_pad, 64);
     HIPCHK(h->X.ensure((size_t)np * dp * sizeof(float)), "alloc X");

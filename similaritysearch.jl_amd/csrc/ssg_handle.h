@@ -89,11 +89,14 @@ struct ssg_handle {
     // list entries and rows share cache lines and DRAM pages), with each list entry
     // carrying its target's {start, degree}.  Built on the first query search after the
     // graph or database changes; searchbatch! uses it, index!/allknn! do not.
-    // precision form of exhaustive k-NN: BF16 rows of 32 * ceil(dim / 32) elements (made
-    // on first use), BF16 queries, per-(query, slice) candidate lists and their lengths;
-    // prec_slices > 0 forces the number of DB slices
+    // precision form of exhaustive k-NN: encoded rows (made on first use in the format
+    // xe_fmt: BF16 rows of 32 * ceil(dim / 32) elements, or FP8 rows of 128 bytes), encoded
+    // queries, per-(query, slice) candidate lists and their lengths; prec_slices > 0 forces
+    // the number of DB slices
     DevBuf Xe, pQe, pList, pListN, pTau0, pFlags, pQf;
     DevBuf Xs, xns;  // rows of every 16th tile of Xe, packed, and their norms (sample pass)
+    DevBuf xAmax;    // FP8 max |value| bits: [0] and [1] the rows, [2] the last queries
+    int32_t xe_fmt = 0;  // SSG_BF16 or SSG_FP8 while Xe holds rows of that format
     int32_t prec_slices = 0;
     int32_t prec_sample = 0;  // 0: sampled thresholds when slices are long, -1: never, T > 0: always, rank T
     bool reorder = true;

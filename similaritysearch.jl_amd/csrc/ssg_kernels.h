@@ -121,13 +121,16 @@ inline int32_t prec_list_cap(int32_t ncand)
     return ncand <= 128 ? 512 : 1024;
 }
 struct PrecArgs {
-    const uint16_t* Xe;     // n_pad rows of dpb BF16 (zero padded)
+    const uint8_t* Xe;      // encoded rows of rowb bytes (zero padded): BF16 or scaled FP8
     const float* xnorm;     // FP32 squared norms of the rows
     const float* X;         // n_pad rows of dp FP32 (the rerank reads them)
     int64_t n;
-    int32_t dpb;            // 32 * KB
+    int32_t rowb;           // bytes per encoded row: 64 * KB (BF16), 128 * KB (FP8)
     int32_t metric;
-    const uint16_t* Qe;     // m_pad rows of dpb BF16 queries
+    const uint8_t* Qe;      // m_pad encoded query rows of rowb bytes
+    int32_t fp8;            // 0: BF16 operands; 1: FP8 e4m3fn operands scaled by 2^e
+    const uint32_t* f8max;  // fp8: [0] bits of max |x| over the rows, [1] over the queries
+                            // (the scale exponents e follow from them: fp8_scale_exp)
     int64_t m_pad;          // multiple of PREC_QB
     int32_t kc;             // candidates kept per (query, slice)
     int32_t cap;            // list capacity per (query, slice)
@@ -162,7 +165,7 @@ inline int64_t prec_sample_rows(int64_t n, int32_t step)
     return groups * 128 + (rest < 128 ? rest : 128);
 }
 hipError_t launch_sample_rows(const uint16_t* Xe, const float* xnorm, int64_t ns_rows, int32_t step,
-                              int32_t dpb, uint16_t* Xs, float* xns, hipStream_t s);
+                              int32_t width, uint16_t* Xs, float* xns, hipStream_t s);
 hipError_t launch_gather_rows_u16(const uint16_t* src, const uint32_t* idx, const unsigned int* count,
                                   int64_t max_rows, int32_t width, uint16_t* dst, hipStream_t s);
 int32_t prec_slices(int64_t n, int64_t m, int num_cu, int kc);
@@ -170,6 +173,32 @@ hipError_t prec_launch_stage1(const PrecArgs& a, hipStream_t s);
 hipError_t prec_launch_stage2(const PrecArgs& a, const PrecOut& o, int32_t dp, hipStream_t s);
 hipError_t launch_rows_to_bf16(const float* src, int64_t rows, int32_t dp, uint16_t* dst, int32_t dpb,
                                hipStream_t s);
+// FP8 operands: amax[0 .. namax) (zeroed by the caller) each receive the bits of max |src|
+// over count values; rows_to_fp8 writes rows of rowb bytes, e4m3fn(src * 2^e) rounded to
+// nearest even with e = fp8_scale_exp(*amax), zero padded.
+hipError_t launch_absmax(const float* src, int64_t count, uint32_t* amax, int32_t namax, hipStream_t s);
+hipError_t launch_rows_to_fp8(const float* src, int64_t rows, int32_t dp, const uint32_t* amax,
+                              uint8_t* dst, int32_t rowb, hipStream_t s);
+// The largest e with max_abs * 2^e <= 448 (the largest finite e4m3fn value), clamped to
+// [-120, 120]; 0 when max_abs is 0 or not finite.
+__host__ __device__ inline int fp8_scale_exp(uint32_t amax_bits)
+{
+    const uint32_t ex = (amax_bits >> 23) & 255u, mant = amax_bits & 0x7fffffu;
+    if (amax_bits == 0u || ex == 255u) return 0;
+    // max_abs = f * 2^E with f in [1, 2): f * 2^(E+e) <= 1.75 * 2^8
+    int E;
+    uint32_t f;  // the 24-bit significand of max_abs, normalised
+    if (ex == 0u) {
+        const int lz = __builtin_clz(mant) - 8;  // shift that brings the leading bit to bit 23
+        f = mant << lz;
+        E = -126 - lz;
+    } else {
+        f = mant | 0x800000u;
+        E = (int)ex - 127;
+    }
+    int e = (f <= 0xe00000u ? 8 : 7) - E;  // 0xe00000 = 1.75 * 2^23
+    return e < -120 ? -120 : (e > 120 ? 120 : e);
+}
 
 hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float* Q, int32_t qdim,
                          int64_t m, int32_t k, uint32_t* ids, float* dists, hipStream_t s);

@@ -1,14 +1,20 @@
 // ssg_prec.hip -- precision form of exhaustive k-NN and allknn for gfx950: a candidate
-// pass with BF16 operands on the matrix cores, then rerank! of the candidates with the
-// canonical FP32 distance.
+// pass with BF16 or FP8 operands on the matrix cores, then rerank! of the candidates with
+// the canonical FP32 distance.
 //
 // The reference computes every distance in Float32 (src/exact/sequential-exhaustive.jl:68-80,
 // src/allknn.jl:66-110); the precision form is the allknn-with-rerank pattern of
 // src/rerank.jl:22-98 with the candidate queue filled from low-precision dot products.
 //
+// Operands: BF16 (round to nearest even) for v_mfma_f32_16x16x32_bf16, or OCP FP8 e4m3fn
+//   of the values scaled by a power of two 2^e (e from the largest |value| of the rows,
+//   and of the queries, so that it maps to at most 448; round to nearest even) for
+//   v_mfma_scale_f32_16x16x128_f8f6f4, whose E8M0 scale operands multiply the product by
+//   2^-e of each side exactly.  FP8 rows are padded to 128 values.
+//
 // Candidate pass (prec_stage1): block = 256 queries x one DB slice, 4 waves of 64
-//   queries.  A wave keeps its queries' BF16 rows as the B operands of
-//   v_mfma_f32_16x16x32_bf16 in registers for the whole slice; DB rows are staged in
+//   queries.  A wave keeps its queries' encoded rows as the B operands of the matrix-core
+//   product in registers for the whole slice; DB rows are staged in
 //   LDS in 128-row tiles (double-buffered, one barrier per tile, the next tile
 //   prefetched into registers while the current one is multiplied) and shared by the 4
 //   waves.  The accumulator starts at -||x||^2/2 (SqL2, L2) or 0 (NormCosine), so the
@@ -34,21 +40,58 @@ namespace ssg {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef int intx4 __attribute__((ext_vector_type(4)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
 
 constexpr int PQ_TR = 128;          // DB rows per LDS tile
 constexpr int PQ_JS = 32;           // merged entries per lane in the rerank stage
 constexpr int PQ_NS = PREC_NCAND_MAX / 64;  // candidate slots per lane in the rerank stage
 
+// Operand format of the candidate pass.  A lane's share of one matrix-core product is a
+// piece of PB bytes of its row: BF16, 8 values for v_mfma_f32_16x16x32_bf16 (lane l: row
+// l % 16, the k group l / 16); FP8 e4m3fn, 32 values for v_mfma_scale_f32_16x16x128_f8f6f4
+// (same lane mapping, 32 k per group).  A row of KB products is split into 4 contiguous
+// quarters, one per k group, so a lane reads KB pieces that lie next to each other; the
+// same split of the query rows keeps the k order of both operands the same.
+template <bool F8> struct PrecOp {
+    typedef bf16x8 T;
+    static constexpr int PB = 16;
+    static __device__ inline T ld(const uint8_t* p) { return *reinterpret_cast<const bf16x8*>(p); }
+    static __device__ inline floatx4 mma(T x, T q, floatx4 c, int, int)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, q, c, 0, 0, 0);
+    }
+};
+template <> struct PrecOp<true> {
+    typedef intx8 T;
+    static constexpr int PB = 32;
+    // two 16-byte reads: rows are 16-byte aligned
+    static __device__ inline T ld(const uint8_t* p)
+    {
+        const intx4 lo = *reinterpret_cast<const intx4*>(p), hi = *reinterpret_cast<const intx4*>(p + 16);
+        return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    }
+    // The E8M0 scale operands (127 - e) undo the power-of-two scaling of each operand
+    // exactly, so the product and the -|x|^2/2 start value are in the same units.
+    static __device__ inline floatx4 mma(T x, T q, floatx4 c, int sx, int sq)
+    {
+        return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(x, q, c, 0, 0, 0, sx, 0, sq);
+    }
+};
+
 // J = cap / 64 list entries per lane in a compaction (8: lists of up to 512 entries, 16:
 // of 1024).  Each kernel calls one compaction instance only: the registers that instance
-// uses are what the kernel has to keep clear around the call.
-template <int KB, int J>
+// uses are what the kernel has to keep clear around the call.  F8 selects the FP8 form.
+template <int KB, int J, bool F8>
 __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
 {
-    constexpr int DPB = 32 * KB;             // BF16 row length
-    constexpr int LSB = DPB * 2 + 16;        // LDS row stride in bytes (staggers banks)
+    typedef PrecOp<F8> Op;
+    typedef typename Op::T OpT;
+    constexpr int PB = Op::PB;
+    constexpr int RB = 4 * PB * KB;          // encoded row bytes
+    constexpr int LSB = RB + 16;             // LDS row stride in bytes (staggers banks)
     constexpr int NCT = 4;                   // 16-query column tiles per wave
-    constexpr int U4 = DPB / 8;              // 16-byte units per row
+    constexpr int U4 = RB / 16;              // 16-byte units per row
     constexpr int PF = PQ_TR * U4 / 256;     // 16-byte units per thread per tile
     static_assert(PQ_TR * U4 % 256 == 0, "tile must split evenly over the block");
     __shared__ __attribute__((aligned(16))) uint8_t s_x[2][PQ_TR * LSB];
@@ -69,12 +112,17 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
         return a.list + ((qb0 + ql) * a.nslices + sl) * (int64_t)cap;
     };
 
-    bf16x8 bq[NCT][KB];
+    OpT bq[NCT][KB];
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
-        const uint16_t* qp = a.Qe + (qb0 + 64 * w + 16 * ct + col) * DPB + g * KB * 8;
+        const uint8_t* qp = a.Qe + (qb0 + 64 * w + 16 * ct + col) * RB + g * KB * PB;
 #pragma unroll
-        for (int s = 0; s < KB; s++) bq[ct][s] = *reinterpret_cast<const bf16x8*>(qp + 8 * s);
+        for (int s = 0; s < KB; s++) bq[ct][s] = Op::ld(qp + PB * s);
+    }
+    int sx = 127, sq = 127;  // FP8: E8M0 scale operands of the rows and of the queries
+    if (F8) {
+        sx = 127 - fp8_scale_exp(a.f8max[0]);
+        sq = 127 - fp8_scale_exp(a.f8max[1]);
     }
     // Appends go through a buffer resource over the block's lists (stores past it are
     // dropped by the hardware).  Each lane keeps, per query, the byte offset of the next
@@ -112,7 +160,7 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
                 const int idx = tid + 256 * I;                                              \
                 const int row = idx / U4, c = idx % U4;                                     \
                 const int64_t gr = min(t + row, r1 - 1); /* rows past the slice: see s_h */ \
-                R = reinterpret_cast<const uint4*>(a.Xe + gr * DPB)[c];                     \
+                R = reinterpret_cast<const uint4*>(a.Xe + gr * RB)[c];                      \
             }
             SSG_PF(0, pf0) SSG_PF(1, pf1) SSG_PF(2, pf2) SSG_PF(3, pf3)
             SSG_PF(4, pf4) SSG_PF(5, pf5) SSG_PF(6, pf6) SSG_PF(7, pf7)
@@ -144,16 +192,15 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
             // MFMA chains of one 16-row block for the wave's 4 query tiles
             auto chain = [&](int rb, floatx4 (&acc)[NCT]) {
                 const floatx4 c0 = *reinterpret_cast<const floatx4*>(&hb[rb * 16 + 4 * g]);
-                bf16x8 af[KB];
-                const uint8_t* ap = xb + (rb * 16 + col) * LSB + g * KB * 16;
+                OpT af[KB];
+                const uint8_t* ap = xb + (rb * 16 + col) * LSB + g * KB * PB;
 #pragma unroll
-                for (int s = 0; s < KB; s++) af[s] = *reinterpret_cast<const bf16x8*>(ap + 16 * s);
+                for (int s = 0; s < KB; s++) af[s] = Op::ld(ap + PB * s);
 #pragma unroll
                 for (int ct = 0; ct < NCT; ct++) {
                     floatx4 c = c0;
 #pragma unroll
-                    for (int s = 0; s < KB; s++)
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[s], bq[ct][s], c, 0, 0, 0);
+                    for (int s = 0; s < KB; s++) c = Op::mma(af[s], bq[ct][s], c, sx, sq);
                     acc[ct] = c;
                 }
             };
@@ -428,28 +475,81 @@ hipError_t launch_gather_rows_u16(const uint16_t* src, const uint32_t* idx, cons
     return hipGetLastError();
 }
 
-// The rows of every step-th 128-row tile (tiles 0, step, 2 step, ...) of the BF16 rows
-// and their norms, packed: sample row i is row (i / 128) * 128 * step + i % 128.  Rows at
-// or past n are not copied (the caller passes the number of sample rows below n).
+// The rows of every step-th 128-row tile (tiles 0, step, 2 step, ...) of the encoded rows
+// (width 16-bit words each) and their norms, packed: sample row i is row
+// (i / 128) * 128 * step + i % 128.  Rows at or past n are not copied (the caller passes
+// the number of sample rows below n).
 __global__ void sample_rows_kernel(const uint16_t* __restrict__ Xe, const float* __restrict__ xnorm,
-                                   int64_t ns_rows, int32_t step, int32_t dpb, uint16_t* __restrict__ Xs,
+                                   int64_t ns_rows, int32_t step, int32_t width, uint16_t* __restrict__ Xs,
                                    float* __restrict__ xns)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t r = i / dpb, c = i % dpb;
+    const int64_t r = i / width, c = i % width;
     if (r >= ns_rows) return;
     const int64_t orig = (r / 128) * 128 * step + r % 128;
-    Xs[i] = Xe[orig * dpb + c];
+    Xs[i] = Xe[orig * width + c];
     if (c == 0) xns[r] = xnorm[orig];
 }
 
 hipError_t launch_sample_rows(const uint16_t* Xe, const float* xnorm, int64_t ns_rows, int32_t step,
-                              int32_t dpb, uint16_t* Xs, float* xns, hipStream_t s)
+                              int32_t width, uint16_t* Xs, float* xns, hipStream_t s)
 {
-    const int64_t tot = ns_rows * dpb;
+    const int64_t tot = ns_rows * width;
     if (tot <= 0) return hipSuccess;
     hipLaunchKernelGGL(sample_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, Xe, xnorm,
-                       ns_rows, step, dpb, Xs, xns);
+                       ns_rows, step, width, Xs, xns);
+    return hipGetLastError();
+}
+
+// max |src| over count values, as float bits (non-negative floats order as their bits),
+// folded into amax[0 .. namax) with atomic maxima.
+__global__ void absmax_kernel(const float* __restrict__ src, int64_t count, uint32_t* __restrict__ amax,
+                              int32_t namax)
+{
+    uint32_t mx = 0u;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        mx = max(mx, __float_as_uint(src[i]) & 0x7fffffffu);
+    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+    if ((threadIdx.x & 63) == 0 && mx != 0u)
+        for (int t = 0; t < namax; t++) atomicMax(&amax[t], mx);
+}
+
+hipError_t launch_absmax(const float* src, int64_t count, uint32_t* amax, int32_t namax, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, 4096);
+    hipLaunchKernelGGL(absmax_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, count, amax, namax);
+    return hipGetLastError();
+}
+
+// Rows of dp floats -> rows of rowb bytes of e4m3fn(v * 2^e), e = fp8_scale_exp(*amax),
+// zero padded.  v_cvt_pk_fp8_f32 rounds to nearest even; |v * 2^e| <= 448 by the choice
+// of e, so nothing saturates.  One thread per 4 bytes.
+__global__ void rows_to_fp8_kernel(const float* __restrict__ src, int64_t rows, int32_t dp,
+                                   const uint32_t* __restrict__ amax, uint8_t* __restrict__ dst, int32_t rowb)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t words = rowb / 4;
+    const int64_t r = i / words;
+    const int32_t c = 4 * (int32_t)(i % words);
+    if (r >= rows) return;
+    const int e = fp8_scale_exp(*amax);
+    float v[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) v[t] = c + t < dp ? __builtin_ldexpf(src[r * dp + c + t], e) : 0.f;
+    int wv = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+    wv = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], wv, true);
+    reinterpret_cast<int*>(dst)[i] = wv;
+}
+
+hipError_t launch_rows_to_fp8(const float* src, int64_t rows, int32_t dp, const uint32_t* amax, uint8_t* dst,
+                              int32_t rowb, hipStream_t s)
+{
+    const int64_t tot = rows * (rowb / 4);
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rows_to_fp8_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, src, rows, dp,
+                       amax, dst, rowb);
     return hipGetLastError();
 }
 
@@ -511,11 +611,19 @@ hipError_t prec_launch_stage1(const PrecArgs& a, hipStream_t s)
     const dim3 grid((unsigned)(a.m_pad / PREC_QB), (unsigned)a.nslices), block(256);
     if (a.cap != 512 && a.cap != 1024) return hipErrorInvalidValue;
     const bool big = a.cap == 1024;
-    switch (a.dpb / 32) {
+    if (a.fp8) {
+        // dim <= 128: one 128-value product per row
+        if (a.rowb != 128 || !a.f8max) return hipErrorInvalidValue;
+        if (big) hipLaunchKernelGGL((prec_stage1<1, 16, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((prec_stage1<1, 8, true>), grid, block, 0, s, a);
+        return hipGetLastError();
+    }
+    if (a.rowb % 64 != 0) return hipErrorInvalidValue;
+    switch (a.rowb / 64) {
 #define SSG_P1(KK)                                                                                  \
     case KK:                                                                                        \
-        if (big) hipLaunchKernelGGL((prec_stage1<KK, 16>), grid, block, 0, s, a);                  \
-        else hipLaunchKernelGGL((prec_stage1<KK, 8>), grid, block, 0, s, a);                       \
+        if (big) hipLaunchKernelGGL((prec_stage1<KK, 16, false>), grid, block, 0, s, a);           \
+        else hipLaunchKernelGGL((prec_stage1<KK, 8, false>), grid, block, 0, s, a);                \
         break;
         SSG_P1(1) SSG_P1(2) SSG_P1(3) SSG_P1(4)
 #undef SSG_P1

@@ -142,15 +142,18 @@ class GenericContext:
         self.verbose = verbose
         self.costdists = np.zeros(0, np.int64)
         self.costblocks = np.zeros(0, np.int64)
-        if precision not in ("f32", "bf16"):
-            raise ValueError("precision must be 'f32' or 'bf16'")
-        # precision form of the exhaustive indexes (no reference counterpart): a BF16
-        # candidate pass keeping max(ncand, 4k) candidates per query, then rerank!
+        if precision not in ("f32", "bf16", "fp8"):
+            raise ValueError("precision must be 'f32', 'bf16' or 'fp8'")
+        # precision form of the exhaustive indexes (no reference counterpart): a BF16 or
+        # FP8 candidate pass keeping max(ncand, 4k) candidates per query, then rerank!
         self.precision = precision
         self.ncand = int(ncand)
 
     def exhaustive_ncand(self, k):
         return min(256, max(k, self.ncand if self.ncand > 0 else 4 * k))
+
+
+_PREC_CODE = {"bf16": 1, "fp8": 2}  # SSG_BF16, SSG_FP8
 
 
 class SearchGraphContext(GenericContext):
@@ -347,10 +350,10 @@ def searchbatch_(index, ctx, Q, ids, dists, sorted=False):
                                     ids.T.ctypes.data, dists.T.ctypes.data, cd.ctypes.data,
                                     cb.ctypes.data), "searchbatch!")
         ctx.costdists, ctx.costblocks = cd, cb
-    elif getattr(ctx, "precision", "f32") == "bf16":
-        _lib.check(L.ssg_exhaustive_knn_prec(h, rows.ctypes.data, m, k, 1, ctx.exhaustive_ncand(k),
-                                             ids.T.ctypes.data, dists.T.ctypes.data, None, None),
-                   "searchbatch!")
+    elif getattr(ctx, "precision", "f32") in _PREC_CODE:
+        _lib.check(L.ssg_exhaustive_knn_prec(h, rows.ctypes.data, m, k, _PREC_CODE[ctx.precision],
+                                             ctx.exhaustive_ncand(k), ids.T.ctypes.data,
+                                             dists.T.ctypes.data, None, None), "searchbatch!")
         ctx.costdists = np.full(m, len(index), np.int64)
         ctx.costblocks = np.zeros(m, np.int64)
     else:
@@ -414,22 +417,25 @@ def allknn(index, ctx, k, sort=True):
         bp = index.beam_params(ctx)
         _lib.check(L.ssg_allknn_graph(h, k, C.byref(bp), ids.ctypes.data, dists.ctypes.data),
                    "allknn")
-    elif getattr(ctx, "precision", "f32") == "bf16":
-        _lib.check(L.ssg_allknn_exhaustive_prec(h, k, 1, ctx.exhaustive_ncand(k), ids.ctypes.data,
-                                                dists.ctypes.data, None, None), "allknn")
+    elif getattr(ctx, "precision", "f32") in _PREC_CODE:
+        _lib.check(L.ssg_allknn_exhaustive_prec(h, k, _PREC_CODE[ctx.precision], ctx.exhaustive_ncand(k),
+                                                ids.ctypes.data, dists.ctypes.data, None, None), "allknn")
     else:
         _lib.check(L.ssg_allknn_exhaustive(h, k, ids.ctypes.data, dists.ctypes.data), "allknn")
     return ids.T, dists.T
 
 
-def exhaustive_candidates(index, k, ncand, Q=None):
+def exhaustive_candidates(index, k, ncand, Q=None, precision="bf16"):
     """Precision form of an exhaustive index with its candidate queue exposed (no reference
     counterpart): returns (ids, dists, cand_ids, cand_dists), all (rows, m) with m the
     number of queries (every database object when Q is None, as allknn).  cand_* hold
-    the ncand BF16-pass candidates in exact (distance, id) order and their approximate
-    distances; ids/dists are the exact top k among them."""
+    the ncand candidates of the BF16 or FP8 pass in exact (distance, id) order and their
+    approximate distances; ids/dists are the exact top k among them."""
     if not isinstance(index, ExhaustiveSearch):
         raise TypeError("the precision form serves exhaustive indexes")
+    if precision not in _PREC_CODE:
+        raise ValueError("precision must be 'bf16' or 'fp8'")
+    pc = _PREC_CODE[precision]
     L = _lib.load()
     h = index._handle()
     if Q is None:
@@ -444,12 +450,12 @@ def exhaustive_candidates(index, k, ncand, Q=None):
     ids, dists = _out(m, k)
     ci, cd = _out(m, ncand)
     if Q is None:
-        _lib.check(L.ssg_allknn_exhaustive_prec(h, k, 1, ncand, ids.ctypes.data, dists.ctypes.data,
+        _lib.check(L.ssg_allknn_exhaustive_prec(h, k, pc, ncand, ids.ctypes.data, dists.ctypes.data,
                                                 ci.ctypes.data, cd.ctypes.data), "allknn")
     else:
         if Q.dim != index.db.dim:
             raise ValueError("query dimension differs from the database")
-        _lib.check(L.ssg_exhaustive_knn_prec(h, Q.rows.ctypes.data, m, k, 1, ncand, ids.ctypes.data,
+        _lib.check(L.ssg_exhaustive_knn_prec(h, Q.rows.ctypes.data, m, k, pc, ncand, ids.ctypes.data,
                                              dists.ctypes.data, ci.ctypes.data, cd.ctypes.data),
                    "searchbatch!")
     return ids.T, dists.T, ci.T, cd.T
