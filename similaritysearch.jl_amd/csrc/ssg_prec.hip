@@ -89,13 +89,20 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
     typedef typename Op::T OpT;
     constexpr int PB = Op::PB;
     constexpr int RB = 4 * PB * KB;          // encoded row bytes
-    constexpr int LSB = RB + 16;             // LDS row stride in bytes (staggers banks)
+    // FP8 rows are 128 bytes: tiles of 256 rows (half the barriers per row), stored
+    // unpadded with the 8 16-byte chunks of row r at positions c ^ (r & 7), so that the 8
+    // rows a quarter-wave reads, or the chunks a row is written in, fall in distinct
+    // banks.  BF16 rows: 128-row tiles, rows padded by 16 bytes.
+    constexpr bool SWZ = F8;
+    constexpr int TR = F8 ? 2 * PQ_TR : PQ_TR;  // DB rows per LDS tile
+    constexpr int LSB = SWZ ? RB : RB + 16;  // LDS row stride in bytes
     constexpr int NCT = 4;                   // 16-query column tiles per wave
     constexpr int U4 = RB / 16;              // 16-byte units per row
-    constexpr int PF = PQ_TR * U4 / 256;     // 16-byte units per thread per tile
-    static_assert(PQ_TR * U4 % 256 == 0, "tile must split evenly over the block");
-    __shared__ __attribute__((aligned(16))) uint8_t s_x[2][PQ_TR * LSB];
-    __shared__ __attribute__((aligned(16))) float s_h[2][PQ_TR];
+    constexpr int PF = TR * U4 / 256;        // 16-byte units per thread per tile
+    static_assert(TR * U4 % 256 == 0, "tile must split evenly over the block");
+    static_assert(!SWZ || (U4 == 8 && KB == 1), "the swizzle covers one 128-byte row");
+    __shared__ __attribute__((aligned(16))) uint8_t s_x[2][TR * LSB];
+    __shared__ __attribute__((aligned(16))) float s_h[2][TR];
     __shared__ __attribute__((aligned(16))) uint32_t s_hist[4][256];
     __shared__ int s_cnt[4][NCT * 64];   // sub-list counts of each wave's queries
     __shared__ float s_tau[4][NCT * 16]; // their thresholds
@@ -146,15 +153,15 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
 
     if (r0 < r1) {
         uint4 pf0, pf1, pf2, pf3, pf4, pf5, pf6, pf7;  // the next tile (PF <= 8 units per thread)
-        float pfx = 0.f;                                // the norm of its row tid % 128
+        float pfx = 0.f;                                // the norm of its row tid % TR
         int64_t pft = r0;                               // its first row
-        static_assert(PF <= 8, "prefetch registers");
+        static_assert(PF <= 8 && TR <= 256, "prefetch registers");
         // Every load is unconditional (clamped indices) and nothing consumes it before
         // stage(): a conditional load consumed in its branch would make the wave wait for
         // all of its loads in flight, the tile prefetch included.
         auto prefetch = [&](int64_t t) {
             pft = t;
-            pfx = a.xnorm[min(t + (tid & (PQ_TR - 1)), r1 - 1)];
+            pfx = a.xnorm[min(t + (tid & (TR - 1)), r1 - 1)];
 #define SSG_PF(I, R)                                                                        \
             if (I < PF) {                                                                   \
                 const int idx = tid + 256 * I;                                              \
@@ -171,19 +178,20 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
             if (I < PF) {                                                                   \
                 const int idx = tid + 256 * I;                                              \
                 const int row = idx / U4, c = idx % U4;                                     \
-                *reinterpret_cast<uint4*>(&s_x[b][row * LSB + c * 16]) = R;                 \
+                const int pc = SWZ ? (c ^ (row & 7)) : c;                                   \
+                *reinterpret_cast<uint4*>(&s_x[b][row * LSB + pc * 16]) = R;                \
             }
             SSG_ST(0, pf0) SSG_ST(1, pf1) SSG_ST(2, pf2) SSG_ST(3, pf3)
             SSG_ST(4, pf4) SSG_ST(5, pf5) SSG_ST(6, pf6) SSG_ST(7, pf7)
 #undef SSG_ST
-            if (tid < PQ_TR)
+            if (tid < TR)
                 s_h[b][tid] = pft + tid < r1 ? (a.metric == M_NORMCOS ? 0.f : -0.5f * pfx) : -__builtin_inff();
         };
         prefetch(r0);
         stage(0);
         __syncthreads();
         int b = 0;
-        const int64_t step = (int64_t)PQ_TR * a.tile_step;
+        const int64_t step = (int64_t)TR * a.tile_step;
         for (int64_t t0 = r0; t0 < r1; t0 += step) {
             const bool more = t0 + step < r1;
             if (more) prefetch(t0 + step);
@@ -193,9 +201,18 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
             auto chain = [&](int rb, floatx4 (&acc)[NCT]) {
                 const floatx4 c0 = *reinterpret_cast<const floatx4*>(&hb[rb * 16 + 4 * g]);
                 OpT af[KB];
-                const uint8_t* ap = xb + (rb * 16 + col) * LSB + g * KB * PB;
+                const int r = rb * 16 + col;
+                if constexpr (SWZ) {
+                    // the lane's 32 bytes are chunks 2g and 2g + 1 of its row
+                    const uint8_t* rp = xb + r * LSB;
+                    const intx4 lo = *reinterpret_cast<const intx4*>(rp + ((2 * g) ^ (r & 7)) * 16);
+                    const intx4 hi = *reinterpret_cast<const intx4*>(rp + ((2 * g + 1) ^ (r & 7)) * 16);
+                    af[0] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+                } else {
+                    const uint8_t* ap = xb + r * LSB + g * KB * PB;
 #pragma unroll
-                for (int s = 0; s < KB; s++) af[s] = Op::ld(ap + PB * s);
+                    for (int s = 0; s < KB; s++) af[s] = Op::ld(ap + PB * s);
+                }
 #pragma unroll
                 for (int ct = 0; ct < NCT; ct++) {
                     floatx4 c = c0;
@@ -207,8 +224,8 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
             floatx4 cur[NCT], nxt[NCT];
             chain(0, cur);
 #pragma unroll
-            for (int rb = 0; rb < PQ_TR / 16; rb++) {
-                if (rb + 1 < PQ_TR / 16) chain(rb + 1, nxt);
+            for (int rb = 0; rb < TR / 16; rb++) {
+                if (rb + 1 < TR / 16) chain(rb + 1, nxt);
                 // Each 16-query tile is tested on its own (the maximum of its 4 x 16 scores
                 // against each query's threshold; 256 scores per decision), and the append
                 // path of a tile runs only when one of its scores passes.
@@ -264,7 +281,7 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
                         tau[ct] = s_tau[w][16 * ct + col];
                     }
                 }
-                if (rb + 1 < PQ_TR / 16) {
+                if (rb + 1 < TR / 16) {
 #pragma unroll
                     for (int ct = 0; ct < NCT; ct++) cur[ct] = nxt[ct];
                 }
