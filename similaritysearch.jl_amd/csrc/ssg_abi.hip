@@ -468,7 +468,9 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     as.n = nsr;
     as.rows_per_slice = rps / STEP;
     as.kc = std::min(T, PREC_NCAND_MAX);
-    as.cap = prec_list_cap(as.kc);
+    // short lists: the sample pass starts with no threshold, and a list that is compacted
+    // after about a hundred appends raises it sooner
+    as.cap = as.kc <= 32 ? 128 : prec_list_cap(as.kc);
     as.tile_step = 1;
     as.sample = 1;
     as.tau0 = h->pTau0.as<float>();

@@ -626,21 +626,25 @@ int32_t prec_slices(int64_t n, int64_t m, int num_cu, int kc)
 hipError_t prec_launch_stage1(const PrecArgs& a, hipStream_t s)
 {
     const dim3 grid((unsigned)(a.m_pad / PREC_QB), (unsigned)a.nslices), block(256);
-    if (a.cap != 512 && a.cap != 1024) return hipErrorInvalidValue;
-    const bool big = a.cap == 1024;
+    // list capacities: 128 (kc <= 64; the sample pass), 512 (kc <= 128), 1024
+    if (a.cap != 128 && a.cap != 512 && a.cap != 1024) return hipErrorInvalidValue;
+    if (a.kc + 64 > a.cap) return hipErrorInvalidValue;
+    const int jc = a.cap / 64;
     if (a.fp8) {
         // dim <= 128: one 128-value product per row
         if (a.rowb != 128 || !a.f8max) return hipErrorInvalidValue;
-        if (big) hipLaunchKernelGGL((prec_stage1<1, 16, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((prec_stage1<1, 8, true>), grid, block, 0, s, a);
+        if (jc == 16) hipLaunchKernelGGL((prec_stage1<1, 16, true>), grid, block, 0, s, a);
+        else if (jc == 8) hipLaunchKernelGGL((prec_stage1<1, 8, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((prec_stage1<1, 2, true>), grid, block, 0, s, a);
         return hipGetLastError();
     }
     if (a.rowb % 64 != 0) return hipErrorInvalidValue;
     switch (a.rowb / 64) {
 #define SSG_P1(KK)                                                                                  \
     case KK:                                                                                        \
-        if (big) hipLaunchKernelGGL((prec_stage1<KK, 16, false>), grid, block, 0, s, a);           \
-        else hipLaunchKernelGGL((prec_stage1<KK, 8, false>), grid, block, 0, s, a);                \
+        if (jc == 16) hipLaunchKernelGGL((prec_stage1<KK, 16, false>), grid, block, 0, s, a);      \
+        else if (jc == 8) hipLaunchKernelGGL((prec_stage1<KK, 8, false>), grid, block, 0, s, a);   \
+        else hipLaunchKernelGGL((prec_stage1<KK, 2, false>), grid, block, 0, s, a);                \
         break;
         SSG_P1(1) SSG_P1(2) SSG_P1(3) SSG_P1(4)
 #undef SSG_P1
