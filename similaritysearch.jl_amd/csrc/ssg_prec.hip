@@ -15,9 +15,9 @@
 // Candidate pass (prec_stage1): block = 256 queries x one DB slice, 4 waves of 64
 //   queries.  A wave keeps its queries' encoded rows as the B operands of the matrix-core
 //   product in registers for the whole slice; DB rows are staged in
-//   LDS in 128-row tiles (double-buffered, one barrier per tile, the next tile
-//   prefetched into registers while the current one is multiplied) and shared by the 4
-//   waves.  The accumulator starts at -||x||^2/2 (SqL2, L2) or 0 (NormCosine), so the
+//   LDS in tiles of 128 rows (BF16) or 256 rows (FP8) (double-buffered, one barrier per
+//   tile, the next tile prefetched into registers while the current one is multiplied)
+//   and shared by the 4 waves.  The accumulator starts at -||x||^2/2 (SqL2, L2) or 0 (NormCosine), so the
 //   MFMA result is the score s = x.q - ||x||^2/2 (or x.q): larger is nearer, and an
 //   invalid row starts at -Inf.  A lane holds 4 rows x 1 query per 16x16 tile; a tile
 //   is tested with two max operations and one compare against the query's threshold,
@@ -79,9 +79,10 @@ template <> struct PrecOp<true> {
     }
 };
 
-// J = cap / 64 list entries per lane in a compaction (8: lists of up to 512 entries, 16:
-// of 1024).  Each kernel calls one compaction instance only: the registers that instance
-// uses are what the kernel has to keep clear around the call.  F8 selects the FP8 form.
+// J = cap / 64 list entries per lane in a compaction (2: lists of 128 entries, 8: of 512,
+// 16: of 1024).  Each kernel calls one compaction instance only: the registers that
+// instance uses are what the kernel has to keep clear around the call.  F8 selects the
+// FP8 form.
 template <int KB, int J, bool F8>
 __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
 {
