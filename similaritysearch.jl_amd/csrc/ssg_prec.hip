@@ -17,9 +17,9 @@
 //   product in registers for the whole slice; DB rows are staged in
 //   LDS in tiles of 128 rows (BF16) or 256 rows (FP8) (double-buffered, one barrier per
 //   tile, the next tile prefetched into registers while the current one is multiplied)
-//   and shared by the 4 waves.  The accumulator starts at -||x||^2/2 (SqL2, L2) or 0 (NormCosine), so the
-//   MFMA result is the score s = x.q - ||x||^2/2 (or x.q): larger is nearer, and an
-//   invalid row starts at -Inf.  A lane holds 4 rows x 1 query per 16x16 tile; a tile
+//   and shared by the 4 waves.  The accumulator starts at -||x||^2/2 (SqL2, L2) or 0
+//   (NormCosine), so the MFMA result is the score s = x.q - ||x||^2/2 (or x.q): larger
+//   is nearer, and an invalid row starts at -Inf.  A lane holds 4 rows x 1 query per 16x16 tile; a tile
 //   is tested with two max operations and one compare against the query's threshold,
 //   and only a wave with a passing score takes the append path.
 //   Each (query, slice) has a list of `cap` entries {score, id} in global memory; a list
