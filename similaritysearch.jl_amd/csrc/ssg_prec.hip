@@ -240,13 +240,15 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
                     if (__builtin_expect(__ballot(mx > tau[ct]) != 0, 0)) {
                         const float t = tau[ct];
                         uint32_t o = wp[ct];
+                        // No branches: every row's entry is stored, a row that does not
+                        // pass at an offset past the buffer's extent (2^31), which the
+                        // hardware drops, and only passing rows advance the offset.
 #pragma unroll
                         for (int r = 0; r < 4; r++) {
-                            if (cur[ct][r] > t) {
-                                const u32x2 e = {__float_as_uint(cur[ct][r]), rowb + r};
-                                __builtin_amdgcn_raw_buffer_store_b64(e, lrs, (int)o, 0, 0);
-                                o += 8u;
-                            }
+                            const bool p = cur[ct][r] > t;
+                            const u32x2 e = {__float_as_uint(cur[ct][r]), rowb + r};
+                            __builtin_amdgcn_raw_buffer_store_b64(e, lrs, (int)(p ? o : 0x80000000u), 0, 0);
+                            o += p ? 8u : 0u;
                         }
                         wp[ct] = o;
                         // A lane adds at most 4 entries per query and row block, so a
