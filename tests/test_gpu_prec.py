@@ -87,11 +87,15 @@ def _check(oracle, metric, X, Q, k, ncand, ids, d, ci, cd, fmt="bf16", Qs=None):
     ("SqL2", 32, 200, 300, 10, 256),
     ("NormCosine", 96, 150, 70, 32, 128),
     ("SqL2", 16, 9, 5, 12, 16),
+    ("SqL2", 1, 40, 7, 5, 40),
+    ("SqL2", 24, 300, 0, 6, 64),   # m = 0: eight all-zero queries
 ])
 def test_every_object_a_candidate_equals_exact(oracle, metric, dim, n, m, k, ncand, fmt):
     ss = _ss()
     X = _unit(n, dim, 1) if metric == "NormCosine" else _uniform(n, dim, 1)
     Q = _unit(m, dim, 2) if metric == "NormCosine" else _uniform(m, dim, 2)
+    if m == 0:
+        Q = np.zeros((8, dim), np.float32)
     E = ss.ExhaustiveSearch(_dist(ss, metric), ss.MatrixDatabase.from_rows(X))
     ids, d, ci, cd = ss.exhaustive_candidates(E, k, ncand, ss.MatrixDatabase.from_rows(Q), fmt)
     gi, gd = oracle.exhaustive(metric, X, Q, k)
@@ -126,6 +130,35 @@ def test_candidates_and_rerank(oracle, metric, dim, n, m, k, ncand, slices, fmt)
     rec = oracle.macrorecall(gi, ids.T)
     print(f"recall {fmt} {metric} dim {dim} n {n} k {k} ncand {ncand}: {rec:.4f}")
     assert rec >= _MIN_RECALL[fmt], rec
+
+
+@pytest.mark.parametrize("fmt", ["bf16", "fp8"])
+def test_l2_precision_form_follows_sqeuclidean(oracle, fmt):
+    """L2 scores the candidates as SqL2 does (the same -|x|^2/2 start value), so its
+    candidate set equals SqL2's, its approximate distances are their square roots (FP32
+    sqrt of the reported SqL2 value, clamped at 0; within one ulp), and its results are
+    the canonical L2 distances of the exact top k among the candidates."""
+    ss = _ss()
+    X = _uniform(5000, 48, 50)
+    Q = _uniform(120, 48, 51)
+    E2 = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(X))
+    E1 = ss.ExhaustiveSearch(ss.L2, ss.MatrixDatabase.from_rows(X))
+    i2, d2, c2, a2 = [t.T for t in ss.exhaustive_candidates(E2, 10, 48, ss.MatrixDatabase.from_rows(Q), fmt)]
+    i1, d1, c1, a1 = [t.T for t in ss.exhaustive_candidates(E1, 10, 48, ss.MatrixDatabase.from_rows(Q), fmt)]
+    for j in range(len(Q)):
+        assert set(c1[j].tolist()) == set(c2[j].tolist()), j
+        o1, o2 = np.argsort(c1[j]), np.argsort(c2[j])
+        # within one ulp: the device's FP32 square root against numpy's
+        np.testing.assert_array_max_ulp(a1[j][o1], np.sqrt(np.maximum(a2[j][o2], np.float32(0))), maxulp=1)
+    D = oracle.dist_batch("L2", X, Q)
+    gi, _ = oracle.exhaustive("L2", X, Q, 10)
+    for j in range(len(Q)):
+        np.testing.assert_array_equal(d1[j], D[j, i1[j].astype(np.int64) - 1])
+        # the exact top 10 among the candidates, in (distance, id) order
+        cd = D[j, c1[j].astype(np.int64) - 1]
+        order = np.lexsort((c1[j], cd))[:10]
+        np.testing.assert_array_equal(i1[j], c1[j][order])
+    assert oracle.macrorecall(gi, i1) >= _MIN_RECALL[fmt]
 
 
 def test_fp8_power_of_two_scaling_is_exact():
