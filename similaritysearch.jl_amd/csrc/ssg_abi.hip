@@ -359,6 +359,7 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     if (h->xe_fmt != precision) {
         h->Xe.release();
         h->Xs.release();
+        h->Xs2.release();
         h->xe_fmt = 0;
     }
     if (f8) HIPCHK(h->xAmax.ensure(4 * sizeof(uint32_t)), "alloc scales");
@@ -414,7 +415,11 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     constexpr int32_t STEP = 16;
     const int64_t rps0 = (n + ns - 1) / ns;
     const bool sample = h->prec_sample > 0 || (h->prec_sample == 0 && rps0 >= (int64_t)STEP * 128 * 64);
-    const int64_t unit = sample ? (int64_t)STEP * 128 : 128;
+    // On slices of at least 256 Ki rows, a seed pass over every 16th tile of the sample
+    // gives the sample pass its starting thresholds (slices then hold whole groups of
+    // STEP^2 tiles, so each slice's seed rows are contiguous too).
+    const bool seed = sample && rps0 >= (int64_t)STEP * STEP * 128 * 8;
+    const int64_t unit = seed ? (int64_t)STEP * STEP * 128 : (sample ? (int64_t)STEP * 128 : 128);
     const int64_t rps = (rps0 + unit - 1) / unit * unit;
     ns = (int32_t)((n + rps - 1) / rps);
     HIPCHK(h->pList.ensure((size_t)m_pad * ns * cap * sizeof(uint2)), "alloc candidate lists");
@@ -474,6 +479,32 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     as.tile_step = 1;
     as.sample = 1;
     as.tau0 = h->pTau0.as<float>();
+    if (seed) {
+        // Seed pass: the T2-th best score of every 16th tile of the sample (about rank
+        // 16 T2 = 128 of the sample for T = 24) starts the sample pass's threshold, so it
+        // appends about 16 T2 rows per query instead of filling its lists from nothing.
+        // A query with fewer than T sample rows above its seed gets no threshold from the
+        // sample pass (its main pass then starts from none): slower, still exact.
+        const int64_t nsr2 = prec_sample_rows(nsr, STEP);
+        if (!h->Xs2.p) {
+            HIPCHK(h->Xs2.ensure((size_t)nsr2 * rowb), "alloc seed rows");
+            HIPCHK(h->xns2.ensure((size_t)nsr2 * sizeof(float)), "alloc seed norms");
+            HIPCHK(launch_sample_rows(h->Xs.as<uint16_t>(), h->xns.as<float>(), nsr2, STEP, rowb / 2,
+                                      h->Xs2.as<uint16_t>(), h->xns2.as<float>(), s), "seed rows");
+        }
+        HIPCHK(h->pTauSeed.ensure((size_t)m_pad * ns * sizeof(float)), "alloc seed thresholds");
+        PrecArgs a2 = as;
+        a2.Xe = h->Xs2.as<uint8_t>();
+        a2.xnorm = h->xns2.as<float>();
+        a2.n = nsr2;
+        a2.rows_per_slice = rps / (STEP * STEP);
+        a2.kc = std::max(4, as.kc / 3);
+        a2.cap = a2.kc <= 32 ? 128 : prec_list_cap(a2.kc);
+        a2.tau0 = h->pTauSeed.as<float>();
+        a2.tau_seed = nullptr;
+        HIPCHK(prec_launch_stage1(a2, s), "precision seed pass");
+        as.tau_seed = a2.tau0;
+    }
     HIPCHK(prec_launch_stage1(as, s), "precision sample pass");
     a.tau0 = as.tau0;
     o.flags = flags;
@@ -713,6 +744,7 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     h->Xh.release();
     h->Xe.release();
     h->Xs.release();
+    h->Xs2.release();
     h->xe_fmt = 0;
     ssg_query_layout_stale(h);
     const int64_t np = std::max<int64_t>(h->n_pad, 64);

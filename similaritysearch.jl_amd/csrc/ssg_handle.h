@@ -95,6 +95,7 @@ struct ssg_handle {
     // the number of DB slices
     DevBuf Xe, pQe, pList, pListN, pTau0, pFlags, pQf;
     DevBuf Xs, xns;  // rows of every 16th tile of Xe, packed, and their norms (sample pass)
+    DevBuf Xs2, xns2, pTauSeed;  // every 16th tile of Xs (seed pass) and its thresholds
     DevBuf xAmax;    // FP8 max |value| bits: [0] and [1] the rows, [2] the last queries
     int32_t xe_fmt = 0;  // SSG_BF16 or SSG_FP8 while Xe holds rows of that format
     int32_t prec_slices = 0;

@@ -143,6 +143,8 @@ struct PrecArgs {
     float* tau0;            // nullable, [m_pad][nslices]: the sample pass writes it; the main
                             // pass starts each query's threshold there (rows at or below it
                             // are dropped) and the rerank stage certifies the result
+    const float* tau_seed;  // nullable, sample pass only: its starting thresholds (from a
+                            // coarser sample); rows at or below them are dropped
 };
 struct PrecOut {
     int64_t m;

@@ -147,8 +147,9 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
     uint32_t wp[NCT];
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
-        tau[ct] = (a.tau0 && !a.sample) ? a.tau0[(qb0 + 64 * w + 16 * ct + col) * a.nslices + sl]
-                                        : -__builtin_inff();
+        // starting threshold: the main pass's sampled one, or a sample pass's seed
+        const float* ts = a.sample ? a.tau_seed : a.tau0;
+        tau[ct] = ts ? ts[(qb0 + 64 * w + 16 * ct + col) * a.nslices + sl] : -__builtin_inff();
         wp[ct] = (uint32_t)(((64 * w + 16 * ct + col) * qstride + g * sc) * (int64_t)sizeof(uint2));
     }
 

@@ -212,6 +212,30 @@ def test_slice_count_does_not_change_results():
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("fmt", ["bf16", "fp8"])
+def test_seeded_sample_pass_does_not_change_results(oracle, fmt):
+    """On slices of 256 Ki rows or more, a seed pass over every 16th tile of the sample
+    starts the sample pass's thresholds.  Results equal those with sampling off, for the
+    search and for allknn, and the candidates pass the oracle's checks."""
+    ss = _ss()
+    from similaritysearch import _lib
+    L = _lib.load()
+    X = _unit(300_000, 32, 60)
+    Q = _unit(300, 32, 61)
+    E = ss.ExhaustiveSearch(ss.NormCosine, ss.MatrixDatabase.from_rows(X))
+    h = E._handle()
+    _lib.check(L.ssg_set_option(h, _lib.SSG_OPT_PREC_SLICES, 1), "option")
+    outs = []
+    for t in (0, -1):  # 0: sampled thresholds with the seed pass (1 slice of 300,000 rows)
+        _lib.check(L.ssg_set_option(h, _lib.SSG_OPT_PREC_SAMPLE, t), "option")
+        outs.append(ss.exhaustive_candidates(E, 16, 64, ss.MatrixDatabase.from_rows(Q), fmt))
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+    ids, d, ci, cd = outs[0]
+    _check(oracle, "NormCosine", X, Q[:12], 16, 64, ids.T[:12], d.T[:12], ci.T[:12], cd.T[:12], fmt, Q)
+    _lib.check(L.ssg_set_option(h, _lib.SSG_OPT_PREC_SAMPLE, 0), "option")
+
+
 @pytest.mark.parametrize("fmt,slices", [("bf16", 1), ("bf16", 3), ("fp8", 2)])
 def test_sampled_thresholds_do_not_change_results(oracle, slices, fmt):
     """The sampled starting thresholds only drop rows that cannot be candidates, and
