@@ -419,3 +419,103 @@ def fp8_approx(metric, X, Q, Qscale=None):
         return 1.0 - S, eb + u * (1.0 + np.abs(S) + eb)
     qn = fma_norms(Q).astype(np.float64)[:, None]
     return qn - 2.0 * S, 2.0 * eb + u * (qn + 2.0 * (np.abs(S) + eb))
+
+
+# ------------------------------------------------------------------ batch consumers
+def bichromatic_closestpair(metric, A, B, min_k=8, samedata=False):
+    """The closest pair (i of A, j of B, dist) as bichromatic_closestpair over an
+    ExhaustiveSearch of A (src/bichromatic/bichromaticclosestpair.jl:52-139): each object
+    of B takes its min_k nearest in (distance, id) order; with samedata a nearest entry
+    equal to the query is dropped (pop_min!); the least distance wins, ties to the lowest
+    j (strict < in object order, first batch on equal minima).  Pinned by brute force
+    over all pairs in tests/test_oracle.py."""
+    ids, d = exhaustive(metric, A, B, min_k)
+    best = (0, 0, np.float32(np.inf))
+    for j in range(len(B)):
+        t = 1 if samedata and ids[j, 0] == j + 1 else 0
+        if t < min_k and ids[j, t] != 0 and d[j, t] < best[2]:
+            best = (int(ids[j, t]), j + 1, d[j, t])
+    return best
+
+
+def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True):
+    """neardup over an exact index (src/neardup.jl:69-256): blocks of blocksize objects;
+    a block's objects search the net built so far (nearest in (distance, id) order; a
+    near duplicate when within eps), the rest are filtered in order against the block's
+    new centers (first center at the least distance), which then join the net.  Returns
+    (map, nn, dist, centers, costdists) as in similaritysearch.neardup.  Pinned in
+    tests/test_oracle.py: one block is the greedy eps-net in object order (brute force),
+    and blocked runs keep the net invariants."""
+    X = _f32c(X)
+    eps = np.float32(eps)
+    n = len(X)
+    blocksize = min(blocksize, n)
+    L = np.zeros(n, np.int32)
+    D = np.zeros(n, np.float32)
+    M, net = [], []
+    cost = 0
+
+    def block(items):
+        nonlocal cost
+        if not filterblocks:
+            for i in items:
+                M.append(i); L[i - 1] = i; D[i - 1] = 0.0
+            return list(items)
+        P = dist_batch(metric, X[np.asarray(items) - 1], X[np.asarray(items) - 1])
+        tmp = [0]
+        M.append(items[0]); L[items[0] - 1] = items[0]; D[items[0] - 1] = 0.0
+        for ii in range(1, len(items)):
+            cost += len(tmp)
+            best_t, best_d = -1, np.float32(np.inf)
+            for t in tmp:
+                if P[ii, t] < best_d:
+                    best_t, best_d = t, P[ii, t]
+            i = items[ii]
+            if best_d > eps:
+                tmp.append(ii); M.append(i); L[i - 1] = i; D[i - 1] = 0.0
+            else:
+                L[i - 1] = items[best_t]; D[i - 1] = best_d
+        return [items[t] for t in tmp]
+
+    for r0 in range(0, n, blocksize):
+        rng = list(range(r0 + 1, min(n, r0 + blocksize) + 1))
+        if not net:
+            new = block(rng)
+        else:
+            # searches of the exact index count no distance evaluations (only the
+            # block filter does: src/neardup.jl:240)
+            ids, d = exhaustive(metric, X[np.asarray(net) - 1], X[r0:r0 + len(rng)], 1)
+            imap = []
+            for t, j in enumerate(rng):
+                if d[t, 0] > eps:
+                    imap.append(j)
+                else:
+                    D[j - 1] = d[t, 0]; L[j - 1] = M[int(ids[t, 0]) - 1]
+            new = block(imap) if imap else []
+        net.extend(new)
+    return np.asarray(M, np.uint32), L, D, np.asarray(net, np.uint32), cost
+
+
+def hsp_queries(metric, X, Q, ids, dists):
+    """hsp_queries (src/hsp.jl:8-33, 77-101) over (m, k) candidate rows (ascending, 0 ends a
+    row): a candidate is kept unless an already kept one is nearer to it than the query
+    (canonical distances).  Returns (m, k) kept ids and distances, then 0 / Inf.  Pinned
+    by the keep/drop invariants in tests/test_oracle.py."""
+    X = _f32c(X)
+    ids = np.asarray(ids, np.uint32)
+    dd = np.asarray(dists, np.float32)
+    m, k = ids.shape
+    oi = np.zeros((m, k), np.uint32)
+    od = np.full((m, k), np.inf, np.float32)
+    for q in range(m):
+        kept = []
+        for j in range(k):
+            if ids[q, j] == 0:
+                break
+            p = X[ids[q, j] - 1]
+            if all(not (dist(metric, p, X[ids[q, h] - 1]) < dd[q, j]) for h in kept):
+                kept.append(j)
+        kept.sort(key=lambda j: dd[q, j])          # the KnnSorted queue: stable by distance
+        oi[q, :len(kept)] = ids[q, kept]
+        od[q, :len(kept)] = dd[q, kept]
+    return oi, od

@@ -509,3 +509,9 @@ def macrorecall(gold, res, k=None):
     res = np.asarray(res)
     k = gold.shape[0] if k is None else k
     return float(np.mean([recallscore(gold[:k, j], res[:k, j]) for j in range(gold.shape[1])]))
+
+
+from .consumers import (bichromatic_closestpair, closestpair, hsp_queries, neardup,  # noqa: E402
+                        pairwise)
+
+__all__ += ["closestpair", "bichromatic_closestpair", "neardup", "hsp_queries", "pairwise"]

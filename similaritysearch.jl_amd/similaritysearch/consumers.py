@@ -1,0 +1,228 @@
+"""Batch consumers of the search path, on the GPU indexes: closest pairs, near duplicates
+and half-space proximal neighborhoods.
+
+Each follows the reference's algorithm step for step; every distance comes from the GPU
+(searchbatch!, allknn! or rerank! through the C ABI) with the canonical FP32 distance,
+and only the reference's sequential bookkeeping runs on the host.
+- closestpair / bichromatic_closestpair: src/bichromatic/closestpair.jl:39-41,
+  src/bichromatic/bichromaticclosestpair.jl:52-139
+- neardup: src/neardup.jl:69-256
+- hsp_queries: src/hsp.jl:8-33, 77-101
+"""
+import numpy as np
+
+from . import (ExhaustiveSearch, GenericContext, MatrixDatabase, SearchGraph, _as_db, rerank_,
+               searchbatch, searchbatch_knns)
+
+__all__ = ["closestpair", "bichromatic_closestpair", "neardup", "hsp_queries", "pairwise"]
+
+
+def _positions(rows, found):
+    """For each r, the position in rows[r] of each id of found[r] (ids distinct within a
+    row; entries of found that are 0 map to -1)."""
+    nq, k = rows.shape
+    off = np.arange(nq, dtype=np.int64)[:, None] << 33
+    keys = (off + rows.astype(np.int64)).ravel()
+    perm = np.argsort(keys, kind="stable")
+    at = np.searchsorted(keys[perm], (off + found.astype(np.int64)).ravel())
+    pos = (perm[np.minimum(at, len(perm) - 1)] % k).reshape(nq, -1)
+    return np.where(found != 0, pos, -1)
+
+
+def pairwise(dist, X, ids):
+    """(m, k, k) canonical distances between the objects of each row of ids ((m, k),
+    1-based, 0 ends a row; pairs with an empty slot are Inf), computed by rerank! on the
+    GPU: object ids[i, j] is the query and row i's ids are the candidates (k <= 64)."""
+    X = _as_db(X)
+    ids = np.asarray(ids, np.uint32)
+    m, k = ids.shape
+    if k > 64:
+        raise ValueError("pairwise distances take at most 64 objects per row")
+    out = np.full((m, k, k), np.inf, np.float32)
+    qi, qj = np.nonzero(ids > 0)
+    if len(qi) == 0:
+        return out
+    Q = X.rows[ids[qi, qj].astype(np.int64) - 1]                     # one query per object
+    cids = np.asfortranarray(ids[qi].T)                               # (k, nq): its row's objects
+    cd = np.asfortranarray(np.zeros((k, len(qi)), np.float32))
+    rerank_(dist, X, MatrixDatabase.from_rows(Q), cids, cd)
+    # rerank! sorts each column by distance: put the distances back by object position
+    pos = _positions(ids[qi], cids.T)
+    r, t = np.nonzero(pos >= 0)
+    out[qi[r], qj[r], pos[r, t]] = cd.T[r, t]
+    return out
+
+
+def bichromatic_closestpair(idxA, ctx, B, min_k=8, samedata=None, seed=0):
+    """bichromatic_closestpair(idxA, ctx, B; min_k, samedata) -> (i, j, dist)
+    (src/bichromatic/bichromaticclosestpair.jl:52-139): the closest pair (a, b) with a an
+    object of idxA and b one of B.  Every object of B is searched with k = min_k; when
+    samedata (idxA indexes B itself) a result whose nearest entry is the query itself
+    drops it (pop_min!).  The pair of least distance wins, ties to the lowest j.
+
+    Over a SearchGraph with samedata, the reference seeds each search with a random
+    neighbor of the object and marks the object visited; here the random neighbor comes
+    from a numpy generator seeded with `seed` and the object is only dropped from the
+    result, so that case matches the reference in kind, not draw for draw."""
+    B = _as_db(B)
+    if samedata is None:
+        samedata = idxA.db is B
+    if samedata and min_k < 2:
+        raise ValueError("min_k must be >= 2 when the index holds B itself")
+    n = len(B)
+    if n == 0:
+        raise ValueError("empty dataset")
+    if isinstance(idxA, SearchGraph) and samedata:
+        rng = np.random.default_rng(seed)
+        hints = []
+        for i in range(1, n + 1):
+            nb = idxA.adj.neighbors(i)
+            hints.append(np.array([nb[rng.integers(len(nb))]] if len(nb) else idxA.hints, np.uint32))
+        res = searchbatch_knns(idxA, ctx, B, np.full(n, min_k, np.int32), hints=hints)
+        ids = np.stack([r[0] for r in res])
+        dists = np.stack([r[1] for r in res])
+    else:
+        ids, dists = searchbatch(idxA, ctx, B, min_k)
+        ids, dists = ids.T, dists.T                                    # (n, k)
+    best = (0, 0, np.float32(np.inf))
+    for j in range(n):
+        row_i, row_d = ids[j], dists[j]
+        t = 0
+        if samedata and row_i[0] == j + 1:
+            t = 1                                                      # pop_min!
+        if t < len(row_i) and row_i[t] != 0 and row_d[t] < best[2]:
+            best = (int(row_i[t]), j + 1, row_d[t])
+    return best
+
+
+def closestpair(idx, ctx, min_k=8, seed=0):
+    """closestpair(idx, ctx; min_k) -> (i, j, dist) (src/bichromatic/closestpair.jl:39-41):
+    the closest pair of distinct objects of idx."""
+    return bichromatic_closestpair(idx, ctx, idx.db, min_k=min_k, samedata=True, seed=seed)
+
+
+def _nearest_rows(E, Q, k):
+    ids, dists = searchbatch(E, GenericContext(), Q, k)
+    return ids[0], dists[0]
+
+
+def neardup(dist, X, eps, k=8, blocksize=256, filterblocks=True):
+    """neardup(dist, X, ϵ; k, blocksize, filterblocks) with an exact index
+    (src/neardup.jl:69-85 with recall = 1.0, then neardup_ :94-171 and neardup_block!
+    :186-256).  X is processed in blocks of blocksize objects: a block is searched against
+    the ϵ-net built so far (nearest center within ϵ: a near duplicate of it); the rest of
+    the block is filtered in order against the block's own new centers (the first center
+    at the least distance, when within ϵ) and the new centers join the net.
+
+    Returns a dict with the reference's fields: map (net position -> object id), nn
+    (each object's covering object, 1-based; itself for a center), dist (the distance to
+    it), centers (the net, in construction order), costdists (distance evaluations)."""
+    X = _as_db(X)
+    eps = np.float32(eps)
+    n = len(X)
+    blocksize = min(blocksize, n)
+    L = np.zeros(n, np.int32)
+    D = np.zeros(n, np.float32)
+    M = []
+    cost = 0
+    E = None  # exact index over the net
+
+    def block_filter(items):
+        nonlocal cost
+        if not filterblocks:
+            for i in items:
+                M.append(i)
+                L[i - 1] = i
+                D[i - 1] = 0.0
+            return list(items)
+        items = list(items)
+        # distances between the block's items, in chunks of 64 candidates (rerank!)
+        P = np.full((len(items), len(items)), np.inf, np.float32)
+        ids = np.asarray(items, np.uint32)
+        Qb = MatrixDatabase.from_rows(X.rows[ids.astype(np.int64) - 1])
+        for c0 in range(0, len(items), 64):
+            chunk = ids[c0:c0 + 64]
+            cids = np.asfortranarray(np.tile(chunk, (len(items), 1)).T)
+            cd = np.asfortranarray(np.zeros((len(chunk), len(items)), np.float32))
+            rerank_(dist, X, Qb, cids, cd)
+            pos = _positions(np.tile(chunk, (len(items), 1)), cids.T)
+            P[np.arange(len(items))[:, None], c0 + pos] = cd.T
+        tmp = [0]                                  # block positions of new centers
+        first = items[0]
+        M.append(first)
+        L[first - 1] = first
+        D[first - 1] = 0.0
+        for ii in range(1, len(items)):
+            i = items[ii]
+            cost += len(tmp)
+            dj = P[ii, tmp]
+            b = int(np.argmin(dj))                 # the first center at the least distance
+            if dj[b] > eps:
+                tmp.append(ii)
+                M.append(i)
+                L[i - 1] = i
+                D[i - 1] = 0.0
+            else:
+                L[i - 1] = items[tmp[b]]
+                D[i - 1] = dj[b]
+        return [items[t] for t in tmp]
+
+    net = []
+    for r0 in range(0, n, blocksize):
+        rng = list(range(r0 + 1, min(n, r0 + blocksize) + 1))
+        if len(net) == 0:
+            new = block_filter(rng)
+        else:
+            Qb = MatrixDatabase.from_rows(X.rows[r0:r0 + len(rng)])
+            kk = min(k, len(net))
+            pid, pdist = _nearest_rows(E, Qb, kk)      # exact searches count no evaluations
+            imap = []
+            for t, j in enumerate(rng):
+                if pdist[t] > eps:
+                    imap.append(j)
+                else:
+                    D[j - 1] = pdist[t]
+                    L[j - 1] = M[int(pid[t]) - 1]
+            new = block_filter(imap) if imap else []
+        if new:
+            net.extend(new)
+            E = ExhaustiveSearch(dist, MatrixDatabase.from_rows(X.rows[np.asarray(net) - 1]))
+    return {"idx": E, "map": np.asarray(M, np.uint32), "nn": L, "dist": D,
+            "centers": np.asarray(net, np.uint32), "costdists": cost}
+
+
+def hsp_queries(dist, X, Q, knns_ids, knns_dists):
+    """hsp_queries(dist, X, Q, knns_ids, knns_dists) -> (hsp_ids, hsp_dists, hsp)
+    (src/hsp.jl:8-33, 77-101): each query's candidates ((k, n) columns, 0 ends a column)
+    are taken in column order and kept unless one already kept is nearer to the candidate
+    than the query is (hsp_should_push).  The kept entries form a sorted queue (stable by
+    distance) at the top of each (k, n) output column, then 0 / Inf; hsp lists each
+    query's (ids, dists).  The distances between a column's candidates come from
+    rerank! on the GPU (pairwise)."""
+    X = _as_db(X)
+    ids = np.asarray(knns_ids, np.uint32).T            # (n, k)
+    dd = np.asarray(knns_dists, np.float32).T
+    if ids.shape != dd.shape:
+        raise ValueError("knns_ids and knns_dists differ in shape")
+    n, k = ids.shape
+    if n != len(_as_db(Q)):
+        raise ValueError("the number of queries is different from the candidate columns")
+    # 0 ends a column: later entries are not candidates
+    ends = np.cumprod(ids != 0, axis=1).astype(bool)
+    ids = np.where(ends, ids, 0).astype(np.uint32)
+    P = pairwise(dist, X, ids)                          # (n, k, k)
+    out_i = np.zeros((n, k), np.uint32)
+    out_d = np.full((n, k), np.inf, np.float32)
+    hsp = []
+    for q in range(n):
+        kept = []
+        for j in range(k):
+            if ids[q, j] == 0:
+                break
+            if all(not (P[q, j, h] < dd[q, j]) for h in kept):
+                kept.append(j)
+        kept.sort(key=lambda j: dd[q, j])              # the KnnSorted queue
+        out_i[q, :len(kept)] = ids[q, kept]
+        out_d[q, :len(kept)] = dd[q, kept]
+        hsp.append((out_i[q, :len(kept)], out_d[q, :len(kept)]))
+    return out_i.T, out_d.T, hsp

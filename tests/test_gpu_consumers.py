@@ -1,0 +1,158 @@
+"""GPU parity of the batch consumers of the search path: closestpair /
+bichromatic_closestpair (src/bichromatic/*.jl), neardup (src/neardup.jl) and hsp_queries
+(src/hsp.jl).  Every distance they use comes from the GPU (searchbatch! or rerank!
+through the C ABI); the results are compared with the oracle's restatements
+(oracle/pyoracle.py, pinned by brute force and invariants in tests/test_oracle.py),
+bit for bit: ids, distances, maps and evaluation counts.
+"""
+import numpy as np
+import pytest
+
+
+def _ss():
+    import similaritysearch as ss
+    return ss
+
+
+def _data(n, dim, seed, metric="SqL2"):
+    X = np.random.default_rng(seed).random((n, dim), dtype=np.float32)
+    if metric == "NormCosine":
+        X /= np.linalg.norm(X, axis=1, keepdims=True)
+    return X
+
+
+def _db(X):
+    return _ss().MatrixDatabase.from_rows(X)         # (n, dim) rows = the reference's (dim, n)
+
+
+def _dist(ss, metric):
+    return {"SqL2": ss.SqL2(), "L2": ss.L2(), "NormCosine": ss.NormCosine()}[metric]
+
+
+def test_positions_host_logic():
+    """The host-side map from rerank!'s sorted columns back to object positions."""
+    from similaritysearch.consumers import _positions
+    rows = np.array([[5, 9, 2, 0], [7, 3, 1, 4]], np.uint32)
+    found = np.array([[2, 5, 9, 0], [4, 1, 7, 3]], np.uint32)
+    np.testing.assert_array_equal(_positions(rows, found), [[2, 0, 1, -1], [3, 2, 0, 1]])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("metric", ["SqL2", "L2", "NormCosine"])
+def test_closestpair_exhaustive(oracle, metric):
+    ss = _ss()
+    X = _data(2000, 12, 80, metric)
+    E = ss.ExhaustiveSearch(_dist(ss, metric), _db(X))
+    got = ss.closestpair(E, ss.GenericContext())
+    assert got == oracle.bichromatic_closestpair(metric, X, X, 8, samedata=True)
+    X[1500] = X[37]                                   # a planted duplicate pair
+    E = ss.ExhaustiveSearch(_dist(ss, metric), _db(X))
+    got = ss.closestpair(E, ss.GenericContext(), min_k=2)
+    assert got == oracle.bichromatic_closestpair(metric, X, X, 2, samedata=True)
+    assert got[:2] == (1501, 38)
+
+
+@pytest.mark.gpu
+def test_bichromatic_closestpair_exhaustive(oracle):
+    ss = _ss()
+    A = _data(3000, 8, 81)
+    B = _data(700, 8, 82)
+    E = ss.ExhaustiveSearch(ss.SqL2(), _db(A))
+    for min_k in (1, 8):
+        got = ss.bichromatic_closestpair(E, ss.GenericContext(), _db(B), min_k=min_k)
+        assert got == oracle.bichromatic_closestpair("SqL2", A, B, min_k)
+    with pytest.raises(ValueError):
+        ss.closestpair(E, ss.GenericContext(), min_k=1)
+
+
+@pytest.mark.gpu
+def test_closestpair_searchgraph():
+    """Over a SearchGraph (approximate): the pair is two distinct objects at their
+    canonical distance; a planted duplicate pair, far below every other distance, is
+    found exactly."""
+    ss = _ss()
+    X = _data(3000, 8, 83)
+    X[2500] = X[100] + np.float32(1e-4)
+    G = ss.SearchGraph.build(ss.SqL2(), _db(X), seed=3)
+    ctx = ss.SearchGraphContext()
+    got = ss.closestpair(G, ctx)
+    E = ss.ExhaustiveSearch(ss.SqL2(), _db(X))
+    exact = ss.closestpair(E, ss.GenericContext())
+    assert got == exact and exact[:2] == (2501, 101)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("metric,eps", [("SqL2", 0.02), ("L2", 0.15), ("NormCosine", 0.002)])
+@pytest.mark.parametrize("blocksize,filterblocks", [(256, True), (100, True), (128, False)])
+def test_neardup_matches_oracle(oracle, metric, eps, blocksize, filterblocks):
+    ss = _ss()
+    X = _data(1500, 3, 84, metric)
+    X[700:760] = X[10:70] + np.float32(1e-5)          # near duplicates across blocks
+    R = ss.neardup(_dist(ss, metric), _db(X), eps, blocksize=blocksize, filterblocks=filterblocks)
+    M, L, D, centers, cost = oracle.neardup(metric, X, eps, blocksize=blocksize,
+                                            filterblocks=filterblocks)
+    np.testing.assert_array_equal(R["map"], M)
+    np.testing.assert_array_equal(R["nn"], L)
+    np.testing.assert_array_equal(R["dist"], D)
+    np.testing.assert_array_equal(R["centers"], centers)
+    assert R["costdists"] == cost
+    assert len(R["idx"]) == len(centers) < 1500
+
+
+@pytest.mark.gpu
+def test_neardup_all_duplicates_and_single_block(oracle):
+    ss = _ss()
+    X = np.tile(_data(5, 6, 85), (40, 1))             # 5 distinct rows, 40 copies each
+    R = ss.neardup(ss.SqL2(), _db(X), 0.0, blocksize=64)
+    M, L, D, centers, cost = oracle.neardup("SqL2", X, 0.0, blocksize=64)
+    np.testing.assert_array_equal(R["centers"], centers)
+    np.testing.assert_array_equal(R["nn"], L)
+    assert list(centers) == [1, 2, 3, 4, 5]
+    R = ss.neardup(ss.SqL2(), _db(X[:50]), 1e9, blocksize=256)
+    assert list(R["centers"]) == [1] and (R["nn"] == 1).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("metric", ["SqL2", "L2", "NormCosine"])
+def test_hsp_queries_matches_oracle(oracle, metric):
+    ss = _ss()
+    X = _data(4000, 4, 86, metric)
+    Q = _data(300, 4, 87, metric)
+    E = ss.ExhaustiveSearch(_dist(ss, metric), _db(X))
+    ids, dists = ss.searchbatch(E, ss.GenericContext(), _db(Q), 32)
+    hi, hd, hsp = ss.hsp_queries(_dist(ss, metric), _db(X), _db(Q), ids, dists)
+    oi, od = oracle.hsp_queries(metric, X, Q, ids.T, dists.T)
+    np.testing.assert_array_equal(hi.T, oi)
+    np.testing.assert_array_equal(hd.T, od)
+    assert [len(h[0]) for h in hsp] == list((oi != 0).sum(1))
+    assert 1 < np.mean((oi != 0).sum(1)) < 32
+
+
+@pytest.mark.gpu
+def test_hsp_queries_short_columns(oracle):
+    """Candidate columns that end early (k above the database size) and the database
+    itself as the query set."""
+    ss = _ss()
+    X = _data(20, 3, 88)
+    E = ss.ExhaustiveSearch(ss.SqL2(), _db(X))
+    ids, dists = ss.searchbatch(E, ss.GenericContext(), _db(X), 24)
+    assert (ids[20:] == 0).all()
+    hi, hd, _ = ss.hsp_queries(ss.SqL2(), _db(X), _db(X), ids, dists)
+    oi, od = oracle.hsp_queries("SqL2", X, X, ids.T, dists.T)
+    np.testing.assert_array_equal(hi.T, oi)
+    np.testing.assert_array_equal(hd.T, od)
+    assert (oi[:, 0] == np.arange(1, 21)).all()       # each object keeps itself first
+
+
+@pytest.mark.gpu
+def test_pairwise_matches_dist_batch(oracle):
+    ss = _ss()
+    X = _data(500, 7, 89)
+    ids = np.random.default_rng(90).permutation(500)[:300].reshape(20, 15).astype(np.uint32) + 1
+    ids[3, 9:] = 0
+    P = ss.pairwise(ss.SqL2(), _db(X), ids)
+    for r in range(20):
+        v = ids[r][ids[r] > 0]
+        ref = oracle.dist_batch("SqL2", X[v - 1], X[v - 1])
+        np.testing.assert_array_equal(P[r, :len(v), :len(v)], ref)
+        assert np.isinf(P[r, len(v):]).all() and np.isinf(P[r, :, len(v):]).all()

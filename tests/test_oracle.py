@@ -450,3 +450,119 @@ def test_macrorecall(oracle):
     assert oracle.macrorecall(g, g + 10) == 0.0
     r = np.array([[3, 9, 1], [6, 7, 8]], np.uint32)
     assert oracle.macrorecall(g, r) == pytest.approx((2 / 3 + 1 / 3) / 2)
+
+
+# ---------------------------------------------------------------- batch consumers
+def _nearest_non_self(D, j):
+    row = D[j].copy()
+    row[j] = np.inf
+    i = int(np.argmin(row))          # least distance, lowest id on ties
+    return i, row[i]
+
+
+@pytest.mark.parametrize("metric", ["SqL2", "L2", "NormCosine"])
+def test_closestpair_against_all_pairs(oracle, metric):
+    """closestpair: every pair's canonical distance (dist_batch) gives the closest pair,
+    the lowest j first on ties; a planted exact duplicate is the pair found."""
+    rng = np.random.default_rng(70)
+    X = rng.random((300, 8), dtype=np.float32)
+    if metric == "NormCosine":
+        X /= np.linalg.norm(X, axis=1, keepdims=True)
+    for planted in (False, True):
+        if planted:
+            X[16] = X[4]
+        D = oracle.dist_batch(metric, X, X)
+        best = (0, 0, np.inf)
+        for j in range(len(X)):
+            i, d = _nearest_non_self(D, j)
+            if d < best[2]:
+                best = (i + 1, j + 1, d)
+        got = oracle.bichromatic_closestpair(metric, X, X, 8, samedata=True)
+        assert got[:2] == best[:2] and got[2] == best[2]
+        if planted:   # NormCosine of a row with itself is 1 - |x|^2, not always 0
+            assert got[:2] == (17, 5) and got[2] == D[16, 4] and (metric == "NormCosine" or got[2] == 0)
+
+
+def test_bichromatic_closestpair_against_all_pairs(oracle):
+    rng = np.random.default_rng(71)
+    A = rng.random((250, 6), dtype=np.float32)
+    B = rng.random((120, 6), dtype=np.float32)
+    D = oracle.dist_batch("SqL2", A, B)          # (|B|, |A|)
+    j, i = np.unravel_index(np.argmin(D), D.shape)
+    got = oracle.bichromatic_closestpair("SqL2", A, B, 4)
+    assert got == (int(i) + 1, int(j) + 1, D[j, i])
+
+
+def _greedy_net(D, eps):
+    centers, nn, dd = [], np.zeros(len(D), np.int32), np.zeros(len(D), np.float32)
+    for i in range(len(D)):
+        if centers:
+            row = D[i, centers]
+            b = int(np.argmin(row))
+            if row[b] <= eps:
+                nn[i], dd[i] = centers[b] + 1, row[b]
+                continue
+        centers.append(i)
+        nn[i], dd[i] = i + 1, 0.0
+    return np.asarray(centers) + 1, nn, dd
+
+
+@pytest.mark.parametrize("metric,eps", [("SqL2", 0.05), ("L2", 0.3), ("NormCosine", 0.02)])
+def test_neardup_one_block_is_the_greedy_net(oracle, metric, eps):
+    """With one block, neardup is the greedy eps-net in object order: an object joins
+    the net unless a center is within eps, else it points to the first nearest center."""
+    rng = np.random.default_rng(72)
+    X = rng.random((400, 4), dtype=np.float32)
+    if metric == "NormCosine":
+        X /= np.linalg.norm(X, axis=1, keepdims=True)
+    M, L, D, centers, cost = oracle.neardup(metric, X, eps, blocksize=400)
+    c, nn, dd = _greedy_net(oracle.dist_batch(metric, X, X), np.float32(eps))
+    np.testing.assert_array_equal(centers, c)
+    np.testing.assert_array_equal(M, c)
+    np.testing.assert_array_equal(L, nn)
+    np.testing.assert_array_equal(D, dd)
+    assert 1 < len(c) < 400
+
+
+def test_neardup_blocked_net_invariants(oracle):
+    """Blocked runs: the centers are the objects that cover themselves, pairwise farther
+    than eps; every other object lies within eps of its center, at the recorded distance;
+    without filtering, every object of the first block is a center."""
+    rng = np.random.default_rng(73)
+    X = rng.random((900, 3), dtype=np.float32)
+    eps = np.float32(0.01)
+    for bs in (64, 200):
+        M, L, D, centers, cost = oracle.neardup("SqL2", X, eps, blocksize=bs)
+        np.testing.assert_array_equal(np.sort(centers), np.nonzero(L == np.arange(1, 901))[0] + 1)
+        C = X[centers - 1]
+        P = oracle.dist_batch("SqL2", C, C)
+        np.fill_diagonal(P, np.inf)
+        assert (P > eps).all()
+        nc = L != np.arange(1, 901)
+        assert (D[nc] <= eps).all()
+        dd = np.array([oracle.dist("SqL2", X[i], X[L[i] - 1]) for i in np.nonzero(nc)[0]])
+        np.testing.assert_array_equal(D[nc], dd)
+        assert set(L[nc].tolist()) <= set(centers.tolist())
+    M, L, D, centers, cost = oracle.neardup("SqL2", X, eps, blocksize=64, filterblocks=False)
+    assert (centers[:64] == np.arange(1, 65)).all()
+
+
+def test_hsp_queries_keep_drop_invariants(oracle):
+    """hsp_queries keeps a query's first candidate, and every later candidate exactly
+    when no kept one is nearer to it than the query is (canonical distances)."""
+    rng = np.random.default_rng(74)
+    X = rng.random((500, 5), dtype=np.float32)
+    Q = rng.random((40, 5), dtype=np.float32)
+    ids, d = oracle.exhaustive("SqL2", X, Q, 16)
+    hi, hd = oracle.hsp_queries("SqL2", X, Q, ids, d)
+    for q in range(len(Q)):
+        P = oracle.dist_batch("SqL2", X[ids[q] - 1], X[ids[q] - 1])
+        kept = []
+        for j in range(16):
+            if not any(P[j, h] < d[q, j] for h in kept):
+                kept.append(j)
+        assert kept[0] == 0
+        np.testing.assert_array_equal(hi[q, :len(kept)], ids[q, kept])
+        np.testing.assert_array_equal(hd[q, :len(kept)], d[q, kept])
+        assert (hi[q, len(kept):] == 0).all()
+        assert 1 <= len(kept) < 16
