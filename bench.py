@@ -58,11 +58,12 @@ def main():
     ap.add_argument("--learned", type=str, default="",
                     help="quantile,rerank,cap,delta (skip tuning)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("search", "allknn-bf16", "allknn-fp8", "exhaustive"),
+    ap.add_argument("--workload", choices=("search", "allknn-bf16", "allknn-fp8", "exhaustive", "hsp"),
                     default="search",
                     help="search: the headline graph-search metric (default); allknn-bf16 / "
                          "allknn-fp8: exhaustive allknn with the BF16 / FP8 candidate pass and "
-                         "FP32 rerank; exhaustive: exact FP32 exhaustive k-NN")
+                         "FP32 rerank; exhaustive: exact FP32 exhaustive k-NN; hsp: hsp_queries "
+                         "over a 1M allknn")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the query-sharded "
@@ -74,6 +75,8 @@ def main():
         return allknn_prec(args, args.workload[len("allknn-"):])
     if args.workload == "exhaustive":
         return exhaustive_fp32(args)
+    if args.workload == "hsp":
+        return hsp_bench(args)
 
     import numpy as np
     import similaritysearch as ss
@@ -547,6 +550,95 @@ def allknn_prec(args, fmt):
                      "kernel": f"prec_stage1<{'3, 8, false' if fmt == 'bf16' else '1, 8, true'}> + "
                                "prec_stage2<6> (whole call)",
                      "kernel_ms": round(ms, 3), "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def hsp_bench(args):
+    """hsp_queries (src/hsp.jl:77-101) of every object of the 1,000,000 x 96 unit-vector
+    database (N(0, I) seed 6, L2-normalised), NormCosine, over its k = 32 nearest
+    neighbours (allknn with the FP8 candidate pass and FP32 rerank, computed once before
+    the timed region).  A step is one ssg_hsp_queries_dev launch over all n columns;
+    `value` is columns (queries) per second.  `roofline`: HBM-bound; the algorithmic bytes
+    of a column are its k candidate rows (k * dim * 4) plus its id and distance columns
+    read and written (16 k), over the launch's HIP-event duration.  Checked bit for bit
+    against the oracle on 5,000 evenly spaced columns; `cpu_baseline`: the oracle's
+    restated hsp_queries (OpenMP over columns) on a bounded sample of the same columns."""
+    import numpy as np
+    import similaritysearch as ss
+    from similaritysearch import datasets, _lib
+    from similaritysearch.device import DeviceArray, Event, Stream
+
+    L = _lib.load()
+    n = args.n
+    dim, k, ncand = 96, 32, 128
+    X = datasets.unit(n, dim, seed=6)
+    E = ss.ExhaustiveSearch(ss.NormCosine, ss.MatrixDatabase.from_rows(X))
+    h = E._handle()
+    st = Stream()
+    sp = st.handle
+    dI = DeviceArray((n, k), np.uint32)
+    dD = DeviceArray((n, k), np.float32)
+    dHI = DeviceArray((n, k), np.uint32)
+    dHD = DeviceArray((n, k), np.float32)
+    _lib.check(L.ssg_allknn_exhaustive_prec_dev(h, k, 2, ncand, dI.ptr, dD.ptr, None, None, sp),
+               "allknn")
+
+    def run():
+        _lib.check(L.ssg_hsp_queries_dev(h, n, k, dI.ptr, dD.ptr, dHI.ptr, dHD.ptr, sp),
+                   "hsp_queries")
+
+    for _ in range(max(1, args.warmup)):
+        run()
+    st.synchronize()
+    e0, e1 = Event(), Event()
+    e0.record(st)
+    for _ in range(args.steps):
+        run()
+    e1.record(st)
+    st.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    ids, dists = dI.numpy(), dD.numpy()
+    hi, hd = dHI.numpy(), dHD.numpy()
+    kept = float((hi != 0).sum(1).mean())
+    log(f"hsp_queries: {ms:.3f} ms per {n} columns of k={k}; mean kept {kept:.2f}")
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sub = np.linspace(0, n - 1, min(n, 5_000)).astype(np.int64)
+    oi, od = pyoracle.hsp_queries("NormCosine", X, None, ids[sub], dists[sub], nthreads=threads)
+    ok = bool((oi == hi[sub]).all() and (od == hd[sub]).all())
+    log(f"oracle match on {len(sub)} columns: {ok}")
+
+    cpu = None
+    if not args.no_cpu:
+        nq, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and nq < n:
+            pyoracle.hsp_queries("NormCosine", X, None, ids[nq:nq + 20_000], dists[nq:nq + 20_000],
+                                 nthreads=threads)
+            nq += 20_000
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(min(nq, n) / cpu_s, 1), "unit": "queries/s", "cores": threads,
+               "kind": "port",
+               "sample": f"restated hsp_queries (hsp_should_push with the FP32 canonical distance, "
+                         f"KnnSorted) of the first {min(nq, n)} columns, OpenMP over columns"}
+
+    bytes_col = k * dim * 4 + 16 * k
+    gbs = n * bytes_col / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "hsp_queries columns/s, 1M×96 NormCosine, k=32 candidates per column",
+        "value": round(n / (ms * 1e-3), 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: N(0, I) rows, seed 6, L2-normalised; candidates = allknn k=32",
+        "config": {"workload": f"hsp_queries over the allknn of {n}x{dim} NormCosine, k={k}",
+                   "mean_kept": round(kept, 3), "oracle_match_5000": ok},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "hsp_kernel<6>", "kernel_ms": round(ms, 3),
+                     "bytes_per_launch": n * bytes_col},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

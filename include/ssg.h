@@ -303,6 +303,18 @@ int ssg_rerank(ssg_handle* h, const float* Q, int64_t m, int32_t k, uint32_t* id
 int ssg_rerank_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, uint32_t* dids,
                    float* ddists, void* stream);
 
+/* hsp_queries(dist, X, Q, knns_ids, knns_dists) (src/hsp.jl:77-101, hsp_should_push
+ * :8-33) over the handle's database and distance: each of the m (k, m) candidate columns
+ * (ascending, 0 ends a column; 1 <= k <= 64) is scanned in order and a candidate is kept
+ * unless an already kept one is strictly nearer to it than its given distance.  The kept
+ * candidates fill the top of the hsp_ids / hsp_dists columns as a sorted queue (stable on
+ * equal distances), then 0 / +Inf.  The queries themselves are not needed: the given
+ * distances stand for them. */
+int ssg_hsp_queries(ssg_handle* h, int64_t m, int32_t k, const uint32_t* ids, const float* dists,
+                    uint32_t* hsp_ids, float* hsp_dists);
+int ssg_hsp_queries_dev(ssg_handle* h, int64_t m, int32_t k, const uint32_t* dids,
+                        const float* ddists, uint32_t* dhsp_ids, float* dhsp_dists, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,7 @@ def lib():
         L.or_build_graph.argtypes = [C.c_int, _p, _i64, C.c_int, C.POINTER(BuildParams),
                                      C.POINTER(Graph)]
         L.or_free.argtypes = [_p]
+        L.or_hsp_queries.argtypes = [C.c_int, _p, C.c_int, _i64, _i32, _p, _p, _p, _p, C.c_int]
         L.or_fp8_mfma.argtypes = [_p, _i64, _p, _i64, C.c_int, _p, C.c_int, C.c_int, C.c_int, _p, _p]
         L.or_fma_norms.argtypes = [_p, _i64, C.c_int, _p]
         L.or_optimize_minrecall.restype = C.c_double
@@ -496,26 +497,18 @@ def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True):
     return np.asarray(M, np.uint32), L, D, np.asarray(net, np.uint32), cost
 
 
-def hsp_queries(metric, X, Q, ids, dists):
-    """hsp_queries (src/hsp.jl:8-33, 77-101) over (m, k) candidate rows (ascending, 0 ends a
-    row): a candidate is kept unless an already kept one is nearer to it than the query
-    (canonical distances).  Returns (m, k) kept ids and distances, then 0 / Inf.  Pinned
+def hsp_queries(metric, X, Q, ids, dists, nthreads=1):
+    """hsp_queries (src/hsp.jl:8-33, 77-101) over (m, k) candidate rows (0 ends a row): a
+    candidate is kept unless an already kept one is strictly nearer to it than its given
+    distance (canonical distances); the kept enter a KnnSorted of capacity k.  Returns
+    (m, k) kept ids and distances, then 0 / Inf (or_hsp_queries in ssg_oracle.c).  Pinned
     by the keep/drop invariants in tests/test_oracle.py."""
     X = _f32c(X)
-    ids = np.asarray(ids, np.uint32)
-    dd = np.asarray(dists, np.float32)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    dd = np.ascontiguousarray(dists, np.float32)
     m, k = ids.shape
     oi = np.zeros((m, k), np.uint32)
     od = np.full((m, k), np.inf, np.float32)
-    for q in range(m):
-        kept = []
-        for j in range(k):
-            if ids[q, j] == 0:
-                break
-            p = X[ids[q, j] - 1]
-            if all(not (dist(metric, p, X[ids[q, h] - 1]) < dd[q, j]) for h in kept):
-                kept.append(j)
-        kept.sort(key=lambda j: dd[q, j])          # the KnnSorted queue: stable by distance
-        oi[q, :len(kept)] = ids[q, kept]
-        od[q, :len(kept)] = dd[q, kept]
+    lib().or_hsp_queries(METRICS.get(metric, metric), _ptr(X), X.shape[1], m, k, _ptr(ids), _ptr(dd),
+                         _ptr(oi), _ptr(od), nthreads)
     return oi, od

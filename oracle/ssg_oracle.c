@@ -843,6 +843,47 @@ static void hsp_filter(int metric, const float* X, int dim, const float* center,
     }
 }
 
+/* hsp_queries hsp.jl:77-101: each (k, m) candidate column (0 ends it) is scanned in
+ * order; hsp_should_push (hsp.jl:8-33) keeps a candidate unless an already kept one
+ * is strictly nearer to it than its given distance; the kept enter a KnnSorted of
+ * capacity k.  Outputs are (k, m) columns, then 0 / +Inf.
+ * Pinned by: the keep/drop invariants in tests/test_oracle.py (each kept candidate has
+ * no earlier kept one nearer than its distance, each dropped one has, the kept form a
+ * stable sort by distance). */
+void or_hsp_queries(int metric, const float* X, int dim, int64_t m, int32_t k,
+                    const uint32_t* ids, const float* dists, uint32_t* out_ids, float* out_d,
+                    int nthreads)
+{
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int64_t q = 0; q < m; q++) {
+        const uint32_t* ci = ids + q * k;
+        const float* cd = dists + q * k;
+        uint32_t* oi = out_ids + q * k;
+        float* od = out_d + q * k;
+        for (int32_t t = 0; t < k; t++) {
+            oi[t] = 0;
+            od[t] = INFINITY;
+        }
+        knn_t out;
+        knn_init(&out, oi, od, k);
+        for (int32_t j = 0; j < k; j++) {
+            const uint32_t pid = ci[j];
+            if (pid == 0) break;
+            const float* pt = X + (int64_t)(pid - 1) * dim;
+            int push = 1;
+            const int32_t ol = knn_len(&out);
+            for (int32_t t = 0; t < ol; t++) {
+                const uint32_t hid = out.ids[out.sp + t];
+                if (or_dist(metric, pt, X + (int64_t)(hid - 1) * dim, dim) < cd[j]) {
+                    push = 0;
+                    break;
+                }
+            }
+            if (push) knn_push(&out, pid, cd[j]);
+        }
+    }
+}
+
 typedef struct {
     int32_t bsize;
     float delta;

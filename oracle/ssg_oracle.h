@@ -65,6 +65,9 @@ void or_allknn_graph(int metric, const float* X, int64_t n, int dim, const int64
                      int nthreads);
 
 /* rerank! batch form: rescore the non-zero-id prefix of every column, stable-sort it */
+void or_hsp_queries(int metric, const float* X, int dim, int64_t m, int32_t k,
+                    const uint32_t* ids, const float* dists, uint32_t* out_ids, float* out_d,
+                    int nthreads);
 void or_rerank(int metric, const float* X, int dim, const float* Q, int64_t m, int32_t k,
                uint32_t* ids, float* dists);
 

@@ -1193,6 +1193,54 @@ int ssg_rerank_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, uint32_
     return SSG_OK;
 }
 
+int ssg_hsp_queries_dev(ssg_handle* h, int64_t m, int32_t k, const uint32_t* dids,
+                        const float* ddists, uint32_t* dhsp_ids, float* dhsp_dists, void* stream)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    if (!h->X.p) return ssg_fail(SSG_E_STATE, "no database uploaded");
+    if (!dids || !ddists || !dhsp_ids || !dhsp_dists) return ssg_fail(SSG_E_ARG, "null buffer");
+    if (m <= 0 || k < 1) return ssg_fail(SSG_E_ARG, "hsp_queries: empty candidate set");
+    if (k > 64) return ssg_fail(SSG_E_LIMIT, "hsp_queries: columns are limited to 64 rows");
+    HIPCHK(launch_hsp(h->X.as<float>(), h->dp, h->metric, m, k, dids, ddists, dhsp_ids, dhsp_dists,
+                      pick(h, stream)), "hsp_queries");
+    return SSG_OK;
+}
+
+int ssg_hsp_queries(ssg_handle* h, int64_t m, int32_t k, const uint32_t* ids, const float* dists,
+                    uint32_t* hsp_ids, float* hsp_dists)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    if (!h->X.p) return ssg_fail(SSG_E_STATE, "no database uploaded");
+    if (!ids || !dists || !hsp_ids || !hsp_dists) return ssg_fail(SSG_E_ARG, "null buffer");
+    if (m <= 0 || k < 1) return ssg_fail(SSG_E_ARG, "hsp_queries: empty candidate set");
+    if (k > 64) return ssg_fail(SSG_E_LIMIT, "hsp_queries: columns are limited to 64 rows");
+    for (int64_t j = 0; j < m; j++)
+        for (int32_t t = 0; t < k; t++) {
+            const uint32_t v = ids[j * k + t];
+            if (v == 0) break;
+            if ((int64_t)v > h->n) return ssg_fail(SSG_E_ARG, "hsp_queries: id out of range");
+        }
+    hipStream_t s = h->stream;
+    const size_t bytes = (size_t)m * k * 4;
+    HIPCHK(h->sIds.ensure(bytes), "alloc");
+    HIPCHK(h->sD.ensure(bytes), "alloc");
+    HIPCHK(h->cIds.ensure(bytes), "alloc");
+    HIPCHK(h->cD.ensure(bytes), "alloc");
+    HIPCHK(hipMemcpyAsync(h->sIds.p, ids, bytes, hipMemcpyHostToDevice, s), "copy");
+    HIPCHK(hipMemcpyAsync(h->sD.p, dists, bytes, hipMemcpyHostToDevice, s), "copy");
+    HIPCHK(launch_hsp(h->X.as<float>(), h->dp, h->metric, m, k, h->sIds.as<uint32_t>(),
+                      h->sD.as<float>(), h->cIds.as<uint32_t>(), h->cD.as<float>(), s),
+           "hsp_queries");
+    HIPCHK(hipMemcpyAsync(hsp_ids, h->cIds.p, bytes, hipMemcpyDeviceToHost, s), "copy");
+    HIPCHK(hipMemcpyAsync(hsp_dists, h->cD.p, bytes, hipMemcpyDeviceToHost, s), "copy");
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    return SSG_OK;
+}
+
 int ssg_rerank(ssg_handle* h, const float* Q, int64_t m, int32_t k, uint32_t* ids, float* dists)
 {
     if (!h) return ssg_fail(SSG_E_ARG, "null handle");

@@ -80,6 +80,7 @@ struct ssg_handle {
     DevBuf qpad, qnorm, cand_d, cand_i, slice_tau, flags, part_i, part_d, xList;
     // host-call staging
     DevBuf sQ, sIds, sD, sCd, sCb, sK, sB, sH, sHc, sO;
+    DevBuf cIds, cD;  // hsp_queries outputs (host entry point)
     // builder scratch
     DevBuf bIds, bD, bOut, bCnt, bPairs, bNodeUpd;
     // learned-search scratch: hints, bucket, confidence, hint counts, beam sizes

@@ -204,6 +204,10 @@ __host__ __device__ inline int fp8_scale_exp(uint32_t amax_bits)
 
 hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float* Q, int32_t qdim,
                          int64_t m, int32_t k, uint32_t* ids, float* dists, hipStream_t s);
+// hsp_queries filter (ssg_consumers.hip): (k, m) candidate columns -> kept columns.
+hipError_t launch_hsp(const float* X, int32_t dp, int32_t metric, int64_t m, int32_t k,
+                      const uint32_t* ids, const float* dists, uint32_t* out_ids, float* out_dists,
+                      hipStream_t s);
 hipError_t launch_pad_rows(const float* src, int64_t rows, int32_t dim, float* dst, int32_t dp,
                            float* norms, hipStream_t s);
 

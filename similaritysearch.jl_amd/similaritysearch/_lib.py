@@ -98,6 +98,8 @@ SIGNATURES = {
     "ssg_graph_destroy": (C.c_int, [_p]),
     "ssg_rerank": (C.c_int, [_p, _p, _i64, _i32, _p, _p]),
     "ssg_rerank_dev": (C.c_int, [_p, _p, _i64, _i32, _p, _p, _p]),
+    "ssg_hsp_queries": (C.c_int, [_p, _i64, _i32, _p, _p, _p, _p]),
+    "ssg_hsp_queries_dev": (C.c_int, [_p, _i64, _i32, _p, _p, _p, _p, _p]),
     "ssg_exhaustive_knn_prec": (C.c_int, [_p, _p, _i64, _i32, _i32, _i32, _p, _p, _p, _p]),
     "ssg_exhaustive_knn_prec_dev": (C.c_int, [_p, _p, _i64, _i32, _i32, _i32, _p, _p, _p, _p, _p]),
     "ssg_allknn_exhaustive_prec": (C.c_int, [_p, _i32, _i32, _i32, _p, _p, _p, _p]),

@@ -11,8 +11,8 @@ and only the reference's sequential bookkeeping runs on the host.
 """
 import numpy as np
 
-from . import (ExhaustiveSearch, GenericContext, MatrixDatabase, SearchGraph, _as_db, rerank_,
-               searchbatch, searchbatch_knns)
+from . import (ExhaustiveSearch, GenericContext, MatrixDatabase, SearchGraph, _as_db, _lib,
+               _metric, rerank_, searchbatch, searchbatch_knns)
 
 __all__ = ["closestpair", "bichromatic_closestpair", "neardup", "hsp_queries", "pairwise"]
 
@@ -191,38 +191,35 @@ def neardup(dist, X, eps, k=8, blocksize=256, filterblocks=True):
             "centers": np.asarray(net, np.uint32), "costdists": cost}
 
 
+_hsp_cache = {}
+
+
 def hsp_queries(dist, X, Q, knns_ids, knns_dists):
     """hsp_queries(dist, X, Q, knns_ids, knns_dists) -> (hsp_ids, hsp_dists, hsp)
-    (src/hsp.jl:8-33, 77-101): each query's candidates ((k, n) columns, 0 ends a column)
-    are taken in column order and kept unless one already kept is nearer to the candidate
-    than the query is (hsp_should_push).  The kept entries form a sorted queue (stable by
-    distance) at the top of each (k, n) output column, then 0 / Inf; hsp lists each
-    query's (ids, dists).  The distances between a column's candidates come from
-    rerank! on the GPU (pairwise)."""
+    (src/hsp.jl:8-33, 77-101): each query's candidates ((k, n) columns, 0 ends a column,
+    k <= 64) are taken in column order and kept unless one already kept is strictly
+    nearer to the candidate than the query is (hsp_should_push).  The kept entries form
+    a sorted queue (stable by distance) at the top of each (k, n) output column, then
+    0 / Inf; hsp lists each query's (ids, dists).  The filter runs on the GPU
+    (ssg_hsp_queries: one wave per query, candidate rows staged in LDS)."""
     X = _as_db(X)
-    ids = np.asarray(knns_ids, np.uint32).T            # (n, k)
-    dd = np.asarray(knns_dists, np.float32).T
+    ids = np.ascontiguousarray(np.asarray(knns_ids, np.uint32).T)     # (n, k) = (k, n) columns
+    dd = np.ascontiguousarray(np.asarray(knns_dists, np.float32).T)
     if ids.shape != dd.shape:
         raise ValueError("knns_ids and knns_dists differ in shape")
     n, k = ids.shape
     if n != len(_as_db(Q)):
         raise ValueError("the number of queries is different from the candidate columns")
-    # 0 ends a column: later entries are not candidates
-    ends = np.cumprod(ids != 0, axis=1).astype(bool)
-    ids = np.where(ends, ids, 0).astype(np.uint32)
-    P = pairwise(dist, X, ids)                          # (n, k, k)
-    out_i = np.zeros((n, k), np.uint32)
-    out_d = np.full((n, k), np.inf, np.float32)
-    hsp = []
-    for q in range(n):
-        kept = []
-        for j in range(k):
-            if ids[q, j] == 0:
-                break
-            if all(not (P[q, j, h] < dd[q, j]) for h in kept):
-                kept.append(j)
-        kept.sort(key=lambda j: dd[q, j])              # the KnnSorted queue
-        out_i[q, :len(kept)] = ids[q, kept]
-        out_d[q, :len(kept)] = dd[q, kept]
-        hsp.append((out_i[q, :len(kept)], out_d[q, :len(kept)]))
-    return out_i.T, out_d.T, hsp
+    key = (id(X), _metric(dist))
+    E = _hsp_cache.get(key)
+    if E is None or E.db is not X:
+        E = ExhaustiveSearch(dist, X)
+        _hsp_cache.clear()
+        _hsp_cache[key] = E
+    oi = np.zeros((n, k), np.uint32)
+    od = np.full((n, k), np.inf, np.float32)
+    _lib.check(_lib.load().ssg_hsp_queries(E._handle(), n, k, ids.ctypes.data, dd.ctypes.data,
+                                           oi.ctypes.data, od.ctypes.data), "hsp_queries")
+    cnt = (oi != 0).sum(1)
+    hsp = [(oi[q, :cnt[q]], od[q, :cnt[q]]) for q in range(n)]
+    return oi.T, od.T, hsp

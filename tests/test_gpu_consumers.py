@@ -156,3 +156,41 @@ def test_pairwise_matches_dist_batch(oracle):
         ref = oracle.dist_batch("SqL2", X[v - 1], X[v - 1])
         np.testing.assert_array_equal(P[r, :len(v), :len(v)], ref)
         assert np.isinf(P[r, len(v):]).all() and np.isinf(P[r, :, len(v):]).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,dim,k", [(3000, 256, 64), (2000, 100, 40), (500, 16, 1)])
+def test_hsp_queries_shapes_and_unsorted_columns(oracle, n, dim, k):
+    """Wide rows with k = 64 (one wave per block), ragged dims, k = 1, and columns given
+    out of distance order (the kept queue sorts them, stable on ties)."""
+    ss = _ss()
+    X = _data(n, dim, 91)
+    Q = _data(64, dim, 92)
+    ids, dists = (np.ascontiguousarray(a.T) for a in oracle.exhaustive("SqL2", X, Q, k))
+    rng = np.random.default_rng(93)
+    perm = rng.permutation(k)
+    ids, dists = np.ascontiguousarray(ids[perm]), np.ascontiguousarray(dists[perm])
+    if k >= 3:
+        dists[1:3] = dists[0]                         # ties among the given distances
+    hi, hd, _ = ss.hsp_queries(ss.SqL2(), _db(X), _db(Q), ids, dists)
+    oi, od = oracle.hsp_queries("SqL2", X, Q, ids.T, dists.T)
+    np.testing.assert_array_equal(hi.T, oi)
+    np.testing.assert_array_equal(hd.T, od)
+    with pytest.raises(Exception, match="64"):
+        ss.hsp_queries(ss.SqL2(), _db(X), _db(Q[:2]), np.ones((65, 2), np.uint32),
+                       np.zeros((65, 2), np.float32))
+
+
+@pytest.mark.gpu
+def test_hsp_queries_hand_case():
+    """Points on a line, query at 0 (the oracle's hand case): equal distance to a kept
+    candidate keeps (strict <); the kept come back sorted, stable on ties."""
+    ss = _ss()
+    X = np.zeros((4, 3), np.float32)
+    X[:, 0] = [2.0, 1.0, 3.0, -1.0]
+    ids = np.array([[1], [2], [3], [4]], np.uint32)                  # (k, n) = (4, 1)
+    d = np.array([[4.0], [1.0], [9.0], [1.0]], np.float32)
+    hi, hd, hsp = ss.hsp_queries(ss.SqL2(), _db(X), _db(np.zeros((1, 3), np.float32)), ids, d)
+    np.testing.assert_array_equal(hi[:, 0], [2, 4, 1, 0])
+    np.testing.assert_array_equal(hd[:, 0], np.array([1.0, 1.0, 4.0, np.inf], np.float32))
+    assert len(hsp[0][0]) == 3

@@ -548,21 +548,48 @@ def test_neardup_blocked_net_invariants(oracle):
 
 
 def test_hsp_queries_keep_drop_invariants(oracle):
-    """hsp_queries keeps a query's first candidate, and every later candidate exactly
-    when no kept one is nearer to it than the query is (canonical distances)."""
+    """hsp_queries, checked through properties of its output that fix it uniquely (by
+    induction over the column order): the first candidate is kept; a kept candidate has
+    no kept candidate before it in the column that is strictly nearer to it than its
+    given distance, and a dropped one has; the kept fill the output as a stable sort by
+    distance (column order on ties), then 0 / Inf.  Columns are given shuffled, with tied
+    distances, and some end early (0)."""
     rng = np.random.default_rng(74)
     X = rng.random((500, 5), dtype=np.float32)
     Q = rng.random((40, 5), dtype=np.float32)
     ids, d = oracle.exhaustive("SqL2", X, Q, 16)
+    for q in range(len(Q)):
+        p = rng.permutation(16)
+        ids[q], d[q] = ids[q, p], d[q, p]
+    d[::3, 2] = d[::3, 5]
+    ids[::7, 12:] = 0
     hi, hd = oracle.hsp_queries("SqL2", X, Q, ids, d)
     for q in range(len(Q)):
-        P = oracle.dist_batch("SqL2", X[ids[q] - 1], X[ids[q] - 1])
-        kept = []
-        for j in range(16):
-            if not any(P[j, h] < d[q, j] for h in kept):
-                kept.append(j)
-        assert kept[0] == 0
-        np.testing.assert_array_equal(hi[q, :len(kept)], ids[q, kept])
-        np.testing.assert_array_equal(hd[q, :len(kept)], d[q, kept])
-        assert (hi[q, len(kept):] == 0).all()
-        assert 1 <= len(kept) < 16
+        L = int(np.argmax(ids[q] == 0)) if (ids[q] == 0).any() else 16
+        P = oracle.dist_batch("SqL2", X[ids[q, :L] - 1], X[ids[q, :L] - 1])
+        nk = int((hi[q] != 0).sum())
+        pos = [int(np.nonzero(ids[q, :L] == v)[0][0]) for v in hi[q, :nk]]
+        kept = set(pos)
+        assert 0 in kept and 1 <= nk < L
+        for c in range(L):
+            blocked = any(P[c, h] < d[q, c] for h in kept if h < c)
+            assert blocked == (c not in kept)
+        order = sorted(kept, key=lambda c: (d[q, c], c))
+        assert pos == order
+        np.testing.assert_array_equal(hd[q, :nk], d[q, order])
+        assert (hi[q, nk:] == 0).all() and np.isinf(hd[q, nk:]).all()
+
+
+def test_hsp_queries_hand_case(oracle):
+    """Points on a line, query at 0: a candidate at distance exactly equal to its
+    distance from a kept one is kept (strict <), a farther-behind one is dropped."""
+    X = np.zeros((4, 3), np.float32)
+    X[:, 0] = [2.0, 1.0, 3.0, -1.0]                 # objects 1..4
+    Q = np.zeros((1, 3), np.float32)
+    ids = np.array([[1, 2, 3, 4]], np.uint32)        # given out of distance order
+    d = np.array([[4.0, 1.0, 9.0, 1.0]], np.float32)
+    hi, hd = oracle.hsp_queries("SqL2", X, Q, ids, d)
+    # 1 kept; 2: d(2,1) = 1, not < 1, kept; 3: d(3,1) = 1 < 9, dropped;
+    # 4: d(4,1) = 9, d(4,2) = 4, neither < 1, kept.  Sorted: 2 (1.0), 4 (1.0), 1 (4.0)
+    np.testing.assert_array_equal(hi[0], [2, 4, 1, 0])
+    np.testing.assert_array_equal(hd[0], np.array([1.0, 1.0, 4.0, np.inf], np.float32))
