@@ -187,6 +187,8 @@ class _GpuIndex:
 
     def _handle(self):
         if self._h is None:
+            if self.db.rows.dtype != np.float32 or not self.db.rows.flags.c_contiguous:
+                raise TypeError("the GPU indexes take a contiguous Float32 database")
             L = _lib.load()
             h = C.c_void_p()
             _lib.check(L.ssg_create(C.byref(h), self.device), "ssg_create")
@@ -515,3 +517,9 @@ from .consumers import (bichromatic_closestpair, closestpair, hsp_queries, neard
                         pairwise)
 
 __all__ += ["closestpair", "bichromatic_closestpair", "neardup", "hsp_queries", "pairwise"]
+
+from .storage import (MMapMatrixDatabase, load_index, load_results, read_matrix,  # noqa: E402
+                      save_index, split_pathkey, write_results)
+
+__all__ += ["MMapMatrixDatabase", "save_index", "load_index", "read_matrix", "split_pathkey",
+            "write_results", "load_results"]
