@@ -11,13 +11,40 @@
 // per wave), so the kept-vs-candidate distances of the scan read LDS, not HBM: the
 // HBM traffic per query is its candidate rows plus the id/distance columns.  Distances
 // are the canonical ones (run_partial + the 16-lane tree in FP64, rounded once), the
-// same evaluation as rerank! and the beam search.
+// same evaluation as rerank! and the beam search; the tree runs on DPP row shifts and
+// its total (lane 15 of each 16-lane group) is the butterfly's, bit for bit.
+//
+// The kept rows are compacted in place: when candidate c is kept as the A-th, its row
+// moves to slot A (slots A..c-1 held candidates already decided), so kept row a sits at
+// slot a.  Each candidate is tested against every kept row, eight per iteration (two
+// independent 4-row steps), with one ballot at the end: the reference stops at the first
+// nearer kept row, which decides the same way.
 #include <hip/hip_runtime.h>
 
 #include "ssg_device.h"
 #include "ssg_kernels.h"
 
 namespace ssg {
+
+// run_partial with the second operand already widened to FP64 (the same operations:
+// the widening is exact, so the sum is run_partial's bit for bit).
+template <int V>
+__device__ __forceinline__ double run_partial_wide(int metric, const double (&p)[V],
+                                                   const float (&x)[V])
+{
+    double s = 0.0;
+    if (metric == M_NORMCOS) {
+#pragma unroll
+        for (int t = 0; t < V; t++) s = __builtin_fma(p[t], (double)x[t], s);
+    } else {
+#pragma unroll
+        for (int t = 0; t < V; t++) {
+            const double m = p[t] - (double)x[t];
+            s = __builtin_fma(m, m, s);
+        }
+    }
+    return s;
+}
 
 template <int V>
 __global__ __launch_bounds__(256) void hsp_kernel(const float* __restrict__ X, int metric, int64_t m,
@@ -39,40 +66,70 @@ __global__ __launch_bounds__(256) void hsp_kernel(const float* __restrict__ X, i
     const uint64_t z = __ballot(lane < k && cid == 0u);
     const int L = z ? (__ffsll((unsigned long long)z) - 1) : k;
 
-    // stage the L candidate rows: four rows per step, 16 lanes per row
-    for (int r0 = 0; r0 < L; r0 += 4) {
-        const int r = r0 + g;
-        const uint32_t id = __shfl(cid, r < L ? r : 0);
-        if (r < L) {
-            float xv[V];
-            load_run<V>(X + (size_t)(id - 1u) * dp + j * V, xv);
+    // stage the L candidate rows, 16 lanes per row: up to 32 rows' loads in flight before
+    // their LDS stores
+    constexpr int SB = V <= 8 ? 8 : 4;  // 4-row steps per batch
+    for (int r0 = 0; r0 < L; r0 += 4 * SB) {
+        float xv[SB][V];
 #pragma unroll
-            for (int t = 0; t < V; t++) R[r * dp + j * V + t] = xv[t];
+        for (int s = 0; s < SB; s++) {
+            const int r = r0 + 4 * s + g;
+            const uint32_t id = __shfl(cid, r < L ? r : 0);
+            if (r < L) {
+                load_run<V>(X + (size_t)(id - 1u) * dp + j * V, xv[s]);
+            } else {
+#pragma unroll
+                for (int t = 0; t < V; t++) xv[s][t] = 0.f;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < SB; s++) {
+            const int r = r0 + 4 * s + g;
+            if (r < L) {
+#pragma unroll
+                for (int t = 0; t < V; t++) R[r * dp + j * V + t] = xv[s][t];
+            }
         }
     }
     wave_sync();
 
     RegQueue<1, false> out;
     out.init(k);
-    uint32_t kp = 0;  // lane a: input position of the a-th kept candidate
     int A = 0;
     for (int c = 0; c < L; c++) {
         float pv[V];
+        double pw[V];
 #pragma unroll
-        for (int t = 0; t < V; t++) pv[t] = R[c * dp + j * V + t];
-        const float pd = rl_f(cd, c);
-        bool ok = true;
-        for (int a0 = 0; a0 < A && ok; a0 += 4) {
-            const int u = a0 + g;
-            const int hp = (int)__shfl(kp, u < A ? u : 0);
-            float xh[V];
-#pragma unroll
-            for (int t = 0; t < V; t++) xh[t] = R[hp * dp + j * V + t];
-            const float dph = finish_dist(metric, group16_sum(run_partial<V>(metric, pv, xh)));
-            if (__ballot(u < A && dph < pd)) ok = false;
+        for (int t = 0; t < V; t++) {
+            pv[t] = R[c * dp + j * V + t];
+            pw[t] = (double)pv[t];
         }
-        if (ok) {
-            if (lane == A) kp = (uint32_t)c;
+        const float pd = rl_f(cd, c);
+        bool hit = false;
+        for (int a0 = 0; a0 < A; a0 += 8) {
+            const int u0 = a0 + g, u1 = a0 + 4 + g;
+            const int r0 = u0 < A ? u0 : 0, r1 = u1 < A ? u1 : 0;  // in-range slots only
+            float x0[V], x1[V];
+#pragma unroll
+            for (int t = 0; t < V; t++) {
+                x0[t] = R[r0 * dp + j * V + t];
+                x1[t] = R[r1 * dp + j * V + t];
+            }
+            const double s0 = group16_tree(run_partial_wide<V>(metric, pw, x0));
+            const double s1 = group16_tree(run_partial_wide<V>(metric, pw, x1));
+            if (j == 15) {
+                hit |= u0 < A && finish_dist(metric, s0) < pd;
+                hit |= u1 < A && finish_dist(metric, s1) < pd;
+            }
+        }
+        if (!__ballot(hit)) {
+            if (A != c) {
+                if (g == 0) {
+#pragma unroll
+                    for (int t = 0; t < V; t++) R[A * dp + j * V + t] = pv[t];
+                }
+                wave_sync();
+            }
             A++;
             out.push(pd, rl_u(cid, c), 0, 0);
         }
