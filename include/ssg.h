@@ -83,6 +83,12 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
  * offset[i] is the inclusive end of node i+1's list in end_point (offset[0] of the
  * reference is implicit 0), end_point holds 1-based ids.  hints are SearchGraph.hints
  * (src/searchgraph/SearchGraph.jl:147-158), 1-based. */
+/* append_items!(index, ctx, X) for an exhaustive index (src/exact/sequential-exhaustive.jl:33-43
+ * appends to its database): add m objects (m x dim row-major FP32, the reference's
+ * column-major (dim, m)) after the uploaded ones; they get ids n+1..n+m.  Device capacity
+ * doubles as needed, so repeated appends move each object once.  Rejected on a handle
+ * that holds a graph. */
+int ssg_append_db(ssg_handle* h, const float* X, int64_t m);
 int ssg_upload_graph(ssg_handle* h, const int64_t* offset, int64_t n, const uint32_t* end_point,
                      int64_t nnz, const uint32_t* hints, int64_t nhints);
 

@@ -781,6 +781,69 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     return SSG_OK;
 }
 
+int ssg_append_db(ssg_handle* h, const float* X, int64_t m)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->X.p) return ssg_fail(SSG_E_STATE, "ssg_append_db: no database uploaded");
+    if (m < 0 || (m > 0 && !X)) return ssg_fail(SSG_E_ARG, "ssg_append_db: bad arguments");
+    if (h->has_graph)
+        return ssg_fail(SSG_E_STATE, "ssg_append_db: the handle holds a graph (inserting into a "
+                                     "SearchGraph is index!, not an append)");
+    if (h->n + m >= (int64_t)0xffffffffLL) return ssg_fail(SSG_E_LIMIT, "ssg_append_db: n must fit UInt32 ids");
+    if (m == 0) return SSG_OK;
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    HIPCHK(hipStreamSynchronize(h->stream), "sync");  // no launch may still read the old rows
+    const int64_t n = h->n, n2 = n + m, dp = h->dp, dim = h->dim;
+    const int64_t np2 = std::max<int64_t>((n2 + 63) / 64 * 64, 64);
+    const size_t rowb = (size_t)dp * sizeof(float);
+    if ((size_t)np2 * rowb > h->X.cap || (size_t)np2 * sizeof(float) > h->xnorm.cap) {
+        // capacity doubles; rows past n stay zero (the padding the kernels read)
+        const int64_t have = (int64_t)(h->X.cap / rowb);
+        const int64_t rows = std::max<int64_t>(np2, (2 * have + 63) / 64 * 64);
+        const size_t xold = h->X.cap, nold = h->xnorm.cap;
+        HIPCHK(h->X.grow((size_t)rows * rowb, xold, h->stream), "grow X");
+        HIPCHK(h->xnorm.grow((size_t)rows * sizeof(float), nold, h->stream), "grow xnorm");
+        if (h->X.cap > xold)
+            HIPCHK(launch_fill_f32(h->X.as<float>() + xold / 4, (int64_t)((h->X.cap - xold) / 4), 0.f,
+                                   h->stream), "fill X");
+        if (h->xnorm.cap > nold)
+            HIPCHK(launch_fill_f32(h->xnorm.as<float>() + nold / 4, (int64_t)((h->xnorm.cap - nold) / 4),
+                                   0.f, h->stream), "fill xnorm");
+    }
+    DevBuf tmp;
+    const int64_t chunk = std::max<int64_t>(1, (int64_t)(256 << 20) / (dim * 4));
+    HIPCHK(tmp.ensure((size_t)std::min(chunk, m) * dim * sizeof(float)), "alloc staging");
+    for (int64_t r = 0; r < m; r += chunk) {
+        const int64_t rows = std::min(chunk, m - r);
+        hipError_t e = hipMemcpyAsync(tmp.p, X + r * dim, (size_t)rows * dim * sizeof(float),
+                                      hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess)
+            e = launch_pad_rows(tmp.as<float>(), rows, (int32_t)dim, h->X.as<float>() + (n + r) * dp,
+                                (int32_t)dp, h->xnorm.as<float>() + n + r, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) {
+            tmp.release();
+            return ssg_hipfail(e, "append X");
+        }
+    }
+    tmp.release();
+    std::vector<float> nr((size_t)m);
+    HIPCHK(hipMemcpy(nr.data(), h->xnorm.as<float>() + n, (size_t)m * sizeof(float),
+                     hipMemcpyDeviceToHost), "read norms");
+    for (float f : nr) h->xnmax = std::max(h->xnmax, f);
+    h->n = n2;
+    h->n_pad = (n2 + 63) / 64 * 64;
+    // encoded copies and layouts of the old rows are rebuilt on next use
+    h->Xh.release();
+    h->Xe.release();
+    h->Xs.release();
+    h->Xs2.release();
+    h->xe_fmt = 0;
+    ssg_query_layout_stale(h);
+    return SSG_OK;
+}
+
 int ssg_upload_graph(ssg_handle* h, const int64_t* offset, int64_t n, const uint32_t* end_point,
                      int64_t nnz, const uint32_t* hints, int64_t nhints)
 {

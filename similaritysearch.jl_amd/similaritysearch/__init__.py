@@ -229,6 +229,34 @@ class ExhaustiveSearch(_GpuIndex):
     def __init__(self, dist=None, db=None, device=0):
         super().__init__(dist if dist is not None else SqL2, db, device)
 
+    def append_items(self, ctx, U):
+        """append_items!(index, ctx, U) (src/exact/sequential-exhaustive.jl:40-43): U's
+        objects get ids n+1..n+len(U).  The host rows live in a buffer whose capacity
+        doubles, and an uploaded index appends only the new rows (ssg_append_db)."""
+        U = _as_db(U)
+        m = len(U)
+        if m == 0:
+            return self
+        if U.dim != self.db.dim:
+            raise ValueError("appended objects differ in dimension")
+        rows = np.ascontiguousarray(U.rows, np.float32)
+        n = len(self.db)
+        buf = getattr(self, "_buf", None)
+        if buf is None or buf.shape[0] < n + m or not np.shares_memory(buf, self.db.rows):
+            cap = max(n + m, 2 * (0 if buf is None else buf.shape[0]), 64)
+            nb = np.empty((cap, self.db.dim), np.float32)
+            nb[:n] = self.db.rows
+            self._buf = buf = nb
+        buf[n:n + m] = rows
+        self.db = MatrixDatabase.from_rows(buf[:n + m])
+        if self._h is not None:
+            _lib.check(_lib.load().ssg_append_db(self._h, rows.ctypes.data, m), "append_items!")
+        return self
+
+    def push_item(self, ctx, u):
+        """push_item!(index, ctx, u) (src/exact/sequential-exhaustive.jl:33-37)."""
+        return self.append_items(ctx, MatrixDatabase.from_rows(np.asarray(u, np.float32)[None, :]))
+
 
 class ParallelExhaustiveSearch(ExhaustiveSearch):
     """Same results as ExhaustiveSearch (src/exact/parallel-exhaustive.jl:64-102);

@@ -125,7 +125,7 @@ def neardup(dist, X, eps, k=8, blocksize=256, filterblocks=True):
     D = np.zeros(n, np.float32)
     M = []
     cost = 0
-    E = None  # exact index over the net
+    E = ExhaustiveSearch(dist, MatrixDatabase.from_rows(np.zeros((0, X.dim), np.float32)))
 
     def block_filter(items):
         nonlocal cost
@@ -186,7 +186,8 @@ def neardup(dist, X, eps, k=8, blocksize=256, filterblocks=True):
             new = block_filter(imap) if imap else []
         if new:
             net.extend(new)
-            E = ExhaustiveSearch(dist, MatrixDatabase.from_rows(X.rows[np.asarray(net) - 1]))
+            # append_items!(idx, ctx, X[tmp]) (:255): the index over the net grows in place
+            E.append_items(None, MatrixDatabase.from_rows(X.rows[np.asarray(new, np.int64) - 1]))
     return {"idx": E, "map": np.asarray(M, np.uint32), "nn": L, "dist": D,
             "centers": np.asarray(net, np.uint32), "costdists": cost}
 
