@@ -58,12 +58,13 @@ def main():
     ap.add_argument("--learned", type=str, default="",
                     help="quantile,rerank,cap,delta (skip tuning)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("search", "allknn-bf16", "allknn-fp8", "exhaustive", "hsp"),
+    ap.add_argument("--workload", choices=("search", "allknn-bf16", "allknn-fp8", "exhaustive", "hsp",
+                                           "neardup"),
                     default="search",
                     help="search: the headline graph-search metric (default); allknn-bf16 / "
                          "allknn-fp8: exhaustive allknn with the BF16 / FP8 candidate pass and "
                          "FP32 rerank; exhaustive: exact FP32 exhaustive k-NN; hsp: hsp_queries "
-                         "over a 1M allknn")
+                         "over a 1M allknn; neardup: neardup over 1M clustered unit vectors")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the query-sharded "
@@ -77,6 +78,8 @@ def main():
         return exhaustive_fp32(args)
     if args.workload == "hsp":
         return hsp_bench(args)
+    if args.workload == "neardup":
+        return neardup_bench(args)
 
     import numpy as np
     import similaritysearch as ss
@@ -639,6 +642,94 @@ def hsp_bench(args):
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "hsp_kernel<6>", "kernel_ms": round(ms, 3),
                      "bytes_per_launch": n * bytes_col},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def neardup_data(n, dim, seed):
+    """n unit vectors in clusters of four: n/4 random bases (N(0, I), L2-normalised) and
+    three copies of each with N(0, 0.01^2) noise, renormalised, in a seeded random order."""
+    import numpy as np
+    from similaritysearch import datasets
+    nb = n // 4
+    B = datasets.unit(nb, dim, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    X = np.empty((n, dim), np.float32)
+    X[:nb] = B
+    for c in range(1, 4):
+        Y = B + rng.standard_normal(B.shape, dtype=np.float32) * np.float32(0.01)
+        X[c * nb:(c + 1) * nb] = Y / np.linalg.norm(Y, axis=1, keepdims=True)
+    return X[rng.permutation(n)]
+
+
+def neardup_bench(args):
+    """neardup(dist, X, eps) with an exact index (src/neardup.jl:69-256) over 1,000,000 unit
+    vectors of 96 dims in 250,000 clusters of four near duplicates (NormCosine, eps = 0.05,
+    blocksize 256, k = 8).  A step is one whole neardup; `value` is objects per second.
+    The net is searched by the exact FP32 exhaustive search on an index that grows by
+    append (ssg_append_db), the in-block distances come from rerank!, and the ε-net
+    decisions are host-side.  `roofline`: the net searches' 2 * dim * block * |net| MFMA
+    flops over the whole call (a floor: host work and the rerank launches are in the
+    time).  Checked bit for bit against the oracle on the first 20,000 objects;
+    `cpu_baseline`: the oracle's neardup (OpenMP exhaustive search of the net) on a
+    prefix."""
+    import numpy as np
+    import similaritysearch as ss
+
+    n, dim, eps = args.n, 96, 0.05
+    X = neardup_data(n, dim, seed=12)
+    db = ss.MatrixDatabase.from_rows(X)
+    ss.neardup(ss.NormCosine, ss.MatrixDatabase.from_rows(X[:4096]), eps)   # warm-up
+    t = []
+    R = None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        R = ss.neardup(ss.NormCosine, db, eps)
+        t.append(time.perf_counter() - t0)
+    s_ = float(np.median(t))
+    centers = len(R["centers"])
+    log(f"neardup: {s_:.2f} s per neardup of {n} ({centers} centers, costdists {R['costdists']})")
+    # net-search flops: each block after the first searches the net built so far
+    flops, net, bs = 0.0, 0, 256
+    cen_pos = np.sort(R["centers"].astype(np.int64))
+    for r0 in range(0, n, bs):
+        net_before = int(np.searchsorted(cen_pos, r0 + 1))
+        if net_before:
+            flops += 2.0 * dim * min(bs, n - r0) * net_before
+    tflops = flops / s_ / 1e12
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sub = 20_000
+    Rs = ss.neardup(ss.NormCosine, ss.MatrixDatabase.from_rows(X[:sub]), eps)
+    M, L, D, cen, cost = pyoracle.neardup("NormCosine", X[:sub], eps, nthreads=threads)
+    ok = bool((Rs["map"] == M).all() and (Rs["nn"] == L).all() and (Rs["dist"] == D).all()
+              and (Rs["centers"] == cen).all() and Rs["costdists"] == cost)
+    log(f"oracle match on the first {sub} objects: {ok}")
+
+    cpu = None
+    if not args.no_cpu:
+        m, t0 = 25_000, time.perf_counter()
+        pyoracle.neardup("NormCosine", X[:m], eps, nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(m / cpu_s, 1), "unit": "objects/s", "cores": threads, "kind": "port",
+               "sample": f"the oracle's neardup of the first {m} objects (exhaustive net searches "
+                         f"with OpenMP, block filter on the host); the net grows with n, so a "
+                         f"whole neardup runs slower per object than this prefix"}
+    out = {
+        "metric": "neardup objects/s, 1M×96 NormCosine, eps=0.05, blocksize 256",
+        "value": round(n / s_, 1), "unit": "objects/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": round(s_ * 1e3, 1), "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: 250,000 N(0, I) unit bases x 4 noisy copies (sigma 0.01), shuffled, seed 12",
+        "config": {"workload": f"neardup with an exact index over {n}x{dim} NormCosine, eps={eps}",
+                   "centers": centers, "costdists": int(R["costdists"]), "oracle_match_20000": ok},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(tflops / 157.3, 5), "traffic": None,
+                     "kernel": "exh_stage1 net searches (whole call, host work included)",
+                     "kernel_ms": round(s_ * 1e3, 1), "flops_per_launch": flops},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -439,7 +439,7 @@ def bichromatic_closestpair(metric, A, B, min_k=8, samedata=False):
     return best
 
 
-def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True):
+def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True, nthreads=1):
     """neardup over an exact index (src/neardup.jl:69-256): blocks of blocksize objects;
     a block's objects search the net built so far (nearest in (distance, id) order; a
     near duplicate when within eps), the rest are filtered in order against the block's
@@ -485,7 +485,7 @@ def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True):
         else:
             # searches of the exact index count no distance evaluations (only the
             # block filter does: src/neardup.jl:240)
-            ids, d = exhaustive(metric, X[np.asarray(net) - 1], X[r0:r0 + len(rng)], 1)
+            ids, d = exhaustive(metric, X[np.asarray(net) - 1], X[r0:r0 + len(rng)], 1, nthreads)
             imap = []
             for t, j in enumerate(rng):
                 if d[t, 0] > eps:
