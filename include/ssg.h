@@ -89,6 +89,18 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
  * doubles as needed, so repeated appends move each object once.  Rejected on a handle
  * that holds a graph. */
 int ssg_append_db(ssg_handle* h, const float* X, int64_t m);
+/* neardup(dist, X, eps; blocksize, filterblocks) with an exact index (src/neardup.jl:69-256,
+ * recall = 1) over the handle's database, run on the device: blocks of blocksize objects
+ * search the net built so far (nearest within eps: a near duplicate of it); the rest of a
+ * block is filtered in order against the block's own new centers (the first center at the
+ * least distance), or with filterblocks = 0 every such object becomes a center; the new
+ * centers join the net.  Outputs: map (n entries of room; *nmap are filled: net position
+ * -> 1-based object id, the centers in construction order), nn (n: each object's
+ * covering object, itself for a center), dist (n: the distance to it), *costdists (the
+ * block filters' distance evaluations; the exact searches count none).  blocksize <= 1024
+ * when filtering. */
+int ssg_neardup(ssg_handle* h, float eps, int32_t blocksize, int32_t filterblocks, uint32_t* map,
+                int64_t* nmap, int32_t* nn, float* dist, int64_t* costdists);
 int ssg_upload_graph(ssg_handle* h, const int64_t* offset, int64_t n, const uint32_t* end_point,
                      int64_t nnz, const uint32_t* hints, int64_t nhints);
 

@@ -781,6 +781,25 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     return SSG_OK;
 }
 
+// Room for n2 padded rows (and norms) in h, keeping the first rows; capacity doubles, and
+// rows past n stay zero (the padding the kernels read).
+static hipError_t grow_rows(ssg_handle* h, int64_t n2, hipStream_t s)
+{
+    const int64_t np2 = std::max<int64_t>((n2 + 63) / 64 * 64, 64);
+    const size_t rowb = (size_t)h->dp * sizeof(float);
+    if ((size_t)np2 * rowb <= h->X.cap && (size_t)np2 * sizeof(float) <= h->xnorm.cap) return hipSuccess;
+    const int64_t have = (int64_t)(h->X.cap / rowb);
+    const int64_t rows = std::max<int64_t>(np2, (2 * have + 63) / 64 * 64);
+    const size_t xold = h->X.cap, nold = h->xnorm.cap;
+    hipError_t e = h->X.grow((size_t)rows * rowb, xold, s);
+    if (e == hipSuccess) e = h->xnorm.grow((size_t)rows * sizeof(float), nold, s);
+    if (e == hipSuccess && h->X.cap > xold)
+        e = launch_fill_f32(h->X.as<float>() + xold / 4, (int64_t)((h->X.cap - xold) / 4), 0.f, s);
+    if (e == hipSuccess && h->xnorm.cap > nold)
+        e = launch_fill_f32(h->xnorm.as<float>() + nold / 4, (int64_t)((h->xnorm.cap - nold) / 4), 0.f, s);
+    return e;
+}
+
 int ssg_append_db(ssg_handle* h, const float* X, int64_t m)
 {
     if (!h) return ssg_fail(SSG_E_ARG, "null handle");
@@ -795,22 +814,7 @@ int ssg_append_db(ssg_handle* h, const float* X, int64_t m)
     HIPCHK(hipSetDevice(h->device), "hipSetDevice");
     HIPCHK(hipStreamSynchronize(h->stream), "sync");  // no launch may still read the old rows
     const int64_t n = h->n, n2 = n + m, dp = h->dp, dim = h->dim;
-    const int64_t np2 = std::max<int64_t>((n2 + 63) / 64 * 64, 64);
-    const size_t rowb = (size_t)dp * sizeof(float);
-    if ((size_t)np2 * rowb > h->X.cap || (size_t)np2 * sizeof(float) > h->xnorm.cap) {
-        // capacity doubles; rows past n stay zero (the padding the kernels read)
-        const int64_t have = (int64_t)(h->X.cap / rowb);
-        const int64_t rows = std::max<int64_t>(np2, (2 * have + 63) / 64 * 64);
-        const size_t xold = h->X.cap, nold = h->xnorm.cap;
-        HIPCHK(h->X.grow((size_t)rows * rowb, xold, h->stream), "grow X");
-        HIPCHK(h->xnorm.grow((size_t)rows * sizeof(float), nold, h->stream), "grow xnorm");
-        if (h->X.cap > xold)
-            HIPCHK(launch_fill_f32(h->X.as<float>() + xold / 4, (int64_t)((h->X.cap - xold) / 4), 0.f,
-                                   h->stream), "fill X");
-        if (h->xnorm.cap > nold)
-            HIPCHK(launch_fill_f32(h->xnorm.as<float>() + nold / 4, (int64_t)((h->xnorm.cap - nold) / 4),
-                                   0.f, h->stream), "fill xnorm");
-    }
+    HIPCHK(grow_rows(h, n2, h->stream), "grow X");
     DevBuf tmp;
     const int64_t chunk = std::max<int64_t>(1, (int64_t)(256 << 20) / (dim * 4));
     HIPCHK(tmp.ensure((size_t)std::min(chunk, m) * dim * sizeof(float)), "alloc staging");
@@ -841,6 +845,175 @@ int ssg_append_db(ssg_handle* h, const float* X, int64_t m)
     h->Xs2.release();
     h->xe_fmt = 0;
     ssg_query_layout_stale(h);
+    return SSG_OK;
+}
+
+int ssg_neardup(ssg_handle* h, float eps, int32_t blocksize, int32_t filterblocks, uint32_t* map,
+                int64_t* nmap, int32_t* nn, float* dist, int64_t* costdists)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->X.p) return ssg_fail(SSG_E_STATE, "ssg_neardup: no database uploaded");
+    if (!map || !nmap || !nn || !dist || !costdists) return ssg_fail(SSG_E_ARG, "null buffer");
+    if (blocksize < 1) return ssg_fail(SSG_E_ARG, "ssg_neardup: blocksize must be >= 1");
+    const int64_t n = h->n;
+    *nmap = 0;
+    *costdists = 0;
+    if (n == 0) return SSG_OK;
+    const int32_t bs = (int32_t)std::min<int64_t>(blocksize, n);
+    if (filterblocks && bs > 1024) return ssg_fail(SSG_E_LIMIT, "ssg_neardup: blocksize is limited to 1024");
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    hipStream_t s = h->stream;
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    // the exact index over the net: its own handle, rows gathered from h on the device
+    ssg_handle* g = nullptr;
+    int rc = ssg_create(&g, h->device);
+    if (rc != SSG_OK) return rc;
+    struct Free {
+        ssg_handle* g;
+        ~Free() { ssg_destroy(g); }
+    } free_g{g};
+    rc = ssg_upload_db(g, nullptr, h->dim, 0, h->metric);
+    if (rc != SSG_OK) return rc;
+    g->xnmax = h->xnmax;  // a bound on the net's norms (the exact stage's error bound)
+    const int64_t dp = h->dp;
+    // Blocks are searched in batches of about 4096 objects against the net as it stood at
+    // the batch start (one exhaustive call); each block then checks the centers added
+    // since (nearest_center, ties to the lower net position), which together is the
+    // nearest over the whole net, as searchbatch! over the current net returns it.
+    const int32_t nbatch = std::max<int32_t>(1, 4096 / bs);
+    const int64_t qmax = (int64_t)nbatch * bs;
+    const int64_t qt = (qmax + EXH_QT - 1) / EXH_QT * EXH_QT;
+    DevBuf qpad, qnorm, rid, rd, items, P, cover, cdist, fresh_d, fpos, fd;
+    HIPCHK(qpad.ensure((size_t)qt * dp * 4), "alloc");
+    HIPCHK(qnorm.ensure((size_t)qt * 4), "alloc");
+    HIPCHK(rid.ensure((size_t)qmax * 4), "alloc");
+    HIPCHK(rd.ensure((size_t)qmax * 4), "alloc");
+    HIPCHK(items.ensure((size_t)bs * 4), "alloc");
+    if (filterblocks) HIPCHK(P.ensure((size_t)bs * bs * 4), "alloc");  // bs <= 1024 here
+    HIPCHK(cover.ensure((size_t)bs * 4), "alloc");
+    HIPCHK(cdist.ensure((size_t)bs * 4), "alloc");
+    HIPCHK(fresh_d.ensure((size_t)qmax * 4), "alloc");
+    HIPCHK(fpos.ensure((size_t)bs * 4), "alloc");
+    HIPCHK(fd.ensure((size_t)bs * 4), "alloc");
+    std::vector<uint32_t> M;       // net position -> 1-based object id (map)
+    std::vector<uint32_t> sid((size_t)qmax), imap, fresh, blk(bs);
+    std::vector<float> sd((size_t)qmax), hcd(bs), hfd(bs);
+    std::vector<int32_t> hcov(bs), hfp(bs);
+    int64_t cost = 0;
+    for (int64_t b0 = 0; b0 < n; b0 += qmax) {
+        const int64_t mq = std::min<int64_t>(qmax, n - b0);
+        const int64_t net0 = g->n;  // net size at the batch start
+        if (net0 > 0) {
+            const int64_t m_pad = (mq + EXH_QT - 1) / EXH_QT * EXH_QT;
+            HIPCHK(hipMemcpyAsync(qpad.p, h->X.as<float>() + b0 * dp, (size_t)mq * dp * 4,
+                                  hipMemcpyDeviceToDevice, s), "copy");
+            HIPCHK(hipMemcpyAsync(qnorm.p, h->xnorm.as<float>() + b0, (size_t)mq * 4,
+                                  hipMemcpyDeviceToDevice, s), "copy");
+            if (m_pad > mq) {
+                HIPCHK(launch_fill_f32(qpad.as<float>() + mq * dp, (m_pad - mq) * dp, 0.f, s), "fill");
+                HIPCHK(launch_fill_f32(qnorm.as<float>() + mq, m_pad - mq, 0.f, s), "fill");
+            }
+            rc = ssg_exh_run(g, nullptr, qpad.as<float>(), qnorm.as<float>(), mq, 1, 1,
+                             rid.as<uint32_t>(), rd.as<float>(), s, net0);
+            if (rc != SSG_OK) return rc;
+            HIPCHK(hipMemcpyAsync(sid.data(), rid.p, (size_t)mq * 4, hipMemcpyDeviceToHost, s), "copy");
+            HIPCHK(hipMemcpyAsync(sd.data(), rd.p, (size_t)mq * 4, hipMemcpyDeviceToHost, s), "copy");
+            HIPCHK(hipStreamSynchronize(s), "sync");
+        }
+        std::vector<uint32_t> F;  // centers added during this batch (object ids, net order)
+        for (int64_t r0 = b0; r0 < b0 + mq; r0 += bs) {
+            const int32_t mb = (int32_t)std::min<int64_t>(bs, b0 + mq - r0);
+            imap.clear();
+            if (g->n == 0) {
+                for (int32_t t = 0; t < mb; t++) imap.push_back((uint32_t)(r0 + t));
+            } else {
+                const int32_t nf = (int32_t)F.size();
+                if (nf > 0) {
+                    for (int32_t t = 0; t < mb; t++) blk[t] = (uint32_t)(r0 + t);
+                    HIPCHK(hipMemcpyAsync(items.p, blk.data(), (size_t)mb * 4, hipMemcpyHostToDevice, s),
+                           "copy");
+                    HIPCHK(launch_nearest_center(h->X.as<float>(), (int32_t)dp, h->metric,
+                                                 items.as<uint32_t>(), mb, fresh_d.as<uint32_t>(), nf,
+                                                 fpos.as<int32_t>(), fd.as<float>(), s), "nearest center");
+                    HIPCHK(hipMemcpyAsync(hfp.data(), fpos.p, (size_t)mb * 4, hipMemcpyDeviceToHost, s),
+                           "copy");
+                    HIPCHK(hipMemcpyAsync(hfd.data(), fd.p, (size_t)mb * 4, hipMemcpyDeviceToHost, s), "copy");
+                    HIPCHK(hipStreamSynchronize(s), "sync");
+                }
+                for (int32_t t = 0; t < mb; t++) {
+                    const int64_t j = r0 + t, q = j - b0;
+                    int64_t pos = net0 > 0 ? (int64_t)sid[q] : 0;  // 1-based net position
+                    float d = net0 > 0 ? sd[q] : __builtin_inff();
+                    if (nf > 0 && hfp[t] > 0 && hfd[t] < d) {  // later positions lose ties
+                        pos = net0 + hfp[t];
+                        d = hfd[t];
+                    }
+                    if (d > eps) {
+                        imap.push_back((uint32_t)j);
+                    } else {
+                        dist[j] = d;
+                        nn[j] = (int32_t)M[pos - 1];
+                    }
+                }
+            }
+            if (imap.empty()) continue;
+            const int32_t mi = (int32_t)imap.size();
+            fresh.clear();
+            if (!filterblocks) {
+                for (uint32_t j : imap) {
+                    M.push_back(j + 1);
+                    nn[j] = (int32_t)(j + 1);
+                    dist[j] = 0.f;
+                    fresh.push_back(j);
+                }
+            } else {
+                HIPCHK(hipMemcpyAsync(items.p, imap.data(), (size_t)mi * 4, hipMemcpyHostToDevice, s),
+                       "copy");
+                HIPCHK(launch_pair_dist(h->X.as<float>(), (int32_t)dp, h->metric, items.as<uint32_t>(), mi,
+                                        P.as<float>(), s), "pair distances");
+                HIPCHK(launch_greedy_net(P.as<float>(), mi, eps, cover.as<int32_t>(), cdist.as<float>(), s),
+                       "greedy net");
+                HIPCHK(hipMemcpyAsync(hcov.data(), cover.p, (size_t)mi * 4, hipMemcpyDeviceToHost, s),
+                       "copy");
+                HIPCHK(hipMemcpyAsync(hcd.data(), cdist.p, (size_t)mi * 4, hipMemcpyDeviceToHost, s), "copy");
+                HIPCHK(hipStreamSynchronize(s), "sync");
+                int64_t T = 0;  // centers of this block so far (add_distance_evaluations!, :240)
+                for (int32_t a = 0; a < mi; a++) {
+                    const uint32_t j = imap[a];
+                    if (a > 0) cost += T;
+                    if (hcov[a] == a) {
+                        T++;
+                        M.push_back(j + 1);
+                        nn[j] = (int32_t)(j + 1);
+                        dist[j] = 0.f;
+                        fresh.push_back(j);
+                    } else {
+                        nn[j] = (int32_t)(imap[hcov[a]] + 1);
+                        dist[j] = hcd[a];
+                    }
+                }
+            }
+            if (fresh.empty()) continue;
+            // append_items!(idx, ctx, X[tmp]) (:255): the new centers join the net (and the
+            // batch's list of centers added since its search)
+            HIPCHK(hipMemcpyAsync(fresh_d.as<uint32_t>() + F.size(), fresh.data(), fresh.size() * 4,
+                                  hipMemcpyHostToDevice, s), "copy");
+            HIPCHK(grow_rows(g, g->n + (int64_t)fresh.size(), s), "grow net");
+            HIPCHK(launch_gather_padded_rows(h->X.as<float>(), h->xnorm.as<float>(), (int32_t)dp,
+                                             fresh_d.as<uint32_t>() + F.size(), (int64_t)fresh.size(),
+                                             g->X.as<float>(), g->xnorm.as<float>(), g->n, s),
+                   "gather net rows");
+            F.insert(F.end(), fresh.begin(), fresh.end());
+            g->n += (int64_t)fresh.size();
+            g->n_pad = (g->n + 63) / 64 * 64;
+            HIPCHK(hipStreamSynchronize(s), "sync");  // fresh is reused by the next block
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    std::copy(M.begin(), M.end(), map);
+    *nmap = (int64_t)M.size();
+    *costdists = cost;
     return SSG_OK;
 }
 

@@ -140,6 +140,282 @@ __global__ __launch_bounds__(256) void hsp_kernel(const float* __restrict__ X, i
     }
 }
 
+// neardup_block! (src/neardup.jl:186-256), in two launches.
+//
+// pair_dist_kernel: P[a * mb + b] = canonical d(X[ids[a]], X[ids[b]]) for b < a, one
+// wave per row a, four b per step (16 lanes each, DPP tree, lane 15 writes).
+template <int V>
+__global__ __launch_bounds__(256) void pair_dist_kernel(const float* __restrict__ X, int metric,
+                                                        const uint32_t* __restrict__ ids, int mb,
+                                                        float* __restrict__ P)
+{
+    constexpr int dp = 16 * V;
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int g = lane >> 4, j = lane & 15;
+    const int a = blockIdx.x * 4 + w;
+    if (a >= mb) return;
+    float qv[V];
+    load_run<V>(X + (size_t)ids[a] * dp + j * V, qv);
+    for (int b0 = 0; b0 < a; b0 += 4) {
+        const int b = b0 + g;
+        const int bb = b < a ? b : 0;  // in-range row only
+        float xv[V];
+        load_run<V>(X + (size_t)ids[bb] * dp + j * V, xv);
+        const double s = group16_tree(run_partial<V>(metric, qv, xv));
+        if (j == 15 && b < a) P[(size_t)a * mb + b] = finish_dist(metric, s);
+    }
+}
+
+// greedy_net_kernel (one wave): items 0..mb-1 in order; item 0 is a center; item a is a
+// center unless the least distance to the centers so far (the first center, i.e. the
+// lowest position, on ties) is <= eps, in which case it is covered by that center.
+// cover[a] is the block position of a's center (a itself for a center), cdist[a] the
+// distance (0 for a center).  Lane l keeps the center flags of items l + 64 i (bit i),
+// so mb <= 1024; row a + 1 of P is loaded while row a is reduced.  The minimum is taken
+// over one 64-bit key per entry (order-preserving distance bits, then the position):
+// the lexicographic minimum, whatever the reduction order.
+__device__ __forceinline__ uint64_t dpp_min_u64_shr(uint64_t v, int ctrl_sel)
+{
+    const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    uint32_t olo, ohi;
+    // lanes without a source keep their own value (bound_ctrl off, old = own)
+    switch (ctrl_sel) {
+    case 1:
+        olo = (uint32_t)__builtin_amdgcn_update_dpp((int)lo, (int)lo, DPP_ROW_SHR1, 0xF, 0xF, false);
+        ohi = (uint32_t)__builtin_amdgcn_update_dpp((int)hi, (int)hi, DPP_ROW_SHR1, 0xF, 0xF, false);
+        break;
+    case 2:
+        olo = (uint32_t)__builtin_amdgcn_update_dpp((int)lo, (int)lo, DPP_ROW_SHR2, 0xF, 0xF, false);
+        ohi = (uint32_t)__builtin_amdgcn_update_dpp((int)hi, (int)hi, DPP_ROW_SHR2, 0xF, 0xF, false);
+        break;
+    case 4:
+        olo = (uint32_t)__builtin_amdgcn_update_dpp((int)lo, (int)lo, DPP_ROW_SHR4, 0xF, 0xF, false);
+        ohi = (uint32_t)__builtin_amdgcn_update_dpp((int)hi, (int)hi, DPP_ROW_SHR4, 0xF, 0xF, false);
+        break;
+    default:
+        olo = (uint32_t)__builtin_amdgcn_update_dpp((int)lo, (int)lo, DPP_ROW_SHR8, 0xF, 0xF, false);
+        ohi = (uint32_t)__builtin_amdgcn_update_dpp((int)hi, (int)hi, DPP_ROW_SHR8, 0xF, 0xF, false);
+        break;
+    }
+    const uint64_t o = ((uint64_t)ohi << 32) | olo;
+    return o < v ? o : v;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v)
+{
+    v = dpp_min_u64_shr(v, 1);
+    v = dpp_min_u64_shr(v, 2);
+    v = dpp_min_u64_shr(v, 4);
+    v = dpp_min_u64_shr(v, 8);  // lane 15 of each row: the row's minimum
+    uint64_t r = ~0ull;
+#pragma unroll
+    for (int row = 0; row < 4; row++) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 16 * row + 15);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 16 * row + 15);
+        const uint64_t x = ((uint64_t)hi << 32) | lo;
+        r = x < r ? x : r;
+    }
+    return r;
+}
+
+// The scan, with row a of P's lower triangle read through rowp(a) (b < a valid).
+template <class RowP>
+__device__ __forceinline__ void greedy_scan(RowP rowp, int mb, float eps, int32_t* __restrict__ cover,
+                                            float* __restrict__ cdist)
+{
+    constexpr int CH = 16;  // mb <= 64 * CH
+    const int lane = threadIdx.x;
+    uint32_t cflags = lane == 0 ? 1u : 0u;  // item 0 is a center
+    if (lane == 0) {
+        cover[0] = 0;
+        cdist[0] = 0.f;
+    }
+    float cur[CH];
+#pragma unroll
+    for (int i = 0; i < CH; i++) cur[i] = 0.f;
+    if (mb > 1 && lane == 0) cur[0] = rowp(1)[0];
+    for (int a = 1; a < mb; a++) {
+        float nxt[CH];
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+            const int b = lane + 64 * i;
+            nxt[i] = (a + 1 < mb && b < a + 1) ? rowp(a + 1)[b] : 0.f;
+        }
+        uint64_t key = ~0ull;
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+            const int b = lane + 64 * i;
+            const float d = cur[i];
+            if (b < a && ((cflags >> i) & 1u) && d == d) {
+                const uint32_t u = d == 0.f ? 0u : __float_as_uint(d);  // -0 ties with +0
+                const uint32_t kd = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+                const uint64_t k = ((uint64_t)kd << 32) | (uint32_t)b;
+                key = k < key ? k : key;
+            }
+        }
+        key = wave_min_u64(key);
+        bool center = key == ~0ull;
+        float bd = 0.f;
+        int bb = 0;
+        if (!center) {
+            const uint32_t kd = (uint32_t)(key >> 32);
+            bd = __uint_as_float((kd & 0x80000000u) ? (kd & 0x7fffffffu) : ~kd);
+            bb = (int)(uint32_t)key;
+            center = bd > eps;
+        }
+        if (center) {
+            if (lane == (a & 63)) cflags |= 1u << (a >> 6);
+            if (lane == 0) {
+                cover[a] = a;
+                cdist[a] = 0.f;
+            }
+        } else if (lane == 0) {
+            cover[a] = bb;
+            cdist[a] = bd;
+        }
+#pragma unroll
+        for (int i = 0; i < CH; i++) cur[i] = nxt[i];
+    }
+}
+
+__global__ __launch_bounds__(64) void greedy_net_kernel(const float* __restrict__ P, int mb, float eps,
+                                                        int32_t* __restrict__ cover,
+                                                        float* __restrict__ cdist)
+{
+    greedy_scan([&](int a) { return P + (size_t)a * mb; }, mb, eps, cover, cdist);
+}
+
+// mb <= 256: the whole workgroup first copies P's lower triangle (row a at a(a-1)/2) into
+// LDS, then wave 0 scans it there.
+constexpr int GREEDY_LDS_MB = 256;
+__global__ __launch_bounds__(1024) void greedy_net_lds_kernel(const float* __restrict__ P, int mb,
+                                                              float eps, int32_t* __restrict__ cover,
+                                                              float* __restrict__ cdist)
+{
+    __shared__ float tri[GREEDY_LDS_MB * (GREEDY_LDS_MB - 1) / 2];
+    const int ntri = mb * (mb - 1) / 2;
+    for (int e = threadIdx.x; e < ntri; e += blockDim.x) {
+        // row a = the largest a with a(a-1)/2 <= e
+        int a = (int)((1.0f + __builtin_sqrtf(1.0f + 8.0f * (float)e)) * 0.5f);
+        while (a * (a - 1) / 2 > e) a--;
+        while ((a + 1) * a / 2 <= e) a++;
+        tri[e] = P[(size_t)a * mb + (e - a * (a - 1) / 2)];
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    greedy_scan([&](int a) { return (const float*)(tri + a * (a - 1) / 2); }, mb, eps, cover, cdist);
+}
+
+// nearest_center_kernel: for item a, the nearest of the centers F[0..nf) (canonical
+// distance; the lowest index on ties) -> pos[a] = 1-based index into F, or 0 when nf = 0,
+// and its distance.  One wave per item, four centers per step (16 lanes each).
+template <int V>
+__global__ __launch_bounds__(256) void nearest_center_kernel(const float* __restrict__ X, int metric,
+                                                             const uint32_t* __restrict__ items, int mb,
+                                                             const uint32_t* __restrict__ F, int nf,
+                                                             int32_t* __restrict__ pos,
+                                                             float* __restrict__ pdist)
+{
+    constexpr int dp = 16 * V;
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int g = lane >> 4, j = lane & 15;
+    const int a = blockIdx.x * 4 + w;
+    if (a >= mb) return;
+    float qv[V];
+    load_run<V>(X + (size_t)items[a] * dp + j * V, qv);
+    float bd = __builtin_inff();
+    int bi = 0x7fffffff;
+    for (int f0 = 0; f0 < nf; f0 += 4) {
+        const int f = f0 + g;
+        const int ff = f < nf ? f : 0;  // in-range center only
+        float xv[V];
+        load_run<V>(X + (size_t)F[ff] * dp + j * V, xv);
+        const float d = finish_dist(metric, group16_tree(run_partial<V>(metric, qv, xv)));
+        if (f < nf && d < bd) {  // each group sees its centers in increasing index
+            bd = d;
+            bi = f;
+        }
+    }
+    // lane 15 of each group holds its group's best; combine by (distance, index)
+#pragma unroll
+    for (int o = 16; o <= 32; o <<= 1) {
+        const float od = __shfl_xor(bd, o);
+        const int oi = __shfl_xor(bi, o);
+        if (od < bd || (od == bd && oi < bi)) {
+            bd = od;
+            bi = oi;
+        }
+    }
+    if (lane == 15) {
+        pos[a] = bi == 0x7fffffff ? 0 : bi + 1;
+        pdist[a] = bd;
+    }
+}
+
+hipError_t launch_nearest_center(const float* X, int32_t dp, int32_t metric, const uint32_t* items,
+                                 int32_t mb, const uint32_t* F, int32_t nf, int32_t* pos, float* pdist,
+                                 hipStream_t s)
+{
+    if (mb <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((mb + 3) / 4)), block(256);
+    switch (dp / 16) {
+#define SSG_NC(VV) \
+    case VV: hipLaunchKernelGGL((nearest_center_kernel<VV>), grid, block, 0, s, X, metric, items, mb, F, nf, pos, pdist); break;
+        SSG_NC(1) SSG_NC(2) SSG_NC(3) SSG_NC(4) SSG_NC(5) SSG_NC(6) SSG_NC(7) SSG_NC(8) SSG_NC(12) SSG_NC(16)
+#undef SSG_NC
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Rows ids[i] of X (padded, with norms) appended at row n0 + i of Y.
+__global__ void gather_padded_rows_kernel(const float* __restrict__ X, const float* __restrict__ xn,
+                                          int dp, const uint32_t* __restrict__ ids, int64_t m,
+                                          float* __restrict__ Y, float* __restrict__ yn, int64_t n0)
+{
+    const int64_t r = blockIdx.x;
+    if (r >= m) return;
+    const size_t src = (size_t)ids[r];
+    for (int c = threadIdx.x; c < dp; c += blockDim.x) Y[(size_t)(n0 + r) * dp + c] = X[src * dp + c];
+    if (threadIdx.x == 0) yn[n0 + r] = xn[src];
+}
+
+hipError_t launch_pair_dist(const float* X, int32_t dp, int32_t metric, const uint32_t* ids, int32_t mb,
+                            float* P, hipStream_t s)
+{
+    if (mb <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((mb + 3) / 4)), block(256);
+    switch (dp / 16) {
+#define SSG_PD(VV) \
+    case VV: hipLaunchKernelGGL((pair_dist_kernel<VV>), grid, block, 0, s, X, metric, ids, mb, P); break;
+        SSG_PD(1) SSG_PD(2) SSG_PD(3) SSG_PD(4) SSG_PD(5) SSG_PD(6) SSG_PD(7) SSG_PD(8) SSG_PD(12) SSG_PD(16)
+#undef SSG_PD
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_greedy_net(const float* P, int32_t mb, float eps, int32_t* cover, float* cdist,
+                             hipStream_t s)
+{
+    if (mb <= 0) return hipSuccess;
+    if (mb > 1024) return hipErrorInvalidValue;
+    if (mb <= GREEDY_LDS_MB)
+        hipLaunchKernelGGL(greedy_net_lds_kernel, dim3(1), dim3(1024), 0, s, P, mb, eps, cover, cdist);
+    else
+        hipLaunchKernelGGL(greedy_net_kernel, dim3(1), dim3(64), 0, s, P, mb, eps, cover, cdist);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_padded_rows(const float* X, const float* xn, int32_t dp, const uint32_t* ids,
+                                     int64_t m, float* Y, float* yn, int64_t n0, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_padded_rows_kernel, dim3((unsigned)m), dim3(64), 0, s, X, xn, dp, ids, m,
+                       Y, yn, n0);
+    return hipGetLastError();
+}
+
 // Waves per block: as many as keep the block's staged rows within 64 KiB (1 to 4).
 static int hsp_waves(int k, int dp)
 {

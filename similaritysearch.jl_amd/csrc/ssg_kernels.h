@@ -208,6 +208,16 @@ hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float
 hipError_t launch_hsp(const float* X, int32_t dp, int32_t metric, int64_t m, int32_t k,
                       const uint32_t* ids, const float* dists, uint32_t* out_ids, float* out_dists,
                       hipStream_t s);
+// neardup_block! pieces (ssg_consumers.hip)
+hipError_t launch_pair_dist(const float* X, int32_t dp, int32_t metric, const uint32_t* ids, int32_t mb,
+                            float* P, hipStream_t s);
+hipError_t launch_greedy_net(const float* P, int32_t mb, float eps, int32_t* cover, float* cdist,
+                             hipStream_t s);
+hipError_t launch_nearest_center(const float* X, int32_t dp, int32_t metric, const uint32_t* items,
+                                 int32_t mb, const uint32_t* F, int32_t nf, int32_t* pos, float* pdist,
+                                 hipStream_t s);
+hipError_t launch_gather_padded_rows(const float* X, const float* xn, int32_t dp, const uint32_t* ids,
+                                     int64_t m, float* Y, float* yn, int64_t n0, hipStream_t s);
 hipError_t launch_pad_rows(const float* src, int64_t rows, int32_t dim, float* dst, int32_t dp,
                            float* norms, hipStream_t s);
 

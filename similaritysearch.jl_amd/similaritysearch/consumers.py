@@ -109,87 +109,36 @@ def _nearest_rows(E, Q, k):
 def neardup(dist, X, eps, k=8, blocksize=256, filterblocks=True):
     """neardup(dist, X, ϵ; k, blocksize, filterblocks) with an exact index
     (src/neardup.jl:69-85 with recall = 1.0, then neardup_ :94-171 and neardup_block!
-    :186-256).  X is processed in blocks of blocksize objects: a block is searched against
-    the ϵ-net built so far (nearest center within ϵ: a near duplicate of it); the rest of
-    the block is filtered in order against the block's own new centers (the first center
-    at the least distance, when within ϵ) and the new centers join the net.
+    :186-256), run on the device by ssg_neardup.  X is processed in blocks of blocksize
+    objects: a block is searched against the ϵ-net built so far (nearest center within ϵ:
+    a near duplicate of it); the rest of the block is filtered in order against the
+    block's own new centers (the first center at the least distance, when within ϵ) and
+    the new centers join the net.  k only sizes the reference's searches; with an exact
+    index their nearest entry does not depend on it.
 
-    Returns a dict with the reference's fields: map (net position -> object id), nn
-    (each object's covering object, 1-based; itself for a center), dist (the distance to
-    it), centers (the net, in construction order), costdists (distance evaluations)."""
+    Returns a dict with the reference's fields: idx (an ExhaustiveSearch over the net),
+    map (net position -> object id), nn (each object's covering object, 1-based; itself
+    for a center), dist (the distance to it), centers (the net, in construction order),
+    costdists (distance evaluations of the block filters)."""
     X = _as_db(X)
-    eps = np.float32(eps)
     n = len(X)
-    blocksize = min(blocksize, n)
+    if n and not 1 <= blocksize:
+        raise ValueError("blocksize must be >= 1")
+    E = ExhaustiveSearch(dist, X)
+    M = np.zeros(max(n, 1), np.uint32)
     L = np.zeros(n, np.int32)
     D = np.zeros(n, np.float32)
-    M = []
-    cost = 0
-    E = ExhaustiveSearch(dist, MatrixDatabase.from_rows(np.zeros((0, X.dim), np.float32)))
-
-    def block_filter(items):
-        nonlocal cost
-        if not filterblocks:
-            for i in items:
-                M.append(i)
-                L[i - 1] = i
-                D[i - 1] = 0.0
-            return list(items)
-        items = list(items)
-        # distances between the block's items, in chunks of 64 candidates (rerank!)
-        P = np.full((len(items), len(items)), np.inf, np.float32)
-        ids = np.asarray(items, np.uint32)
-        Qb = MatrixDatabase.from_rows(X.rows[ids.astype(np.int64) - 1])
-        for c0 in range(0, len(items), 64):
-            chunk = ids[c0:c0 + 64]
-            cids = np.asfortranarray(np.tile(chunk, (len(items), 1)).T)
-            cd = np.asfortranarray(np.zeros((len(chunk), len(items)), np.float32))
-            rerank_(dist, X, Qb, cids, cd)
-            pos = _positions(np.tile(chunk, (len(items), 1)), cids.T)
-            P[np.arange(len(items))[:, None], c0 + pos] = cd.T
-        tmp = [0]                                  # block positions of new centers
-        first = items[0]
-        M.append(first)
-        L[first - 1] = first
-        D[first - 1] = 0.0
-        for ii in range(1, len(items)):
-            i = items[ii]
-            cost += len(tmp)
-            dj = P[ii, tmp]
-            b = int(np.argmin(dj))                 # the first center at the least distance
-            if dj[b] > eps:
-                tmp.append(ii)
-                M.append(i)
-                L[i - 1] = i
-                D[i - 1] = 0.0
-            else:
-                L[i - 1] = items[tmp[b]]
-                D[i - 1] = dj[b]
-        return [items[t] for t in tmp]
-
-    net = []
-    for r0 in range(0, n, blocksize):
-        rng = list(range(r0 + 1, min(n, r0 + blocksize) + 1))
-        if len(net) == 0:
-            new = block_filter(rng)
-        else:
-            Qb = MatrixDatabase.from_rows(X.rows[r0:r0 + len(rng)])
-            kk = min(k, len(net))
-            pid, pdist = _nearest_rows(E, Qb, kk)      # exact searches count no evaluations
-            imap = []
-            for t, j in enumerate(rng):
-                if pdist[t] > eps:
-                    imap.append(j)
-                else:
-                    D[j - 1] = pdist[t]
-                    L[j - 1] = M[int(pid[t]) - 1]
-            new = block_filter(imap) if imap else []
-        if new:
-            net.extend(new)
-            # append_items!(idx, ctx, X[tmp]) (:255): the index over the net grows in place
-            E.append_items(None, MatrixDatabase.from_rows(X.rows[np.asarray(new, np.int64) - 1]))
-    return {"idx": E, "map": np.asarray(M, np.uint32), "nn": L, "dist": D,
-            "centers": np.asarray(net, np.uint32), "costdists": cost}
+    nm = np.zeros(1, np.int64)
+    cost = np.zeros(1, np.int64)
+    if n:
+        _lib.check(_lib.load().ssg_neardup(E._handle(), float(np.float32(eps)), int(blocksize),
+                                           int(bool(filterblocks)), M.ctypes.data, nm.ctypes.data,
+                                           L.ctypes.data, D.ctypes.data, cost.ctypes.data), "neardup")
+    E.close()
+    M = M[:int(nm[0])].copy()
+    net = ExhaustiveSearch(dist, MatrixDatabase.from_rows(X.rows[M.astype(np.int64) - 1]))
+    return {"idx": net, "map": M, "nn": L, "dist": D, "centers": M.copy(),
+            "costdists": int(cost[0])}
 
 
 _hsp_cache = {}

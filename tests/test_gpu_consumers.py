@@ -194,3 +194,22 @@ def test_hsp_queries_hand_case():
     np.testing.assert_array_equal(hi[:, 0], [2, 4, 1, 0])
     np.testing.assert_array_equal(hd[:, 0], np.array([1.0, 1.0, 4.0, np.inf], np.float32))
     assert len(hsp[0][0]) == 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,blocksize,filterblocks", [(10000, 64, True), (9000, 1000, True),
+                                                      (9000, 700, False)])
+def test_neardup_several_batches(oracle, n, blocksize, filterblocks):
+    """Long enough that blocks are searched in several batches (about 4096 objects each)
+    against the net as of each batch start, plus the centers added since."""
+    ss = _ss()
+    X = _data(n, 4, 94)
+    X[n // 2:n // 2 + 300] = X[:300] + np.float32(1e-5)
+    R = ss.neardup(ss.SqL2(), _db(X), 0.01, blocksize=blocksize, filterblocks=filterblocks)
+    M, L, D, centers, cost = oracle.neardup("SqL2", X, 0.01, blocksize=blocksize,
+                                            filterblocks=filterblocks, nthreads=8)
+    np.testing.assert_array_equal(R["map"], M)
+    np.testing.assert_array_equal(R["nn"], L)
+    np.testing.assert_array_equal(R["dist"], D)
+    assert R["costdists"] == cost
+    assert (R["nn"][n // 2:n // 2 + 300] != np.arange(n // 2 + 1, n // 2 + 301)).all()
