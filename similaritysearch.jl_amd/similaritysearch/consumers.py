@@ -84,15 +84,18 @@ def bichromatic_closestpair(idxA, ctx, B, min_k=8, samedata=None, seed=0):
     else:
         ids, dists = searchbatch(idxA, ctx, B, min_k)
         ids, dists = ids.T, dists.T                                    # (n, k)
-    best = (0, 0, np.float32(np.inf))
-    for j in range(n):
-        row_i, row_d = ids[j], dists[j]
-        t = 0
-        if samedata and row_i[0] == j + 1:
-            t = 1                                                      # pop_min!
-        if t < len(row_i) and row_i[t] != 0 and row_d[t] < best[2]:
-            best = (int(row_i[t]), j + 1, row_d[t])
-    return best
+    # each object's nearest after pop_min! of itself, then the least distance, lowest j on
+    # ties (strict < in object order)
+    t = (samedata & (ids[:, 0] == np.arange(1, n + 1, dtype=np.uint32))).astype(np.int64)
+    ok = t < ids.shape[1]
+    tc = np.minimum(t, ids.shape[1] - 1)
+    cid = ids[np.arange(n), tc]
+    cd = np.where(ok & (cid != 0), dists[np.arange(n), tc], np.float32(np.inf))
+    cd = np.where(np.isnan(cd), np.float32(np.inf), cd)
+    if not (cd < np.inf).any():
+        return (0, 0, np.float32(np.inf))
+    j = int(np.argmin(cd))                                             # first minimum
+    return (int(cid[j]), j + 1, cd[j])
 
 
 def closestpair(idx, ctx, min_k=8, seed=0):
