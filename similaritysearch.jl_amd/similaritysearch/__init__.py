@@ -498,10 +498,13 @@ def rerank_(dist, db, Q, ids, dists):
     """rerank!(dist, db, queries, ids, dists) batch form, in place (src/rerank.jl:77-86)."""
     Q = _as_db(Q)
     key = (id(db), _metric(dist))
-    idx = _rerank_cache.get(key)
-    if idx is None or idx.db is not (db if isinstance(db, MatrixDatabase) else idx.db):
+    hit = _rerank_cache.get(key)
+    if hit is not None and hit[0] is db:   # the entry holds db, so its id cannot be reused
+        idx = hit[1]
+    else:
         idx = ExhaustiveSearch(dist, db)
-        _rerank_cache[key] = idx
+        _rerank_cache.clear()              # one uploaded database at a time
+        _rerank_cache[key] = (db, idx)
     m = len(Q)
     k = ids.shape[0]
     if not (ids.T.flags.c_contiguous and dists.T.flags.c_contiguous):

@@ -165,7 +165,7 @@ def hsp_queries(dist, X, Q, knns_ids, knns_dists):
         raise ValueError("the number of queries is different from the candidate columns")
     key = (id(X), _metric(dist))
     E = _hsp_cache.get(key)
-    if E is None or E.db is not X:
+    if E is None or E.db is not X:         # E holds X, so its id cannot be reused
         E = ExhaustiveSearch(dist, X)
         _hsp_cache.clear()
         _hsp_cache[key] = E
