@@ -68,7 +68,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the query-sharded "
-              "multi-GPU layout is described in DESIGN.md and not run here", file=sys.stderr)
+              "multi-GPU layout (one database replica per GPU) is not run here", file=sys.stderr)
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be >= 1")
