@@ -59,12 +59,13 @@ def main():
                     help="quantile,rerank,cap,delta (skip tuning)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("search", "allknn-bf16", "allknn-fp8", "exhaustive", "hsp",
-                                           "neardup"),
+                                           "neardup", "copy"),
                     default="search",
                     help="search: the headline graph-search metric (default); allknn-bf16 / "
                          "allknn-fp8: exhaustive allknn with the BF16 / FP8 candidate pass and "
                          "FP32 rerank; exhaustive: exact FP32 exhaustive k-NN; hsp: hsp_queries "
-                         "over a 1M allknn; neardup: neardup over 1M clustered unit vectors")
+                         "over a 1M allknn; neardup: neardup over 1M clustered unit vectors; copy: "
+                         "1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the query-sharded "
@@ -80,6 +81,8 @@ def main():
         return hsp_bench(args)
     if args.workload == "neardup":
         return neardup_bench(args)
+    if args.workload == "copy":
+        return copy_bench(args)
 
     import numpy as np
     import similaritysearch as ss
@@ -630,6 +633,13 @@ def hsp_bench(args):
 
     bytes_col = k * dim * 4 + 16 * k
     gbs = n * bytes_col / (ms * 1e-3) / 1e9
+    # HBM traffic of this launch from the committed counter measurement of the same kernel
+    # and workload (FETCH_SIZE / WRITE_SIZE passes scaled by a 1 GiB copy)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hsp_queries_1m96_k32_traffic.json")
+    if os.path.exists(tfile) and n == 1_000_000:
+        with open(tfile) as fh:
+            traffic = int(json.load(fh)["hbm_bytes_per_column"] * n)
     out = {
         "metric": "hsp_queries columns/s, 1M×96 NormCosine, k=32 candidates per column",
         "value": round(n / (ms * 1e-3), 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
@@ -639,12 +649,41 @@ def hsp_bench(args):
         "config": {"workload": f"hsp_queries over the allknn of {n}x{dim} NormCosine, k={k}",
                    "mean_kept": round(kept, 3), "oracle_match_5000": ok},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/hsp_queries_1m96_k32.md",
                      "kernel": "hsp_kernel<6>", "kernel_ms": round(ms, 3),
                      "bytes_per_launch": n * bytes_col},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_bench(args):
+    """Calibration for the HBM counters: one 1 GiB device-to-device copy (ssg_memcpy) per
+    step.  Under `rocprofv3 --pmc FETCH_SIZE` (then WRITE_SIZE) restricted to the copy
+    kernel, the counters' readings for 2^30 bytes moved give the factor that turns a
+    kernel's readings into HBM bytes."""
+    import numpy as np
+    from similaritysearch import _lib
+    from similaritysearch.device import DeviceArray, synchronize
+
+    L = _lib.load()
+    nb = 1 << 30
+    a = DeviceArray((nb // 4,), np.float32)
+    b = DeviceArray((nb // 4,), np.float32)
+    for _ in range(max(1, args.warmup)):
+        _lib.check(L.ssg_memcpy(b.ptr, a.ptr, nb, 2), "copy")
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _lib.check(L.ssg_memcpy(b.ptr, a.ptr, nb, 2), "copy")
+    synchronize()
+    s_ = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"metric": "device-to-device copy GB/s (read + write)", "value": round(2 * nb / s_ / 1e9, 1),
+                      "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": round(s_ * 1e3, 3), "higher_is_better": True, "scaling": "strong",
+                      "vs_baseline": None, "dtype": "u8", "data": "1 GiB buffer",
+                      "config": {"workload": "ssg_memcpy 1 GiB device to device"}}), flush=True)
 
 
 def neardup_data(n, dim, seed):
