@@ -361,6 +361,16 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _traffic(name, key, applies):
+    """HBM bytes per call from a committed counter measurement (profiles/<name>) of the same
+    kernel and workload, or None when it does not apply."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not applies or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return int(json.load(fh)[key])
+
+
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 MFMA_FP8_PEAK_TFLOPS = 5000.0
 MFMA_FP32_PEAK_TFLOPS = 157.3
@@ -434,7 +444,9 @@ def exhaustive_fp32(args):
                    "gmm_1m128": main, "uniform_100k32": res["uniform_100k32"]},
         "roofline": {"bound": "mfma", "achieved": main["tflops"], "peak": MFMA_FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(main["tflops"] / MFMA_FP32_PEAK_TFLOPS, 4),
-                     "traffic": None, "kernel": "exh_stage1_l<128> + exh_stage2<8> (whole call)",
+                     "traffic": _traffic("exhaustive_fp32_1m128_traffic.json", "hbm_bytes_per_call",
+                                         main["n"] == 1_000_000),
+                     "traffic_source": "profiles/exhaustive_fp32_1m128.md", "kernel": "exh_stage1_l<128> + exh_stage2<8> (whole call)",
                      "kernel_ms": main["ms"], "flops_per_launch": 2.0 * 128 * main["n"] * main["m"]},
         "cpu_baseline": res.get("cpu"),
     }
@@ -552,7 +564,9 @@ def allknn_prec(args, fmt):
                    "error_sample": len(sub)},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 1), "peak": peak,
                      "unit": "TFLOP/s", "frac": round(tflops / peak, 4),
-                     "traffic": None,
+                     "traffic": _traffic("allknn_fp8_1m96_traffic.json", "hbm_bytes_per_call",
+                                         fmt == "fp8" and n == 1_000_000),
+                     "traffic_source": "profiles/allknn_fp8_1m96.md" if fmt == "fp8" else None,
                      "kernel": f"prec_stage1<{'3, 8, false' if fmt == 'bf16' else '1, 8, true'}> + "
                                "prec_stage2<6> (whole call)",
                      "kernel_ms": round(ms, 3), "flops_per_launch": flops},
