@@ -197,16 +197,18 @@ def test_hsp_queries_hand_case():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,blocksize,filterblocks", [(10000, 64, True), (9000, 1000, True),
-                                                      (9000, 700, False)])
-def test_neardup_several_batches(oracle, n, blocksize, filterblocks):
+@pytest.mark.parametrize("metric,eps,n,blocksize,filterblocks", [
+    ("SqL2", 0.01, 10000, 64, True), ("SqL2", 0.01, 9000, 1000, True),
+    ("SqL2", 0.01, 9000, 700, False), ("L2", 0.1, 9000, 256, True),
+    ("NormCosine", 0.004, 9000, 200, True)])
+def test_neardup_several_batches(oracle, metric, eps, n, blocksize, filterblocks):
     """Long enough that blocks are searched in several batches (about 4096 objects each)
     against the net as of each batch start, plus the centers added since."""
     ss = _ss()
-    X = _data(n, 4, 94)
+    X = _data(n, 4, 94, metric)
     X[n // 2:n // 2 + 300] = X[:300] + np.float32(1e-5)
-    R = ss.neardup(ss.SqL2(), _db(X), 0.01, blocksize=blocksize, filterblocks=filterblocks)
-    M, L, D, centers, cost = oracle.neardup("SqL2", X, 0.01, blocksize=blocksize,
+    R = ss.neardup(_dist(ss, metric), _db(X), eps, blocksize=blocksize, filterblocks=filterblocks)
+    M, L, D, centers, cost = oracle.neardup(metric, X, eps, blocksize=blocksize,
                                             filterblocks=filterblocks, nthreads=8)
     np.testing.assert_array_equal(R["map"], M)
     np.testing.assert_array_equal(R["nn"], L)
