@@ -174,6 +174,11 @@ typedef struct {
     const uint32_t* order;   /* [m] nullable, no reference counterpart: the order in which
                                 queries are taken (a permutation of 0..m-1; results do not
                                 depend on it) */
+    const uint32_t* visited; /* [m] nullable: a 1-based object marked visited before query j's
+                                search starts (0 = none), as bichromatic_search! marks the
+                                query's own vertex (src/bichromatic/bichromaticclosestpair.jl:
+                                107-113); the n < 64 brute-force branch ignores it, as the
+                                reference's does */
 } ssg_query_opts;
 
 /* searchbatch!(::SearchGraph, ctx, Q, ids, dists) with search(::BeamSearch, ...)

@@ -62,7 +62,7 @@ def lib():
         L.or_exhaustive.argtypes = [C.c_int, _p, _i64, C.c_int, _p, _i64, _i32, _p, _p, C.c_int]
         L.or_parallel_exhaustive.argtypes = [C.c_int, _p, _i64, C.c_int, _p, _i64, _i32, _i32,
                                              _p, _p]
-        L.or_beamsearch.argtypes = [C.c_int, _p, _i64, C.c_int, _p, _p, _p, _i64, _p, _p, _i32,
+        L.or_beamsearch.argtypes = [C.c_int, _p, _i64, C.c_int, _p, _p, _p, _i64, _p, _p, _i32, _p,
                                     _p, _i64, _i32, _p, _i32, _i32, _p, _f32, _i64, _i32, _p,
                                     _p, _p, _p, C.c_int]
         L.or_allknn_graph.argtypes = [C.c_int, _p, _i64, C.c_int, _p, _p, _i32, _i32, _f32, _i64,
@@ -150,8 +150,9 @@ def parallel_exhaustive(metric, X, Q, k, nbatches):
 
 
 def beamsearch(metric, X, offs, adj, hints, Q, k, bsize, delta, maxvisits, beamcap=32, kq=None,
-               bq=None, qhints=None, qh_count=None, nthreads=1):
-    """Restated searchbatch over a SearchGraph.  Returns ids, dists, costdists, costblocks."""
+               bq=None, qhints=None, qh_count=None, nthreads=1, qvisit=None):
+    """Restated searchbatch over a SearchGraph.  Returns ids, dists, costdists, costblocks.
+    qvisit: per query a 1-based object marked visited before its search (0 = none)."""
     X, Q = _f32c(X), _f32c(Q)
     offs = np.ascontiguousarray(offs, np.int64)
     adj = np.ascontiguousarray(adj, np.uint32)
@@ -162,13 +163,14 @@ def beamsearch(metric, X, offs, adj, hints, Q, k, bsize, delta, maxvisits, beamc
     bq_a = None if bq is None else np.ascontiguousarray(bq, np.int32)
     qh = None if qhints is None else np.ascontiguousarray(qhints, np.uint32)
     qc = None if qh_count is None else np.ascontiguousarray(qh_count, np.int32)
+    qv = None if qvisit is None else np.ascontiguousarray(qvisit, np.uint32)
     ids = np.zeros((m, kstride), np.uint32)
     d = np.full((m, kstride), np.inf, np.float32)
     cd = np.zeros(m, np.int64)
     cb = np.zeros(m, np.int64)
     lib().or_beamsearch(METRICS.get(metric, metric), _ptr(X), X.shape[0], X.shape[1], _ptr(offs),
                         _ptr(adj), _ptr(hints), len(hints), _ptr(qh), _ptr(qc),
-                        0 if qh is None else qh.shape[1], _ptr(Q), m, int(k), _ptr(kq_a), kstride,
+                        0 if qh is None else qh.shape[1], _ptr(qv), _ptr(Q), m, int(k), _ptr(kq_a), kstride,
                         int(bsize), _ptr(bq_a), float(delta), int(maxvisits), int(beamcap),
                         _ptr(ids), _ptr(d), _ptr(cd), _ptr(cb), nthreads)
     return ids, d, cd, cb
@@ -431,10 +433,18 @@ def bichromatic_closestpair(metric, A, B, min_k=8, samedata=False):
     j (strict < in object order, first batch on equal minima).  Pinned by brute force
     over all pairs in tests/test_oracle.py."""
     ids, d = exhaustive(metric, A, B, min_k)
+    return closest_of_rows(ids, d, samedata)
+
+
+def closest_of_rows(ids, d, samedata):
+    """The pair bichromatic_closestpair picks from each object's (m, k) result row
+    (bichromaticclosestpair.jl:75-81, 103-106): drop the row's nearest entry when it is
+    the object itself (samedata), then the least distance, lowest j on ties."""
     best = (0, 0, np.float32(np.inf))
-    for j in range(len(B)):
+    k = ids.shape[1]
+    for j in range(ids.shape[0]):
         t = 1 if samedata and ids[j, 0] == j + 1 else 0
-        if t < min_k and ids[j, t] != 0 and d[j, t] < best[2]:
+        if t < k and ids[j, t] != 0 and d[j, t] < best[2]:
             best = (int(ids[j, t]), j + 1, d[j, t])
     return best
 

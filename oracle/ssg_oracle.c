@@ -371,10 +371,10 @@ finish:
 void or_beamsearch(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
                    const uint32_t* adj, const uint32_t* hints, int64_t nhints,
                    const uint32_t* qhints, const int32_t* qh_count, int32_t qh_stride,
-                   const float* Q, int64_t m, int32_t k, const int32_t* kq, int32_t kstride,
-                   int32_t bsize, const int32_t* bq, float delta, int64_t maxvisits,
-                   int32_t beamcap, uint32_t* ids, float* dists, int64_t* costdists,
-                   int64_t* costblocks, int nthreads)
+                   const uint32_t* qvisit, const float* Q, int64_t m, int32_t k,
+                   const int32_t* kq, int32_t kstride, int32_t bsize, const int32_t* bq,
+                   float delta, int64_t maxvisits, int32_t beamcap, uint32_t* ids, float* dists,
+                   int64_t* costdists, int64_t* costblocks, int nthreads)
 {
     gidx_t g = {metric, X, n, dim, offs, adj};
     const int64_t nw = (n + 63) / 64 + 1;
@@ -399,6 +399,8 @@ void or_beamsearch(int metric, const float* X, int64_t n, int dim, const int64_t
                 od[i] = INFINITY;
             }
             memset(vs, 0, sizeof(uint64_t) * (size_t)nw); /* getvstate, context.jl:209-211 */
+            /* visit!(vstate, i) before the search (bichromaticclosestpair.jl:107-113) */
+            if (qvisit && qvisit[j] != 0 && (int64_t)qvisit[j] <= n) (void)visit_test_set(vs, qvisit[j]);
             knn_t res;
             knn_init(&res, oid, od, kk);
             int64_t cd = 0, cb = 0;
@@ -733,7 +735,7 @@ static double optimize_core(const gidx_t* g, const uint32_t* hints, int64_t nh, 
             const int32_t bs = 2 + 2 * bi;
             const float delta = (float)((900.0 + 25.0 * di) / 1000.0);
             or_beamsearch(g->metric, g->X, g->n, g->dim, g->offs, g->adj, hints, nh, NULL, NULL,
-                          0, Q, m, ks, NULL, ks, bs, NULL, delta, 2 * cur_maxvisits, beamcap, rid,
+                          0, NULL, Q, m, ks, NULL, ks, bs, NULL, delta, 2 * cur_maxvisits, beamcap, rid,
                           rd, cdv, cbv, nthreads);
             perf_t* p = &P[bi * ND + di];
             int64_t full = 0;

@@ -53,9 +53,10 @@ void or_parallel_exhaustive(int metric, const float* X, int64_t n, int dim, cons
 void or_beamsearch(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
                    const uint32_t* adj, const uint32_t* hints, int64_t nhints,
                    const uint32_t* qhints, const int32_t* qh_count, int32_t qh_stride,
-                   const float* Q, int64_t m, int32_t k, const int32_t* kq, int32_t kstride,
-                   int32_t bsize, const int32_t* bq, float delta, int64_t maxvisits,
-                   int32_t beamcap, uint32_t* ids, float* dists, int64_t* costdists,
+                   const uint32_t* qvisit, const float* Q, int64_t m, int32_t k,
+                   const int32_t* kq, int32_t kstride, int32_t bsize, const int32_t* bq,
+                   float delta, int64_t maxvisits, int32_t beamcap, uint32_t* ids, float* dists,
+                   int64_t* costdists,
                    int64_t* costblocks, int nthreads);
 
 /* allknn over a SearchGraph (neighbours of i are the hints, one shared visited set) */

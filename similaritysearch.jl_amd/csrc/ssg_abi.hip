@@ -216,6 +216,7 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     a.qhints = qo ? qo->hints : nullptr;
     a.qhcount = qo ? qo->nhints : nullptr;
     a.qhstride = qo ? qo->hint_stride : 0;
+    a.qvisit = qo ? qo->visited : nullptr;
     a.ids = dids;
     a.dists = ddists;
     a.costd = dcd;
@@ -1273,6 +1274,14 @@ int ssg_beamsearch(ssg_handle* h, const float* Q, int64_t m, int32_t k, int32_t 
             dq.hints = h->sH.as<uint32_t>();
             dq.nhints = h->sHc.as<int32_t>();
             dq.hint_stride = qo->hint_stride;
+        }
+        if (qo->visited) {
+            for (int64_t j = 0; j < m; j++)
+                if ((int64_t)qo->visited[j] > h->n) return ssg_fail(SSG_E_ARG, "visited id out of range");
+            HIPCHK(h->sV.ensure((size_t)m * 4), "alloc");
+            HIPCHK(hipMemcpyAsync(h->sV.p, qo->visited, (size_t)m * 4, hipMemcpyHostToDevice, s),
+                   "copy visited");
+            dq.visited = h->sV.as<uint32_t>();
         }
         if (qo->order) {
             std::vector<uint8_t> seen((size_t)m, 0);

@@ -415,6 +415,15 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
             epoch = 1;
         }
         S.epoch = epoch;
+        if (a.qvisit) {  // the query's own vertex, marked before seeding (bichromatic_search!)
+            const uint32_t x = rfl_u(a.qvisit[qi]);
+            if (x != 0u && (int64_t)x <= a.n) {
+                uint32_t c = x - 1u;
+                if (a.ipos) c = rfl_u(a.ipos[c]);
+                if (lane == 0) vis_put(S.vis + c, (uint8_t)epoch);
+                vm_drain();
+            }
+        }
         if (a.allknn) {
             load_run<V>(a.X + (size_t)qi * dp + S.j * V, S.qv);
         } else {

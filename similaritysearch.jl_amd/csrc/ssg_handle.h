@@ -81,6 +81,7 @@ struct ssg_handle {
     // host-call staging
     DevBuf sQ, sIds, sD, sCd, sCb, sK, sB, sH, sHc, sO;
     DevBuf cIds, cD;  // hsp_queries outputs (host entry point)
+    DevBuf sV;        // per-query visited ids (host entry point)
     // builder scratch
     DevBuf bIds, bD, bOut, bCnt, bPairs, bNodeUpd;
     // learned-search scratch: hints, bucket, confidence, hint counts, beam sizes

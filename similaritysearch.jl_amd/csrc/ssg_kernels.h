@@ -33,6 +33,7 @@ struct BeamArgs {
     float delta;
     int64_t maxvisits;
     int32_t beamcap;
+    const uint32_t* qvisit;  // 1-based, [m], 0 = none: marked visited before the search
     const uint32_t* qhints;  // 1-based, [m * qhstride]
     const int32_t* qhcount;
     int32_t qhstride;

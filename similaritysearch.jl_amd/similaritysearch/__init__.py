@@ -394,10 +394,12 @@ def searchbatch_(index, ctx, Q, ids, dists, sorted=False):
     return ids, dists
 
 
-def searchbatch_knns(index, ctx, Q, ks, bsizes=None, hints=None):
+def searchbatch_knns(index, ctx, Q, ks, bsizes=None, hints=None, visited=None):
     """Queue-vector form searchbatch!(index, ctx, Q, knns) (src/SimilaritySearch.jl:384-398)
     with per-query k (ks), optional per-query beam sizes and per-query hint lists
-    (1-based; `hints` is a list of sequences).  Returns a list of (ids, dists) pairs."""
+    (1-based; `hints` is a list of sequences).  `visited` gives per query a 1-based object
+    marked visited before its search starts (0 = none), as bichromatic_search! does with
+    the query's own vertex.  Returns a list of (ids, dists) pairs."""
     if not isinstance(index, SearchGraph):
         raise TypeError("per-query queues are supported over SearchGraph")
     Q = _as_db(Q)
@@ -413,6 +415,12 @@ def searchbatch_knns(index, ctx, Q, ks, bsizes=None, hints=None):
     cb = np.zeros(m, np.int64)
     keep = [ks]
     qo = QueryOpts(ks.ctypes.data, None, None, None, 0)
+    if visited is not None:
+        v = np.ascontiguousarray(visited, np.uint32)
+        if len(v) != m:
+            raise ValueError("visited needs one entry per query")
+        keep.append(v)
+        qo.visited = v.ctypes.data
     if bsizes is not None:
         b = np.ascontiguousarray(bsizes, np.int32)
         keep.append(b)

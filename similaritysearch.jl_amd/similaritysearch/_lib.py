@@ -29,7 +29,7 @@ class BeamParams(C.Structure):
 
 class QueryOpts(C.Structure):
     _fields_ = [("k", _p), ("bsize", _p), ("hints", _p), ("nhints", _p),
-                ("hint_stride", C.c_int32), ("order", _p)]
+                ("hint_stride", C.c_int32), ("order", _p), ("visited", _p)]
 
 
 class BuildParams(C.Structure):

@@ -14,7 +14,8 @@ import numpy as np
 from . import (ExhaustiveSearch, GenericContext, MatrixDatabase, SearchGraph, _as_db, _lib,
                _metric, rerank_, searchbatch, searchbatch_knns)
 
-__all__ = ["closestpair", "bichromatic_closestpair", "neardup", "hsp_queries", "pairwise"]
+__all__ = ["closestpair", "bichromatic_closestpair", "closestpair_hints", "neardup", "hsp_queries",
+           "pairwise"]
 
 
 def _positions(rows, found):
@@ -60,10 +61,10 @@ def bichromatic_closestpair(idxA, ctx, B, min_k=8, samedata=None, seed=0):
     samedata (idxA indexes B itself) a result whose nearest entry is the query itself
     drops it (pop_min!).  The pair of least distance wins, ties to the lowest j.
 
-    Over a SearchGraph with samedata, the reference seeds each search with a random
-    neighbor of the object and marks the object visited; here the random neighbor comes
-    from a numpy generator seeded with `seed` and the object is only dropped from the
-    result, so that case matches the reference in kind, not draw for draw."""
+    Over a SearchGraph with samedata, each object's search is seeded with a random
+    neighbor of it and starts with the object itself marked visited, as the reference
+    does; the random neighbors come from a numpy generator seeded with `seed`
+    (closestpair_hints), not Julia's, so the draws differ from the reference's."""
     B = _as_db(B)
     if samedata is None:
         samedata = idxA.db is B
@@ -73,12 +74,9 @@ def bichromatic_closestpair(idxA, ctx, B, min_k=8, samedata=None, seed=0):
     if n == 0:
         raise ValueError("empty dataset")
     if isinstance(idxA, SearchGraph) and samedata:
-        rng = np.random.default_rng(seed)
-        hints = []
-        for i in range(1, n + 1):
-            nb = idxA.adj.neighbors(i)
-            hints.append(np.array([nb[rng.integers(len(nb))]] if len(nb) else idxA.hints, np.uint32))
-        res = searchbatch_knns(idxA, ctx, B, np.full(n, min_k, np.int32), hints=hints)
+        res = searchbatch_knns(idxA, ctx, B, np.full(n, min_k, np.int32),
+                               hints=closestpair_hints(idxA, seed),
+                               visited=np.arange(1, n + 1, dtype=np.uint32))
         ids = np.stack([r[0] for r in res])
         dists = np.stack([r[1] for r in res])
     else:
@@ -96,6 +94,18 @@ def bichromatic_closestpair(idxA, ctx, B, min_k=8, samedata=None, seed=0):
         return (0, 0, np.float32(np.inf))
     j = int(np.argmin(cd))                                             # first minimum
     return (int(cid[j]), j + 1, cd[j])
+
+
+def closestpair_hints(G, seed=0):
+    """The per-object hint lists bichromatic_search! uses over SearchGraph G with samedata
+    (bichromaticclosestpair.jl:111): one random neighbor of each object (numpy generator
+    seeded with `seed`), or G's hints for an object without neighbors."""
+    rng = np.random.default_rng(seed)
+    hints = []
+    for i in range(1, len(G) + 1):
+        nb = G.adj.neighbors(i)
+        hints.append(np.array([nb[rng.integers(len(nb))]] if len(nb) else G.hints, np.uint32))
+    return hints
 
 
 def closestpair(idx, ctx, min_k=8, seed=0):

@@ -66,19 +66,57 @@ def test_bichromatic_closestpair_exhaustive(oracle):
 
 
 @pytest.mark.gpu
-def test_closestpair_searchgraph():
-    """Over a SearchGraph (approximate): the pair is two distinct objects at their
-    canonical distance; a planted duplicate pair, far below every other distance, is
-    found exactly."""
+def test_closestpair_searchgraph(oracle):
+    """Over a SearchGraph: each object's search starts from a random neighbor with the
+    object marked visited (bichromaticclosestpair.jl:107-113); the same hints and marks in
+    the oracle's beam search give the same rows and the same pair.  A planted duplicate
+    pair, far below every other distance, is found (from either side)."""
     ss = _ss()
+    from similaritysearch.consumers import closestpair_hints
     X = _data(3000, 8, 83)
     X[2500] = X[100] + np.float32(1e-4)
     G = ss.SearchGraph.build(ss.SqL2(), _db(X), seed=3)
     ctx = ss.SearchGraphContext()
-    got = ss.closestpair(G, ctx)
+    got = ss.closestpair(G, ctx, seed=5)
+    H = closestpair_hints(G, 5)
+    qh = np.stack([h[:1] for h in H]).astype(np.uint32)
+    offs = np.concatenate([[0], G.adj.offset]).astype(np.int64)
+    ids, d, _, _ = oracle.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, X, 8, G.algo.bsize,
+                                     G.algo.delta, G.algo.maxvisits, beamcap=ctx.beamcap, qhints=qh,
+                                     qh_count=np.ones(3000, np.int32),
+                                     qvisit=np.arange(1, 3001, dtype=np.uint32), nthreads=8)
+    assert not (ids == np.arange(1, 3001, dtype=np.uint32)[:, None]).any()   # never itself
+    assert got == oracle.closest_of_rows(ids, d, True)
     E = ss.ExhaustiveSearch(ss.SqL2(), _db(X))
     exact = ss.closestpair(E, ss.GenericContext())
-    assert got == exact and exact[:2] == (2501, 101)
+    # the approximate search may find the pair from either side
+    assert exact[:2] == (2501, 101) and {got[0], got[1]} == {101, 2501} and got[2] == exact[2]
+
+
+@pytest.mark.gpu
+def test_searchbatch_knns_visited_marks(oracle):
+    """Per-query visited marks: the marked object never enters the result or the beam,
+    and the search equals the oracle's with the same marks."""
+    ss = _ss()
+    X = _data(5000, 16, 95)
+    Q = X[:400] + np.float32(1e-3)
+    G = ss.SearchGraph.build(ss.SqL2(), _db(X), seed=6)
+    ctx = ss.SearchGraphContext()
+    vis = np.arange(1, 401, dtype=np.uint32)
+    vis[::5] = 0                                                  # some queries mark nothing
+    res = ss.searchbatch_knns(G, ctx, _db(Q), np.full(400, 10, np.int32), visited=vis)
+    ids = np.stack([r[0] for r in res])
+    dists = np.stack([r[1] for r in res])
+    offs = np.concatenate([[0], G.adj.offset]).astype(np.int64)
+    gi, gd, gcd, _ = oracle.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, Q, 10, G.algo.bsize,
+                                       G.algo.delta, G.algo.maxvisits, beamcap=ctx.beamcap,
+                                       qvisit=vis, nthreads=8)
+    np.testing.assert_array_equal(ids, gi)
+    np.testing.assert_array_equal(dists, gd)
+    np.testing.assert_array_equal(ctx.costdists, gcd)
+    marked = vis != 0
+    assert not (ids[marked] == vis[marked][:, None]).any()
+    assert (ids[~marked][:, 0] == np.nonzero(~marked)[0] + 1).mean() > 0.9   # unmarked find their source
 
 
 @pytest.mark.gpu
