@@ -320,7 +320,8 @@ int ssg_learned_search_dev(ssg_handle* h, ssg_mlp* mlp, const uint32_t* dlandmar
                            uint32_t* dids, float* ddists, int64_t* dcd, int64_t* dcb, void* stream);
 
 /* rerank!(dist, db, queries, ids, dists) batch form (src/rerank.jl:77-86): recompute the
- * non-zero-id prefix of each column with the handle's distance and stable-sort it.
+ * non-zero-id prefix of each column with the handle's distance and stable-sort it
+ * (columns of up to 1024 rows).
  * ids / dists are updated in place. */
 int ssg_rerank(ssg_handle* h, const float* Q, int64_t m, int32_t k, uint32_t* ids, float* dists);
 int ssg_rerank_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, uint32_t* dids,

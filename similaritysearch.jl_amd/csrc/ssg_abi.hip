@@ -1432,7 +1432,7 @@ int ssg_rerank_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, uint32_
     HIPCHK(hipSetDevice(h->device), "hipSetDevice");
     if (!h->X.p) return ssg_fail(SSG_E_STATE, "no database uploaded");
     if (m <= 0 || k < 1) return ssg_fail(SSG_E_ARG, "empty rerank");
-    if (k > 64) return ssg_fail(SSG_E_LIMIT, "rerank columns are limited to 64 rows");
+    if (k > 1024) return ssg_fail(SSG_E_LIMIT, "rerank columns are limited to 1024 rows");
     HIPCHK(launch_rerank(h->X.as<float>(), h->dp, h->metric, dQ, h->dim, m, k, dids, ddists,
                          pick(h, stream)), "rerank");
     return SSG_OK;
@@ -1494,7 +1494,7 @@ int ssg_rerank(ssg_handle* h, const float* Q, int64_t m, int32_t k, uint32_t* id
     if (!h->X.p) return ssg_fail(SSG_E_STATE, "no database uploaded");
     if (!Q || !ids || !dists) return ssg_fail(SSG_E_ARG, "null buffer");
     if (m <= 0 || k < 1) return ssg_fail(SSG_E_ARG, "empty rerank");
-    if (k > 64) return ssg_fail(SSG_E_LIMIT, "rerank columns are limited to 64 rows");
+    if (k > 1024) return ssg_fail(SSG_E_LIMIT, "rerank columns are limited to 1024 rows");
     for (int64_t j = 0; j < m; j++)
         for (int32_t t = 0; t < k; t++) {
             const uint32_t v = ids[j * k + t];

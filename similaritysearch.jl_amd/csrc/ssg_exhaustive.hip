@@ -770,6 +770,67 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ X
     }
 }
 
+// rerank! for columns of 65..1024 rows: one wave per column; the prefix's ids and
+// canonical distances are staged in LDS (k * 8 bytes per wave), then every lane ranks its
+// entries by counting, over LDS broadcast reads, the entries that sort before them
+// ((distance, position) order: a stable sort) and writes them at their rank.
+template <int V>
+__global__ __launch_bounds__(256) void rerank_wide_kernel(const float* __restrict__ X, int metric,
+                                                          const float* __restrict__ Q, int qdim,
+                                                          int64_t m, int k, uint32_t* ids, float* dists)
+{
+    extern __shared__ float rr_lds[];
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int g = lane >> 4, j = lane & 15;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= m) return;
+    const int dp = 16 * V;
+    float* sd = rr_lds + (size_t)w * 2 * k;
+    uint32_t* si = reinterpret_cast<uint32_t*>(sd + k);
+    uint32_t* row = ids + q * k;
+    float* drow = dists + q * k;
+    int len = k;
+    for (int c0 = 0; c0 < k; c0 += 64) {
+        const int c = c0 + lane;
+        const uint32_t v = c < k ? row[c] : 1u;
+        if (c < k) si[c] = v;
+        const uint64_t z = __ballot(c < k && v == 0u);
+        if (z) {
+            len = c0 + __ffsll((unsigned long long)z) - 1;
+            break;
+        }
+    }
+    float qv[V];
+#pragma unroll
+    for (int t = 0; t < V; t++) {
+        const int i = j * V + t;
+        qv[t] = i < qdim ? Q[q * qdim + i] : 0.f;
+    }
+    wave_sync();
+    for (int r0 = 0; r0 < len; r0 += 4) {
+        const int u = r0 + g;
+        const uint32_t id = si[u < len ? u : 0];  // in-range entry only
+        float xv[V];
+        load_run<V>(X + (size_t)(id - 1u) * dp + j * V, xv);
+        const double s = group16_tree(run_partial<V>(metric, xv, qv));
+        if (j == 15 && u < len) sd[u] = finish_dist(metric, s);
+    }
+    wave_sync();
+    for (int i0 = 0; i0 < len; i0 += 64) {
+        const int i = i0 + lane;
+        const float di = i < len ? sd[i] : 0.f;
+        int rank = 0;
+        for (int t = 0; t < len; t++) {
+            const float dt = sd[t];
+            rank += (dt < di || (dt == di && t < i)) ? 1 : 0;
+        }
+        if (i < len) {
+            row[rank] = si[i];
+            drow[rank] = di;
+        }
+    }
+}
+
 // Stage-1 blocks resident per CU: LDS-bound (row tile + candidate lists), at most two
 // by registers.
 // Stage-1 blocks resident per CU: two for the list kernel (dp <= 128); LDS-bound (row
@@ -886,6 +947,18 @@ hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float
                          int64_t m, int32_t k, uint32_t* ids, float* dists, hipStream_t s)
 {
     const dim3 grid((unsigned)((m + 3) / 4)), block(256);
+    if (k > 1024) return hipErrorInvalidValue;
+    if (k > 64) {
+        const size_t lds = (size_t)4 * 2 * k * sizeof(float);
+        switch (dp / 16) {
+#define SSG_RW(VV) \
+    case VV: hipLaunchKernelGGL((rerank_wide_kernel<VV>), grid, block, lds, s, X, metric, Q, qdim, m, k, ids, dists); break;
+            SSG_RW(1) SSG_RW(2) SSG_RW(3) SSG_RW(4) SSG_RW(5) SSG_RW(6) SSG_RW(7) SSG_RW(8) SSG_RW(12) SSG_RW(16)
+#undef SSG_RW
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (dp / 16) {
 #define SSG_RR(VV) \
     case VV: hipLaunchKernelGGL((rerank_kernel<VV>), grid, block, 0, s, X, metric, Q, qdim, m, k, ids, dists); break;
