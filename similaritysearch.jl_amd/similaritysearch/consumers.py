@@ -33,12 +33,12 @@ def _positions(rows, found):
 def pairwise(dist, X, ids):
     """(m, k, k) canonical distances between the objects of each row of ids ((m, k),
     1-based, 0 ends a row; pairs with an empty slot are Inf), computed by rerank! on the
-    GPU: object ids[i, j] is the query and row i's ids are the candidates (k <= 64)."""
+    GPU: object ids[i, j] is the query and row i's ids are the candidates (k <= 1024)."""
     X = _as_db(X)
     ids = np.asarray(ids, np.uint32)
     m, k = ids.shape
-    if k > 64:
-        raise ValueError("pairwise distances take at most 64 objects per row")
+    if k > 1024:
+        raise ValueError("pairwise distances take at most 1024 objects per row")
     out = np.full((m, k, k), np.inf, np.float32)
     qi, qj = np.nonzero(ids > 0)
     if len(qi) == 0:

@@ -183,10 +183,11 @@ def test_hsp_queries_short_columns(oracle):
 
 
 @pytest.mark.gpu
-def test_pairwise_matches_dist_batch(oracle):
+@pytest.mark.parametrize("k", [15, 100])
+def test_pairwise_matches_dist_batch(oracle, k):
     ss = _ss()
-    X = _data(500, 7, 89)
-    ids = np.random.default_rng(90).permutation(500)[:300].reshape(20, 15).astype(np.uint32) + 1
+    X = _data(5000, 7, 89)
+    ids = np.random.default_rng(90).permutation(5000)[:20 * k].reshape(20, k).astype(np.uint32) + 1
     ids[3, 9:] = 0
     P = ss.pairwise(ss.SqL2(), _db(X), ids)
     for r in range(20):
