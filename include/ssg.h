@@ -329,7 +329,7 @@ int ssg_rerank_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, uint32_
 
 /* hsp_queries(dist, X, Q, knns_ids, knns_dists) (src/hsp.jl:77-101, hsp_should_push
  * :8-33) over the handle's database and distance: each of the m (k, m) candidate columns
- * (ascending, 0 ends a column; 1 <= k <= 64) is scanned in order and a candidate is kept
+ * (ascending, 0 ends a column; 1 <= k <= 256 with k * padded dim * 4 <= 160 KiB) is scanned in order and a candidate is kept
  * unless an already kept one is strictly nearer to it than its given distance.  The kept
  * candidates fill the top of the hsp_ids / hsp_dists columns as a sorted queue (stable on
  * equal distances), then 0 / +Inf.  The queries themselves are not needed: the given

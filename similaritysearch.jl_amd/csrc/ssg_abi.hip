@@ -1447,7 +1447,8 @@ int ssg_hsp_queries_dev(ssg_handle* h, int64_t m, int32_t k, const uint32_t* did
     if (!h->X.p) return ssg_fail(SSG_E_STATE, "no database uploaded");
     if (!dids || !ddists || !dhsp_ids || !dhsp_dists) return ssg_fail(SSG_E_ARG, "null buffer");
     if (m <= 0 || k < 1) return ssg_fail(SSG_E_ARG, "hsp_queries: empty candidate set");
-    if (k > 64) return ssg_fail(SSG_E_LIMIT, "hsp_queries: columns are limited to 64 rows");
+    if (k > 256 || (int64_t)k * h->dp * 4 > 160 * 1024)
+        return ssg_fail(SSG_E_LIMIT, "hsp_queries: columns are limited to 256 rows and k * padded dim * 4 to 160 KiB");
     HIPCHK(launch_hsp(h->X.as<float>(), h->dp, h->metric, m, k, dids, ddists, dhsp_ids, dhsp_dists,
                       pick(h, stream)), "hsp_queries");
     return SSG_OK;
@@ -1462,7 +1463,8 @@ int ssg_hsp_queries(ssg_handle* h, int64_t m, int32_t k, const uint32_t* ids, co
     if (!h->X.p) return ssg_fail(SSG_E_STATE, "no database uploaded");
     if (!ids || !dists || !hsp_ids || !hsp_dists) return ssg_fail(SSG_E_ARG, "null buffer");
     if (m <= 0 || k < 1) return ssg_fail(SSG_E_ARG, "hsp_queries: empty candidate set");
-    if (k > 64) return ssg_fail(SSG_E_LIMIT, "hsp_queries: columns are limited to 64 rows");
+    if (k > 256 || (int64_t)k * h->dp * 4 > 160 * 1024)
+        return ssg_fail(SSG_E_LIMIT, "hsp_queries: columns are limited to 256 rows and k * padded dim * 4 to 160 KiB");
     for (int64_t j = 0; j < m; j++)
         for (int32_t t = 0; t < k; t++) {
             const uint32_t v = ids[j * k + t];

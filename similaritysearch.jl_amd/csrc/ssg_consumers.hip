@@ -46,7 +46,7 @@ __device__ __forceinline__ double run_partial_wide(int metric, const double (&p)
     return s;
 }
 
-template <int V>
+template <int V, int NB>
 __global__ __launch_bounds__(256) void hsp_kernel(const float* __restrict__ X, int metric, int64_t m,
                                                   int k, const uint32_t* __restrict__ ids,
                                                   const float* __restrict__ dists,
@@ -61,10 +61,22 @@ __global__ __launch_bounds__(256) void hsp_kernel(const float* __restrict__ X, i
     if (q >= m) return;
     float* R = hsp_rows + (size_t)w * k * dp;
 
-    const uint32_t cid = lane < k ? ids[q * k + lane] : 0u;
-    const float cd = lane < k ? dists[q * k + lane] : __builtin_inff();
-    const uint64_t z = __ballot(lane < k && cid == 0u);
-    const int L = z ? (__ffsll((unsigned long long)z) - 1) : k;
+    // candidate c = 64 b + lane in register b (k <= 64 NB)
+    uint32_t cid[NB];
+    float cd[NB];
+    int L = k;
+    bool found = false;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int c = 64 * b + lane;
+        cid[b] = c < k ? ids[q * k + c] : 0u;
+        cd[b] = c < k ? dists[q * k + c] : __builtin_inff();
+        const uint64_t z = __ballot(c < k && cid[b] == 0u);
+        if (!found && z) {
+            L = 64 * b + __ffsll((unsigned long long)z) - 1;
+            found = true;
+        }
+    }
 
     // stage the L candidate rows, 16 lanes per row: up to 32 rows' loads in flight before
     // their LDS stores
@@ -74,8 +86,8 @@ __global__ __launch_bounds__(256) void hsp_kernel(const float* __restrict__ X, i
 #pragma unroll
         for (int s = 0; s < SB; s++) {
             const int r = r0 + 4 * s + g;
-            const uint32_t id = __shfl(cid, r < L ? r : 0);
             if (r < L) {
+                const uint32_t id = ids[q * k + r];
                 load_run<V>(X + (size_t)(id - 1u) * dp + j * V, xv[s]);
             } else {
 #pragma unroll
@@ -93,18 +105,26 @@ __global__ __launch_bounds__(256) void hsp_kernel(const float* __restrict__ X, i
     }
     wave_sync();
 
-    RegQueue<1, false> out;
+    RegQueue<NB, false> out;
     out.init(k);
     int A = 0;
     for (int c = 0; c < L; c++) {
         float pv[V];
         double pw[V];
+        float cdc = cd[0];
+        uint32_t cic = cid[0];
+#pragma unroll
+        for (int b = 1; b < NB; b++)
+            if ((c >> 6) == b) {  // wave-uniform selection of the register holding c
+                cdc = cd[b];
+                cic = cid[b];
+            }
 #pragma unroll
         for (int t = 0; t < V; t++) {
             pv[t] = R[c * dp + j * V + t];
             pw[t] = (double)pv[t];
         }
-        const float pd = rl_f(cd, c);
+        const float pd = rl_f(cdc, c & 63);
         bool hit = false;
         for (int a0 = 0; a0 < A; a0 += 8) {
             const int u0 = a0 + g, u1 = a0 + 4 + g;
@@ -131,12 +151,16 @@ __global__ __launch_bounds__(256) void hsp_kernel(const float* __restrict__ X, i
                 wave_sync();
             }
             A++;
-            out.push(pd, rl_u(cid, c), 0, 0);
+            out.push(pd, rl_u(cic, c & 63), 0, 0);
         }
     }
-    if (lane < k) {
-        out_ids[q * k + lane] = lane < out.len ? out.id[0] : 0u;
-        out_dists[q * k + lane] = lane < out.len ? out.d[0] : __builtin_inff();
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int c = 64 * b + lane;
+        if (c < k) {
+            out_ids[q * k + c] = c < out.len ? out.id[b] : 0u;
+            out_dists[q * k + c] = c < out.len ? out.d[b] : __builtin_inff();
+        }
     }
 }
 
@@ -416,7 +440,8 @@ hipError_t launch_gather_padded_rows(const float* X, const float* xn, int32_t dp
     return hipGetLastError();
 }
 
-// Waves per block: as many as keep the block's staged rows within 64 KiB (1 to 4).
+// Waves per block: as many as keep the block's staged rows within 64 KiB (1 to 4); one
+// wave per block beyond that, within the 160 KiB of LDS a gfx950 workgroup may use.
 static int hsp_waves(int k, int dp)
 {
     const int per_wave = k * dp * 4;
@@ -424,22 +449,40 @@ static int hsp_waves(int k, int dp)
     return w < 1 ? 1 : (w > 4 ? 4 : w);
 }
 
-hipError_t launch_hsp(const float* X, int32_t dp, int32_t metric, int64_t m, int32_t k,
-                      const uint32_t* ids, const float* dists, uint32_t* out_ids, float* out_dists,
-                      hipStream_t s)
+template <int NB>
+static hipError_t launch_hsp_nb(const float* X, int32_t dp, int32_t metric, int64_t m, int32_t k,
+                                const uint32_t* ids, const float* dists, uint32_t* out_ids,
+                                float* out_dists, hipStream_t s)
 {
-    if (k < 1 || k > 64 || m <= 0) return hipErrorInvalidValue;
     const int W = hsp_waves(k, dp);
     const dim3 grid((unsigned)((m + W - 1) / W)), block(64 * W);
     const size_t lds = (size_t)W * k * dp * 4;
     switch (dp / 16) {
-#define SSG_HSP(VV) \
-    case VV: hipLaunchKernelGGL((hsp_kernel<VV>), grid, block, lds, s, X, metric, m, k, ids, dists, out_ids, out_dists); break;
+#define SSG_HSP(VV)                                                                                  \
+    case VV:                                                                                         \
+        if (lds > 65536) {                                                                           \
+            hipError_t e = hipFuncSetAttribute((const void*)hsp_kernel<VV, NB>,                      \
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+            if (e != hipSuccess) return e;                                                           \
+        }                                                                                            \
+        hipLaunchKernelGGL((hsp_kernel<VV, NB>), grid, block, lds, s, X, metric, m, k, ids, dists,   \
+                           out_ids, out_dists);                                                      \
+        break;
         SSG_HSP(1) SSG_HSP(2) SSG_HSP(3) SSG_HSP(4) SSG_HSP(5) SSG_HSP(6) SSG_HSP(7) SSG_HSP(8) SSG_HSP(12) SSG_HSP(16)
 #undef SSG_HSP
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_hsp(const float* X, int32_t dp, int32_t metric, int64_t m, int32_t k,
+                      const uint32_t* ids, const float* dists, uint32_t* out_ids, float* out_dists,
+                      hipStream_t s)
+{
+    if (k < 1 || k > 256 || m <= 0 || (size_t)k * dp * 4 > 160 * 1024) return hipErrorInvalidValue;
+    if (k <= 64) return launch_hsp_nb<1>(X, dp, metric, m, k, ids, dists, out_ids, out_dists, s);
+    if (k <= 128) return launch_hsp_nb<2>(X, dp, metric, m, k, ids, dists, out_ids, out_dists, s);
+    return launch_hsp_nb<4>(X, dp, metric, m, k, ids, dists, out_ids, out_dists, s);
 }
 
 }  // namespace ssg

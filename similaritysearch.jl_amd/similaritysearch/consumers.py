@@ -160,7 +160,7 @@ _hsp_cache = {}
 def hsp_queries(dist, X, Q, knns_ids, knns_dists):
     """hsp_queries(dist, X, Q, knns_ids, knns_dists) -> (hsp_ids, hsp_dists, hsp)
     (src/hsp.jl:8-33, 77-101): each query's candidates ((k, n) columns, 0 ends a column,
-    k <= 64) are taken in column order and kept unless one already kept is strictly
+    k <= 256) are taken in column order and kept unless one already kept is strictly
     nearer to the candidate than the query is (hsp_should_push).  The kept entries form
     a sorted queue (stable by distance) at the top of each (k, n) output column, then
     0 / Inf; hsp lists each query's (ids, dists).  The filter runs on the GPU

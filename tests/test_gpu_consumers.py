@@ -198,10 +198,12 @@ def test_pairwise_matches_dist_batch(oracle, k):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,dim,k", [(3000, 256, 64), (2000, 100, 40), (500, 16, 1)])
+@pytest.mark.parametrize("n,dim,k", [(3000, 256, 64), (2000, 100, 40), (500, 16, 1), (3000, 96, 100),
+                                     (3000, 96, 256), (4000, 256, 160)])
 def test_hsp_queries_shapes_and_unsorted_columns(oracle, n, dim, k):
-    """Wide rows with k = 64 (one wave per block), ragged dims, k = 1, and columns given
-    out of distance order (the kept queue sorts them, stable on ties)."""
+    """Wide rows with k = 64 (one wave per block), ragged dims, k = 1, columns of more than
+    64 rows (two and four queue registers per lane, LDS beyond 64 KiB at k = 160, dim 256),
+    and columns given out of distance order (the kept queue sorts them, stable on ties)."""
     ss = _ss()
     X = _data(n, dim, 91)
     Q = _data(64, dim, 92)
@@ -215,9 +217,9 @@ def test_hsp_queries_shapes_and_unsorted_columns(oracle, n, dim, k):
     oi, od = oracle.hsp_queries("SqL2", X, Q, ids.T, dists.T)
     np.testing.assert_array_equal(hi.T, oi)
     np.testing.assert_array_equal(hd.T, od)
-    with pytest.raises(Exception, match="64"):
-        ss.hsp_queries(ss.SqL2(), _db(X), _db(Q[:2]), np.ones((65, 2), np.uint32),
-                       np.zeros((65, 2), np.float32))
+    with pytest.raises(Exception, match="256"):
+        ss.hsp_queries(ss.SqL2(), _db(X), _db(Q[:2]), np.ones((257, 2), np.uint32),
+                       np.zeros((257, 2), np.float32))
 
 
 @pytest.mark.gpu
