@@ -86,8 +86,14 @@ __global__ __launch_bounds__(256) void hsp_kernel(const float* __restrict__ X, i
 #pragma unroll
         for (int s = 0; s < SB; s++) {
             const int r = r0 + 4 * s + g;
+            // the row's id: from the lanes' registers when the column fits one register
+            uint32_t id;
+            if constexpr (NB == 1) {
+                id = __shfl(cid[0], r < L ? r : 0);
+            } else {
+                id = r < L ? ids[q * k + r] : 1u;
+            }
             if (r < L) {
-                const uint32_t id = ids[q * k + r];
                 load_run<V>(X + (size_t)(id - 1u) * dp + j * V, xv[s]);
             } else {
 #pragma unroll
