@@ -21,7 +21,8 @@ def test_header_declares_the_api():
     names = declared()
     for must in ["ssg_create", "ssg_destroy", "ssg_upload_db", "ssg_upload_graph",
                  "ssg_exhaustive_knn", "ssg_beamsearch", "ssg_allknn_graph", "ssg_rerank",
-                 "ssg_build_graph", "ssg_optimize_minrecall"]:
+                 "ssg_build_graph", "ssg_optimize_minrecall", "ssg_append_db", "ssg_neardup",
+                 "ssg_hsp_queries", "ssg_hsp_queries_dev"]:
         assert must in names
 
 
