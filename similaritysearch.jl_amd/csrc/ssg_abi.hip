@@ -1008,7 +1008,8 @@ int ssg_neardup(ssg_handle* h, float eps, int32_t blocksize, int32_t filterblock
             F.insert(F.end(), fresh.begin(), fresh.end());
             g->n += (int64_t)fresh.size();
             g->n_pad = (g->n + 63) / 64 * 64;
-            HIPCHK(hipStreamSynchronize(s), "sync");  // fresh is reused by the next block
+            // no synchronisation here: fresh (the copy's source) is next modified after the
+            // next block's fix-up or batch-search synchronisation, which orders this copy
         }
     }
     HIPCHK(hipStreamSynchronize(s), "sync");
