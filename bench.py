@@ -665,7 +665,7 @@ def hsp_bench(args):
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": "profiles/hsp_queries_1m96_k32.md",
-                     "kernel": "hsp_kernel<6>", "kernel_ms": round(ms, 3),
+                     "kernel": "hsp_kernel<6, 1>", "kernel_ms": round(ms, 3),
                      "bytes_per_launch": n * bytes_col},
         "cpu_baseline": cpu,
     }
