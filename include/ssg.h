@@ -121,7 +121,8 @@ typedef struct {
     int32_t precision; /* SSG_F32: canonical distances.  SSG_BF16 (no reference
                           counterpart): BF16 vector gathers with FP32 arithmetic during the
                           search, then rerank! of the result list with the canonical distance */
-    int32_t rerank;    /* SSG_BF16: result-list length reranked (max(k, rerank) <= 64) */
+    int32_t rerank;    /* SSG_BF16: query j keeps and reranks max(k_j, rerank) result entries
+                          (max(kstride, rerank) <= 64) */
     int32_t expand;    /* SSG_BF16 only (no reference counterpart): beam entries popped and
                           expanded together per step, 0..4; 0 and 1 are the reference step */
 } ssg_beam_params;

@@ -65,6 +65,13 @@ def lib():
         L.or_beamsearch.argtypes = [C.c_int, _p, _i64, C.c_int, _p, _p, _p, _i64, _p, _p, _i32, _p,
                                     _p, _i64, _i32, _p, _i32, _i32, _p, _f32, _i64, _i32, _p,
                                     _p, _p, _p, C.c_int]
+        L.or_beamsearch_bf16.argtypes = [C.c_int, _p, _i64, C.c_int, _p, _p, _p, _i64, _p, _p,
+                                         _i32, _p, _p, _i64, _i32, _p, _i32, _i32, _p, _f32,
+                                         _i64, _i32, _i32, _p, _p, _p, _p, C.c_int]
+        L.or_bf16_round.restype = C.c_float
+        L.or_bf16_round.argtypes = [C.c_float]
+        L.or_dist_bf16.restype = C.c_float
+        L.or_dist_bf16.argtypes = [C.c_int, _p, _p, C.c_int]
         L.or_allknn_graph.argtypes = [C.c_int, _p, _i64, C.c_int, _p, _p, _i32, _i32, _f32, _i64,
                                       _i32, _p, _p, C.c_int]
         L.or_rerank.argtypes = [C.c_int, _p, C.c_int, _p, _i64, _i32, _p, _p]
@@ -150,9 +157,12 @@ def parallel_exhaustive(metric, X, Q, k, nbatches):
 
 
 def beamsearch(metric, X, offs, adj, hints, Q, k, bsize, delta, maxvisits, beamcap=32, kq=None,
-               bq=None, qhints=None, qh_count=None, nthreads=1, qvisit=None):
+               bq=None, qhints=None, qh_count=None, nthreads=1, qvisit=None, bf16_rerank=None):
     """Restated searchbatch over a SearchGraph.  Returns ids, dists, costdists, costblocks.
-    qvisit: per query a 1-based object marked visited before its search (0 = none)."""
+    qvisit: per query a 1-based object marked visited before its search (0 = none).
+    bf16_rerank (an int): the BF16 form instead (no reference counterpart): the search runs
+    on dist_bf16 with a result queue of max(k, bf16_rerank) entries, reranked with the
+    canonical distance before the first k are written."""
     X, Q = _f32c(X), _f32c(Q)
     offs = np.ascontiguousarray(offs, np.int64)
     adj = np.ascontiguousarray(adj, np.uint32)
@@ -168,12 +178,23 @@ def beamsearch(metric, X, offs, adj, hints, Q, k, bsize, delta, maxvisits, beamc
     d = np.full((m, kstride), np.inf, np.float32)
     cd = np.zeros(m, np.int64)
     cb = np.zeros(m, np.int64)
-    lib().or_beamsearch(METRICS.get(metric, metric), _ptr(X), X.shape[0], X.shape[1], _ptr(offs),
-                        _ptr(adj), _ptr(hints), len(hints), _ptr(qh), _ptr(qc),
-                        0 if qh is None else qh.shape[1], _ptr(qv), _ptr(Q), m, int(k), _ptr(kq_a), kstride,
-                        int(bsize), _ptr(bq_a), float(delta), int(maxvisits), int(beamcap),
-                        _ptr(ids), _ptr(d), _ptr(cd), _ptr(cb), nthreads)
+    common = (METRICS.get(metric, metric), _ptr(X), X.shape[0], X.shape[1], _ptr(offs),
+              _ptr(adj), _ptr(hints), len(hints), _ptr(qh), _ptr(qc),
+              0 if qh is None else qh.shape[1], _ptr(qv), _ptr(Q), m, int(k), _ptr(kq_a), kstride,
+              int(bsize), _ptr(bq_a), float(delta), int(maxvisits), int(beamcap))
+    if bf16_rerank is None:
+        lib().or_beamsearch(*common, _ptr(ids), _ptr(d), _ptr(cd), _ptr(cb), nthreads)
+    else:
+        lib().or_beamsearch_bf16(*common, int(bf16_rerank), _ptr(ids), _ptr(d), _ptr(cd),
+                                 _ptr(cb), nthreads)
     return ids, d, cd, cb
+
+
+def dist_bf16(metric, x, q):
+    """The BF16 form's search distance: row x rounded to BF16, FP32 query and arithmetic
+    in the canonical run/tree order (oracle/ssg_oracle.c, or_dist_bf16)."""
+    x, q = _f32c(x), _f32c(q)
+    return lib().or_dist_bf16(METRICS.get(metric, metric), _ptr(x), _ptr(q), x.shape[-1])
 
 
 def allknn_graph(metric, X, offs, adj, k, bsize, delta, maxvisits, beamcap=32, nthreads=1):

@@ -70,6 +70,57 @@ float or_dist(int metric, const float* a, const float* b, int dim)
     return finish(metric, tree16(metric, a, b, dim));
 }
 
+/* ------------------------------------------------------------------------ */
+/* Search distance of the BF16 form (no reference counterpart): the object  */
+/* row x is rounded to BF16 (nearest, ties to even), the query q stays FP32, */
+/* and the arithmetic is FP32: the same 16 runs of Dp/16 elements, each      */
+/* accumulated with fmaf in increasing index from 0, the same xor-butterfly  */
+/* 8,4,2,1 over the 16 partial sums (FP32 adds), then sqrtf (L2) or 1 - s    */
+/* (NormCosine) in FP32.                                                     */
+/* Pinned by: on BF16-representable integer rows every FP32 step is exact,  */
+/* so it equals the canonical distance; on general rows it stays within the  */
+/* FP32 accumulation bound of the FP64 distance between the BF16 row and q   */
+/* (tests/test_oracle.py).                                                   */
+/* ------------------------------------------------------------------------ */
+float or_bf16_round(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+
+float or_dist_bf16(int metric, const float* x, const float* q, int dim)
+{
+    const int dp = (dim + 15) & ~15;
+    const int v = dp / 16;
+    float p[16], t[16];
+    for (int j = 0; j < 16; j++) {
+        float s = 0.f;
+        for (int u = 0; u < v; u++) {
+            const int i = j * v + u;
+            const float xf = i < dim ? or_bf16_round(x[i]) : 0.f;
+            const float qf = i < dim ? q[i] : 0.f;
+            if (metric == OR_NORMCOS) {
+                s = fmaf(xf, qf, s);
+            } else {
+                const float m = xf - qf;
+                s = fmaf(m, m, s);
+            }
+        }
+        p[j] = s;
+    }
+    for (int w = 8; w >= 1; w >>= 1) {
+        for (int j = 0; j < 16; j++) t[j] = p[j] + p[j ^ w];
+        memcpy(p, t, sizeof(p));
+    }
+    if (metric == OR_L2) return sqrtf(p[0]);
+    if (metric == OR_NORMCOS) return 1.0f - p[0];
+    return p[0];
+}
+
 double or_dist_f64(int metric, const float* a, const float* b, int dim)
 {
     double s = 0.0;
@@ -292,7 +343,15 @@ typedef struct {
     int dim;
     const int64_t* offs;
     const uint32_t* adj; /* 1-based ids */
+    int prec;            /* 0: canonical distance; 1: the BF16 form's search distance */
 } gidx_t;
+
+/* distance of the object with 1-based id `id1` to the query, as the search evaluates it */
+static inline float gdist(const gidx_t* g, const float* q, uint32_t id1)
+{
+    const float* x = g->X + (int64_t)(id1 - 1) * g->dim;
+    return g->prec ? or_dist_bf16(g->metric, x, q, g->dim) : or_dist(g->metric, q, x, g->dim);
+}
 
 static inline int visit_test_set(uint64_t* vs, uint32_t id1)
 {
@@ -314,7 +373,7 @@ static inline void enqueue_item(const gidx_t* g, const float* q, knn_t* res, uin
                                 uint64_t* vs, int64_t* cd)
 {
     if (visit_test_set(vs, id1)) return;
-    const float d = or_dist(g->metric, q, g->X + (int64_t)(id1 - 1) * g->dim, g->dim);
+    const float d = gdist(g, q, id1);
     knn_push(res, id1, d);
     (*cd)++;
 }
@@ -329,7 +388,7 @@ static void beam_search(const gidx_t* g, const float* q, knn_t* res, const uint3
     if (n == 0) return;
     if (n < 64) {
         for (int64_t i = 0; i < n; i++)
-            knn_push(res, (uint32_t)(i + 1), or_dist(g->metric, q, g->X + i * g->dim, g->dim));
+            knn_push(res, (uint32_t)(i + 1), gdist(g, q, (uint32_t)(i + 1)));
         *cd += n;
         return;
     }
@@ -354,7 +413,7 @@ static void beam_search(const gidx_t* g, const float* q, knn_t* res, const uint3
         for (int64_t t = s; t < e; t++) {
             const uint32_t child = g->adj[t];
             if (visit_test_set(vs, child)) continue;
-            const float d = or_dist(g->metric, q, g->X + (int64_t)(child - 1) * g->dim, g->dim);
+            const float d = gdist(g, q, child);
             knn_push(res, child, d);
             costdists++;
             if (costdists > maxvisits) goto finish;
@@ -368,16 +427,24 @@ finish:
     *cb += costblocks;
 }
 
-void or_beamsearch(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
-                   const uint32_t* adj, const uint32_t* hints, int64_t nhints,
-                   const uint32_t* qhints, const int32_t* qh_count, int32_t qh_stride,
-                   const uint32_t* qvisit, const float* Q, int64_t m, int32_t k,
-                   const int32_t* kq, int32_t kstride, int32_t bsize, const int32_t* bq,
-                   float delta, int64_t maxvisits, int32_t beamcap, uint32_t* ids, float* dists,
-                   int64_t* costdists, int64_t* costblocks, int nthreads)
+/* searchbatch! over all queries.  prec 1 is the BF16 form (no reference
+ * counterpart): the search runs on the BF16 search distance with a result queue
+ * of max(k, krr) entries, which rerank! (src/rerank.jl:22-41) then rescores with
+ * the canonical distance and stable-sorts; the first k are written.
+ * Pinned (prec 1) by: on rows whose BF16 distance is exact it equals the FP32
+ * search with k = max(k, krr) cut to its first k (tests/test_oracle.py). */
+static void beamsearch_impl(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
+                            const uint32_t* adj, const uint32_t* hints, int64_t nhints,
+                            const uint32_t* qhints, const int32_t* qh_count, int32_t qh_stride,
+                            const uint32_t* qvisit, const float* Q, int64_t m, int32_t k,
+                            const int32_t* kq, int32_t kstride, int32_t bsize, const int32_t* bq,
+                            float delta, int64_t maxvisits, int32_t beamcap, uint32_t* ids,
+                            float* dists, int64_t* costdists, int64_t* costblocks, int nthreads,
+                            int prec, int32_t krr)
 {
-    gidx_t g = {metric, X, n, dim, offs, adj};
+    gidx_t g = {metric, X, n, dim, offs, adj, prec};
     const int64_t nw = (n + 63) / 64 + 1;
+    const int32_t rcap = (prec && krr > kstride ? krr : kstride) + 1;
     if (nthreads < 1) nthreads = 1;
 #ifdef _OPENMP
 #pragma omp parallel num_threads(nthreads)
@@ -386,6 +453,10 @@ void or_beamsearch(int metric, const float* X, int64_t n, int dim, const int64_t
         uint64_t* vs = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)nw);
         uint32_t* bids = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(beamcap + 1));
         float* bd = (float*)malloc(sizeof(float) * (size_t)(beamcap + 1));
+        uint32_t* rid = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)rcap);
+        float* rd = (float*)malloc(sizeof(float) * (size_t)rcap);
+        float* dex = (float*)malloc(sizeof(float) * (size_t)rcap);
+        int32_t* perm = (int32_t*)malloc(sizeof(int32_t) * (size_t)rcap);
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 8)
 #endif
@@ -402,23 +473,79 @@ void or_beamsearch(int metric, const float* X, int64_t n, int dim, const int64_t
             /* visit!(vstate, i) before the search (bichromaticclosestpair.jl:107-113) */
             if (qvisit && qvisit[j] != 0 && (int64_t)qvisit[j] <= n) (void)visit_test_set(vs, qvisit[j]);
             knn_t res;
-            knn_init(&res, oid, od, kk);
+            if (prec)
+                knn_init(&res, rid, rd, kk > krr ? kk : krr);
+            else
+                knn_init(&res, oid, od, kk);
             int64_t cd = 0, cb = 0;
             const uint32_t* h = hints;
             int64_t nh = nhints;
-            if (qhints) {
+            if (qhints && qh_count[j] >= 0) { /* a negative count keeps the index's hints */
                 h = qhints + j * (int64_t)qh_stride;
                 nh = qh_count[j];
             }
             beam_search(&g, Q + j * dim, &res, h, nh, bs, delta, maxvisits, beamcap, vs, bids, bd,
                         &cd, &cb);
+            if (prec) {
+                /* rerank!: canonical distances of the queue in its order, stable sort */
+                const int32_t len = knn_len(&res);
+                for (int32_t i = 0; i < len; i++) {
+                    const uint32_t id = res.ids[res.sp + i];
+                    dex[i] = or_dist(metric, Q + j * dim, X + (int64_t)(id - 1) * dim, dim);
+                    perm[i] = i;
+                }
+                for (int32_t i = 1; i < len; i++) { /* insertion sort: stable */
+                    const int32_t p = perm[i];
+                    int32_t t = i;
+                    while (t > 0 && dex[perm[t - 1]] > dex[p]) {
+                        perm[t] = perm[t - 1];
+                        t--;
+                    }
+                    perm[t] = p;
+                }
+                for (int32_t i = 0; i < len && i < kk; i++) {
+                    oid[i] = res.ids[res.sp + perm[i]];
+                    od[i] = dex[perm[i]];
+                }
+            }
             if (costdists) costdists[j] = cd;
             if (costblocks) costblocks[j] = cb;
         }
         free(vs);
         free(bids);
         free(bd);
+        free(rid);
+        free(rd);
+        free(dex);
+        free(perm);
     }
+}
+
+void or_beamsearch(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
+                   const uint32_t* adj, const uint32_t* hints, int64_t nhints,
+                   const uint32_t* qhints, const int32_t* qh_count, int32_t qh_stride,
+                   const uint32_t* qvisit, const float* Q, int64_t m, int32_t k,
+                   const int32_t* kq, int32_t kstride, int32_t bsize, const int32_t* bq,
+                   float delta, int64_t maxvisits, int32_t beamcap, uint32_t* ids, float* dists,
+                   int64_t* costdists, int64_t* costblocks, int nthreads)
+{
+    beamsearch_impl(metric, X, n, dim, offs, adj, hints, nhints, qhints, qh_count, qh_stride, qvisit,
+                    Q, m, k, kq, kstride, bsize, bq, delta, maxvisits, beamcap, ids, dists,
+                    costdists, costblocks, nthreads, 0, 0);
+}
+
+void or_beamsearch_bf16(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
+                        const uint32_t* adj, const uint32_t* hints, int64_t nhints,
+                        const uint32_t* qhints, const int32_t* qh_count, int32_t qh_stride,
+                        const uint32_t* qvisit, const float* Q, int64_t m, int32_t k,
+                        const int32_t* kq, int32_t kstride, int32_t bsize, const int32_t* bq,
+                        float delta, int64_t maxvisits, int32_t beamcap, int32_t krr,
+                        uint32_t* ids, float* dists, int64_t* costdists, int64_t* costblocks,
+                        int nthreads)
+{
+    beamsearch_impl(metric, X, n, dim, offs, adj, hints, nhints, qhints, qh_count, qh_stride, qvisit,
+                    Q, m, k, kq, kstride, bsize, bq, delta, maxvisits, beamcap, ids, dists,
+                    costdists, costblocks, nthreads, 1, krr);
 }
 
 /* allknn_single_search!(g::SearchGraph, ...) src/allknn.jl:93-106: each
@@ -428,7 +555,7 @@ void or_allknn_graph(int metric, const float* X, int64_t n, int dim, const int64
                      int64_t maxvisits, int32_t beamcap, uint32_t* ids, float* dists,
                      int nthreads)
 {
-    gidx_t g = {metric, X, n, dim, offs, adj};
+    gidx_t g = {metric, X, n, dim, offs, adj, 0};
     const int64_t nw = (n + 63) / 64 + 1;
     if (nthreads < 1) nthreads = 1;
 #ifdef _OPENMP
@@ -797,7 +924,7 @@ double or_optimize_minrecall(int metric, const float* X, int64_t n, int dim, con
                              int64_t cur_maxvisits, int32_t beamcap, int32_t* bsize_out,
                              float* delta_out, int64_t* maxvisits_out, int nthreads)
 {
-    gidx_t g = {metric, X, n, dim, offs, adj};
+    gidx_t g = {metric, X, n, dim, offs, adj, 0};
     return optimize_core(&g, hints, nhints, Q, m, ksearch, minrecall, cur_maxvisits, beamcap,
                          bsize_out, delta_out, maxvisits_out, nthreads);
 }
@@ -920,7 +1047,7 @@ static void run_callbacks(const or_build_params* p, int metric, const float* X, 
         float* Q = (float*)malloc(sizeof(float) * (size_t)m * (size_t)dim);
         for (int32_t i = 0; i < m; i++)
             memcpy(Q + (int64_t)i * dim, X + (smp[i] - 1) * dim, sizeof(float) * (size_t)dim);
-        gidx_t g = {metric, X, len, dim, c->offs, c->adj};
+        gidx_t g = {metric, X, len, dim, c->offs, c->adj, 0};
         int32_t bs;
         float dl;
         int64_t mv;
@@ -974,7 +1101,7 @@ int or_build_graph(int metric, const float* X, int64_t n, int dim, const or_buil
             const float* item = X + (sp - 1) * dim;
             if (len > 0) {
                 snapshot(L, len, &c);
-                gidx_t g = {metric, X, len, dim, c.offs, c.adj};
+                gidx_t g = {metric, X, len, dim, c.offs, c.adj, 0};
                 memset(vs, 0, sizeof(uint64_t) * (size_t)nw);
                 int64_t cd = 0, cb = 0;
                 beam_search(&g, item, &tmp, hints, nh, algo.bsize, algo.delta, algo.maxvisits,
@@ -1000,7 +1127,7 @@ int or_build_graph(int metric, const float* X, int64_t n, int dim, const or_buil
             const int64_t ep = (sp + PB - 1 < n) ? sp + PB - 1 : n;
             const int32_t ks = nb_size(p, ep);
             if (len > 0) snapshot(L, len, &c);
-            gidx_t g = {metric, X, len, dim, c.offs, c.adj};
+            gidx_t g = {metric, X, len, dim, c.offs, c.adj, 0};
             for (int64_t obj = sp; obj <= ep; obj++) {
                 knn_t tmp, nbr;
                 knn_init(&tmp, ti, td, ks);

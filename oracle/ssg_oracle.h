@@ -48,7 +48,7 @@ void or_parallel_exhaustive(int metric, const float* X, int64_t n, int dim, cons
 /* BeamSearch over a CSR graph.  offs has n+1 entries (exclusive prefix, offs[0]=0) and adj
  * holds 1-based neighbour ids (the reference's StaticAdjList end_point).  hints are 1-based.
  * qhints (nullable): per-query hint lists, query j uses qhints[j*qh_stride .. +qh_count[j]-1]
- * instead of the global hints.  kq / bq (nullable) give per-query k and bsize; else k / bsize.
+ * instead of the global hints (a negative qh_count[j] keeps the global hints).  kq / bq (nullable) give per-query k and bsize; else k / bsize.
  * The output column stride is kstride (>= every k). */
 void or_beamsearch(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
                    const uint32_t* adj, const uint32_t* hints, int64_t nhints,
@@ -58,6 +58,21 @@ void or_beamsearch(int metric, const float* X, int64_t n, int dim, const int64_t
                    float delta, int64_t maxvisits, int32_t beamcap, uint32_t* ids, float* dists,
                    int64_t* costdists,
                    int64_t* costblocks, int nthreads);
+/* The BF16 form of the same search (no reference counterpart): search distance
+ * or_dist_bf16, result queue of max(k, krr) entries, then rerank! with or_dist and the
+ * first k written. */
+void or_beamsearch_bf16(int metric, const float* X, int64_t n, int dim, const int64_t* offs,
+                        const uint32_t* adj, const uint32_t* hints, int64_t nhints,
+                        const uint32_t* qhints, const int32_t* qh_count, int32_t qh_stride,
+                        const uint32_t* qvisit, const float* Q, int64_t m, int32_t k,
+                        const int32_t* kq, int32_t kstride, int32_t bsize, const int32_t* bq,
+                        float delta, int64_t maxvisits, int32_t beamcap, int32_t krr,
+                        uint32_t* ids, float* dists, int64_t* costdists, int64_t* costblocks,
+                        int nthreads);
+/* BF16 rounding (nearest, ties to even) and the BF16 form's FP32 search distance of row x
+ * (rounded to BF16) to the FP32 query q */
+float or_bf16_round(float x);
+float or_dist_bf16(int metric, const float* x, const float* q, int dim);
 
 /* allknn over a SearchGraph (neighbours of i are the hints, one shared visited set) */
 void or_allknn_graph(int metric, const float* X, int64_t n, int dim, const int64_t* offs,

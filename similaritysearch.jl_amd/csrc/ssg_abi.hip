@@ -181,8 +181,9 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     if (bp->expand < 0 || bp->expand > 4) return ssg_fail(SSG_E_ARG, "expand must be in 0..4");
     if (bp->expand > 1 && bp->precision != SSG_BF16)
         return ssg_fail(SSG_E_ARG, "wide steps (expand > 1) belong to the BF16 form");
-    const int32_t krr = std::max(k, bp->rerank);
-    if (bp->precision == SSG_BF16 && (krr > 64 || kstride > 64))
+    // each query reranks max(its own k, rerank) entries; kstride bounds every k
+    const int32_t krr = std::max(kstride, bp->rerank);
+    if (bp->precision == SSG_BF16 && krr > 64)
         return ssg_fail(SSG_E_LIMIT, "BF16 rerank list is limited to 64");
     int rc = ensure_vis(h, s);
     if (rc) return rc;
@@ -230,7 +231,7 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     a.internal = internal ? 1 : 0;
     a.prec = bp->precision == SSG_BF16 ? 1 : 0;
     a.Xh = h->Xh.as<uint16_t>();
-    a.krr = krr;
+    a.krr = std::max(bp->rerank, 0);
     a.expand = bp->expand > 1 ? bp->expand : 1;
     a.order = qo ? qo->order : nullptr;
     if (use_q) {

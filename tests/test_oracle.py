@@ -202,6 +202,74 @@ def test_beam_small_n_is_brute_force(oracle):
     assert (cd == 40).all() and (cb == 0).all()
 
 
+# ---------------------------------------------------------------- BF16 form of the search
+@pytest.mark.parametrize("metric", ["SqL2", "L2", "NormCosine"])
+@pytest.mark.parametrize("dim", [8, 32, 96, 128])
+def test_dist_bf16_exact_on_integer_rows(oracle, metric, dim):
+    """Small integers are BF16 values and every FP32 step on them is exact, so the BF16
+    search distance equals the canonical one bit for bit."""
+    rng = np.random.default_rng(dim)
+    X = rng.integers(-7, 8, (50, dim)).astype(np.float32)
+    Q = rng.integers(-7, 8, (5, dim)).astype(np.float32)
+    for q in Q:
+        for x in X:
+            assert oracle.dist_bf16(metric, x, q) == oracle.dist(metric, x, q)
+
+
+@pytest.mark.parametrize("dim", [16, 96, 128, 256])
+def test_dist_bf16_within_fp32_bound_of_rounded_rows(oracle, dim):
+    """General rows: within the FP32 accumulation bound ((dim/16 + 4 + 3) units of 2^-24 of
+    the sum of squares) of the FP64 distance between the BF16-rounded row and the FP32
+    query; a restatement that skipped the BF16 rounding misses it by far."""
+    rng = np.random.default_rng(dim + 1)
+    X = rng.standard_normal((200, dim)).astype(np.float32)
+    q = rng.standard_normal(dim).astype(np.float32)
+    Xb = oracle.bf16_round(X)
+    u = 2.0 ** -24
+    worst_unrounded = 0.0
+    for x, xb in zip(X, Xb):
+        d = oracle.dist_bf16("SqL2", x, q)
+        ref = ((xb.astype(np.float64) - q) ** 2).sum()
+        assert abs(d - ref) <= (dim / 16 + 7) * u * ref
+        ref_unrounded = ((x.astype(np.float64) - q) ** 2).sum()
+        worst_unrounded = max(worst_unrounded, abs(d - ref_unrounded) / ((dim / 16 + 7) * u * ref))
+    assert worst_unrounded > 10
+
+
+def test_bf16_beamsearch_is_fp32_search_with_longer_queue_on_exact_rows(oracle):
+    """On integer rows the BF16 search distance is exact, so the BF16 form with a result
+    queue of max(k, rerank) entries is the FP32 search with k = rerank cut to its first
+    k: same ids, distances and cost counters (the rerank keeps the queue's order)."""
+    rng = np.random.default_rng(17)
+    X = rng.integers(0, 12, (1500, 8)).astype(np.float32)
+    Q = rng.integers(0, 12, (60, 8)).astype(np.float32)
+    g = oracle.build_graph("SqL2", X, parallel_block=32, seed=4)
+    args = ("SqL2", X, g["offs"], g["adj"], g["hints"], Q)
+    for k, rr, bs in ((10, 14, 8), (5, 5, 16), (3, 20, 4)):
+        bi, bd, bcd, bcb = oracle.beamsearch(*args, k, bs, 1.1, 10**6, bf16_rerank=rr)
+        fi, fd, fcd, fcb = oracle.beamsearch(*args, max(k, rr), bs, 1.1, 10**6)
+        np.testing.assert_array_equal(bi, fi[:, :k])
+        np.testing.assert_array_equal(bd, fd[:, :k])
+        np.testing.assert_array_equal(bcd, fcd)
+        np.testing.assert_array_equal(bcb, fcb)
+
+
+def test_bf16_beamsearch_returns_canonical_sorted_distances(oracle):
+    """General rows: every returned distance is the canonical distance of its id, each
+    column ascending, and the ids are distinct."""
+    rng = np.random.default_rng(18)
+    X = rng.random((1200, 16), dtype=np.float32)
+    Q = rng.random((40, 16), dtype=np.float32)
+    g = oracle.build_graph("SqL2", X, parallel_block=32, seed=6)
+    ids, d, cd, cb = oracle.beamsearch("SqL2", X, g["offs"], g["adj"], g["hints"], Q, 10, 8, 1.0,
+                                       10**6, bf16_rerank=16)
+    for j in range(len(Q)):
+        assert len(set(ids[j].tolist())) == 10
+        ex = [oracle.dist("SqL2", X[i - 1], Q[j]) for i in ids[j]]
+        np.testing.assert_array_equal(d[j], np.array(ex, np.float32))
+        assert np.all(np.diff(d[j]) >= 0)
+
+
 @pytest.fixture(scope="module")
 def built(oracle):
     rng = np.random.default_rng(11)
