@@ -201,6 +201,23 @@ int ssg_beamsearch_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k, int
                        const ssg_beam_params* bp, const ssg_query_opts* qo, uint32_t* dids,
                        float* ddists, int64_t* dcostdists, int64_t* dcostblocks, void* stream);
 
+/* The form of ssg_beamsearch_dev that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * ssg_beamsearch_dev; its visited slots, slot epochs and work counter live in a static
+ * 256 MiB device array that each GPU the library runs on holds until the process ends, so
+ * it serves databases of at most 67,076,096 objects (SSG_E_LIMIT beyond; fewer wave slots
+ * for larger databases: 255 at 1M objects against 32 per CU in ssg_beamsearch_dev).  It
+ * searches the query layout only if an earlier ssg_beamsearch_dev call built it (results
+ * do not depend on the layout), and the BF16 form needs the BF16 rows an earlier BF16
+ * ssg_beamsearch_dev call on the handle made (SSG_E_STATE otherwise).  The array keeps the
+ * last call's contents and each call clears what it uses first; calls whose kernels can
+ * run at the same time on one GPU (other streams, other handles) race on it. */
+int ssg_beamsearch_noalloc_dev(ssg_handle* h, const float* dQ, int64_t m, int32_t k,
+                               int32_t kstride, const ssg_beam_params* bp,
+                               const ssg_query_opts* qo, uint32_t* dids, float* ddists,
+                               int64_t* dcostdists, int64_t* dcostblocks, void* stream);
+
 /* HIP graphs (no reference counterpart): launches issued by ssg_*_dev calls on `stream`
  * between ssg_capture_begin and ssg_capture_end are recorded into a graph that
  * ssg_graph_launch replays on any stream.  Every buffer those calls use must already be

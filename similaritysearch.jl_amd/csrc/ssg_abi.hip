@@ -161,7 +161,7 @@ static int ensure_query_layout(ssg_handle* h, hipStream_t s, bool bf16)
 int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_t k,
                  int32_t kstride, const ssg_beam_params* bp, const ssg_query_opts* qo,
                  uint32_t* dids, float* ddists, int64_t* dcd, int64_t* dcb, hipStream_t s,
-                 bool allknn, int64_t n_eff, bool internal)
+                 bool allknn, int64_t n_eff, bool internal, const BeamScratch* ext)
 {
     if (!h->X.p) return ssg_fail(SSG_E_STATE, "no database uploaded");
     if (!h->has_graph) return ssg_fail(SSG_E_STATE, "no graph uploaded");
@@ -185,9 +185,16 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     const int32_t krr = std::max(kstride, bp->rerank);
     if (bp->precision == SSG_BF16 && krr > 64)
         return ssg_fail(SSG_E_LIMIT, "BF16 rerank list is limited to 64");
-    int rc = ensure_vis(h, s);
+    int rc = ext ? SSG_OK : ensure_vis(h, s);
     if (rc) return rc;
-    const bool use_q = h->reorder && !allknn && !internal && n_eff == h->n && h->n >= 64;
+    bool use_q = h->reorder && !allknn && !internal && n_eff == h->n && h->n >= 64;
+    if (ext) {
+        // nothing may be allocated: the query layout only if it is already there
+        use_q = use_q && h->q_ready && (bp->precision != SSG_BF16 || h->q_bf16);
+        if (bp->precision == SSG_BF16 && !use_q && !h->Xh.p)
+            return ssg_fail(SSG_E_STATE, "the no-allocation form needs the BF16 rows that an "
+                                         "earlier BF16 ssg_beamsearch_dev call on this handle makes");
+    }
     if (bp->precision == SSG_BF16 && !use_q && !h->Xh.p) {
         HIPCHK(h->Xh.ensure((size_t)std::max<int64_t>(h->n_pad, 64) * h->dp * 2), "alloc BF16 copy");
         HIPCHK(launch_to_bf16(h->X.as<float>(), std::max<int64_t>(h->n_pad, 64) * h->dp,
@@ -246,7 +253,19 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
         a.perm = h->qperm.as<uint32_t>();
         a.ipos = h->qipos.as<uint32_t>();
     }
-    const int slots = (int)std::min<int64_t>(h->nslots, (m + 3) / 4 * 4);
+    int slots = (int)std::min<int64_t>(h->nslots, (m + 3) / 4 * 4);
+    if (ext) {
+        // the scratch keeps whatever the last call on this GPU left: clear the slots used
+        slots = (int)std::min<int64_t>(ext->nslots, (m + 3) / 4 * 4);
+        a.vis = ext->vis;
+        a.epochs = ext->epochs;
+        a.counter = ext->counter;
+        a.vis_stride = ext->vis_stride;
+        HIPCHK(launch_zero_u32(reinterpret_cast<unsigned int*>(ext->vis),
+                               (int64_t)slots * ext->vis_stride / 4, s), "clear visited slots");
+        HIPCHK(launch_zero_u32(reinterpret_cast<unsigned int*>(ext->epochs), ((int64_t)slots + 3) / 4, s),
+               "clear epochs");
+    }
     HIPCHK(launch_zero_u32(a.counter, 1, s), "zero counter");
     HIPCHK(launch_beam(a, slots, s), "beam kernel launch");
     return SSG_OK;
@@ -694,6 +713,11 @@ int ssg_create(ssg_handle** out, int device)
         return ssg_hipfail(e, "hipGetDeviceProperties");
     }
     h->num_cu = prop.multiProcessorCount;
+    const int rc = ssg_noalloc_init(device);
+    if (rc) {
+        delete h;
+        return rc;
+    }
     e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete h;

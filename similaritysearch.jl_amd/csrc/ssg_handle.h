@@ -123,13 +123,32 @@ int ssg_hipfail(hipError_t e, const char* what);
         if (_e != hipSuccess) return ssg_hipfail(_e, what);     \
     } while (0)
 
+// Visited slots, their epochs and the work counter of a beam-search launch held in
+// memory the caller provides (the no-allocation entry point): nslots slots of
+// vis_stride bytes at vis, nslots epoch bytes and a 4-byte counter, all cleared by the
+// launch itself before the search.
+struct BeamScratch {
+    uint8_t* vis;
+    uint8_t* epochs;
+    unsigned int* counter;
+    int64_t vis_stride;
+    int nslots;
+};
+
 // Beam search over the handle's graph with the first n_eff objects as the index
 // (n_eff = h->n outside graph construction).  Queries are rows of dQ with qdim
-// floats each.
+// floats each.  With `ext` the launch uses that scratch and allocates nothing: it
+// searches the query layout only if it is already built (the results do not depend on
+// the layout) and the BF16 form needs BF16 rows made by an earlier BF16 search.
 int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_t k,
                  int32_t kstride, const ssg_beam_params* bp, const ssg_query_opts* qo,
                  uint32_t* dids, float* ddists, int64_t* dcd, int64_t* dcb, hipStream_t s,
-                 bool allknn, int64_t n_eff, bool internal = false);
+                 bool allknn, int64_t n_eff, bool internal = false,
+                 const BeamScratch* ext = nullptr);
+
+// Resolves the device address of the no-allocation scratch on `device` (current); called
+// once per device by ssg_create.
+int ssg_noalloc_init(int device);
 
 // Exhaustive k-NN over the first n_eff objects.  Either dQ (m rows of h->dim
 // floats) or a prepared (qpad, qnorm) pair with m_pad = roundup(m, 64) rows of dp

@@ -88,6 +88,8 @@ SIGNATURES = {
                                  C.POINTER(QueryOpts), _p, _p, _p, _p]),
     "ssg_beamsearch_dev": (C.c_int, [_p, _p, _i64, _i32, _i32, C.POINTER(BeamParams),
                                      C.POINTER(QueryOpts), _p, _p, _p, _p, _p]),
+    "ssg_beamsearch_noalloc_dev": (C.c_int, [_p, _p, _i64, _i32, _i32, C.POINTER(BeamParams),
+                                             C.POINTER(QueryOpts), _p, _p, _p, _p, _p]),
     "ssg_allknn_graph": (C.c_int, [_p, _i32, C.POINTER(BeamParams), _p, _p]),
     "ssg_allknn_exhaustive": (C.c_int, [_p, _i32, _p, _p]),
     "ssg_order_by_bsize_dev": (C.c_int, [_p, _p, _i64, _p, _p]),
