@@ -17,7 +17,10 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.environ.get("SSG_ORACLE_LIB", os.path.join(_HERE, "_build", "libssg_oracle.so"))
-_lib = None
+# the same restatement built with the reference's FP32 SIMD distance, for timing the CPU
+# baseline only (never a checker: its rounding differs from the canonical distance)
+_BASE_PATH = os.path.join(_HERE, "_build", "libssg_cpubase.so")
+_libs = {}
 
 SQL2, L2, NORMCOS = 0, 1, 2
 METRICS = {"sqL2": SQL2, "SqL2": SQL2, "L2": L2, "NormCosine": NORMCOS, "normcos": NORMCOS}
@@ -46,12 +49,12 @@ class Graph(C.Structure):
                 ("bsize", C.c_int32), ("delta", C.c_float), ("maxvisits", C.c_int64)]
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(baseline=False):
+    path = _BASE_PATH if baseline else _LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(path)
         L.or_dist.restype = C.c_float
         L.or_dist.argtypes = [C.c_int, _p, _p, C.c_int]
         L.or_dist_f64.restype = C.c_double
@@ -91,8 +94,8 @@ def lib():
         L.or_optimize_minrecall.argtypes = [C.c_int, _p, _i64, C.c_int, _p, _p, _p, _i64, _p,
                                             _i64, _i32, C.c_float, _i64, _i32, _p, _p, _p,
                                             C.c_int]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _ptr(a):
@@ -136,12 +139,14 @@ def knn_run(cap, ops, ids, dists):
     return oi[:ln[0]].copy(), od[:ln[0]].copy(), acc.astype(bool)
 
 
-def exhaustive(metric, X, Q, k, nthreads=1):
+def exhaustive(metric, X, Q, k, nthreads=1, baseline=False):
+    """Restated ExhaustiveSearch over every query.  baseline=True runs the CPU-baseline
+    build (the reference's FP32 SIMD distance; timing only)."""
     X, Q = _f32c(X), _f32c(Q)
     m = Q.shape[0]
     ids = np.zeros((m, k), np.uint32)
     d = np.full((m, k), np.inf, np.float32)
-    lib().or_exhaustive(METRICS.get(metric, metric), _ptr(X), X.shape[0], X.shape[1], _ptr(Q), m,
+    lib(baseline).or_exhaustive(METRICS.get(metric, metric), _ptr(X), X.shape[0], X.shape[1], _ptr(Q), m,
                         k, _ptr(ids), _ptr(d), nthreads)
     return ids, d
 
@@ -157,12 +162,14 @@ def parallel_exhaustive(metric, X, Q, k, nbatches):
 
 
 def beamsearch(metric, X, offs, adj, hints, Q, k, bsize, delta, maxvisits, beamcap=32, kq=None,
-               bq=None, qhints=None, qh_count=None, nthreads=1, qvisit=None, bf16_rerank=None):
+               bq=None, qhints=None, qh_count=None, nthreads=1, qvisit=None, bf16_rerank=None,
+               baseline=False):
     """Restated searchbatch over a SearchGraph.  Returns ids, dists, costdists, costblocks.
     qvisit: per query a 1-based object marked visited before its search (0 = none).
     bf16_rerank (an int): the BF16 form instead (no reference counterpart): the search runs
     on dist_bf16 with a result queue of max(k, bf16_rerank) entries, reranked with the
-    canonical distance before the first k are written."""
+    canonical distance before the first k are written.  baseline=True runs the CPU-baseline
+    build (the reference's FP32 SIMD distance; timing only)."""
     X, Q = _f32c(X), _f32c(Q)
     offs = np.ascontiguousarray(offs, np.int64)
     adj = np.ascontiguousarray(adj, np.uint32)
@@ -183,7 +190,7 @@ def beamsearch(metric, X, offs, adj, hints, Q, k, bsize, delta, maxvisits, beamc
               0 if qh is None else qh.shape[1], _ptr(qv), _ptr(Q), m, int(k), _ptr(kq_a), kstride,
               int(bsize), _ptr(bq_a), float(delta), int(maxvisits), int(beamcap))
     if bf16_rerank is None:
-        lib().or_beamsearch(*common, _ptr(ids), _ptr(d), _ptr(cd), _ptr(cb), nthreads)
+        lib(baseline).or_beamsearch(*common, _ptr(ids), _ptr(d), _ptr(cd), _ptr(cb), nthreads)
     else:
         lib().or_beamsearch_bf16(*common, int(bf16_rerank), _ptr(ids), _ptr(d), _ptr(cd),
                                  _ptr(cb), nthreads)
@@ -528,18 +535,19 @@ def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True, nthreads=1):
     return np.asarray(M, np.uint32), L, D, np.asarray(net, np.uint32), cost
 
 
-def hsp_queries(metric, X, Q, ids, dists, nthreads=1):
+def hsp_queries(metric, X, Q, ids, dists, nthreads=1, baseline=False):
     """hsp_queries (src/hsp.jl:8-33, 77-101) over (m, k) candidate rows (0 ends a row): a
     candidate is kept unless an already kept one is strictly nearer to it than its given
     distance (canonical distances); the kept enter a KnnSorted of capacity k.  Returns
     (m, k) kept ids and distances, then 0 / Inf (or_hsp_queries in ssg_oracle.c).  Pinned
-    by the keep/drop invariants in tests/test_oracle.py."""
+    by the keep/drop invariants in tests/test_oracle.py.  baseline=True runs the CPU-baseline
+    build (the reference's FP32 SIMD distance; timing only)."""
     X = _f32c(X)
     ids = np.ascontiguousarray(ids, np.uint32)
     dd = np.ascontiguousarray(dists, np.float32)
     m, k = ids.shape
     oi = np.zeros((m, k), np.uint32)
     od = np.full((m, k), np.inf, np.float32)
-    lib().or_hsp_queries(METRICS.get(metric, metric), _ptr(X), X.shape[1], m, k, _ptr(ids), _ptr(dd),
+    lib(baseline).or_hsp_queries(METRICS.get(metric, metric), _ptr(X), X.shape[1], m, k, _ptr(ids), _ptr(dd),
                          _ptr(oi), _ptr(od), nthreads)
     return oi, od

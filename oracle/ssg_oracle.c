@@ -65,10 +65,43 @@ static inline float finish(int metric, double s)
     return (float)s;
 }
 
+#ifndef SSG_CPU_BASELINE
 float or_dist(int metric, const float* a, const float* b, int dim)
 {
     return finish(metric, tree16(metric, a, b, dim));
 }
+#else
+/* CPU-baseline build (libssg_cpubase.so, timing only, never a checker): the reference's own
+ * arithmetic, FP32 accumulation reassociated into 16 SIMD lanes as @fastmath @simd does
+ * (src/dist/vecs.jl:52-67,93-102, src/dist/cos.jl:78-85), vectorised for the host's widest
+ * unit.  Its rounding differs from the canonical distance. */
+__attribute__((target_clones("avx512f", "avx2", "default")))
+float or_dist(int metric, const float* a, const float* b, int dim)
+{
+    float acc[16] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int i = 0;
+    if (metric == OR_NORMCOS) {
+        for (; i + 16 <= dim; i += 16)
+            for (int j = 0; j < 16; j++) acc[j] += a[i + j] * b[i + j];
+        for (; i < dim; i++) acc[0] += a[i] * b[i];
+    } else {
+        for (; i + 16 <= dim; i += 16)
+            for (int j = 0; j < 16; j++) {
+                const float m = a[i + j] - b[i + j];
+                acc[j] += m * m;
+            }
+        for (; i < dim; i++) {
+            const float m = a[i] - b[i];
+            acc[0] += m * m;
+        }
+    }
+    float s = 0.f;
+    for (int j = 0; j < 16; j++) s += acc[j];
+    if (metric == OR_L2) return sqrtf(s);
+    if (metric == OR_NORMCOS) return 1.0f - s;
+    return s;
+}
+#endif
 
 /* ------------------------------------------------------------------------ */
 /* Search distance of the BF16 form (no reference counterpart): the object  */
