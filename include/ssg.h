@@ -103,8 +103,10 @@ int ssg_append_db(ssg_handle* h, const float* X, int64_t m);
  * centers join the net.  Outputs: map (n entries of room; *nmap are filled: net position
  * -> 1-based object id, the centers in construction order), nn (n: each object's
  * covering object, itself for a center), dist (n: the distance to it), *costdists (the
- * block filters' distance evaluations; the exact searches count none).  blocksize <= 1024
- * when filtering. */
+ * distance evaluations: each block searched against a non-empty net charges the net's
+ * size per object, as the exhaustive search does, plus the block filters' evaluations).
+ * An object with no finite nearest center (NaN distances) becomes a center.  blocksize
+ * <= 1024 when filtering. */
 int ssg_neardup(ssg_handle* h, float eps, int32_t blocksize, int32_t filterblocks, uint32_t* map,
                 int64_t* nmap, int32_t* nn, float* dist, int64_t* costdists);
 int ssg_upload_graph(ssg_handle* h, const int64_t* offset, int64_t n, const uint32_t* end_point,

@@ -521,12 +521,13 @@ def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True, nthreads=1):
         if not net:
             new = block(rng)
         else:
-            # searches of the exact index count no distance evaluations (only the
-            # block filter does: src/neardup.jl:240)
+            # searchbatch! of the block against the net (src/neardup.jl:121-127); each exact
+            # search charges the net's length (src/exact/sequential-exhaustive.jl:78)
+            cost += len(rng) * len(net)
             ids, d = exhaustive(metric, X[np.asarray(net) - 1], X[r0:r0 + len(rng)], 1, nthreads)
             imap = []
             for t, j in enumerate(rng):
-                if d[t, 0] > eps:
+                if ids[t, 0] == 0 or d[t, 0] > eps:
                     imap.append(j)
                 else:
                     D[j - 1] = d[t, 0]; L[j - 1] = M[int(ids[t, 0]) - 1]

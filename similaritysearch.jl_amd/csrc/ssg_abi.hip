@@ -954,6 +954,9 @@ int ssg_neardup(ssg_handle* h, float eps, int32_t blocksize, int32_t filterblock
             if (g->n == 0) {
                 for (int32_t t = 0; t < mb; t++) imap.push_back((uint32_t)(r0 + t));
             } else {
+                // searchbatch! of the block against the net (neardup.jl:121-127): each
+                // exhaustive search charges the net's length (sequential-exhaustive.jl:78)
+                cost += (int64_t)mb * g->n;
                 const int32_t nf = (int32_t)F.size();
                 if (nf > 0) {
                     for (int32_t t = 0; t < mb; t++) blk[t] = (uint32_t)(r0 + t);
@@ -975,7 +978,7 @@ int ssg_neardup(ssg_handle* h, float eps, int32_t blocksize, int32_t filterblock
                         pos = net0 + hfp[t];
                         d = hfd[t];
                     }
-                    if (d > eps) {
+                    if (pos < 1 || d > eps) {  // no nearest center (NaN or +Inf distances): a center
                         imap.push_back((uint32_t)j);
                     } else {
                         dist[j] = d;

@@ -497,7 +497,8 @@ int ssg_build_graph(ssg_handle* h, const ssg_build_params* p)
         if (p->verbose > 0) {
             const int64_t step = std::max<int64_t>(1, n * p->verbose / 100);
             if (ep / step != len / step || ep == n) {
-                fprintf(stderr, "index!: %lld / %lld inserted, nnz %lld, algo bsize=%d delta=%.3f maxvisits=%lld\n",
+                fprintf(stderr, "index!: %lld / %lld inserted, adjacency pool %lld slots (lists and their "
+                        "free room), algo bsize=%d delta=%.3f maxvisits=%lld\n",
                         (long long)ep, (long long)n, (long long)h->pool_used, h->algo.bsize,
                         h->algo.delta, (long long)h->algo.maxvisits);
                 fflush(stderr);

@@ -87,6 +87,23 @@ class MatrixDatabase:
         """Object i (1-based, like the reference)."""
         return self.rows[i - 1]
 
+    def __setitem__(self, i, v):
+        """setindex! of object i (1-based); bumps the version that device copies check."""
+        self.rows[i - 1] = np.asarray(v, np.float32)
+        self.touch()
+
+    def touch(self):
+        """Marks the rows as changed, so cached device copies (rerank_, hsp_queries) are
+        uploaded again; call it after writing into `rows` directly."""
+        self._version = getattr(self, "_version", 0) + 1
+
+
+def _db_key(db):
+    """What a cached device copy of db depends on: the object, its length, where its rows
+    live and its version (bumped by setindex!, push_item!, append_items! and touch())."""
+    rows = db.rows
+    return (id(db), len(db), rows.ctypes.data if len(db) else 0, getattr(db, "_version", 0))
+
 
 def _as_db(Q):
     return Q if isinstance(Q, MatrixDatabase) else MatrixDatabase(Q)
@@ -505,7 +522,7 @@ _rerank_cache = {}
 def rerank_(dist, db, Q, ids, dists):
     """rerank!(dist, db, queries, ids, dists) batch form, in place (src/rerank.jl:77-86)."""
     Q = _as_db(Q)
-    key = (id(db), _metric(dist))
+    key = (_db_key(db), _metric(dist))
     hit = _rerank_cache.get(key)
     if hit is not None and hit[0] is db:   # the entry holds db, so its id cannot be reused
         idx = hit[1]

@@ -12,7 +12,7 @@ and only the reference's sequential bookkeeping runs on the host.
 import numpy as np
 
 from . import (ExhaustiveSearch, GenericContext, MatrixDatabase, SearchGraph, _as_db, _lib,
-               _metric, rerank_, searchbatch, searchbatch_knns)
+               _db_key, _metric, rerank_, searchbatch, searchbatch_knns)
 
 __all__ = ["closestpair", "bichromatic_closestpair", "closestpair_hints", "neardup", "hsp_queries",
            "pairwise"]
@@ -173,7 +173,7 @@ def hsp_queries(dist, X, Q, knns_ids, knns_dists):
     n, k = ids.shape
     if n != len(_as_db(Q)):
         raise ValueError("the number of queries is different from the candidate columns")
-    key = (id(X), _metric(dist))
+    key = (_db_key(X), _metric(dist))
     E = _hsp_cache.get(key)
     if E is None or E.db is not X:         # E holds X, so its id cannot be reused
         E = ExhaustiveSearch(dist, X)

@@ -151,6 +151,7 @@ class MMapMatrixDatabase(MatrixDatabase):
         if not 1 <= i <= self._n:
             raise IndexError(i)
         self._data[i - 1] = value
+        self.touch()
 
     def push_item(self, v):
         """push_item!: write v, flush it, then persist the new length (:247-256)."""
@@ -162,6 +163,7 @@ class MMapMatrixDatabase(MatrixDatabase):
         self._data.flush()
         self._n = n
         self._persist_n(n)
+        self.touch()
         return self
 
     def append_items(self, B):
@@ -185,6 +187,7 @@ class MMapMatrixDatabase(MatrixDatabase):
             self._data.flush()
             self._n = n
             self._persist_n(n)
+            self.touch()
         return self
 
     def close(self):

@@ -148,6 +148,13 @@ def test_neardup_all_duplicates_and_single_block(oracle):
     assert list(centers) == [1, 2, 3, 4, 5]
     R = ss.neardup(ss.SqL2(), _db(X[:50]), 1e9, blocksize=256)
     assert list(R["centers"]) == [1] and (R["nn"] == 1).all()
+    # eps = +Inf over several blocks: one center, every later block is covered by the net
+    # search, and each searched block charges the net's size per object
+    R = ss.neardup(ss.SqL2(), _db(X), np.inf, blocksize=64)
+    M, L, D, centers, cost = oracle.neardup("SqL2", X, np.inf, blocksize=64)
+    assert list(R["centers"]) == [1] and (R["nn"] == 1).all()
+    np.testing.assert_array_equal(R["dist"], D)
+    assert R["costdists"] == cost == 63 + (200 - 64) * 1
 
 
 @pytest.mark.gpu

@@ -592,6 +592,20 @@ def test_neardup_one_block_is_the_greedy_net(oracle, metric, eps):
     assert 1 < len(c) < 400
 
 
+def test_neardup_cost_counts_net_searches_hand_case(oracle):
+    """Blocks of 2 on a line (rows v * ones(4), SqL2, eps 1e-3): block 1 {0, 10} makes two
+    centers and its filter evaluates object 2 against center 1 (1); block 2 {0.01, 20}
+    searches the 2-center net (2 x 2 = 4): 0.01 is covered by 0, 20 becomes a center with no
+    filter evaluation; block 3 {10.01, 20.01} searches the 3-center net (2 x 3 = 6), both
+    covered.  costdists = 1 + 4 + 6 = 11 (sequential-exhaustive.jl:78, neardup.jl:240)."""
+    v = np.array([0.0, 10.0, 0.01, 20.0, 10.01, 20.01], np.float32)
+    X = (v[:, None] * np.ones((1, 4), np.float32)).astype(np.float32)
+    M, L, D, cen, cost = oracle.neardup("SqL2", X, 1e-3, blocksize=2)
+    assert cost == 11
+    np.testing.assert_array_equal(cen, [1, 2, 4])
+    np.testing.assert_array_equal(L, [1, 2, 1, 4, 2, 4])
+
+
 def test_neardup_blocked_net_invariants(oracle):
     """Blocked runs: the centers are the objects that cover themselves, pairwise farther
     than eps; every other object lies within eps of its center, at the recorded distance;
