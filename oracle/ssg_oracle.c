@@ -848,8 +848,15 @@ int64_t or_random_hints(int64_t n, const int64_t* offs, const uint32_t* adj, flo
 /* SearchModels.jl 0.5 (evolutionary, not vendored here); this restatement   */
 /* scores every point of the BeamSearchSpace grid (bsize 2:2:16, Δ           */
 /* 0.9:0.025:1.1) with the same cost function and keeps the first minimum.   */
-/* Per-query costs stand for the per-batch counters (one query per batch).  */
-/* parity unpinned: SearchModels' population dynamics are not reproduced.    */
+/* Per-query costs stand for the per-batch counters (one query per batch,   */
+/* which the reference's getminbatch gives when 4 x nthreads >= queries).    */
+/* Pinned by: an independent evaluation of every grid point with the pinned  */
+/* beam search and exhaustive gold; the choice is the least mean cost among  */
+/* points reaching r, else the highest recall, with maxvisits = ceil(2 x max) */
+/* (tests/test_oracle.py, checked to catch a cost on the maximum, a missing  */
+/* factor 2 and an unscaled runconfig maxvisits).  Which grid points the     */
+/* reference's evolutionary search visits (random, SearchModels.jl 0.5.0,    */
+/* not vendored) stays parity unpinned: it can stop at another point.        */
 /* ------------------------------------------------------------------------ */
 typedef struct {
     double recall, vmean;
@@ -969,10 +976,15 @@ double or_optimize_minrecall(int metric, const float* X, int64_t n, int dim, con
 /* never fires), connect_reverse_links! (neighborhood.jl:52-89; the          */
 /* reference appends from parallel tasks in unspecified order, here in       */
 /* ascending node order) and execute_callbacks! (callbacks.jl:20-25).        */
-/* Pinned by: structural invariants (every list sorted by insertion, HSP     */
-/* property of every forward list, symmetric reverse links) and n < 64       */
-/* closed cases.  The callbacks' random draws (RandomHints, the query sample */
-/* of optimize_index!) and the SearchModels optimiser are parity unpinned.   */
+/* Pinned by: the first 64 objects (inserted while the index holds fewer   */
+/* than 64, so their candidates are the exact k-NN of the earlier objects):  */
+/* each list starts with a brute-force HSP filter of its ksearch nearest     */
+/* earlier objects and continues with reverse links from later objects only */
+/* (tests/test_oracle.py, on tied, long-list and all-kept data: a <= in the  */
+/* filter or a neighbourhood size off by one fails); symmetric lists without */
+/* self loops; the reference's recall floor.  The callbacks' random draws    */
+/* (RandomHints, optimize_index!'s query sample) are parity unpinned: a      */
+/* splitmix64 stream replaces Julia's Xoshiro.                               */
 /* ------------------------------------------------------------------------ */
 static int32_t nb_size(const or_build_params* p, int64_t n)
 {

@@ -310,6 +310,98 @@ def test_builder_hints_are_well_connected(built):
     assert len(set(g["hints"].tolist())) == len(g["hints"])
 
 
+def _hsp_forward(D, i, ksearch):
+    """find_neighborhood! of object i (0-based) when every earlier object is a candidate
+    (the n < 64 brute-force branch plus the interblock pushes, neighborhood.jl:25-45): the
+    ksearch nearest earlier objects by (distance, id), then the proximal HSP filter
+    (hsp.jl:103-119 with hsp_should_push's strict <, hsp.jl:8-33)."""
+    prev = np.arange(i)
+    order = prev[np.lexsort((prev, D[i, prev]))][:ksearch]
+    kept = []
+    for p in order:
+        if all(D[p, h] >= D[i, p] for h in kept):
+            kept.append(p)
+    return [int(p) + 1 for p in kept]
+
+
+@pytest.mark.parametrize("data", ["grid", "dim32", "axes"])
+def test_builder_forward_lists_are_hsp_of_exact_candidates(oracle, data):
+    """Objects 1..64 (blocks of 32) are inserted while the index holds fewer than 64
+    objects, so their candidate lists are exact; each final list must start with the HSP
+    filter of the ksearch = ceil(2 + log_1.3(block end)) nearest earlier objects
+    (neighborhood.jl:5, insertions.jl:61), and carry every reverse link from a later
+    object among the first 64 whose forward list names it.  Integer grid rows make exact
+    distance ties (the filter's strict <, ties to the lower id); 32-dim rows keep long
+    lists; unit basis vectors are all equally far apart, so the filter keeps every
+    candidate and each list is exactly the neighbourhood size."""
+    rng = np.random.default_rng(43)
+    if data == "grid":
+        X = rng.integers(0, 4, (300, 3)).astype(np.float32)
+    elif data == "dim32":
+        X = rng.random((300, 32), dtype=np.float32)
+    else:
+        X = np.concatenate([np.eye(64, dtype=np.float32), rng.random((236, 64), dtype=np.float32)])
+    g = oracle.build_graph("SqL2", X, parallel_block=32, seed=5)
+    n0 = 64
+    D = oracle.dist_batch("SqL2", X[:n0], X[:n0]).reshape(n0, n0)
+    fwd = {}
+    for i in range(n0):
+        ep = (i // 32 + 1) * 32
+        ks = int(np.ceil(np.float32(2) + np.log(np.float32(ep)) / np.log(np.float32(1.3))))
+        fwd[i] = _hsp_forward(D, i, ks)
+    offs, adj = g["offs"], g["adj"]
+    for i in range(n0):
+        L = adj[offs[i]:offs[i + 1]].tolist()
+        assert L[:len(fwd[i])] == fwd[i], i
+        # the rest are reverse links, which only later objects make (connect_reverse_links!,
+        # neighborhood.jl:52-89): among the first 64, exactly those naming i
+        tail = L[len(fwd[i]):]
+        assert all(x > i + 1 for x in tail), i
+        assert {x for x in tail if x <= n0} == {j + 1 for j in range(n0) if (i + 1) in fwd[j]}, i
+    assert sum(len(f) for f in fwd.values()) > 64
+
+
+def test_optimizer_picks_the_minrecall_optimum_of_the_grid(oracle, built):
+    """or_optimize_minrecall against an independent evaluation of every grid point
+    (bsize 2:2:16 x delta 0.9:0.025:1.1, optbs.jl:28-33) with the pinned beam search and
+    exhaustive gold: among points with recall >= r the least mean distance evaluations
+    (first in grid order on ties), else the highest recall (opt.jl:281), and
+    maxvisits = ceil(2 * visited.max) of the chosen point (optbs.jl:141-145), where the
+    candidates ran with twice the current maxvisits (optbs.jl:153-156).  The visited
+    statistics are per query: the reference's are per batch slot, which is per query when
+    4 x nthreads >= the 64 sample queries."""
+    X, g = built
+    Q = np.random.default_rng(41).random((64, 4), dtype=np.float32)
+    gold, _ = oracle.exhaustive("SqL2", X, Q, 10)
+    for cur in (3000, 25):  # the second cuts searches at 2 * 25 evaluations
+        _check_optimizer_grid(oracle, X, g, Q, gold, cur)
+
+
+def _check_optimizer_grid(oracle, X, g, Q, gold, cur):
+    pts = []
+    for b in range(2, 17, 2):
+        for di in range(9):
+            d = np.float32((900.0 + 25.0 * di) / 1000.0)
+            ids, _, cd, _ = oracle.beamsearch("SqL2", X, g["offs"], g["adj"], g["hints"], Q, 10, b,
+                                              d, 2 * cur)
+            rec = np.mean([len(set(gold[j]) & set(ids[j][ids[j] != 0])) / 10 for j in range(64)])
+            full = int(((ids != 0).sum(1) == 10).sum())
+            act = cd[cd > 0]
+            pts.append(dict(b=b, d=d, rec=rec, valid=full > 3, vmean=act.mean(), vmax=int(act.max())))
+    for r in (0.9, 0.99, 1.01):
+        o = oracle.optimize_minrecall("SqL2", X, g["offs"], g["adj"], g["hints"], Q, 10, r,
+                                      cur_maxvisits=cur)
+        valid = [p for p in pts if p["valid"]]
+        ok = [p for p in valid if p["rec"] >= np.float32(r)]
+        if ok:
+            best = min(ok, key=lambda p: p["vmean"])   # min keeps the first in grid order
+        else:
+            best = max(valid, key=lambda p: p["rec"])  # max keeps the first in grid order
+        assert (o["bsize"], np.float32(o["delta"])) == (best["b"], best["d"]), r
+        assert o["maxvisits"] == int(np.ceil(2.0 * best["vmax"]))
+        assert o["recall"] == pytest.approx(best["rec"])
+
+
 # ---------------------------------------------------------------- fft, rerank, recall
 def test_fft_invariants(oracle):
     rng = np.random.default_rng(13)
