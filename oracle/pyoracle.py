@@ -197,6 +197,20 @@ def beamsearch(metric, X, offs, adj, hints, Q, k, bsize, delta, maxvisits, beamc
     return ids, d, cd, cb
 
 
+def unit_rows(R):
+    """Cosine (src/dist/cos.jl:103) as searched: each row divided by its FP64 Euclidean
+    norm and rounded to Float32 once (zero rows stay zero); the canonical NormCosine
+    distance of such rows is the cosine distance.  Pinned in tests/test_oracle.py by the
+    closed forms of test/testdistances.jl:61-76 and an FP64 bound."""
+    R = np.asarray(R, np.float64)
+    out = np.zeros(R.shape, np.float32)
+    for i, r in enumerate(R):
+        nr = np.sqrt(np.dot(r, r))
+        if nr > 0:
+            out[i] = (r / nr).astype(np.float32)
+    return out
+
+
 def dist_bf16(metric, x, q):
     """The BF16 form's search distance: row x rounded to BF16, FP32 query and arithmetic
     in the canonical run/tree order (oracle/ssg_oracle.c, or_dist_bf16)."""

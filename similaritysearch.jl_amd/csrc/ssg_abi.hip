@@ -278,7 +278,7 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
     if (!h->X.p) return ssg_fail(SSG_E_STATE, "no database uploaded");
     if (m <= 0) return ssg_fail(SSG_E_ARG, "empty set of queries");
     if (k < 1 || kstride < k) return ssg_fail(SSG_E_ARG, "k must satisfy 1 <= k <= kstride");
-    if (k > EXH_KC) return ssg_fail(SSG_E_LIMIT, "exhaustive k is limited to 32");
+    if (k > EXH_WIDE_KMAX) return ssg_fail(SSG_E_LIMIT, "exhaustive k is limited to 3840");
     if (kstride > k || n_eff == 0) {
         HIPCHK(launch_fill_f32(ddists, m * kstride, __builtin_inff(), s), "fill");
         HIPCHK(launch_zero_u32(dids, m * kstride, s), "fill");
@@ -298,6 +298,12 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
         HIPCHK(launch_fill_f32(w.qpad + m * h->dp, (m_pad - m) * h->dp, 0.f, s), "fill qpad");
         HIPCHK(launch_fill_f32(w.qnorm + m, m_pad - m, 0.f, s), "fill qnorm");
         HIPCHK(launch_pad_rows(dQ, m, h->dim, w.qpad, h->dp, w.qnorm, s), "pad queries");
+    }
+    if (k > EXH_KC) {
+        // more than stage 1 keeps per slice: the one-block-per-query scan
+        HIPCHK(launch_exh_wide(h->X.as<float>(), n_eff, h->dp, h->metric, w.qpad, m, k, dids, ddists,
+                               kstride, s), "exhaustive k > 32");
+        return SSG_OK;
     }
     if (h->dp <= 128) {
         HIPCHK(h->xList.ensure((size_t)m_pad * ns * 512 * sizeof(uint2)), "alloc candidate lists");

@@ -943,6 +943,109 @@ hipError_t exh_launch_rescan(const ExhArgs& a, const ExhWork& w, int f0, int nf,
     return hipGetLastError();
 }
 
+// Exact exhaustive k-NN for k > EXH_KC (more than stage 1 keeps per slice): one block per
+// query scans the rows in ascending id order with the canonical distance (16 lanes per
+// row, 16 rows per round, 256 rows per tile).  Keys (d, id) below the current k-th best
+// are appended to a pending run in LDS; when the run may overflow, list and run are merged
+// by a bitonic sort of the whole buffer and the first k kept.  The result is the first k
+// in (d, id) order, which is what the ascending scan with KnnSorted pushes leaves.
+// Dynamic LDS: P2 keys (a power of two >= k + 256): [0, k) the sorted list, [k, P2) the
+// pending run.
+__device__ __forceinline__ void bitonic_sort_lds(uint64_t* B, int P2)
+{
+    for (int size = 2; size <= P2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < (P2 >> 1); i += blockDim.x) {
+                const int pos = 2 * i - (i & (stride - 1));
+                const int partner = pos + stride;
+                const bool up = (pos & size) == 0;
+                const uint64_t a = B[pos], b = B[partner];
+                if ((a > b) == up) {
+                    B[pos] = b;
+                    B[partner] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int V>
+__global__ __launch_bounds__(256) void exh_wide_kernel(const float* __restrict__ X, int64_t n,
+                                                       int metric, const float* __restrict__ Qp,
+                                                       int k, int P2, uint32_t* ids, float* dists,
+                                                       int kstride)
+{
+    extern __shared__ uint64_t B[];
+    __shared__ int s_pend, s_len;
+    __shared__ uint64_t s_thr;
+    const int tid = threadIdx.x, grp = tid >> 4, j = tid & 15;
+    const int64_t q = blockIdx.x;
+    const int dp = 16 * V;
+    float qv[V];
+    load_run<V>(Qp + (size_t)q * dp + j * V, qv);
+    if (tid == 0) {
+        s_pend = 0;
+        s_len = 0;
+        s_thr = ~0ull;
+    }
+    __syncthreads();
+    const int pend_cap = P2 - k;
+    for (int64_t r0 = 0; r0 < n; r0 += 256) {
+        const uint64_t thr = s_thr;
+        for (int rr = 0; rr < 16; rr++) {
+            const int64_t row = r0 + rr * 16 + grp;
+            float xv[V];
+            load_run<V>(X + (size_t)(row < n ? row : 0) * dp + j * V, xv);
+            const float d = finish_dist(metric, group16_sum(run_partial<V>(metric, xv, qv)));
+            if (j == 0 && row < n) {
+                const uint64_t key = ((uint64_t)sort_key(d) << 32) | (uint64_t)(uint32_t)row;
+                if (key < thr) B[k + atomicAdd(&s_pend, 1)] = key;
+            }
+        }
+        __syncthreads();
+        const int pend = s_pend;
+        if (pend > pend_cap - 256 || (r0 + 256 >= n && pend > 0)) {
+            const int len = s_len;
+            for (int i = tid; i < P2; i += 256)
+                if ((i >= len && i < k) || i >= k + pend) B[i] = ~0ull;
+            __syncthreads();
+            bitonic_sort_lds(B, P2);
+            if (tid == 0) {
+                s_len = len + pend < k ? len + pend : k;
+                s_pend = 0;
+                s_thr = len + pend >= k ? B[k - 1] : ~0ull;
+            }
+            __syncthreads();
+        }
+    }
+    const int len = s_len;
+    for (int i = tid; i < kstride; i += 256) {
+        const bool has = i < len;
+        ids[(size_t)q * kstride + i] = has ? (uint32_t)(B[i] & 0xffffffffu) + 1u : 0u;
+        dists[(size_t)q * kstride + i] = has ? key_float((uint32_t)(B[i] >> 32)) : __builtin_inff();
+    }
+}
+
+hipError_t launch_exh_wide(const float* X, int64_t n, int32_t dp, int32_t metric, const float* Qp,
+                           int64_t m, int32_t k, uint32_t* ids, float* dists, int32_t kstride,
+                           hipStream_t s)
+{
+    if (k < 1 || k > EXH_WIDE_KMAX) return hipErrorInvalidValue;
+    int P2 = 512;
+    while (P2 < k + 256) P2 <<= 1;
+    const size_t lds = (size_t)P2 * sizeof(uint64_t);
+    const dim3 grid((unsigned)m), block(256);
+    switch (dp / 16) {
+#define SSG_EW(VV) \
+    case VV: hipLaunchKernelGGL((exh_wide_kernel<VV>), grid, block, lds, s, X, n, metric, Qp, k, P2, ids, dists, kstride); break;
+        SSG_EW(1) SSG_EW(2) SSG_EW(3) SSG_EW(4) SSG_EW(5) SSG_EW(6) SSG_EW(7) SSG_EW(8) SSG_EW(12) SSG_EW(16)
+#undef SSG_EW
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float* Q, int32_t qdim,
                          int64_t m, int32_t k, uint32_t* ids, float* dists, hipStream_t s)
 {

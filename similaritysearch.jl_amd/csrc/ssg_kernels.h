@@ -109,6 +109,7 @@ hipError_t exh_launch_rescan(const ExhArgs& a, const ExhWork& w, int f0, int nf,
                              hipStream_t s);
 int32_t exhaustive_slices(int64_t n, int64_t m, int num_cu, int dp);
 constexpr int EXH_KC = 32;        // candidates kept per (query, slice)
+constexpr int EXH_WIDE_KMAX = 3840;  // k of the one-block-per-query path (k > EXH_KC)
 constexpr int EXH_QT = 128;       // queries per block of exhaustive stage 1 (query padding)
 
 // Precision form of exhaustive k-NN: BF16 MFMA candidate pass + exact rerank.
@@ -203,6 +204,11 @@ __host__ __device__ inline int fp8_scale_exp(uint32_t amax_bits)
     return e < -120 ? -120 : (e > 120 ? 120 : e);
 }
 
+// Exact exhaustive k-NN for EXH_KC < k <= EXH_WIDE_KMAX: one block per query over the n
+// padded rows of X; Qp holds m padded query rows; results (kstride, m), 0 / +Inf past n.
+hipError_t launch_exh_wide(const float* X, int64_t n, int32_t dp, int32_t metric, const float* Qp,
+                           int64_t m, int32_t k, uint32_t* ids, float* dists, int32_t kstride,
+                           hipStream_t s);
 hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float* Q, int32_t qdim,
                          int64_t m, int32_t k, uint32_t* ids, float* dists, hipStream_t s);
 // hsp_queries filter (ssg_consumers.hip): (k, m) candidate columns -> kept columns.

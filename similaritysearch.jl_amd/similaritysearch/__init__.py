@@ -20,10 +20,12 @@ import numpy as np
 from . import _lib
 from ._lib import SSGError, BeamParams, QueryOpts
 
-__all__ = ["Dist", "SqL2", "L2", "NormCosine", "MatrixDatabase", "StaticAdjList", "BeamSearch",
-           "GenericContext", "SearchGraphContext", "ExhaustiveSearch", "ParallelExhaustiveSearch",
-           "SearchGraph", "searchbatch", "searchbatch_", "searchbatch_knns", "allknn", "rerank_",
-           "macrorecall", "recallscore", "SSGError"]
+__all__ = ["Dist", "SqL2", "L2", "NormCosine", "Cosine", "MatrixDatabase", "StaticAdjList",
+           "AdjList", "AdjDict", "BeamSearch", "GenericContext", "SearchGraphContext",
+           "ExhaustiveSearch", "ParallelExhaustiveSearch", "SearchGraph", "searchbatch",
+           "searchbatch_", "searchbatch_knns", "searchbatch_queues", "allknn", "rerank_",
+           "macrorecall", "recallscore", "SSGError", "KnnSorted", "KnnHeap", "knnqueue",
+           "unit_rows", "static_adjlist"]
 
 
 # --------------------------------------------------------------------------- distances
@@ -42,10 +44,39 @@ class NormCosine:
     metric = _lib.SSG_NORMCOS
 
 
+class Cosine:
+    """1 - dot(u, v) / (norm(u) norm(v)) (src/dist/cos.jl:103).  Bound as NormCosine over
+    copies of the database and the queries whose rows are divided by their norm in FP64
+    and rounded to Float32 once; a zero row stays zero (distance 1 to everything, where
+    the reference's 0/0 gives NaN).  The reference's own value goes through BLAS dot and
+    norm, whose rounding is not reproduced."""
+    metric = _lib.SSG_NORMCOS
+    normalises = True
+
+
 class Dist:
     SqL2 = SqL2
     L2 = L2
     NormCosine = NormCosine
+    Cosine = Cosine
+
+
+def unit_rows(R):
+    """Rows divided by their Euclidean norm in FP64, rounded to Float32 (zero rows stay
+    zero): what Cosine indexes and queries are searched as."""
+    R64 = np.asarray(R, np.float64)
+    nrm = np.sqrt((R64 * R64).sum(axis=1, keepdims=True))
+    return np.ascontiguousarray(np.where(nrm > 0, R64 / np.where(nrm > 0, nrm, 1.0), 0.0), np.float32)
+
+
+def _rows_for(dist, rows):
+    """The rows an index of distance `dist` stores or searches with."""
+    return unit_rows(rows) if getattr(dist, "normalises", False) else rows
+
+
+def _qrows(index, Q):
+    """Query rows as the index searches them (normalised for Cosine)."""
+    return _rows_for(index.dist, _as_db(Q).rows)
 
 
 def _metric(dist):
@@ -134,6 +165,62 @@ class StaticAdjList:
         return int(self.offset[i - 1]) - (0 if i == 1 else int(self.offset[i - 2]))
 
 
+class AdjList:
+    """AdjList (src/adj/adjlist.jl): one neighbour vector per node, 1-based ids; a node
+    never assigned a list has none.  SearchGraph freezes it into a StaticAdjList."""
+
+    def __init__(self, lists):
+        self.end_point = [None if x is None else np.asarray(x, np.uint32) for x in lists]
+
+    def __len__(self):
+        return len(self.end_point)
+
+    def neighbors(self, i):
+        return self.end_point[i - 1]
+
+    def neighbors_length(self, i):
+        x = self.end_point[i - 1]
+        return 0 if x is None else len(x)
+
+
+class AdjDict:
+    """AdjDict (src/adj/adjdict.jl): node id -> neighbour vector; absent nodes have none.
+    n is the number of nodes (ids 1..n)."""
+
+    def __init__(self, d, n):
+        self.end_point = {int(k): np.asarray(v, np.uint32) for k, v in d.items()}
+        self.n = int(n)
+
+    def __len__(self):
+        return self.n
+
+    def neighbors(self, i):
+        return self.end_point.get(int(i))
+
+    def neighbors_length(self, i):
+        x = self.end_point.get(int(i))
+        return 0 if x is None else len(x)
+
+
+def static_adjlist(adj):
+    """A StaticAdjList holding the same lists as adj (StaticAdjList, AdjList or AdjDict),
+    in node order, each list in its own order (StaticAdjList(adj), src/adj/adjstatic.jl:
+    69-120)."""
+    if isinstance(adj, StaticAdjList):
+        return adj
+    if not isinstance(adj, (AdjList, AdjDict)):
+        raise TypeError("adj must be a StaticAdjList, AdjList or AdjDict")
+    n = len(adj)
+    lens = np.array([adj.neighbors_length(i) for i in range(1, n + 1)], np.int64)
+    ep = np.zeros(int(lens.sum()), np.uint32)
+    pos = 0
+    for i in range(1, n + 1):
+        if lens[i - 1]:
+            ep[pos:pos + lens[i - 1]] = adj.neighbors(i)
+            pos += int(lens[i - 1])
+    return StaticAdjList(np.cumsum(lens), ep)
+
+
 # --------------------------------------------------------------------------- search params
 class BeamSearch:
     """BeamSearch(; bsize=4, Δ=1.0, maxvisits=10^6) (src/searchgraph/SearchGraph.jl:94-100)."""
@@ -210,7 +297,7 @@ class _GpuIndex:
             h = C.c_void_p()
             _lib.check(L.ssg_create(C.byref(h), self.device), "ssg_create")
             self._h = h
-            X = self.db.rows
+            X = _rows_for(self.dist, self.db.rows)
             _lib.check(L.ssg_upload_db(h, X.ctypes.data, X.shape[1], X.shape[0],
                                        _metric(self.dist)), "ssg_upload_db")
             self._after_upload()
@@ -267,7 +354,8 @@ class ExhaustiveSearch(_GpuIndex):
         buf[n:n + m] = rows
         self.db = MatrixDatabase.from_rows(buf[:n + m])
         if self._h is not None:
-            _lib.check(_lib.load().ssg_append_db(self._h, rows.ctypes.data, m), "append_items!")
+            up = _rows_for(self.dist, rows)
+            _lib.check(_lib.load().ssg_append_db(self._h, up.ctypes.data, m), "append_items!")
         return self
 
     def push_item(self, ctx, u):
@@ -285,8 +373,7 @@ class SearchGraph(_GpuIndex):
 
     def __init__(self, dist, db, adj, hints=(), algo=None, device=0):
         super().__init__(dist, db, device)
-        if not isinstance(adj, StaticAdjList):
-            raise TypeError("adj must be a StaticAdjList")
+        adj = static_adjlist(adj)
         if len(adj) != len(self.db):
             raise ValueError("graph and database sizes differ")
         self.adj = adj
@@ -344,7 +431,8 @@ class SearchGraph(_GpuIndex):
         Q = _as_db(queries)
         bp = self.beam_params(ctx)
         rec = C.c_double()
-        _lib.check(_lib.load().ssg_optimize_minrecall(self._handle(), Q.rows.ctypes.data, len(Q),
+        rows = _qrows(self, Q)
+        _lib.check(_lib.load().ssg_optimize_minrecall(self._handle(), rows.ctypes.data, len(Q),
                                                       ksearch, minrecall, C.byref(bp),
                                                       C.byref(rec)), "optimize_index!")
         self.algo = BeamSearch(bp.bsize, bp.delta, bp.maxvisits)
@@ -386,7 +474,7 @@ def searchbatch_(index, ctx, Q, ids, dists, sorted=False):
     k = ids.shape[0]
     L = _lib.load()
     h = index._handle()
-    rows = Q.rows
+    rows = _qrows(index, Q)
     if rows.shape[1] != index.db.dim:
         raise ValueError("query dimension differs from the database")
     if isinstance(index, SearchGraph):
@@ -452,12 +540,41 @@ def searchbatch_knns(index, ctx, Q, ks, bsizes=None, hints=None, visited=None):
         keep += [H, cnt]
         qo.hints, qo.nhints, qo.hint_stride = H.ctypes.data, cnt.ctypes.data, stride
     bp = index.beam_params(ctx)
-    _lib.check(_lib.load().ssg_beamsearch(index._handle(), Q.rows.ctypes.data, m, kstride,
+    rows = _qrows(index, Q)
+    _lib.check(_lib.load().ssg_beamsearch(index._handle(), rows.ctypes.data, m, kstride,
                                           kstride, C.byref(bp), C.byref(qo), ids.ctypes.data,
                                           dists.ctypes.data, cd.ctypes.data, cb.ctypes.data),
                "searchbatch!")
     ctx.costdists, ctx.costblocks = cd, cb
     return [(ids[j, :ks[j]].copy(), dists[j, :ks[j]].copy()) for j in range(m)]
+
+
+def searchbatch_queues(index, ctx, Q, knns):
+    """searchbatch!(index, ctx, Q, knns) with a vector of KnnSorted / KnnHeap queues
+    (src/SimilaritySearch.jl:384-398): query j's first maxlength(knns[j]) results in
+    (distance, id) order are pushed into knns[j] with the queue's own push_item!
+    (similaritysearch.pqueue).  Over a SearchGraph every query runs with its own k in one
+    launch; over an exhaustive index the batch runs once at the largest k."""
+    from .pqueue import fill
+    Q = _as_db(Q)
+    m = len(Q)
+    if m == 0:
+        raise ValueError("empty set of queries")
+    if len(knns) != m:
+        raise ValueError("the number of queries is different from the given output containers")
+    ks = np.array([r.maxlength() for r in knns], np.int32)
+    if (ks < 1).any():
+        raise ValueError("every queue needs room for one item")
+    if isinstance(index, SearchGraph):
+        runs = searchbatch_knns(index, ctx, Q, ks)
+    else:
+        kmax = int(ks.max())
+        ids, dists = _out(m, kmax)
+        searchbatch_(index, ctx, Q, ids.T, dists.T)
+        runs = [(ids[j, :ks[j]], dists[j, :ks[j]]) for j in range(m)]
+    for r, (i, d) in zip(knns, runs):
+        fill(r, i, d)
+    return knns
 
 
 def allknn(index, ctx, k, sort=True):
@@ -510,7 +627,8 @@ def exhaustive_candidates(index, k, ncand, Q=None, precision="bf16"):
     else:
         if Q.dim != index.db.dim:
             raise ValueError("query dimension differs from the database")
-        _lib.check(L.ssg_exhaustive_knn_prec(h, Q.rows.ctypes.data, m, k, pc, ncand, ids.ctypes.data,
+        rows = _qrows(index, Q)
+        _lib.check(L.ssg_exhaustive_knn_prec(h, rows.ctypes.data, m, k, pc, ncand, ids.ctypes.data,
                                              dists.ctypes.data, ci.ctypes.data, cd.ctypes.data),
                    "searchbatch!")
     return ids.T, dists.T, ci.T, cd.T
@@ -534,7 +652,8 @@ def rerank_(dist, db, Q, ids, dists):
     k = ids.shape[0]
     if not (ids.T.flags.c_contiguous and dists.T.flags.c_contiguous):
         raise ValueError("outputs must be Fortran-ordered (k, m) arrays")
-    _lib.check(_lib.load().ssg_rerank(idx._handle(), Q.rows.ctypes.data, m, k, ids.T.ctypes.data,
+    rows = _qrows(idx, Q)
+    _lib.check(_lib.load().ssg_rerank(idx._handle(), rows.ctypes.data, m, k, ids.T.ctypes.data,
                                       dists.T.ctypes.data), "rerank!")
     return ids, dists
 
@@ -574,6 +693,7 @@ from .consumers import (bichromatic_closestpair, closestpair, hsp_queries, neard
 
 __all__ += ["closestpair", "bichromatic_closestpair", "neardup", "hsp_queries", "pairwise"]
 
+from .pqueue import KnnHeap, KnnSorted, knnqueue  # noqa: E402
 from .storage import (MMapMatrixDatabase, load_index, load_results, read_matrix,  # noqa: E402
                       save_index, split_pathkey, write_results)
 

@@ -133,8 +133,10 @@ class LearnedSearch:
         """Labels (landmark index, bucket index) for each training query row: the bucket is
         the smallest beam size whose search from the label landmark finds at least
         `min_hits` (default k) of the exact k nearest neighbours."""
+        from . import _rows_for
         min_hits = k if min_hits is None else int(min_hits)
         G = self.G
+        Qtrain = _rows_for(G.dist, np.ascontiguousarray(Qtrain, np.float32))
         h = G._handle()
         L = _lib.load()
         m = len(Qtrain)
@@ -176,9 +178,10 @@ class LearnedSearch:
         if verbose:
             verbose(f"traces: {len(Qtrain)} queries, exact top-10 reached for {found.mean():.3f}, "
                     f"bucket histogram {np.bincount(yb, minlength=len(self.buckets)).tolist()}")
+        from . import _rows_for
         D = Qtrain.shape[1]
         self.mlp = Mlp(D, self.hidden, self.hidden, self.L, len(self.buckets), seed=self.seed)
-        dX = DeviceArray.from_numpy(np.ascontiguousarray(Qtrain, np.float32))
+        dX = DeviceArray.from_numpy(_rows_for(self.G.dist, np.ascontiguousarray(Qtrain, np.float32)))
         dyl = DeviceArray.from_numpy(yl)
         dyb = DeviceArray.from_numpy(yb)
         losses = self.mlp.train_dev(dX.ptr, dyl.ptr, dyb.ptr, len(Qtrain), epochs, batch,
@@ -227,10 +230,10 @@ class LearnedSearch:
         return LearnedGraph(g)
 
     def search(self, ctx, Q, k, **kw):
-        from . import _as_db
+        from . import _as_db, _qrows
         Q = _as_db(Q)
         m = len(Q)
-        dQ = DeviceArray.from_numpy(Q.rows)
+        dQ = DeviceArray.from_numpy(_qrows(self.G, Q))
         dI = DeviceArray((m, k), np.uint32)
         dD = DeviceArray((m, k), np.float32)
         dcd = DeviceArray((m,), np.int64)

@@ -202,6 +202,36 @@ def test_beam_small_n_is_brute_force(oracle):
     assert (cd == 40).all() and (cb == 0).all()
 
 
+# ---------------------------------------------------------------- Cosine
+def test_cosine_closed_forms(oracle):
+    """test/testdistances.jl:61-76: orthogonal vectors are at cosine distance 1, a vector
+    (any length) from itself and from a positive multiple at 0."""
+    e = np.array([[3.0, 0.0, 0.0]], np.float32)
+    f = np.array([[0.0, 0.5, 0.0]], np.float32)
+    E, F = oracle.unit_rows(e), oracle.unit_rows(f)
+    assert oracle.dist("NormCosine", E[0], F[0]) == 1.0
+    assert oracle.dist("NormCosine", E[0], E[0]) == 0.0
+    assert oracle.dist("NormCosine", E[0], oracle.unit_rows(7 * e)[0]) == 0.0
+    np.testing.assert_array_equal(oracle.unit_rows(np.zeros((1, 3), np.float32)), 0)
+
+
+def test_cosine_within_fp32_bound_of_fp64(oracle):
+    """The NormCosine distance of the normalised rows stays within 2 dim units of 2^-24 of
+    the FP64 1 - a.b / (|a| |b|) (one rounding per normalised component and the final
+    one), and a restatement that skipped the normalisation misses by far."""
+    rng = np.random.default_rng(51)
+    A = (rng.standard_normal((300, 96)) * rng.uniform(0.1, 10, (300, 1))).astype(np.float32)
+    B = (rng.standard_normal((20, 96)) * 3).astype(np.float32)
+    UA, UB = oracle.unit_rows(A), oracle.unit_rows(B)
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    for j in range(len(B)):
+        for i in range(len(A)):
+            ref = 1 - A64[i] @ B64[j] / (np.linalg.norm(A64[i]) * np.linalg.norm(B64[j]))
+            assert abs(oracle.dist("NormCosine", UA[i], UB[j]) - ref) <= 2 * 96 * 2.0 ** -24
+    assert abs(oracle.dist("NormCosine", A[0], B[0]) -
+               (1 - A64[0] @ B64[0] / (np.linalg.norm(A64[0]) * np.linalg.norm(B64[0])))) > 1e-3
+
+
 # ---------------------------------------------------------------- BF16 form of the search
 @pytest.mark.parametrize("metric", ["SqL2", "L2", "NormCosine"])
 @pytest.mark.parametrize("dim", [8, 32, 96, 128])
