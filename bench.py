@@ -19,7 +19,9 @@ trained on the GPU on traces of 500,000 training queries; per query, the predict
 landmark as entry point and the beam size of the smallest bucket whose predicted
 cumulative probability reaches a quantile; BF16 search with an exact FP32 rerank.  Its
 knobs (quantile, rerank length, beam-size cap, delta) are tuned on the 20,000 tuning
-queries.  All steps run as one replay of a captured HIP graph (inference, planning,
+queries by the product's tuner (LearnedSearch.tune), which also sets the fallback: a
+query whose recall, predicted from its landmark confidence, is below the target takes the
+plain BF16 search instead.  All steps run as one replay of a captured HIP graph (inference, planning,
 longest-first ordering and one persistent search launch over every step's queries).
 `value` is that form's QPS over the timed steps.  `config.ablation_plain_bf16` is the
 same BF16 search and rerank without the network (global hints, one beam size tuned to
@@ -267,42 +269,24 @@ def search_bench(args):
     fit_s = time.time() - t0
     log(f"learned model in {fit_s:.1f}s, losses {np.round(losses, 3).tolist()}")
     full_buckets = LS.buckets
-
-    def set_cap(cap):
-        LS.buckets = tuple(min(b, cap) for b in full_buckets)
-
-    def learned_tune_run(q, rr, cap, delta):
-        set_cap(cap)
-        LS.search_dev(ctx, dQt.ptr, len(Qtune), k, dI.ptr, dD.ptr, nhints=1, rerank=rr,
-                      quantile=q, delta=delta, stream=sp, dcd=dcd.ptr)
-        st.synchronize()
-        return recall(gold_tune, dI.numpy()[:len(Qtune)]), float(dcd.numpy()[:len(Qtune)].mean())
-
     margin = 0.004
     if args.learned:
         q, rr, cap, delta = args.learned.split(",")
         q, rr, cap, delta = float(q), int(rr), int(cap), float(delta)
+        LS.buckets = tuple(min(b, cap) for b in full_buckets)
+        tuned = {"quantile": q, "rerank": rr, "cap": cap, "delta": delta, "min_conf": -1.0,
+                 "fallback_bsize": palgo.bsize, "fallback_fraction": None, "calibration": None}
     else:
-        # cheapest (mean distance evaluations) setting whose tuning recall clears the
-        # target with a margin
-        best = None
-        for delta in (1.05, 1.1):
-            for rr in (10, 12, 14, 20):
-                for cap in (64, 128):
-                    for q in (0.5, 0.55, 0.6, 0.65, 0.7, 0.75, 0.8, 0.85, 0.9):
-                        r, cost = learned_tune_run(q, rr, cap, delta)
-                        if r >= args.target_recall + margin:
-                            log(f"tune q={q} rr={rr} cap={cap} delta={delta}: recall {r:.4f} "
-                                f"mean costdists {cost:.0f}")
-                            if best is None or cost < best[0]:
-                                best = (cost, q, rr, cap, delta)
-                            break
-        if best is None:
-            best = (0, 0.9, 20, 128, 1.1)
-        _, q, rr, cap, delta = best
-    set_cap(cap)
-    kw = dict(nhints=1, rerank=rr, quantile=q, delta=delta)
-    log(f"learned: quantile={q} rerank={rr} cap={cap} delta={delta}")
+        # the product's tuner: cheapest setting whose tuning recall clears the target with a
+        # margin, queries of predicted recall below the target falling back to plain search
+        t0 = time.time()
+        tuned = LS.tune(ctx, Qtune, gold_tune, target=args.target_recall, margin=margin, k=k,
+                        verbose=log)
+        log(f"tuned in {time.time() - t0:.1f}s: {tuned}")
+    q, rr, cap, delta = tuned["quantile"], tuned["rerank"], tuned["cap"], tuned["delta"]
+    min_conf, fb = tuned["min_conf"], tuned["fallback_bsize"]
+    kw = dict(nhints=1, rerank=rr, quantile=q, delta=delta, min_conf=min_conf, fallback_bsize=fb)
+    log(f"learned: quantile={q} rerank={rr} cap={cap} delta={delta} min_conf={min_conf} fallback bsize {fb}")
 
     # batched: one graph replay covers every step
     g = LS.capture(ctx, dQ.ptr, m, k, dI.ptr, dD.ptr, st, **kw)
@@ -331,14 +315,14 @@ def search_bench(args):
     dS = DeviceArray((m,), np.int32)
     dO = DeviceArray((m,), np.uint32)
     table = LS.table(0)
-    bp_l = ss.BeamParams(G.algo.bsize, delta, 10**6, max(ctx.beamcap, int(table.max())), 1, rr, 1)
+    bp_l = ss.BeamParams(fb, delta, 10**6, max(ctx.beamcap, int(table.max()), fb), 1, rr, 1)
     qo = ss.QueryOpts(None, dS.ptr, dH.ptr, dN.ptr, 1, dO.ptr)
     e2, e3, e4 = Event(), Event(), Event()
     e2.record(st)
     _lib.check(L.ssg_mlp_infer_dev(LS.mlp._p, dQ.ptr, m, LS.dlandmarks.ptr, 1, float(q), dH.ptr,
                                    dB.ptr, dC.ptr, sp), "infer")
-    _lib.check(L.ssg_learned_plan_dev(h, dB.ptr, dC.ptr, m, 1, 0.0, table.ctypes.data, nb,
-                                      G.algo.bsize, dN.ptr, dS.ptr, sp), "plan")
+    _lib.check(L.ssg_learned_plan_dev(h, dB.ptr, dC.ptr, m, 1, float(min_conf), table.ctypes.data, nb,
+                                      fb, dN.ptr, dS.ptr, sp), "plan")
     _lib.check(L.ssg_order_by_bsize_dev(h, dS.ptr, m, dO.ptr, sp), "order")
     e3.record(st)
     _lib.check(L.ssg_beamsearch_dev(h, dQ.ptr, m, k, k, C.byref(bp_l), C.byref(qo), dI.ptr,
@@ -347,6 +331,7 @@ def search_bench(args):
     st.synchronize()
     pre_ms, sms = e2.elapsed_ms(e3), e3.elapsed_ms(e4)
     lcostd, lcostb = dcd.numpy().copy(), dcb.numpy().copy()
+    fb_frac = float((dN.numpy() < 0).mean())
 
     # per-step (unbatched) learned launches, for reference
     for w in range(args.warmup):
@@ -362,16 +347,10 @@ def search_bench(args):
 
     # ------------------------------------------------------------------ ablation: no network
     # the same BF16 search and rerank from the global hints with one beam size, the smallest
-    # reaching the learned form's tuning margin; all steps in one graph replay
+    # reaching the learned form's tuning margin (the tuner's fallback search) for every
+    # query; all steps in one graph replay
     actx = ss.SearchGraphContext(beamcap=256, precision="bf16", rerank=rr)
-    ab = None
-    for b in (12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128):
-        r, cost = plain_recall(ss.BeamSearch(b, delta, 10**6), actx)
-        log(f"ablation bsize={b}: tuning recall {r:.4f} mean costdists {cost:.0f}")
-        if r >= args.target_recall + margin:
-            ab = b
-            break
-    ab = ab or 128
+    ab = fb
     G.algo = ss.BeamSearch(ab, delta, 10**6)
     bp_a = G.beam_params(actx)
     ga = _capture(L, _lib, st, lambda: plain_run(bp_a, dQ, m))
@@ -449,6 +428,10 @@ def search_bench(args):
                     "in one HIP graph replay (one persistent search launch)",
             "recall@10": round(lrec, 4),
             "learned": {"quantile": q, "rerank": rr, "beam_cap": cap, "delta": delta, "nhints": 1,
+                        "min_conf": min_conf, "fallback_bsize": fb,
+                        "fallback_fraction_test": round(fb_frac, 4),
+                        "fallback_fraction_tune": tuned["fallback_fraction"],
+                        "predicted_recall_calibration": tuned["calibration"],
                         "buckets": list(full_buckets), "landmarks": LS.L,
                         "mean_costdists": round(float(lcostd.mean()), 1),
                         "mean_costblocks": round(float(lcostb.mean()), 1),

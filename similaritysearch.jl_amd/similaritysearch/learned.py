@@ -13,6 +13,14 @@ Pipeline, all on the GPU:
      bucket's beam size; queries whose top-landmark probability is below `min_conf`
      fall back to the plain search (SearchGraph.hints and the plain beam size).
      The search itself runs in the BF16 precision form with an exact rerank.
+  5. tune: the knobs (quantile, rerank length, beam-size cap, delta) and the fallback are
+     chosen on tuning queries against exact ground truth.  Each query's recall is
+     predicted from its landmark confidence by an isotonic (monotone) fit of the recall
+     the learned search reached on the tuning queries; queries whose predicted recall is
+     below the target take the plain search with the smallest beam size whose own recall
+     reaches the target.  The cheapest setting (mean distance evaluations) whose overall
+     tuning recall clears the target with a margin wins: the recall-targeted cost of
+     optimize_index! (src/opt.jl:220-317, MinRecall) over these knobs.
 """
 import ctypes as C
 
@@ -207,11 +215,128 @@ class LearnedSearch:
         table = self.table(shift)
         fb = fallback_bsize or G.algo.bsize
         d = G.algo.delta if delta is None else float(delta)
-        bp = BeamParams(fb, d, 10**6, max(ctx.beamcap, int(table.max())), 1, rerank, expand)
+        bp = BeamParams(fb, d, 10**6, max(ctx.beamcap, int(table.max()), fb), 1, rerank, expand)
         _lib.check(_lib.load().ssg_learned_search_dev(
             G._handle(), self.mlp._p, self.dlandmarks.ptr, dQ, m, k, k, C.byref(bp),
             nhints, min_conf, float(quantile), table.ctypes.data, len(table), dI, dD, dcd, dcb,
             stream), "learned search")
+
+    def confidence_dev(self, dQ, m, stream):
+        """Top-landmark softmax probability of each of the m device-resident queries."""
+        dH = DeviceArray((m, 1), np.uint32)
+        dB = DeviceArray((m,), np.int32)
+        dC = DeviceArray((m,), np.float32)
+        _lib.check(_lib.load().ssg_mlp_infer_dev(self.mlp._p, dQ, m, self.dlandmarks.ptr, 1, 0.0,
+                                                 dH.ptr, dB.ptr, dC.ptr, stream.handle), "infer")
+        stream.synchronize()
+        return dC.numpy()
+
+    @staticmethod
+    def isotonic(x, y):
+        """Non-decreasing least-squares fit of y on x (pool adjacent violators): returns the
+        sorted x and the fitted values."""
+        o = np.argsort(x, kind="stable")
+        xs, ys = np.asarray(x, np.float64)[o], np.asarray(y, np.float64)[o]
+        val, wt, cnt = [], [], []
+        for v in ys:
+            val.append(v)
+            wt.append(1.0)
+            cnt.append(1)
+            while len(val) > 1 and val[-2] > val[-1]:
+                w = wt[-2] + wt[-1]
+                val[-2] = (val[-2] * wt[-2] + val[-1] * wt[-1]) / w
+                wt[-2] = w
+                cnt[-2] += cnt[-1]
+                val.pop()
+                wt.pop()
+                cnt.pop()
+        return xs, np.repeat(val, cnt)
+
+    def tune(self, ctx, Qtune, gold, target=0.95, margin=0.004, k=10, deltas=(1.05, 1.1),
+             reranks=(10, 12, 14, 20), caps=(64, 128),
+             quantiles=(0.5, 0.55, 0.6, 0.65, 0.7, 0.75, 0.8, 0.85, 0.9),
+             fallback_ladder=(8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128), verbose=None):
+        """Chooses the learned knobs and the predicted-recall fallback on tuning queries.
+
+        Qtune: (m, dim) rows; gold: (m, k) exact nearest ids (any order).  Returns a dict
+        with quantile, rerank, cap, delta, min_conf (queries with a lower confidence have
+        a predicted recall below `target` and fall back), fallback_bsize, the tuning recall
+        and mean distance evaluations, the fallback fraction, and the calibration (confidence
+        -> predicted recall).  The chosen cap is installed (self.buckets); pass min_conf and
+        fallback_bsize to search_dev."""
+        from . import _rows_for
+        if not hasattr(self, "full_buckets"):
+            self.full_buckets = self.buckets
+        G = self.G
+        Qr = _rows_for(G.dist, np.ascontiguousarray(Qtune, np.float32))
+        m = len(Qr)
+        gold = np.sort(np.asarray(gold, np.uint32)[:, :k], axis=1)
+        st = Stream()
+        dQ = DeviceArray.from_numpy(Qr)
+        dI, dD = DeviceArray((m, k), np.uint32), DeviceArray((m, k), np.float32)
+        dcd = DeviceArray((m,), np.int64)
+        L = _lib.load()
+
+        def per_query_recall(ids):
+            return (np.sort(ids, axis=1)[:, :, None] == gold[:, None, :]).any(axis=2).mean(axis=1)
+
+        conf = self.confidence_dev(dQ.ptr, m, st)
+        plain_cache = {}
+
+        def plain(rr, delta):
+            if (rr, delta) not in plain_cache:
+                res = None
+                for b in fallback_ladder:
+                    bp = BeamParams(b, delta, 10**6, max(ctx.beamcap, b), 1, rr, 1)
+                    _lib.check(L.ssg_beamsearch_dev(G._handle(), dQ.ptr, m, k, k, C.byref(bp), None,
+                                                    dI.ptr, dD.ptr, dcd.ptr, None, st.handle),
+                               "plain search")
+                    st.synchronize()
+                    r, c = per_query_recall(dI.numpy()), dcd.numpy().astype(np.float64)
+                    res = (b, r, c)
+                    if r.mean() >= target + margin:
+                        break
+                plain_cache[(rr, delta)] = res
+            return plain_cache[(rr, delta)]
+
+        best = None
+        for delta in deltas:
+            for rr in reranks:
+                fb, fr, fc = plain(rr, delta)
+                for cap in caps:
+                    self.buckets = tuple(min(b, cap) for b in self.full_buckets)
+                    for q in quantiles:
+                        self.search_dev(ctx, dQ.ptr, m, k, dI.ptr, dD.ptr, nhints=1, min_conf=-1.0,
+                                        rerank=rr, quantile=q, delta=delta, fallback_bsize=fb,
+                                        stream=st.handle, dcd=dcd.ptr)
+                        st.synchronize()
+                        r, c = per_query_recall(dI.numpy()), dcd.numpy().astype(np.float64)
+                        xs, pred = self.isotonic(conf, r)
+                        ok = np.nonzero(pred >= target)[0]
+                        min_conf = float(xs[ok[0]]) if len(ok) else float("inf")
+                        use = conf >= min_conf
+                        rec = float(np.where(use, r, fr).mean())
+                        cost = float(np.where(use, c, fc).mean())
+                        if rec >= target + margin and (best is None or cost < best["mean_costdists"]):
+                            edges = np.quantile(xs, np.linspace(0, 1, 11))
+                            best = {"quantile": q, "rerank": rr, "cap": cap, "delta": delta,
+                                    "min_conf": min_conf, "fallback_bsize": fb, "recall": rec,
+                                    "mean_costdists": cost, "fallback_fraction": float(1 - use.mean()),
+                                    "calibration": {"confidence": edges.tolist(),
+                                                    "predicted_recall": np.interp(edges, xs, pred).tolist()}}
+                            if verbose:
+                                verbose(f"tune q={q} rr={rr} cap={cap} delta={delta}: recall {rec:.4f} "
+                                        f"mean costdists {cost:.0f}, fallback {1 - use.mean():.3f} "
+                                        f"(bsize {fb}, min_conf {min_conf:.3f})")
+        if best is None:
+            fb, fr, fc = plain(max(reranks), max(deltas))
+            best = {"quantile": max(quantiles), "rerank": max(reranks), "cap": max(caps),
+                    "delta": max(deltas), "min_conf": float("inf"), "fallback_bsize": fb,
+                    "recall": float(fr.mean()), "mean_costdists": float(fc.mean()),
+                    "fallback_fraction": 1.0, "calibration": None}
+        self.buckets = tuple(min(b, best["cap"]) for b in self.full_buckets)
+        self.tuned = best
+        return best
 
     def capture(self, ctx, dQ, m, k, dI, dD, stream, **kw):
         """Records search_dev over the m device-resident queries into a HIP graph (one
