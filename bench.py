@@ -247,6 +247,10 @@ def search_bench(args):
             log(f"widened: {palgo} tuning recall@10={prec:.4f}")
     G.algo = palgo
     bp_plain = G.beam_params(ctx)
+    # launch shape from the product's autotuner (ssg_optimize_launch) on the tuning queries
+    pshape, _, _ = G.optimize_launch_dev(bp_plain, dQt.ptr, len(Qtune), k, k, gold_tune,
+                                         args.target_recall)
+    log(f"plain launch shape (slots per CU, gather rounds): {pshape}")
     for w in range(args.warmup):
         plain_run(bp_plain, dQ, STEP, (w % args.steps) * STEP)
     st.synchronize()
@@ -288,6 +292,28 @@ def search_bench(args):
     kw = dict(nhints=1, rerank=rr, quantile=q, delta=delta, min_conf=min_conf, fallback_bsize=fb)
     log(f"learned: quantile={q} rerank={rr} cap={cap} delta={delta} min_conf={min_conf} fallback bsize {fb}")
 
+    # launch shape for the learned search, from the autotuner over the tuning queries with
+    # their planned hints and beam sizes (the rerank and wide-step knobs it may also raise)
+    mt = len(Qtune)
+    tH, tB, tC = DeviceArray((mt, 1), np.uint32), DeviceArray((mt,), np.int32), DeviceArray((mt,), np.float32)
+    tN, tS = DeviceArray((mt,), np.int32), DeviceArray((mt,), np.int32)
+    ttab = LS.table(0)
+    _lib.check(L.ssg_mlp_infer_dev(LS.mlp._p, dQt.ptr, mt, LS.dlandmarks.ptr, 1, float(q), tH.ptr,
+                                   tB.ptr, tC.ptr, sp), "infer")
+    _lib.check(L.ssg_learned_plan_dev(h, tB.ptr, tC.ptr, mt, 1, float(min_conf), ttab.ctypes.data,
+                                      len(ttab), fb, tN.ptr, tS.ptr, sp), "plan")
+    st.synchronize()
+    bp_t = ss.BeamParams(fb, delta, 10**6, max(ctx.beamcap, int(ttab.max()), fb), 1, rr, 1)
+    lshape, lrec_t, _ = G.optimize_launch_dev(bp_t, dQt.ptr, mt, k, k, gold_tune,
+                                              args.target_recall + margin,
+                                              ss.QueryOpts(None, tS.ptr, tH.ptr, tN.ptr, 1, None))
+    if bp_t.rerank != rr or bp_t.expand != 1:
+        log(f"launch tuner raised rerank {rr} -> {bp_t.rerank}, expand {bp_t.expand}")
+    rr, expand = bp_t.rerank, bp_t.expand
+    kw.update(rerank=rr, expand=expand)
+    log(f"learned launch shape {lshape}, tuning recall {lrec_t:.4f}")
+    del tH, tB, tC, tN, tS
+
     # batched: one graph replay covers every step
     g = LS.capture(ctx, dQ.ptr, m, k, dI.ptr, dD.ptr, st, **kw)
     for _ in range(max(1, args.warmup)):
@@ -315,7 +341,7 @@ def search_bench(args):
     dS = DeviceArray((m,), np.int32)
     dO = DeviceArray((m,), np.uint32)
     table = LS.table(0)
-    bp_l = ss.BeamParams(fb, delta, 10**6, max(ctx.beamcap, int(table.max()), fb), 1, rr, 1)
+    bp_l = ss.BeamParams(fb, delta, 10**6, max(ctx.beamcap, int(table.max()), fb), 1, rr, expand)
     qo = ss.QueryOpts(None, dS.ptr, dH.ptr, dN.ptr, 1, dO.ptr)
     e2, e3, e4 = Event(), Event(), Event()
     e2.record(st)
@@ -353,13 +379,16 @@ def search_bench(args):
     ab = fb
     G.algo = ss.BeamSearch(ab, delta, 10**6)
     bp_a = G.beam_params(actx)
+    ashape, _, _ = G.optimize_launch_dev(bp_a, dQt.ptr, len(Qtune), k, k, gold_tune,
+                                         args.target_recall + margin)
     ga = _capture(L, _lib, st, lambda: plain_run(bp_a, dQ, m))
     awall, ams = _time_graph(L, _lib, st, ga, args.warmup)
     L.ssg_graph_destroy(ga)
     arec = recall(gold, dI.numpy())
     acd = float(dcd.numpy().mean())
     ablation = {"qps": round(m / awall, 1), "recall@10": round(arec, 4), "bsize": ab, "delta": delta,
-                "rerank": rr, "mean_costdists": round(acd, 1), "graph_replay_ms": round(ams, 3),
+                "rerank": bp_a.rerank, "expand": bp_a.expand, "launch_shape": list(ashape),
+                "mean_costdists": round(acd, 1), "graph_replay_ms": round(ams, 3),
                 "learned_over_this": round(learned_qps / (m / awall), 3),
                 "costdists_saved_by_network": round(1.0 - float(lcostd.mean()) / acd, 4)}
     log(f"ablation plain BF16 batched: {m / awall:.0f} QPS, recall@10 {arec:.4f}, bsize {ab}")
@@ -427,7 +456,8 @@ def search_bench(args):
             "form": "learned entry points and beam sizes + BF16 beam search + FP32 rerank, all steps "
                     "in one HIP graph replay (one persistent search launch)",
             "recall@10": round(lrec, 4),
-            "learned": {"quantile": q, "rerank": rr, "beam_cap": cap, "delta": delta, "nhints": 1,
+            "learned": {"quantile": q, "rerank": rr, "expand": expand, "beam_cap": cap,
+                        "delta": delta, "nhints": 1, "launch_shape": list(lshape),
                         "min_conf": min_conf, "fallback_bsize": fb,
                         "fallback_fraction_test": round(fb_frac, 4),
                         "fallback_fraction_tune": tuned["fallback_fraction"],
@@ -439,6 +469,7 @@ def search_bench(args):
                         "search_ms": round(sms, 3), "per_step_launch_qps": round(step_qps, 1),
                         "train_queries": args.train, "epochs": args.epochs, "fit_s": round(fit_s, 1)},
             "plain": {"qps": round(plain_qps, 1), "recall@10": round(prec_dev, 4),
+                      "launch_shape": list(pshape),
                       "bsize": palgo.bsize, "delta": palgo.delta, "maxvisits": palgo.maxvisits,
                       "precision": "fp32", "launch": "one per step",
                       "mean_costdists": round(float(pcostd.mean()), 1),
@@ -491,7 +522,10 @@ def batching_run(args, G, X, compact):
     """C4, the batching form: 200,000 queries from the mixture (seed 7) with per-query beam
     sizes ~ U{16, 32, 64, 128, 256} and k ~ U{1, 10, 50} (seed 8) against the 1M x 128
     graph, all in one persistent launch captured in a HIP graph (longest-first order).
-    FP32 (`value`) and the BF16 form (rerank 20).  Recall: per query the share of its exact
+    FP32 (`value`) and the BF16 form (rerank 20 or more).  The launch shape (wave slots per
+    CU, gather rounds) and the BF16 rerank / wide steps come from the product's autotuner
+    (ssg_optimize_launch) on 20,000 tuning queries drawn the same way (seeds 17, 18) with a
+    0.95 recall target.  Recall: per query the share of its exact
     min(k, 10) nearest neighbours among its first min(k, 10) results, averaged.  Checked
     against the oracle on 2,000 queries; `cpu_baseline`: the restated search with the
     reference's FP32 SIMD distance, per-query k and bsize, 256-row beam buffers."""
@@ -532,9 +566,24 @@ def batching_run(args, G, X, compact):
     dcd, dcb, dO = DeviceArray((m,), np.int64), DeviceArray((m,), np.int64), DeviceArray((m,), np.uint32)
     _lib.check(L.ssg_order_by_bsize_dev(h, db.ptr, m, dO.ptr, st.handle), "order")
     qo = ss.QueryOpts(dk.ptr, db.ptr, None, None, 0, dO.ptr)
+    # tuning queries drawn the same way (seeds 17, 18) for the launch autotuner
+    mt = 20_000
+    Qt = datasets.gmm(mt, dim, seed=17)
+    trng = np.random.default_rng(18)
+    tbq = trng.choice([16, 32, 64, 128, 256], mt).astype(np.int32)
+    tkq = trng.choice([1, 10, 50], mt).astype(np.int32)
+    tgold = np.zeros((mt, 10), np.uint32)
+    tgd = np.zeros((mt, 10), np.float32)
+    _lib.check(L.ssg_exhaustive_knn(h, np.ascontiguousarray(Qt).ctypes.data, mt, 10, tgold.ctypes.data,
+                                    tgd.ctypes.data), "exhaustive")
+    dQt, dtk, dtb = DeviceArray.from_numpy(Qt), DeviceArray.from_numpy(tkq), DeviceArray.from_numpy(tbq)
+    qot = ss.QueryOpts(dtk.ptr, dtb.ptr, None, None, 0, None)
     forms = {}
     for name, prec, rr in (("fp32", 0, 0), ("bf16", 1, 20)):
         bp = ss.BeamParams(16, G.algo.delta, 10**6, 256, prec, rr, 1)
+        shape, trec, _ = G.optimize_launch_dev(bp, dQt.ptr, mt, KS, KS, tgold, 0.95, qot)
+        log(f"C4 {name}: launch shape {shape}, rerank {bp.rerank}, expand {bp.expand}, "
+            f"tuning recall {trec:.4f}")
 
         def run():
             _lib.check(L.ssg_beamsearch_dev(h, dQ.ptr, m, KS, KS, C.byref(bp), C.byref(qo), dI.ptr,
@@ -544,9 +593,10 @@ def batching_run(args, G, X, compact):
         L.ssg_graph_destroy(g)
         ids, cd, cb = dI.numpy(), dcd.numpy(), dcb.numpy()
         nbytes = beam_bytes(cd, cb, deg, dim, 4 if prec == 0 else 2, kq,
-                            np.maximum(kq, rr) if prec else 0)
+                            np.maximum(kq, bp.rerank) if prec else 0)
         forms[name] = {"qps": round(m / wall, 1), "graph_replay_ms": round(ms, 3),
-                       "recall": round(recall(ids), 4),
+                       "recall": round(recall(ids), 4), "launch_shape": list(shape),
+                       "rerank": bp.rerank, "expand": bp.expand,
                        "recall@10_k>=10": round(float((ids[kq >= 10, :10][:, :, None] ==
                                                        gold[kq >= 10][:, None, :]).any(2).mean()), 4),
                        "mean_costdists": round(float(cd.mean()), 1),
@@ -562,7 +612,7 @@ def batching_run(args, G, X, compact):
     nc = 2000
     oi, od, ocd, ocb = po.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, Q[:nc], 10, 16,
                                      G.algo.delta, 10**6, beamcap=256, kq=kq[:nc], bq=bq[:nc],
-                                     nthreads=threads)
+                                     nthreads=threads)   # the FP32 form (bit-exact at any shape)
     ks = oi.shape[1]
     mism = int(((oi != I32[:nc, :ks]).any(1) | (od != D32[:nc, :ks]).any(1) | (ocd != CD32[:nc])
                 | (ocb != CB32[:nc])).sum())

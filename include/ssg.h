@@ -232,6 +232,35 @@ int ssg_capture_end(void* stream, ssg_graph** graph);
 int ssg_graph_launch(ssg_graph* graph, void* stream);
 int ssg_graph_destroy(ssg_graph* graph);
 
+/* Launch shape of the handle's beam searches (no reference counterpart): persistent wave
+ * slots per CU (each wave owns a visited slot of n bytes; 4..64, default 32) and gather
+ * rounds per batch (8, the default, or 4: four rows in flight per 16-lane group per round,
+ * fewer registers; the 4-round kernels serve beam caps <= 64).  Results do not depend on
+ * it. */
+typedef struct {
+    int32_t slots_per_cu;
+    int32_t rounds;
+} ssg_launch_params;
+int ssg_set_launch(ssg_handle* h, const ssg_launch_params* lp);
+int ssg_get_launch(ssg_handle* h, ssg_launch_params* lp);
+
+/* Launch autotuner (no reference counterpart; it extends optimize_index!'s recall-targeted
+ * search, src/opt.jl:220-317, from BeamSearch's (bsize, Δ) to the launch).  Over m
+ * device-resident queries with per-query options qo (device arrays, nullable), it runs
+ * the search of *bp for each recall-changing knob -- in the BF16 form the rerank length
+ * (r, r + 2, r + 4, 1.5 r and 2 r from r = bp->rerank, or k when it is 0; max(kstride,
+ * rerank) <= 64) and expand 1 or 2 -- and
+ * measures recall against gold (host, m x kg 1-based ids in any order; query j is scored
+ * on its first min(k_j, kg) results against its first min(k_j, kg) gold ids).  Among the
+ * knobs reaching minrecall (else the best recall) it times every launch shape
+ * (slots_per_cu in {8, 16, 24, 32, 48, 64} x rounds in {4, 8}) with HIP events and keeps
+ * the fastest.  It installs that shape, writes the chosen rerank/expand into *bp, and
+ * returns the shape, its recall and its time (ms).  Uses the handle's stream. */
+int ssg_optimize_launch(ssg_handle* h, const float* dQ, int64_t m, int32_t k, int32_t kstride,
+                        ssg_beam_params* bp, const ssg_query_opts* qo, const uint32_t* gold,
+                        int32_t kg, float minrecall, ssg_launch_params* lp_out, double* recall_out,
+                        float* ms_out);
+
 /* Handle options (no reference counterpart).
  * SSG_OPT_QUERY_LAYOUT: 1 (default) lets searchbatch! run on a copy of the graph and the
  * vectors renumbered breadth-first from the hints, each list entry carrying its target's

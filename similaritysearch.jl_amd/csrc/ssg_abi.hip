@@ -40,7 +40,7 @@ int ssg_hipfail(hipError_t e, const char* what)
 static int ensure_vis(ssg_handle* h, hipStream_t s)
 {
     const int64_t stride = (std::max<int64_t>(h->n, 1) + 1023) / 1024 * 1024;
-    int want = h->num_cu * 32;
+    int want = h->num_cu * h->slots_per_cu;
     const int64_t budget = (int64_t)8 << 30;
     if ((int64_t)want * stride > budget) want = (int)std::max<int64_t>(4, budget / stride);
     want = want / 4 * 4;
@@ -239,6 +239,7 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     a.prec = bp->precision == SSG_BF16 ? 1 : 0;
     a.Xh = h->Xh.as<uint16_t>();
     a.krr = std::max(bp->rerank, 0);
+    a.rounds = internal ? 8 : h->rounds;
     a.expand = bp->expand > 1 ? bp->expand : 1;
     a.order = qo ? qo->order : nullptr;
     if (use_q) {

@@ -66,11 +66,11 @@ __device__ __forceinline__ void vis_put(uint8_t* p, uint8_t v)
 
 // PREC 0: canonical distance on the FP32 vectors.  PREC 1: BF16 vectors with FP32
 // arithmetic during the search, then an exact (canonical) rerank of the result list.
-template <int V, int NB, int PREC>
+// R: gather rounds per batch (the launch autotuner picks 8 or 4; fewer rounds hold fewer
+// rows in registers, so more waves fit a SIMD, each with fewer loads in flight).
+template <int V, int NB, int PREC, int R = 8>
 struct Searcher {
-    // gather rounds per batch: 8 keeps the BF16 kernel at 4 waves per SIMD (more waves in
-    // flight beat fewer, larger batches for this latency-bound loop)
-    static constexpr int ROUNDS = 8;
+    static constexpr int ROUNDS = R;
     static constexpr int BATCH = 4 * ROUNDS;           // children per batch
     const BeamArgs& a;
     float qv[V];
@@ -375,7 +375,7 @@ struct Searcher {
     }
 };
 
-template <int V, int NB, int PREC>
+template <int V, int NB, int PREC, int R>
 __device__ __forceinline__ void beam_body(const BeamArgs& a)
 {
     __shared__ uint32_t s_cid[WAVES][64];
@@ -384,7 +384,7 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
     __shared__ uint32_t s_dg[WAVES][64];
     const int w = threadIdx.x >> 6;
     const int64_t slot = (int64_t)blockIdx.x * WAVES + w;
-    Searcher<V, NB, PREC> S(a);
+    Searcher<V, NB, PREC, R> S(a);
     S.lane = lane_id();
     S.g = S.lane >> 4;
     S.j = S.lane & 15;
@@ -471,17 +471,17 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
 }
 
 // searchbatch! / allknn! launches
-template <int V, int NB>
+template <int V, int NB, int R = 8>
 __global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
 {
-    beam_body<V, NB, 0>(a);
+    beam_body<V, NB, 0, R>(a);
 }
 
 // precision form: BF16 gathers, FP32 arithmetic, exact rerank of the result list
-template <int V, int NB>
+template <int V, int NB, int R = 8>
 __global__ __launch_bounds__(256) void beam_bf16_kernel(BeamArgs a)
 {
-    beam_body<V, NB, 1>(a);
+    beam_body<V, NB, 1, R>(a);
 }
 
 // the same search issued by index! and optimize_index! (a separate symbol keeps
@@ -489,27 +489,32 @@ __global__ __launch_bounds__(256) void beam_bf16_kernel(BeamArgs a)
 template <int V, int NB>
 __global__ __launch_bounds__(256) void beam_build_kernel(BeamArgs a)
 {
-    beam_body<V, NB, 0>(a);
+    beam_body<V, NB, 0, 8>(a);
 }
 
 template <int V>
 hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
 {
     const dim3 grid((unsigned)(nslots / WAVES)), block(256);
+    const bool r4 = a.rounds == 4 && a.beamcap <= 64;  // the 4-round variants exist for one beam register
     if (a.internal) {
         if (a.beamcap <= 64)
             hipLaunchKernelGGL((beam_build_kernel<V, 1>), grid, block, 0, s, a);
         else
             hipLaunchKernelGGL((beam_build_kernel<V, 4>), grid, block, 0, s, a);
     } else if (a.prec == 1) {
-        if (a.beamcap <= 64)
+        if (r4)
+            hipLaunchKernelGGL((beam_bf16_kernel<V, 1, 4>), grid, block, 0, s, a);
+        else if (a.beamcap <= 64)
             hipLaunchKernelGGL((beam_bf16_kernel<V, 1>), grid, block, 0, s, a);
         else if (a.beamcap <= 128)
             hipLaunchKernelGGL((beam_bf16_kernel<V, 2>), grid, block, 0, s, a);
         else
             hipLaunchKernelGGL((beam_bf16_kernel<V, 4>), grid, block, 0, s, a);
     } else {
-        if (a.beamcap <= 64)
+        if (r4)
+            hipLaunchKernelGGL((beam_kernel<V, 1, 4>), grid, block, 0, s, a);
+        else if (a.beamcap <= 64)
             hipLaunchKernelGGL((beam_kernel<V, 1>), grid, block, 0, s, a);
         else
             hipLaunchKernelGGL((beam_kernel<V, 4>), grid, block, 0, s, a);

@@ -76,6 +76,10 @@ struct ssg_handle {
     DevBuf vis, epochs, counter;
     int64_t vis_stride = 0;
     int nslots = 0;
+    // launch shape of the beam searches (ssg_set_launch / ssg_optimize_launch): persistent
+    // wave slots (each with its visited slot) per CU, and gather rounds per batch
+    int32_t slots_per_cu = 32;
+    int32_t rounds = 8;
     // exhaustive work
     DevBuf qpad, qnorm, cand_d, cand_i, slice_tau, flags, part_i, part_d, xList;
     // host-call staging

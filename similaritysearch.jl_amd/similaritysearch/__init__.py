@@ -425,6 +425,30 @@ class SearchGraph(_GpuIndex):
         G.algo = BeamSearch(bp.bsize, bp.delta, bp.maxvisits)
         return G
 
+    def set_launch(self, slots_per_cu=32, rounds=8):
+        """Launch shape of this index's searches (ssg_set_launch; results do not depend on it)."""
+        _lib.check(_lib.load().ssg_set_launch(self._handle(), C.byref(_lib.LaunchParams(
+            int(slots_per_cu), int(rounds)))), "set launch")
+
+    def launch(self):
+        lp = _lib.LaunchParams()
+        _lib.check(_lib.load().ssg_get_launch(self._handle(), C.byref(lp)), "get launch")
+        return lp.slots_per_cu, lp.rounds
+
+    def optimize_launch_dev(self, bp, dQ, m, k, kstride, gold, minrecall, qo=None):
+        """ssg_optimize_launch over m device-resident queries (qo: device QueryOpts) with
+        host gold ids (m, kg): installs the fastest launch shape among the recall-changing
+        knobs that reach minrecall and writes the chosen rerank/expand into bp.  Returns
+        ((slots_per_cu, rounds), recall, ms)."""
+        gold = np.ascontiguousarray(gold, np.uint32)
+        lp = _lib.LaunchParams()
+        rec, ms = C.c_double(), C.c_float()
+        _lib.check(_lib.load().ssg_optimize_launch(
+            self._handle(), dQ, m, k, kstride, C.byref(bp), None if qo is None else C.byref(qo),
+            gold.ctypes.data, gold.shape[1], float(minrecall), C.byref(lp), C.byref(rec),
+            C.byref(ms)), "optimize launch")
+        return (lp.slots_per_cu, lp.rounds), rec.value, ms.value
+
     def optimize(self, ctx, minrecall, queries, ksearch=10):
         """optimize_index!(G, ctx, MinRecall(r); queries, ksearch) (src/opt.jl:220-317);
         installs and returns the chosen BeamSearch and the sample recall."""

@@ -41,6 +41,10 @@ class BuildParams(C.Structure):
                 ("verbose", C.c_int32)]
 
 
+class LaunchParams(C.Structure):
+    _fields_ = [("slots_per_cu", C.c_int32), ("rounds", C.c_int32)]
+
+
 # name -> (restype, argtypes); the exported surface of include/ssg.h
 SIGNATURES = {
     "ssg_dev_malloc": (C.c_int, [C.POINTER(_p), C.c_size_t]),
@@ -90,6 +94,12 @@ SIGNATURES = {
                                      C.POINTER(QueryOpts), _p, _p, _p, _p, _p]),
     "ssg_beamsearch_noalloc_dev": (C.c_int, [_p, _p, _i64, _i32, _i32, C.POINTER(BeamParams),
                                              C.POINTER(QueryOpts), _p, _p, _p, _p, _p]),
+    "ssg_set_launch": (C.c_int, [_p, C.POINTER(LaunchParams)]),
+    "ssg_get_launch": (C.c_int, [_p, C.POINTER(LaunchParams)]),
+    "ssg_optimize_launch": (C.c_int, [_p, _p, _i64, _i32, _i32, C.POINTER(BeamParams),
+                                      C.POINTER(QueryOpts), _p, _i32, C.c_float,
+                                      C.POINTER(LaunchParams), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_float)]),
     "ssg_allknn_graph": (C.c_int, [_p, _i32, C.POINTER(BeamParams), _p, _p]),
     "ssg_allknn_exhaustive": (C.c_int, [_p, _i32, _p, _p]),
     "ssg_order_by_bsize_dev": (C.c_int, [_p, _p, _i64, _p, _p]),

@@ -8,6 +8,9 @@ against the oracle on a 20,000 x 128 Gaussian mixture with a graph built on the 
   arithmetic, then rerank! with the canonical distance); its recall is within 0.002 of
   the FP32 search's.
 - Mixed per-query k / bsize batch, longest-first order and HIP-graph replay.
+- The 4-round gather kernels and other launch shapes (ssg_set_launch) give the same bits;
+  the launch autotuner (ssg_optimize_launch) installs a shape and reports the recall the
+  search reaches.
 - Learned form at D = 128, H = 256, L = 4096: one training step's gradients against the
   FP64 oracle (the forward output layer runs on the 128-tile GEMM), inference against a
   NumPy forward, and the planned search bit-exact against the oracle with the same hints
@@ -153,6 +156,74 @@ def test_mixed_batch_dim128_under_graph_replay(oracle, g128, prec):
             check()
     finally:
         L.ssg_graph_destroy(g)
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+@pytest.mark.parametrize("shape", [(8, 4), (16, 4), (64, 8), (4, 8)])
+def test_launch_shapes_are_bit_exact(oracle, g128, prec, shape):
+    import similaritysearch as ss
+    from similaritysearch import _lib
+    X, Q, G, offs = g128
+    L = _lib.load()
+    h = G._handle()
+    _lib.check(L.ssg_set_launch(h, C.byref(_lib.LaunchParams(*shape))), "set launch")
+    try:
+        G.algo = ss.BeamSearch(24, 1.1, 10**6)
+        rr = 14 if prec == "bf16" else None
+        ctx = ss.SearchGraphContext(precision=prec, rerank=rr or 0)
+        ids, d = ss.searchbatch(G, ctx, ss.MatrixDatabase.from_rows(Q), 10)
+        gi, gd, gcd, gcb = oracle.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, Q, 10, 24,
+                                             1.1, 10**6, nthreads=8, bf16_rerank=rr)
+        np.testing.assert_array_equal(ids.T, gi)
+        np.testing.assert_array_equal(d.T, gd)
+        np.testing.assert_array_equal(ctx.costdists, gcd)
+        np.testing.assert_array_equal(ctx.costblocks, gcb)
+    finally:
+        _lib.check(L.ssg_set_launch(h, C.byref(_lib.LaunchParams(32, 8))), "set launch")
+
+
+def test_launch_autotuner(oracle, g128):
+    """ssg_optimize_launch over a mixed batch (per-query k and bsize) in the BF16 form:
+    the shape it installs is one of the grid, the rerank it picks keeps the recall it
+    reports, and that recall is the one recomputed from a search with the tuned knobs."""
+    import similaritysearch as ss
+    from similaritysearch import _lib, datasets
+    from similaritysearch.device import DeviceArray, Stream
+    X, _, G, offs = g128
+    L = _lib.load()
+    h = G._handle()
+    rng = np.random.default_rng(208)
+    Q = datasets.gmm(1200, DIM, seed=209)
+    m = len(Q)
+    kq = rng.choice([1, 10, 50], m).astype(np.int32)
+    bq = rng.choice([16, 32, 64], m).astype(np.int32)
+    gold, _ = oracle.exhaustive("SqL2", X, Q, 10, nthreads=8)
+    dQ, dk, db = DeviceArray.from_numpy(Q), DeviceArray.from_numpy(kq), DeviceArray.from_numpy(bq)
+    qo = ss.QueryOpts(dk.ptr, db.ptr, None, None, 0, None)
+    bp = ss.BeamParams(16, 1.1, 10**6, 64, 1, 12, 1)
+    lp = _lib.LaunchParams()
+    rec, ms = C.c_double(), C.c_float()
+    gold = np.ascontiguousarray(gold, np.uint32)
+    try:
+        _lib.check(L.ssg_optimize_launch(h, dQ.ptr, m, 50, 50, C.byref(bp), C.byref(qo),
+                                         gold.ctypes.data, 10, 0.95, C.byref(lp), C.byref(rec),
+                                         C.byref(ms)), "optimize launch")
+        assert lp.slots_per_cu in (8, 16, 24, 32, 48, 64) and lp.rounds in (4, 8)
+        assert bp.rerank in (12, 14, 16, 18, 24) and bp.expand in (1, 2) and ms.value > 0
+        got = _lib.LaunchParams()
+        _lib.check(L.ssg_get_launch(h, C.byref(got)), "get launch")
+        assert (got.slots_per_cu, got.rounds) == (lp.slots_per_cu, lp.rounds)
+        dI, dD = DeviceArray((m, 50), np.uint32), DeviceArray((m, 50), np.float32)
+        st = Stream()
+        _lib.check(L.ssg_beamsearch_dev(h, dQ.ptr, m, 50, 50, C.byref(bp), C.byref(qo), dI.ptr,
+                                        dD.ptr, None, None, st.handle), "search")
+        st.synchronize()
+        ids = dI.numpy()
+        kk = np.minimum(kq, 10)
+        r = np.mean([len(set(ids[j, :kk[j]]) & set(gold[j, :kk[j]])) / kk[j] for j in range(m)])
+        assert r == pytest.approx(rec.value)
+    finally:
+        _lib.check(L.ssg_set_launch(h, C.byref(_lib.LaunchParams(32, 8))), "set launch")
 
 
 def test_mlp_gradients_at_learned_shape(oracle):
