@@ -234,9 +234,9 @@ int ssg_graph_destroy(ssg_graph* graph);
 
 /* Launch shape of the handle's beam searches (no reference counterpart): persistent wave
  * slots per CU (each wave owns a visited slot of n bytes; 4..64, default 32) and gather
- * rounds per batch (8, the default, or 4: four rows in flight per 16-lane group per round,
- * fewer registers; the 4-round kernels serve beam caps <= 64).  Results do not depend on
- * it. */
+ * rounds per batch (each round gathers one row per 16-lane group: 8, the default; 4, fewer
+ * registers; 10, in the BF16 form, 40 children per batch; the 4- and 10-round kernels serve
+ * beam caps <= 64, others run 8).  Results do not depend on it. */
 typedef struct {
     int32_t slots_per_cu;
     int32_t rounds;
@@ -253,7 +253,7 @@ int ssg_get_launch(ssg_handle* h, ssg_launch_params* lp);
  * measures recall against gold (host, m x kg 1-based ids in any order; query j is scored
  * on its first min(k_j, kg) results against its first min(k_j, kg) gold ids).  Among the
  * knobs reaching minrecall (else the best recall) it times every launch shape
- * (slots_per_cu in {8, 16, 24, 32, 48, 64} x rounds in {4, 8}) with HIP events and keeps
+ * (slots_per_cu in {8, 16, 24, 32, 48, 64} x rounds in {4, 8, 10 (BF16)}) with HIP events and keeps
  * the fastest.  It installs that shape, writes the chosen rerank/expand into *bp, and
  * returns the shape, its recall and its time (ms).  Uses the handle's stream. */
 int ssg_optimize_launch(ssg_handle* h, const float* dQ, int64_t m, int32_t k, int32_t kstride,

@@ -66,8 +66,9 @@ __device__ __forceinline__ void vis_put(uint8_t* p, uint8_t v)
 
 // PREC 0: canonical distance on the FP32 vectors.  PREC 1: BF16 vectors with FP32
 // arithmetic during the search, then an exact (canonical) rerank of the result list.
-// R: gather rounds per batch (the launch autotuner picks 8 or 4; fewer rounds hold fewer
-// rows in registers, so more waves fit a SIMD, each with fewer loads in flight).
+// R: gather rounds per batch (the launch autotuner picks 4, 8 or, in the BF16 form, 10;
+// fewer rounds hold fewer rows in registers, so more waves may fit a SIMD, each with fewer
+// loads in flight; more rounds take an expansion's children in fewer round trips).
 template <int V, int NB, int PREC, int R = 8>
 struct Searcher {
     static constexpr int ROUNDS = R;
@@ -496,7 +497,10 @@ template <int V>
 hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
 {
     const dim3 grid((unsigned)(nslots / WAVES)), block(256);
-    const bool r4 = a.rounds == 4 && a.beamcap <= 64;  // the 4-round variants exist for one beam register
+    // the 4- and 10-round variants exist for one beam register (beam caps <= 64); 10 rounds
+    // (40 children per batch) only in the BF16 form, where its rows still leave 4 waves per SIMD
+    const bool r4 = a.rounds == 4 && a.beamcap <= 64;
+    const bool r10 = a.rounds == 10 && a.beamcap <= 64;
     if (a.internal) {
         if (a.beamcap <= 64)
             hipLaunchKernelGGL((beam_build_kernel<V, 1>), grid, block, 0, s, a);
@@ -505,6 +509,8 @@ hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
     } else if (a.prec == 1) {
         if (r4)
             hipLaunchKernelGGL((beam_bf16_kernel<V, 1, 4>), grid, block, 0, s, a);
+        else if (r10)
+            hipLaunchKernelGGL((beam_bf16_kernel<V, 1, 10>), grid, block, 0, s, a);
         else if (a.beamcap <= 64)
             hipLaunchKernelGGL((beam_bf16_kernel<V, 1>), grid, block, 0, s, a);
         else if (a.beamcap <= 128)

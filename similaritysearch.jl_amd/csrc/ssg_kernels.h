@@ -51,7 +51,7 @@ struct BeamArgs {
     int32_t prec;            // 0: canonical FP32 path, 1: BF16 gathers + exact rerank
     const uint16_t* Xh;      // BF16 copy of X (n rows of dp), prec 1
     int32_t krr;             // prec 1: query j reranks max(k_j, krr) result entries exactly
-    int32_t rounds;          // gather rounds per batch: 8, or 4 (beam caps <= 64)
+    int32_t rounds;          // gather rounds per batch: 8, or 4 / 10 (beam caps <= 64; 10: BF16 form)
     int32_t expand;          // prec 1: beam entries popped per step (1 = reference step)
     const uint32_t* order;   // nullable: order[t] = the t-th query to search (0-based)
     // Query layout (nullable): objects stored in another order.  adjn[e] = {start, degree}

@@ -18,7 +18,8 @@ int check_shape(const ssg_launch_params* lp)
     if (!lp) return ssg_fail(SSG_E_ARG, "null launch parameters");
     if (lp->slots_per_cu < 4 || lp->slots_per_cu > 64)
         return ssg_fail(SSG_E_ARG, "slots_per_cu must be in 4..64");
-    if (lp->rounds != 4 && lp->rounds != 8) return ssg_fail(SSG_E_ARG, "rounds must be 4 or 8");
+    if (lp->rounds != 4 && lp->rounds != 8 && lp->rounds != 10)
+        return ssg_fail(SSG_E_ARG, "rounds must be 4, 8 or 10");
     return SSG_OK;
 }
 
@@ -156,7 +157,8 @@ int ssg_optimize_launch(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     bool have = false;
     for (size_t kn : keep) {
         for (int32_t sp : {8, 16, 24, 32, 48, 64}) {
-            for (int32_t rd : {4, 8}) {
+            for (int32_t rd : {4, 8, 10}) {
+                if (rd == 10 && knobs[kn].precision != SSG_BF16) continue;  // BF16 kernels only
                 h->slots_per_cu = sp;
                 h->rounds = rd;
                 float ms = 0.f;
