@@ -99,6 +99,14 @@ def _traffic(name, key, applies, scale=1):
         return int(json.load(fh)[key] * scale)
 
 
+def _kname(prec, beamcap, rounds):
+    """The beam kernel instantiation a launch runs (ssg_beam.h launch_v): beam registers
+    1/2/4 by beam cap, gather rounds 4 or 10 only with one beam register (10: BF16 form)."""
+    nb = 1 if beamcap <= 64 else 2 if (prec and beamcap <= 128) else 4
+    r = rounds if nb == 1 and (rounds == 4 or (rounds == 10 and prec)) else 8
+    return f"{'beam_bf16_kernel' if prec else 'beam_kernel'}<8, {nb}, {r}>"
+
+
 def _capture(L, _lib, st, fn):
     """Runs fn() once (sizing every buffer), then records it into a HIP graph."""
     fn()
@@ -375,8 +383,8 @@ def search_bench(args):
     # the same BF16 search and rerank from the global hints with one beam size, the smallest
     # reaching the learned form's tuning margin (the tuner's fallback search) for every
     # query; all steps in one graph replay
-    actx = ss.SearchGraphContext(beamcap=256, precision="bf16", rerank=rr)
     ab = fb
+    actx = ss.SearchGraphContext(beamcap=max(32, ab), precision="bf16", rerank=rr)
     G.algo = ss.BeamSearch(ab, delta, 10**6)
     bp_a = G.beam_params(actx)
     ashape, _, _ = G.optimize_launch_dev(bp_a, dQt.ptr, len(Qtune), k, k, gold_tune,
@@ -486,7 +494,7 @@ def search_bench(args):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": "profiles/beam_search_learned_bf16_1m128_traffic.md" if traffic else None,
-                     "kernel": f"beam_bf16_kernel<8,{1 if cap <= 64 else 2}>",
+                     "kernel": _kname(1, bp_l.beamcap, lshape[1]),
                      "kernel_ms": round(sms, 3), "algorithmic_bytes_per_launch": int(lbytes)},
         "cpu_baseline": cpu,
     }
@@ -647,7 +655,7 @@ def batching_run(args, G, X, compact):
                      "frac": round(f["achieved_GBs"] / HBM_PEAK_GBS, 4),
                      "traffic": _traffic("beam_search_batching_1m128_traffic.json", "hbm_bytes_per_launch",
                                          n == 1_000_000 and m == 200_000),
-                     "kernel": "beam_kernel<8,4>", "kernel_ms": f["graph_replay_ms"],
+                     "kernel": _kname(0, 256, f["launch_shape"][1]), "kernel_ms": f["graph_replay_ms"],
                      "algorithmic_bytes_per_launch": f["algorithmic_bytes_per_launch"]},
         "cpu_baseline": cpu,
     }
