@@ -390,6 +390,216 @@ __global__ __launch_bounds__(64) void topk_heads_kernel(const float* logits, int
     }
 }
 
+// Fused output layer of inference (hidden width K in {64, 128, 192, 256}, H <= 4 hints,
+// NB <= 16 buckets): the logits z = A2 . W3^T + b3 of a block of 128 queries are formed 32
+// output units at a time (W3 tiles staged in LDS, double-buffered) and folded at once into
+// each query's H best landmark units (value descending, index ascending), the softmax sum
+// of its landmark head and its bucket logits; none of the L + NB logits per query is
+// stored.  The products are those of gemm_f16_kernel (v_mfma_f32_16x16x16f16 in ascending
+// k, queries as the A operand and units as B, the bias added after), so the logits, the
+// landmarks and the buckets equal the unfused path's; the softmax sum (the confidence) is
+// added in another order.
+//
+// A lane holds, per 16 x 16 product, 4 queries (rows 4g .. 4g+3 of a query tile) of one
+// unit (column `col`), so it folds units col, col + 16, ... of each of its queries; the 16
+// lanes of a group are merged by butterfly at the end.  W3 rows are staged in LDS with the
+// 4 k of each lane group and product placed next to each other, so a lane reads its share
+// of a unit row as one run of K / 2 bytes.
+constexpr int HT_TU = 32;  // output units per LDS tile
+constexpr int HT_NQ = 2;   // 16-query tiles per wave: 32 queries per wave, 128 per block
+
+template <int H>
+__device__ __forceinline__ void top_insert(float (&tv)[H], int (&ti)[H], float v, int u)
+{
+#pragma unroll
+    for (int j = 0; j < H; j++) {
+        const bool sw = v > tv[j] || (v == tv[j] && u < ti[j]);
+        const float tf = tv[j];
+        const int tn = ti[j];
+        tv[j] = sw ? v : tf;
+        ti[j] = sw ? u : tn;
+        v = sw ? tf : v;
+        u = sw ? tn : u;
+    }
+}
+
+template <int K, int H>
+__global__ __launch_bounds__(256, 2) void head_top_kernel(const _Float16* __restrict__ A2, int B,
+                                                          const _Float16* __restrict__ W,
+                                                          const float* __restrict__ bias, int L, int NB,
+                                                          float tau, const uint32_t* __restrict__ landmarks,
+                                                          uint32_t* __restrict__ hints,
+                                                          int32_t* __restrict__ bucket, float* __restrict__ conf)
+{
+    constexpr int KS = K / 16;         // products per 16 x 16 output tile
+    constexpr int LB = K / 2;          // bytes of a lane group's share of a unit row
+    constexpr int RS = 2 * K + 16;     // LDS bytes per unit row (16 bytes of padding)
+    constexpr int CH = HT_TU * K / 8;  // 16-byte chunks per tile
+    constexpr int PF = CH / 256;       // chunks per thread
+    static_assert(K % 64 == 0 && K <= 256 && CH % 256 == 0 && PF <= 4, "hidden width");
+    __shared__ __attribute__((aligned(16))) uint8_t s_w[2][HT_TU * RS];
+    __shared__ float s_b[2][HT_TU];
+    __shared__ float s_zb[4][16 * HT_NQ][16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int col = lane & 15, g = lane >> 4;
+    const int O = L + NB;
+    const int ntiles = (O + HT_TU - 1) / HT_TU;
+    const int64_t wq0 = (int64_t)blockIdx.x * (64 * HT_NQ) + (int64_t)w * (16 * HT_NQ);
+
+    // the wave's queries as A operands: row col of query tile q, k = 16 s + 4 g .. + 3
+    half4 aq[HT_NQ][KS];
+#pragma unroll
+    for (int q = 0; q < HT_NQ; q++) {
+        const int64_t r = min(wq0 + 16 * q + col, (int64_t)B - 1);
+        const _Float16* p = A2 + r * K + 4 * g;
+#pragma unroll
+        for (int s = 0; s < KS; s++) aq[q][s] = *reinterpret_cast<const half4*>(p + 16 * s);
+    }
+    // per (query tile, row i): the H best landmark units and the softmax sum
+    float tv[HT_NQ][4][H], se[HT_NQ][4];
+    int ti[HT_NQ][4][H];
+#pragma unroll
+    for (int q = 0; q < HT_NQ; q++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            se[q][i] = 0.f;
+#pragma unroll
+            for (int j = 0; j < H; j++) {
+                tv[q][i][j] = -__builtin_inff();
+                ti[q][i][j] = 0x7fffffff;
+            }
+        }
+
+    uint4 pf[PF];
+    float pfb = 0.f;
+    auto prefetch = [&](int t) {
+        const int u0 = t * HT_TU;
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+            const int c = tid + 256 * i, ur = c / (K / 8), cr = c % (K / 8);
+            const int64_t u = min(u0 + ur, O - 1);  // units past O: see the epilogue
+            pf[i] = reinterpret_cast<const uint4*>(W + u * K)[cr];
+        }
+        if (tid < HT_TU) pfb = bias[min(u0 + tid, O - 1)];
+    };
+    // chunk cr holds k = 8 cr .. 8 cr + 7 of its row: product cr / 2, lane groups
+    // 2 (cr % 2) (its first 4 k) and 2 (cr % 2) + 1 (the last 4)
+    auto stage = [&](int b) {
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+            const int c = tid + 256 * i, ur = c / (K / 8), cr = c % (K / 8);
+            uint8_t* rp = s_w[b] + ur * RS + (cr >> 1) * 8;
+            *reinterpret_cast<uint2*>(rp + (2 * (cr & 1)) * LB) = make_uint2(pf[i].x, pf[i].y);
+            *reinterpret_cast<uint2*>(rp + (2 * (cr & 1) + 1) * LB) = make_uint2(pf[i].z, pf[i].w);
+        }
+        if (tid < HT_TU) s_b[b][tid] = pfb;
+    };
+
+    prefetch(0);
+    stage(0);
+    __syncthreads();
+    int b = 0;
+    for (int t = 0; t < ntiles; t++) {
+        const bool more = t + 1 < ntiles;
+        if (more) prefetch(t + 1);
+#pragma unroll
+        for (int us = 0; us < HT_TU / 16; us++) {
+            const uint8_t* rp = s_w[b] + (us * 16 + col) * RS + g * LB;
+            floatx4v acc[HT_NQ];
+#pragma unroll
+            for (int q = 0; q < HT_NQ; q++) acc[q] = floatx4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s2 = 0; s2 < KS / 2; s2++) {
+                const uint4 v = *reinterpret_cast<const uint4*>(rp + 16 * s2);
+                const half4 b0 = __builtin_bit_cast(half4, make_uint2(v.x, v.y));
+                const half4 b1 = __builtin_bit_cast(half4, make_uint2(v.z, v.w));
+#pragma unroll
+                for (int q = 0; q < HT_NQ; q++) {
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x16f16(aq[q][2 * s2], b0, acc[q], 0, 0, 0);
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x16f16(aq[q][2 * s2 + 1], b1, acc[q], 0, 0, 0);
+                }
+            }
+            const int u = t * HT_TU + us * 16 + col;  // this lane's unit
+            const float bu = s_b[b][us * 16 + col];
+            const bool land = u < L, buck = u >= L && u < O;
+#pragma unroll
+            for (int q = 0; q < HT_NQ; q++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float v = acc[q][i] + bu;
+                    if (buck) s_zb[w][16 * q + 4 * g + i][u - L] = v;
+                    if (land) {
+                        // online softmax sum against the running maximum, then the top-H list
+                        const float m0 = tv[q][i][0];
+                        const bool up = v > m0;
+                        const float e = __expf(up ? m0 - v : v - m0);
+                        se[q][i] = up ? se[q][i] * e + 1.f : se[q][i] + e;
+                        top_insert<H>(tv[q][i], ti[q][i], v, u);
+                    }
+                }
+        }
+        if (more) stage(b ^ 1);
+        __syncthreads();
+        b ^= 1;
+    }
+    // merge the 16 lanes of each group (units of every residue mod 16)
+#pragma unroll
+    for (int q = 0; q < HT_NQ; q++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                const float ose = __shfl_xor(se[q][i], off);
+                float ov[H];
+                int oi[H];
+#pragma unroll
+                for (int j = 0; j < H; j++) {
+                    ov[j] = __shfl_xor(tv[q][i][j], off);
+                    oi[j] = __shfl_xor(ti[q][i][j], off);
+                }
+                const float ma = tv[q][i][0], mb = ov[0], nm = fmaxf(ma, mb);
+                se[q][i] = (ma == -__builtin_inff() ? 0.f : se[q][i] * __expf(ma - nm)) +
+                           (mb == -__builtin_inff() ? 0.f : ose * __expf(mb - nm));
+#pragma unroll
+                for (int j = 0; j < H; j++) top_insert<H>(tv[q][i], ti[q][i], ov[j], oi[j]);
+            }
+        }
+    // lane col < 4 of group g writes query row 4g + col of each query tile
+#pragma unroll
+    for (int q = 0; q < HT_NQ; q++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int64_t row = wq0 + 16 * q + 4 * g + i;
+            if (col != i || row >= B) continue;
+            conf[row] = 1.f / se[q][i];
+#pragma unroll
+            for (int j = 0; j < H; j++) hints[row * H + j] = landmarks[ti[q][i][j]];
+            const float* z = s_zb[w][16 * q + 4 * g + i];
+            float bb = -__builtin_inff();
+            int bix = 0;
+            for (int c = 0; c < NB; c++)
+                if (z[c] > bb) {
+                    bb = z[c];
+                    bix = c;
+                }
+            if (tau > 0.f) {
+                // smallest bucket whose cumulative predicted probability reaches tau
+                float tot = 0.f;
+                for (int c = 0; c < NB; c++) tot += __expf(z[c] - bb);
+                float ac = 0.f;
+                bix = NB - 1;
+                for (int c = 0; c < NB; c++) {
+                    ac += __expf(z[c] - bb);
+                    if (ac >= tau * tot) {
+                        bix = c;
+                        break;
+                    }
+                }
+            }
+            bucket[row] = bix;
+        }
+}
+
 struct BucketTable {
     int32_t v[16];
 };
@@ -484,7 +694,8 @@ struct ssg_mlp {
     DevBuf X16, C1, P1, A1, C2, P2, A2, Z, dZ16, dA32, dA16, loss;
     DevBuf xin, yl, yb, perm;  // training inputs of one batch / one epoch
     DevBuf colpart;            // column-sum partials
-    int cap_b = 0;
+    DevBuf X16i, A1i, A2i;     // inference activations (fused output layer)
+    int cap_b = 0, cap_i = 0;
     int64_t step = 0;
     std::vector<uint32_t> landmarks;
     DevBuf dland;
@@ -823,6 +1034,73 @@ int ssg_mlp_grad(ssg_mlp* p, int32_t layer, float* gW, float* gb)
     return SSG_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The fused output layer (head_top_kernel) serves hidden widths 64, 128, 192 and 256, up
+// to 4 hints and 16 buckets; other shapes store the logits and reduce them afterwards.
+bool fused_head_ok(const ssg_mlp* p, int h)
+{
+    return h <= 4 && p->NB <= 16 && (p->H2 == 64 || p->H2 == 128 || p->H2 == 192 || p->H2 == 256);
+}
+
+template <int K>
+void launch_head_top(ssg_mlp* p, int B, const uint32_t* dland, int h, float tau, uint32_t* dhints,
+                     int32_t* dbucket, float* dconf)
+{
+    const dim3 grid((unsigned)((B + 64 * HT_NQ - 1) / (64 * HT_NQ)));
+    const _Float16* A2 = p->A2i.as<_Float16>();
+    const _Float16* W = p->Wh[2].as<_Float16>();
+    const float* bias = p->b[2].as<float>();
+#define SSG_HT(HH)                                                                                  \
+    hipLaunchKernelGGL((head_top_kernel<K, HH>), grid, dim3(256), 0, p->stream, A2, B, W, bias, p->L, \
+                       p->NB, tau, dland, dhints, dbucket, dconf)
+    switch (h) {
+    case 1: SSG_HT(1); break;
+    case 2: SSG_HT(2); break;
+    case 3: SSG_HT(3); break;
+    default: SSG_HT(4); break;
+    }
+#undef SSG_HT
+}
+
+// Inference with the fused output layer: the hidden layers by gemm_f16_kernel (FP16
+// activations only), then head_top_kernel, in passes of up to 2^18 rows.
+int infer_fused(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dland, int h, float tau,
+                uint32_t* dhints, int32_t* dbucket, float* dconf)
+{
+    const int chunk = 1 << 18;
+    hipStream_t s = p->stream;
+    for (int64_t r0 = 0; r0 < m; r0 += chunk) {
+        const int B = (int)std::min<int64_t>(chunk, m - r0);
+        if (B > p->cap_i) {
+            HIPCHK(p->X16i.ensure((size_t)B * p->D * 2), "alloc");
+            HIPCHK(p->A1i.ensure((size_t)B * p->H1 * 2), "alloc");
+            HIPCHK(p->A2i.ensure((size_t)B * p->H2 * 2), "alloc");
+            p->cap_i = B;
+        }
+        hipLaunchKernelGGL(gather_rows_f16_kernel, dim3(B), dim3(128), 0, s, dQ + r0 * p->D, p->D,
+                           (const int32_t*)nullptr, B, p->X16i.as<_Float16>());
+        HIPCHK(gemm(s, B, p->H1, p->D, p->X16i.as<_Float16>(), p->D, 1, p->Wh[0].as<_Float16>(), 1, p->D,
+                    nullptr, p->H1, 0, p->b[0].as<float>(), 1, p->A1i.as<_Float16>()), "gemm");
+        HIPCHK(gemm(s, B, p->H2, p->H1, p->A1i.as<_Float16>(), p->H1, 1, p->Wh[1].as<_Float16>(), 1, p->H1,
+                    nullptr, p->H2, 0, p->b[1].as<float>(), 1, p->A2i.as<_Float16>()), "gemm");
+        switch (p->H2) {
+        case 64: launch_head_top<64>(p, B, dland, h, tau, dhints + r0 * h, dbucket + r0, dconf + r0); break;
+        case 128: launch_head_top<128>(p, B, dland, h, tau, dhints + r0 * h, dbucket + r0, dconf + r0); break;
+        case 192: launch_head_top<192>(p, B, dland, h, tau, dhints + r0 * h, dbucket + r0, dconf + r0); break;
+        default: launch_head_top<256>(p, B, dland, h, tau, dhints + r0 * h, dbucket + r0, dconf + r0); break;
+        }
+        HIPCHK(hipGetLastError(), "inference head");
+    }
+    return SSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 // Inference on device query rows dQ [m x D] (FP32): logits, then per query the h
 // best landmarks (as 1-based object ids through `landmarks`), bucket and confidence.
 int ssg_mlp_infer_dev(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dlandmarks,
@@ -836,7 +1114,12 @@ int ssg_mlp_infer_dev(ssg_mlp* p, const float* dQ, int64_t m, const uint32_t* dl
     HIPCHK(hipSetDevice(p->device), "hipSetDevice");
     hipStream_t saved = p->stream;
     if (stream) p->stream = (hipStream_t)stream;
-    const int chunk = 32768;  // rows per inference pass
+    if (fused_head_ok(p, h)) {
+        const int rc = infer_fused(p, dQ, m, dlandmarks, h, bucket_quantile, dhints, dbucket, dconf);
+        p->stream = saved;
+        return rc;
+    }
+    const int chunk = 32768;  // rows per inference pass (the logits of a pass are stored)
     int rc = SSG_OK;
     for (int64_t r0 = 0; r0 < m && rc == SSG_OK; r0 += chunk) {
         const int B = (int)std::min<int64_t>(chunk, m - r0);
