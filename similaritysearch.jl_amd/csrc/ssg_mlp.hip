@@ -521,22 +521,36 @@ __global__ __launch_bounds__(256, 2) void head_top_kernel(const _Float16* __rest
             }
             const int u = t * HT_TU + us * 16 + col;  // this lane's unit
             const float bu = s_b[b][us * 16 + col];
-            const bool land = u < L, buck = u >= L && u < O;
-#pragma unroll
-            for (int q = 0; q < HT_NQ; q++)
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const float v = acc[q][i] + bu;
-                    if (buck) s_zb[w][16 * q + 4 * g + i][u - L] = v;
-                    if (land) {
-                        // online softmax sum against the running maximum, then the top-H list
-                        const float m0 = tv[q][i][0];
-                        const bool up = v > m0;
-                        const float e = __expf(up ? m0 - v : v - m0);
-                        se[q][i] = up ? se[q][i] * e + 1.f : se[q][i] + e;
-                        top_insert<H>(tv[q][i], ti[q][i], v, u);
-                    }
+            // online softmax sum against the running maximum (e = exp(-|v - max|)), then the
+            // top-H list; u exceeds every unit the lane has folded, so a tie keeps the entry
+            auto fold = [&](int q, int i, float v) {
+                const float m0 = tv[q][i][0];
+                const bool up = v > m0;
+                const float e = __expf(-__builtin_fabsf(v - m0));
+                se[q][i] = up ? __builtin_fmaf(se[q][i], e, 1.f) : se[q][i] + e;
+                if constexpr (H == 1) {
+                    ti[q][i][0] = up ? u : ti[q][i][0];
+                    tv[q][i][0] = up ? v : m0;
+                } else {
+                    top_insert<H>(tv[q][i], ti[q][i], v, u);
                 }
+            };
+            if (H == 1 && t * HT_TU + us * 16 + 16 <= L) {  // every unit a landmark
+#pragma unroll
+                for (int q = 0; q < HT_NQ; q++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) fold(q, i, acc[q][i] + bu);
+            } else {
+                const bool land = u < L, buck = u >= L && u < O;
+#pragma unroll
+                for (int q = 0; q < HT_NQ; q++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const float v = acc[q][i] + bu;
+                        if (buck) s_zb[w][16 * q + 4 * g + i][u - L] = v;
+                        if (land) fold(q, i, v);
+                    }
+            }
         }
         if (more) stage(b ^ 1);
         __syncthreads();
