@@ -233,13 +233,20 @@ int ssg_graph_launch(ssg_graph* graph, void* stream);
 int ssg_graph_destroy(ssg_graph* graph);
 
 /* Launch shape of the handle's beam searches (no reference counterpart): persistent wave
- * slots per CU (each wave owns a visited slot of n bytes; 4..64, default 32) and gather
+ * slots per CU (each wave owns a visited slot of n bytes; 4..64, default 32), gather
  * rounds per batch (each round gathers one row per 16-lane group: 8, the default; 4, fewer
  * registers; 10, in the BF16 form, 40 children per batch; the 4- and 10-round kernels serve
- * beam caps <= 64, others run 8).  Results do not depend on it. */
+ * beam caps <= 64, others run 8) and the LDS visited table of each wave (vbuckets buckets
+ * of eight 16-bit tags, 16 bytes each: a visited mark goes to the table while its bucket
+ * has room, else to the wave's HBM slot; -1, the default: the largest table that leaves
+ * the kernel's occupancy unchanged; 0: every mark in HBM; 1..2048: that many buckets,
+ * the LDS of a block, 4 * 16 * vbuckets bytes, then bounding how many waves share a CU.
+ * A table is used only when n <= 2^24 and vbuckets >= n / 2^15).  Results do not depend
+ * on it. */
 typedef struct {
     int32_t slots_per_cu;
     int32_t rounds;
+    int32_t vbuckets;
 } ssg_launch_params;
 int ssg_set_launch(ssg_handle* h, const ssg_launch_params* lp);
 int ssg_get_launch(ssg_handle* h, ssg_launch_params* lp);
@@ -253,7 +260,8 @@ int ssg_get_launch(ssg_handle* h, ssg_launch_params* lp);
  * measures recall against gold (host, m x kg 1-based ids in any order; query j is scored
  * on its first min(k_j, kg) results against its first min(k_j, kg) gold ids).  Among the
  * knobs reaching minrecall (else the best recall) it times every launch shape
- * (slots_per_cu in {8, 16, 24, 32, 48, 64} x rounds in {4, 8, 10 (BF16)}) with HIP events and keeps
+ * (slots_per_cu in {8, 16, 24, 32, 48, 64} x rounds in {4, 8, 10 (BF16)} x vbuckets in
+ * {-1, 0}) with HIP events and keeps
  * the fastest.  It installs that shape, writes the chosen rerank/expand into *bp, and
  * returns the shape, its recall and its time (ms).  Uses the handle's stream. */
 int ssg_optimize_launch(ssg_handle* h, const float* dQ, int64_t m, int32_t k, int32_t kstride,

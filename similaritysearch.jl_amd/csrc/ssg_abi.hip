@@ -240,6 +240,10 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     a.Xh = h->Xh.as<uint16_t>();
     a.krr = std::max(bp->rerank, 0);
     a.rounds = internal ? 8 : h->rounds;
+    // LDS visited tables: tags name at most 2^24 objects
+    a.vhb = 15;
+    while (a.vhb < 24 && ((int64_t)1 << a.vhb) < n_eff) a.vhb++;
+    a.vnb = n_eff <= ((int64_t)1 << 24) ? h->vbuckets : 0;
     a.expand = bp->expand > 1 ? bp->expand : 1;
     a.order = qo ? qo->order : nullptr;
     if (use_q) {

@@ -23,9 +23,13 @@
 //     order-dependent part (pushes into res and beam, the maxvisits cut) is then
 //     replayed serially over the children in list order with readlane, skipping
 //     children that provably change neither queue.
-//   * Visited set: one epoch byte per (wave slot, object).  A query takes the
-//     next epoch of its slot; the slot is cleared once every 255 queries.  A
-//     byte equal to the current epoch means visited, so no per-query reset.
+//   * Visited set: a per-wave table in LDS (vnb buckets of eight 16-bit tags; each
+//     object has one bucket and one tag) takes a query's marks while their buckets
+//     have room; the others go to the wave slot's epoch bytes in HBM (one byte per
+//     (wave slot, object); a query takes the next epoch of its slot, the slot is
+//     cleared once every 255 queries, a byte equal to the current epoch means
+//     visited), which are read only for ids whose bucket is full.  vnb = 0 keeps
+//     every mark in HBM.
 #pragma once
 #include <algorithm>
 
@@ -82,6 +86,9 @@ struct Searcher {
     int64_t cb;        // costblocks
     uint32_t epoch;
     uint8_t* vis;  // wave-private slot; every access is a workgroup-scope relaxed atomic
+    uint4* vt;         // wave-private LDS visited table: vnb buckets of 8 tags
+    uint32_t vnb;      // 0: every mark goes to vis
+    uint32_t vhb;      // objects are named by vhb bits
     uint32_t* cidbuf;  // 64 ids, wave-private LDS
     float* dbuf;       // 64 distances, wave-private LDS
     uint32_t* stbuf;   // 64 list starts of the compacted children (adjn layout), LDS
@@ -89,6 +96,109 @@ struct Searcher {
     int lane, g, j;
 
     __device__ __forceinline__ Searcher(const BeamArgs& args) : a(args) {}
+
+    // ---- LDS visited table ----
+    // Object c < 2^hb (15 <= hb <= 24) maps bijectively to h = c * 0x9E3779B1 mod 2^hb;
+    // its bucket is h * vnb >> hb (vnb >= 2^(hb-15), so a bucket covers at most 2^15
+    // consecutive h) and its tag (h mod 2^15) + 1, which with the bucket names c.  A bucket holds 8 tags in
+    // 16 bytes (0 = free, VT_ERASED = an erased tag, occupied for good), filled left to
+    // right and never emptied.  A mark goes to the bucket while it has a free slot and to
+    // the HBM epoch byte otherwise, so c is visited iff its tag is in its bucket, or its
+    // bucket is full and its byte holds the epoch.
+    static constexpr uint32_t VT_ERASED = 0xFFFFu;
+    __device__ __forceinline__ void vt_key(uint32_t c, uint32_t& b, uint32_t& tag) const
+    {
+        const uint32_t h = (c * 0x9E3779B1u) & ((1u << vhb) - 1u);
+        b = (uint32_t)(((uint64_t)h * vnb) >> vhb);
+        tag = (h & 0x7FFFu) + 1u;
+    }
+    __device__ __forceinline__ static bool vt_has(uint32_t w, uint32_t tag)
+    {
+        return (w & 0xFFFFu) == tag || (w >> 16) == tag;
+    }
+    __device__ __forceinline__ static bool vt_fullw(uint32_t w)
+    {
+        return (w & 0xFFFFu) != 0u && (w >> 16) != 0u;
+    }
+    // 1: visited, 0: not visited, 2: not in the table and the bucket is full (ask HBM)
+    __device__ __forceinline__ int vt_test(uint32_t c) const
+    {
+        uint32_t b, tag;
+        vt_key(c, b, tag);
+        const uint4 e = vt[b];
+        if (vt_has(e.x, tag) || vt_has(e.y, tag) || vt_has(e.z, tag) || vt_has(e.w, tag)) return 1;
+        return vt_fullw(e.x) && vt_fullw(e.y) && vt_fullw(e.z) && vt_fullw(e.w) ? 2 : 0;
+    }
+    // Insert c (absent): claim the first free slot by compare-and-swap of its 32-bit word
+    // (lanes of one call may race for a bucket; a loser looks again).  False: bucket full.
+    __device__ __forceinline__ bool vt_insert(uint32_t c)
+    {
+        uint32_t b, tag;
+        vt_key(c, b, tag);
+        uint32_t* wp = reinterpret_cast<uint32_t*>(vt + b);
+        for (;;) {
+            const uint4 e = vt[b];
+            const uint32_t ws[4] = {e.x, e.y, e.z, e.w};
+            int q = -1;
+            uint32_t ow = 0u, nw = 0u;
+#pragma unroll
+            for (int t = 3; t >= 0; t--) {
+                if ((ws[t] & 0xFFFFu) == 0u) {
+                    q = t;
+                    ow = ws[t];
+                    nw = ws[t] | tag;
+                } else if ((ws[t] >> 16) == 0u) {
+                    q = t;
+                    ow = ws[t];
+                    nw = ws[t] | (tag << 16);
+                }
+            }
+            if (q < 0) return false;
+            if (atomicCAS(wp + q, ow, nw) == ow) return true;
+        }
+    }
+    // Erase c: its tag becomes VT_ERASED (still occupied).  False: c is not in the table.
+    __device__ __forceinline__ bool vt_erase(uint32_t c)
+    {
+        uint32_t b, tag;
+        vt_key(c, b, tag);
+        uint16_t* tp = reinterpret_cast<uint16_t*>(vt + b);
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            if (tp[t] == tag) {
+                tp[t] = (uint16_t)VT_ERASED;
+                return true;
+            }
+        return false;
+    }
+    // Start of a query: an empty table.
+    __device__ __forceinline__ void vt_reset()
+    {
+        if (!vnb) return;
+        for (uint32_t b = (uint32_t)lane; b < vnb; b += 64) vt[b] = make_uint4(0u, 0u, 0u, 0u);
+        lds_order();
+    }
+    // Has c been visited by this query (the same answer in every lane)?
+    __device__ __forceinline__ bool visited_uniform(uint32_t c)
+    {
+        int st = vnb ? vt_test(c) : 2;
+        if (st == 2) {
+            vm_drain();
+            st = vis_get(vis + c) == (uint8_t)epoch ? 1 : 0;
+        }
+        return rfl_u((uint32_t)st) == 1u;
+    }
+    // Mark one id (the same c in every lane) visited.
+    __device__ __forceinline__ void mark_uniform(uint32_t c)
+    {
+        bool in_t = false;
+        if (vnb) {
+            if (lane == 0) in_t = vt_insert(c);
+            lds_order();
+        }
+        if (!in_t && lane == 0) vis_put(vis + c, (uint8_t)epoch);
+        vm_drain();
+    }
 
     // Evaluate cidbuf[ub .. ub+cnt) (1 <= cnt <= BATCH).  Child u = 4r + q is gathered
     // by row q in round r; rows past the batch repeat its last child, so every load is
@@ -152,8 +262,17 @@ struct Searcher {
         valid = valid && (int64_t)cid < a.n;
         bool unv = valid;
         if (mode != MODE_BRUTE) {
-            vm_drain();
-            if (valid) unv = vis_get(vis + cid) != (uint8_t)epoch;
+            if (vnb) {
+                int st = valid ? vt_test(cid) : 1;
+                if (__ballot(st == 2)) {
+                    vm_drain();
+                    if (st == 2) st = vis_get(vis + cid) == (uint8_t)epoch ? 1 : 0;
+                }
+                unv = st == 0;
+            } else {
+                vm_drain();
+                if (valid) unv = vis_get(vis + cid) != (uint8_t)epoch;
+            }
             if (dedupe) {
                 // a later copy of an id in the chunk is visited by its first copy; only
                 // unvisited lanes can be first copies (a visited id is visited in every copy)
@@ -165,7 +284,12 @@ struct Searcher {
                     if (lane > t && cid == c) unv = false;
                 }
             }
-            if (unv) vis_put(vis + cid, (uint8_t)epoch);
+            if (vnb) {
+                if (unv && !vt_insert(cid)) vis_put(vis + cid, (uint8_t)epoch);
+                lds_order();
+            } else if (unv) {
+                vis_put(vis + cid, (uint8_t)epoch);
+            }
         }
         const uint64_t mask = __ballot(unv);
         const int U = __popcll(mask);
@@ -212,7 +336,10 @@ struct Searcher {
                 cd_inner += lim;
                 if (stop >= 0) {
                     // children after the cut were marked ahead of time: unmark them
-                    if (lane > ub + stop && lane < U) vis_put(vis + cidbuf[lane], 0);
+                    if (lane > ub + stop && lane < U) {
+                        const uint32_t cl = cidbuf[lane];
+                        if (!vnb || !vt_erase(cl)) vis_put(vis + cl, 0);
+                    }
                     stopped = true;
                 }
             } else {
@@ -383,6 +510,7 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
     __shared__ float s_dist[WAVES][64];
     __shared__ uint32_t s_st[WAVES][64];
     __shared__ uint32_t s_dg[WAVES][64];
+    extern __shared__ uint4 s_vt[];  // WAVES * a.vnb buckets (dynamic)
     const int w = threadIdx.x >> 6;
     const int64_t slot = (int64_t)blockIdx.x * WAVES + w;
     Searcher<V, NB, PREC, R> S(a);
@@ -394,6 +522,9 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
     S.stbuf = s_st[w];
     S.dgbuf = s_dg[w];
     S.vis = a.vis + slot * a.vis_stride;
+    S.vnb = (uint32_t)a.vnb;
+    S.vhb = (uint32_t)a.vhb;
+    S.vt = s_vt + rfl_u((uint32_t)w * (uint32_t)a.vnb);
     uint32_t epoch = rfl_u(a.epochs[slot]);
     const int lane = S.lane;
     const int dp = 16 * V;
@@ -416,13 +547,13 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
             epoch = 1;
         }
         S.epoch = epoch;
+        S.vt_reset();
         if (a.qvisit) {  // the query's own vertex, marked before seeding (bichromatic_search!)
             const uint32_t x = rfl_u(a.qvisit[qi]);
             if (x != 0u && (int64_t)x <= a.n) {
                 uint32_t c = x - 1u;
                 if (a.ipos) c = rfl_u(a.ipos[c]);
-                if (lane == 0) vis_put(S.vis + c, (uint8_t)epoch);
-                vm_drain();
+                S.mark_uniform(c);
             }
         }
         if (a.allknn) {
@@ -444,9 +575,7 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
             const uint2 ni = a.node[qi];
             for (int64_t t = ni.x; t < (int64_t)ni.x + ni.y; t++) {
                 const uint32_t h = a.adj[t];
-                vm_drain();
-                const uint32_t seen = rfl_u((uint32_t)vis_get(S.vis + h));
-                if (seen == epoch) continue;
+                if (S.visited_uniform(h)) continue;
                 const uint32_t h1 = h + 1u;
                 S.search(&h1, 1, true, bs);
                 S.cd_inner = 0;
@@ -478,9 +607,11 @@ __global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
     beam_body<V, NB, 0, R>(a);
 }
 
-// precision form: BF16 gathers, FP32 arithmetic, exact rerank of the result list
+// precision form: BF16 gathers, FP32 arithmetic, exact rerank of the result list.  Four
+// waves per SIMD: the launch is bound by loads in flight, and below four waves it slows
+// more than a few spilled registers cost.
 template <int V, int NB, int R = 8>
-__global__ __launch_bounds__(256) void beam_bf16_kernel(BeamArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void beam_bf16_kernel(BeamArgs a)
 {
     beam_body<V, NB, 1, R>(a);
 }
@@ -493,6 +624,42 @@ __global__ __launch_bounds__(256) void beam_build_kernel(BeamArgs a)
     beam_body<V, NB, 0, 8>(a);
 }
 
+// Launch with the visited tables as dynamic LDS (WAVES * vnb * 16 bytes per block).  vnb < 0
+// picks the largest table that keeps the kernel's occupancy: the LDS a CU has, shared by the
+// blocks its registers let run, less their static LDS.  A table smaller than 2^(vhb-15)
+// buckets (or than 256, when chosen automatically) is not used.
+template <typename K>
+hipError_t launch_lds(K kern, dim3 grid, dim3 block, const BeamArgs& args, hipStream_t s)
+{
+    BeamArgs a = args;
+    if (a.vnb < 0) {
+        int blocks = 0, dev = 0;
+        size_t lds_cu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, 256, 0);
+        if (e != hipSuccess) return e;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        hipDeviceProp_t prop;
+        if ((e = hipGetDeviceProperties(&prop, dev)) != hipSuccess) return e;
+        lds_cu = prop.maxSharedMemoryPerMultiProcessor;
+        hipFuncAttributes fa;
+        if ((e = hipFuncGetAttributes(&fa, (const void*)kern)) != hipSuccess) return e;
+        int64_t v = blocks > 0 ? ((int64_t)(lds_cu / (size_t)blocks) - (int64_t)fa.sharedSizeBytes) /
+                                     (int64_t)(WAVES * sizeof(uint4))
+                               : 0;
+        v = std::min<int64_t>(v, 2048);
+        a.vnb = v >= 256 ? (int32_t)v : 0;
+    }
+    if (a.vnb > 0 && ((int64_t)a.vnb << 15) < ((int64_t)1 << a.vhb)) a.vnb = 0;
+    const size_t lds = (size_t)WAVES * (size_t)a.vnb * sizeof(uint4);
+    if (lds > 65536) {
+        const hipError_t e =
+            hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, grid, block, lds, s, a);
+    return hipGetLastError();
+}
+
 template <int V>
 hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
 {
@@ -502,30 +669,19 @@ hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
     const bool r4 = a.rounds == 4 && a.beamcap <= 64;
     const bool r10 = a.rounds == 10 && a.beamcap <= 64;
     if (a.internal) {
-        if (a.beamcap <= 64)
-            hipLaunchKernelGGL((beam_build_kernel<V, 1>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((beam_build_kernel<V, 4>), grid, block, 0, s, a);
-    } else if (a.prec == 1) {
-        if (r4)
-            hipLaunchKernelGGL((beam_bf16_kernel<V, 1, 4>), grid, block, 0, s, a);
-        else if (r10)
-            hipLaunchKernelGGL((beam_bf16_kernel<V, 1, 10>), grid, block, 0, s, a);
-        else if (a.beamcap <= 64)
-            hipLaunchKernelGGL((beam_bf16_kernel<V, 1>), grid, block, 0, s, a);
-        else if (a.beamcap <= 128)
-            hipLaunchKernelGGL((beam_bf16_kernel<V, 2>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((beam_bf16_kernel<V, 4>), grid, block, 0, s, a);
-    } else {
-        if (r4)
-            hipLaunchKernelGGL((beam_kernel<V, 1, 4>), grid, block, 0, s, a);
-        else if (a.beamcap <= 64)
-            hipLaunchKernelGGL((beam_kernel<V, 1>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((beam_kernel<V, 4>), grid, block, 0, s, a);
+        if (a.beamcap <= 64) return launch_lds(beam_build_kernel<V, 1>, grid, block, a, s);
+        return launch_lds(beam_build_kernel<V, 4>, grid, block, a, s);
     }
-    return hipGetLastError();
+    if (a.prec == 1) {
+        if (r4) return launch_lds(beam_bf16_kernel<V, 1, 4>, grid, block, a, s);
+        if (r10) return launch_lds(beam_bf16_kernel<V, 1, 10>, grid, block, a, s);
+        if (a.beamcap <= 64) return launch_lds(beam_bf16_kernel<V, 1>, grid, block, a, s);
+        if (a.beamcap <= 128) return launch_lds(beam_bf16_kernel<V, 2>, grid, block, a, s);
+        return launch_lds(beam_bf16_kernel<V, 4>, grid, block, a, s);
+    }
+    if (r4) return launch_lds(beam_kernel<V, 1, 4>, grid, block, a, s);
+    if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1>, grid, block, a, s);
+    return launch_lds(beam_kernel<V, 4>, grid, block, a, s);
 }
 
 }  // namespace ssg

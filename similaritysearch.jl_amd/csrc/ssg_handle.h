@@ -80,6 +80,9 @@ struct ssg_handle {
     // wave slots (each with its visited slot) per CU, and gather rounds per batch
     int32_t slots_per_cu = 32;
     int32_t rounds = 8;
+    // LDS visited-table buckets per wave (8 tags each; -1 = as many as leave the
+    // occupancy unchanged, 0 = visited bytes in HBM only)
+    int32_t vbuckets = -1;
     // exhaustive work
     DevBuf qpad, qnorm, cand_d, cand_i, slice_tau, flags, part_i, part_d, xList;
     // host-call staging

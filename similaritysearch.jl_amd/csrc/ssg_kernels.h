@@ -53,6 +53,8 @@ struct BeamArgs {
     int32_t krr;             // prec 1: query j reranks max(k_j, krr) result entries exactly
     int32_t rounds;          // gather rounds per batch: 8, or 4 / 10 (beam caps <= 64; 10: BF16 form)
     int32_t expand;          // prec 1: beam entries popped per step (1 = reference step)
+    int32_t vnb;             // LDS visited-table buckets per wave (8 tags each; 0 none, < 0 auto)
+    int32_t vhb;             // objects < 2^vhb (15 <= vhb <= 24) when vnb != 0
     const uint32_t* order;   // nullable: order[t] = the t-th query to search (0-based)
     // Query layout (nullable): objects stored in another order.  adjn[e] = {start, degree}
     // of the object adj[e] names; perm[stored] = original id (0-based); ipos = inverse.

@@ -4,7 +4,7 @@
 // prefers the cheapest) from BeamSearch's (bsize, Δ) to the launch: the recall-changing
 // knobs of the BF16 form (rerank length, wide steps) are scored on recall against gold ids,
 // and among those that reach the target the launch shape (wave slots per CU, gather rounds
-// per batch), which changes no result, is chosen by measured time.
+// per batch, LDS visited table or none), which changes no result, is chosen by measured time.
 #include <algorithm>
 #include <unordered_set>
 #include <vector>
@@ -20,6 +20,8 @@ int check_shape(const ssg_launch_params* lp)
         return ssg_fail(SSG_E_ARG, "slots_per_cu must be in 4..64");
     if (lp->rounds != 4 && lp->rounds != 8 && lp->rounds != 10)
         return ssg_fail(SSG_E_ARG, "rounds must be 4, 8 or 10");
+    if (lp->vbuckets < -1 || lp->vbuckets > 2048)
+        return ssg_fail(SSG_E_ARG, "vbuckets must be -1 (automatic), 0 (none) or at most 2048");
     return SSG_OK;
 }
 
@@ -52,6 +54,7 @@ int ssg_set_launch(ssg_handle* h, const ssg_launch_params* lp)
     std::lock_guard<std::mutex> lk(h->mu);
     h->slots_per_cu = lp->slots_per_cu;
     h->rounds = lp->rounds;
+    h->vbuckets = lp->vbuckets;
     return SSG_OK;
 }
 
@@ -61,6 +64,7 @@ int ssg_get_launch(ssg_handle* h, ssg_launch_params* lp)
     std::lock_guard<std::mutex> lk(h->mu);
     lp->slots_per_cu = h->slots_per_cu;
     lp->rounds = h->rounds;
+    lp->vbuckets = h->vbuckets;
     return SSG_OK;
 }
 
@@ -98,7 +102,7 @@ int ssg_optimize_launch(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
         }
     } guard{e0, e1};
 
-    const int32_t sp0 = h->slots_per_cu, r0 = h->rounds;
+    const int32_t sp0 = h->slots_per_cu, r0 = h->rounds, vb0 = h->vbuckets;
     auto run = [&](const ssg_beam_params& p, float* ms) -> int {
         HIPCHK(hipEventRecord(e0, s), "event");
         int rc = ssg_beam_run(h, dQ, h->dim, m, k, kstride, &p, qo, dI.as<uint32_t>(), dD.as<float>(),
@@ -153,34 +157,39 @@ int ssg_optimize_launch(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     // 2. launch shapes for those knobs, by measured time
     float best_ms = 0.f;
     size_t best_knob = keep[0];
-    ssg_launch_params best{sp0, r0};
+    ssg_launch_params best{sp0, r0, vb0};
     bool have = false;
     for (size_t kn : keep) {
         for (int32_t sp : {8, 16, 24, 32, 48, 64}) {
             for (int32_t rd : {4, 8, 10}) {
                 if (rd == 10 && knobs[kn].precision != SSG_BF16) continue;  // BF16 kernels only
                 if (rd != 8 && knobs[kn].beamcap > 64) continue;  // one-register beams only
-                h->slots_per_cu = sp;
-                h->rounds = rd;
-                float ms = 0.f;
-                int rc = run(knobs[kn], nullptr);
-                if (!rc) rc = run(knobs[kn], &ms);
-                if (rc) {
-                    h->slots_per_cu = sp0;
-                    h->rounds = r0;
-                    return rc;
-                }
-                if (!have || ms < best_ms) {
-                    have = true;
-                    best_ms = ms;
-                    best = {sp, rd};
-                    best_knob = kn;
+                for (int32_t vb : {-1, 0}) {
+                    h->slots_per_cu = sp;
+                    h->rounds = rd;
+                    h->vbuckets = vb;
+                    float ms = 0.f;
+                    int rc = run(knobs[kn], nullptr);
+                    if (!rc) rc = run(knobs[kn], &ms);
+                    if (rc) {
+                        h->slots_per_cu = sp0;
+                        h->rounds = r0;
+                        h->vbuckets = vb0;
+                        return rc;
+                    }
+                    if (!have || ms < best_ms) {
+                        have = true;
+                        best_ms = ms;
+                        best = {sp, rd, vb};
+                        best_knob = kn;
+                    }
                 }
             }
         }
     }
     h->slots_per_cu = best.slots_per_cu;
     h->rounds = best.rounds;
+    h->vbuckets = best.vbuckets;
     bp->rerank = knobs[best_knob].rerank;
     bp->expand = knobs[best_knob].expand;
     *lp_out = best;

@@ -425,21 +425,21 @@ class SearchGraph(_GpuIndex):
         G.algo = BeamSearch(bp.bsize, bp.delta, bp.maxvisits)
         return G
 
-    def set_launch(self, slots_per_cu=32, rounds=8):
+    def set_launch(self, slots_per_cu=32, rounds=8, vbuckets=-1):
         """Launch shape of this index's searches (ssg_set_launch; results do not depend on it)."""
         _lib.check(_lib.load().ssg_set_launch(self._handle(), C.byref(_lib.LaunchParams(
-            int(slots_per_cu), int(rounds)))), "set launch")
+            int(slots_per_cu), int(rounds), int(vbuckets)))), "set launch")
 
     def launch(self):
         lp = _lib.LaunchParams()
         _lib.check(_lib.load().ssg_get_launch(self._handle(), C.byref(lp)), "get launch")
-        return lp.slots_per_cu, lp.rounds
+        return lp.slots_per_cu, lp.rounds, lp.vbuckets
 
     def optimize_launch_dev(self, bp, dQ, m, k, kstride, gold, minrecall, qo=None):
         """ssg_optimize_launch over m device-resident queries (qo: device QueryOpts) with
         host gold ids (m, kg): installs the fastest launch shape among the recall-changing
         knobs that reach minrecall and writes the chosen rerank/expand into bp.  Returns
-        ((slots_per_cu, rounds), recall, ms)."""
+        ((slots_per_cu, rounds, vbuckets), recall, ms)."""
         gold = np.ascontiguousarray(gold, np.uint32)
         lp = _lib.LaunchParams()
         rec, ms = C.c_double(), C.c_float()
@@ -447,7 +447,7 @@ class SearchGraph(_GpuIndex):
             self._handle(), dQ, m, k, kstride, C.byref(bp), None if qo is None else C.byref(qo),
             gold.ctypes.data, gold.shape[1], float(minrecall), C.byref(lp), C.byref(rec),
             C.byref(ms)), "optimize launch")
-        return (lp.slots_per_cu, lp.rounds), rec.value, ms.value
+        return (lp.slots_per_cu, lp.rounds, lp.vbuckets), rec.value, ms.value
 
     def optimize(self, ctx, minrecall, queries, ksearch=10):
         """optimize_index!(G, ctx, MinRecall(r); queries, ksearch) (src/opt.jl:220-317);

@@ -42,7 +42,7 @@ class BuildParams(C.Structure):
 
 
 class LaunchParams(C.Structure):
-    _fields_ = [("slots_per_cu", C.c_int32), ("rounds", C.c_int32)]
+    _fields_ = [("slots_per_cu", C.c_int32), ("rounds", C.c_int32), ("vbuckets", C.c_int32)]
 
 
 # name -> (restype, argtypes); the exported surface of include/ssg.h

@@ -8,7 +8,10 @@ against the oracle on a 20,000 x 128 Gaussian mixture with a graph built on the 
   arithmetic, then rerank! with the canonical distance); its recall is within 0.002 of
   the FP32 search's.
 - Mixed per-query k / bsize batch, longest-first order and HIP-graph replay.
-- The 4-round gather kernels and other launch shapes (ssg_set_launch) give the same bits;
+- The 4-round gather kernels and other launch shapes (ssg_set_launch) give the same bits,
+  among them LDS visited tables of every size: none, a tiny one whose buckets fill at once
+  (most marks in HBM, erased tags after the maxvisits cut), the automatic one and the
+  largest;
   the launch autotuner (ssg_optimize_launch) installs a shape and reports the recall the
   search reaches.
 - Learned form at D = 128, H = 256, L = 4096: one training step's gradients against the
@@ -159,7 +162,8 @@ def test_mixed_batch_dim128_under_graph_replay(oracle, g128, prec):
 
 
 @pytest.mark.parametrize("prec", ["f32", "bf16"])
-@pytest.mark.parametrize("shape", [(8, 4), (16, 4), (64, 8), (4, 8), (24, 10)])
+@pytest.mark.parametrize("shape", [(8, 4, -1), (16, 4, 0), (64, 8, -1), (4, 8, 32), (24, 10, -1),
+                                   (16, 8, 2048), (16, 10, 32)])
 def test_launch_shapes_are_bit_exact(oracle, g128, prec, shape):
     import similaritysearch as ss
     from similaritysearch import _lib
@@ -179,7 +183,52 @@ def test_launch_shapes_are_bit_exact(oracle, g128, prec, shape):
         np.testing.assert_array_equal(ctx.costdists, gcd)
         np.testing.assert_array_equal(ctx.costblocks, gcb)
     finally:
-        _lib.check(L.ssg_set_launch(h, C.byref(_lib.LaunchParams(32, 8))), "set launch")
+        _lib.check(L.ssg_set_launch(h, C.byref(_lib.LaunchParams(32, 8, -1))), "set launch")
+
+
+@pytest.fixture(scope="module")
+def allknn128(oracle):
+    """A 3,000 x 128 graph from the oracle's builder and the oracle's allknn over it."""
+    import similaritysearch as ss
+    from similaritysearch import datasets
+    X = datasets.gmm(3000, DIM, seed=211)
+    g = oracle.build_graph("SqL2", X, parallel_block=64, seed=3)
+    G = ss.SearchGraph(ss.SqL2, ss.MatrixDatabase.from_rows(X),
+                       ss.StaticAdjList.from_csr(g["offs"], g["adj"]), g["hints"],
+                       ss.BeamSearch(16, 1.1, 10**6), device=0)
+    ai, ad = oracle.allknn_graph("SqL2", X, g["offs"], g["adj"], 8, 16, 1.1, 10**6, nthreads=8)
+    return G, ai, ad
+
+
+@pytest.mark.parametrize("vbuckets", [0, 1, 32, 512, -1])
+def test_visited_table_sizes_bit_exact(oracle, g128, allknn128, vbuckets):
+    """Wide beams (cap 256) with a maxvisits cut, and allknn over a graph (seeds skipped
+    when already visited, one visited set per object across its searches): every LDS table
+    size gives the oracle's bits."""
+    import similaritysearch as ss
+    from similaritysearch import _lib
+    X, Q, G, offs = g128
+    L = _lib.load()
+    h = G._handle()
+    Ga, ai, ad = allknn128
+    for hh in (h, Ga._handle()):
+        _lib.check(L.ssg_set_launch(hh, C.byref(_lib.LaunchParams(32, 8, vbuckets))), "set launch")
+    try:
+        G.algo = ss.BeamSearch(128, 1.1, 900)
+        ctx = ss.SearchGraphContext(beamcap=256)
+        ids, d = ss.searchbatch(G, ctx, ss.MatrixDatabase.from_rows(Q), 10)
+        gi, gd, gcd, gcb = oracle.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, Q, 10, 128,
+                                             1.1, 900, beamcap=256, nthreads=8)
+        np.testing.assert_array_equal(ids.T, gi)
+        np.testing.assert_array_equal(d.T, gd)
+        np.testing.assert_array_equal(ctx.costdists, gcd)
+        np.testing.assert_array_equal(ctx.costblocks, gcb)
+        ids, d = ss.allknn(Ga, ss.SearchGraphContext(), 8)
+        np.testing.assert_array_equal(ids.T, ai)
+        np.testing.assert_array_equal(d.T, ad)
+    finally:
+        for hh in (h, Ga._handle()):
+            _lib.check(L.ssg_set_launch(hh, C.byref(_lib.LaunchParams(32, 8, -1))), "set launch")
 
 
 def test_launch_autotuner(oracle, g128):
@@ -209,10 +258,12 @@ def test_launch_autotuner(oracle, g128):
                                          gold.ctypes.data, 10, 0.95, C.byref(lp), C.byref(rec),
                                          C.byref(ms)), "optimize launch")
         assert lp.slots_per_cu in (8, 16, 24, 32, 48, 64) and lp.rounds in (4, 8, 10)
+        assert lp.vbuckets in (-1, 0)
         assert bp.rerank in (12, 14, 16, 18, 24) and bp.expand in (1, 2) and ms.value > 0
         got = _lib.LaunchParams()
         _lib.check(L.ssg_get_launch(h, C.byref(got)), "get launch")
-        assert (got.slots_per_cu, got.rounds) == (lp.slots_per_cu, lp.rounds)
+        assert (got.slots_per_cu, got.rounds, got.vbuckets) == (lp.slots_per_cu, lp.rounds,
+                                                                lp.vbuckets)
         dI, dD = DeviceArray((m, 50), np.uint32), DeviceArray((m, 50), np.float32)
         st = Stream()
         _lib.check(L.ssg_beamsearch_dev(h, dQ.ptr, m, 50, 50, C.byref(bp), C.byref(qo), dI.ptr,
@@ -223,7 +274,7 @@ def test_launch_autotuner(oracle, g128):
         r = np.mean([len(set(ids[j, :kk[j]]) & set(gold[j, :kk[j]])) / kk[j] for j in range(m)])
         assert r == pytest.approx(rec.value)
     finally:
-        _lib.check(L.ssg_set_launch(h, C.byref(_lib.LaunchParams(32, 8))), "set launch")
+        _lib.check(L.ssg_set_launch(h, C.byref(_lib.LaunchParams(32, 8, -1))), "set launch")
 
 
 def test_mlp_gradients_at_learned_shape(oracle):
