@@ -23,17 +23,25 @@
 /* Canonical distance.                                                       */
 /* Follows src/dist/vecs.jl:93-102 (SqL2), :59-67 (L2), src/dist/cos.jl:78-85 */
 /* (NormCosine).  The reference sums in FP32 in a CPU-width-dependent order  */
-/* (@fastmath @simd); the contract here fixes one order: the padded dim      */
-/* Dp = roundup(dim, 16) is cut into 16 contiguous runs of Dp/16 elements,   */
+/* (@fastmath @simd); the contract here fixes one order: the padded dim Dp   */
+/* (roundup(dim, 16) up to 128, else 192 or 256: or_padded_dim) is cut into  */
+/* 16 contiguous runs of Dp/16 elements,                                     */
 /* each run is accumulated in FP64 with fma in increasing index, and the 16  */
 /* partial sums are combined by the xor-butterfly 8,4,2,1 (the GPU's 16-lane */
 /* reduction).  The FP64 result is rounded once to Float32.                  */
 /* Pinned by: closed forms of test/testdistances.jl:34-76 and an FP64        */
 /* recompute (tests/test_oracle_distances.py).                               */
 /* ------------------------------------------------------------------------ */
-static double tree16(int metric, const float* a, const float* b, int dim)
+int or_padded_dim(int dim)
 {
     const int dp = (dim + 15) & ~15;
+    if (dp <= 128) return dp;
+    return dp <= 192 ? 192 : dp <= 256 ? 256 : dp;
+}
+
+static double tree16(int metric, const float* a, const float* b, int dim)
+{
+    const int dp = or_padded_dim(dim);
     const int v = dp / 16;
     double p[16], t[16];
     for (int j = 0; j < 16; j++) {
@@ -127,7 +135,7 @@ float or_bf16_round(float x)
 
 float or_dist_bf16(int metric, const float* x, const float* q, int dim)
 {
-    const int dp = (dim + 15) & ~15;
+    const int dp = or_padded_dim(dim);
     const int v = dp / 16;
     float p[16], t[16];
     for (int j = 0; j < 16; j++) {

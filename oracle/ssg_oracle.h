@@ -69,6 +69,9 @@ void or_beamsearch_bf16(int metric, const float* X, int64_t n, int dim, const in
                         float delta, int64_t maxvisits, int32_t beamcap, int32_t krr,
                         uint32_t* ids, float* dists, int64_t* costdists, int64_t* costblocks,
                         int nthreads);
+/* The padded dimension of the canonical distance's 16 runs: roundup(dim, 16) up to 128,
+ * else 192 or 256 (the sizes the GPU kernels are built for). */
+int or_padded_dim(int dim);
 /* BF16 rounding (nearest, ties to even) and the BF16 form's FP32 search distance of row x
  * (rounded to BF16) to the FP32 query q */
 float or_bf16_round(float x);

@@ -167,8 +167,10 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
     if (!h->has_graph) return ssg_fail(SSG_E_STATE, "no graph uploaded");
     if (!bp) return ssg_fail(SSG_E_ARG, "null beam parameters");
     if (m <= 0) return ssg_fail(SSG_E_ARG, "empty set of queries");
-    if (k < 1 || kstride < k || kstride > 64)
-        return ssg_fail(SSG_E_LIMIT, "k must satisfy 1 <= k <= kstride <= 64");
+    if (k < 1 || kstride < k || kstride > 256)
+        return ssg_fail(SSG_E_LIMIT, "k must satisfy 1 <= k <= kstride <= 256");
+    if (kstride > 64 && (bp->precision == SSG_BF16 || internal))
+        return ssg_fail(SSG_E_LIMIT, "k is limited to 64 in the BF16 form and in index!");
     if (bp->bsize < 1 || bp->beamcap < 1 || bp->beamcap > 256)
         return ssg_fail(SSG_E_ARG, "bsize >= 1 and 1 <= beamcap <= 256 required");
     if (!(bp->delta >= 0.f) || !std::isfinite(bp->delta))
@@ -763,10 +765,11 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     if (dim <= 0 || n < 0) return ssg_fail(SSG_E_ARG, "ssg_upload_db: bad shape");
     if (metric < SSG_SQL2 || metric > SSG_NORMCOS)
         return ssg_fail(SSG_E_ARG, "ssg_upload_db: unknown distance");
-    const int32_t dp = (dim + 15) / 16 * 16;
-    const int v = dp / 16;
-    if (!(v <= 8 || v == 12 || v == 16))
-        return ssg_fail(SSG_E_LIMIT, "ssg_upload_db: dim must pad to 16..128, 192 or 256");
+    // rows are padded to the next size the kernels are built for (roundup(dim, 16) up to
+    // 128, else 192 or 256); the canonical distance's 16 runs cover the padded row
+    int32_t dp = (dim + 15) / 16 * 16;
+    if (dp > 128) dp = dp <= 192 ? 192 : 256;
+    if (dim > 256) return ssg_fail(SSG_E_LIMIT, "ssg_upload_db: dim is limited to 256");
     if (n >= (int64_t)0xffffffffLL) return ssg_fail(SSG_E_LIMIT, "ssg_upload_db: n must fit UInt32 ids");
     HIPCHK(hipSetDevice(h->device), "hipSetDevice");
     HIPCHK(hipStreamSynchronize(h->stream), "sync");  // no launch may still read the old database
@@ -1277,8 +1280,8 @@ int ssg_beamsearch(ssg_handle* h, const float* Q, int64_t m, int32_t k, int32_t 
     HIPCHK(hipSetDevice(h->device), "hipSetDevice");
     if (!Q || !ids || !dists) return ssg_fail(SSG_E_ARG, "null buffer");
     if (m <= 0) return ssg_fail(SSG_E_ARG, "empty set of queries");
-    if (k < 1 || kstride < k || kstride > 64)
-        return ssg_fail(SSG_E_LIMIT, "k must satisfy 1 <= k <= kstride <= 64");
+    if (k < 1 || kstride < k || kstride > 256)
+        return ssg_fail(SSG_E_LIMIT, "k must satisfy 1 <= k <= kstride <= 256");
     hipStream_t s = h->stream;
     ssg_query_opts dq{};
     if (qo) {

@@ -11,7 +11,8 @@
 //   the child into the beam when deg(child) > 1 and d <= Δ * maximum(res).
 //
 // GPU shape: one wave64 per query, persistent over an atomic work counter.
-//   * res (k <= 64) and the beam (<= 256 slots) live in registers (RegQueue).
+//   * res (k <= 64, or k <= 256 in four registers) and the beam (<= 256 slots) live
+//     in registers (RegQueue).
 //   * Neighbour lists are expanded 64 ids at a time: one coalesced id load, one
 //     visited-byte load per id, then the unvisited ids are compacted in LDS and
 //     their vectors gathered in batches (one child per 16-lane group per round,
@@ -73,13 +74,14 @@ __device__ __forceinline__ void vis_put(uint8_t* p, uint8_t v)
 // R: gather rounds per batch (the launch autotuner picks 4, 8 or, in the BF16 form, 10;
 // fewer rounds hold fewer rows in registers, so more waves may fit a SIMD, each with fewer
 // loads in flight; more rounds take an expansion's children in fewer round trips).
-template <int V, int NB, int PREC, int R = 8>
+// NR: registers of the result queue (k <= 64 * NR; 4 only in the FP32 form).
+template <int V, int NB, int PREC, int R = 8, int NR = 1>
 struct Searcher {
     static constexpr int ROUNDS = R;
     static constexpr int BATCH = 4 * ROUNDS;           // children per batch
     const BeamArgs& a;
     float qv[V];
-    RegQueue<1, false> res;
+    RegQueue<NR, false> res;
     RegQueue<NB, true> beam;
     int64_t cd_init;   // distance evaluations charged by enqueue_item!
     int64_t cd_inner;  // costdists of beamsearch_inner_beam
@@ -503,7 +505,7 @@ struct Searcher {
     }
 };
 
-template <int V, int NB, int PREC, int R>
+template <int V, int NB, int PREC, int R, int NR = 1>
 __device__ __forceinline__ void beam_body(const BeamArgs& a)
 {
     __shared__ uint32_t s_cid[WAVES][64];
@@ -513,7 +515,7 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
     extern __shared__ uint4 s_vt[];  // WAVES * a.vnb buckets (dynamic)
     const int w = threadIdx.x >> 6;
     const int64_t slot = (int64_t)blockIdx.x * WAVES + w;
-    Searcher<V, NB, PREC, R> S(a);
+    Searcher<V, NB, PREC, R, NR> S(a);
     S.lane = lane_id();
     S.g = S.lane >> 4;
     S.j = S.lane & 15;
@@ -587,10 +589,16 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
         }
         if constexpr (PREC == 1) {
             S.rerank_out(qi, kk);
-        } else if (lane < a.kstride) {
-            const bool has = lane < S.res.len;
-            a.ids[(size_t)qi * a.kstride + lane] = has ? S.out_id(S.res.id[0]) : 0u;
-            a.dists[(size_t)qi * a.kstride + lane] = has ? S.res.d[0] : __builtin_inff();
+        } else {
+#pragma unroll
+            for (int b = 0; b < NR; b++) {
+                const int e = 64 * b + lane;
+                if (e < a.kstride) {
+                    const bool has = e < S.res.len;
+                    a.ids[(size_t)qi * a.kstride + e] = has ? S.out_id(S.res.id[b]) : 0u;
+                    a.dists[(size_t)qi * a.kstride + e] = has ? S.res.d[b] : __builtin_inff();
+                }
+            }
         }
         if (lane == 0) {
             if (a.costd) a.costd[qi] = S.cd_init + S.cd_inner;
@@ -600,16 +608,16 @@ __device__ __forceinline__ void beam_body(const BeamArgs& a)
     if (lane == 0) a.epochs[slot] = (uint8_t)epoch;
 }
 
-// searchbatch! / allknn! launches
-template <int V, int NB, int R = 8>
+// searchbatch! / allknn! launches (NR = 4: k up to 256)
+template <int V, int NB, int R = 8, int NR = 1>
 __global__ __launch_bounds__(256) void beam_kernel(BeamArgs a)
 {
-    beam_body<V, NB, 0, R>(a);
+    beam_body<V, NB, 0, R, NR>(a);
 }
 
 // precision form: BF16 gathers, FP32 arithmetic, exact rerank of the result list.  Four
-// waves per SIMD: the launch is bound by loads in flight, and below four waves it slows
-// more than a few spilled registers cost.
+// waves per SIMD: below four the launch has too few loads in flight (3 waves: 12% slower
+// at 1M x 128), and the few registers spilled to hold four cost less.
 template <int V, int NB, int R = 8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void beam_bf16_kernel(BeamArgs a)
 {
@@ -678,6 +686,10 @@ hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
         if (a.beamcap <= 64) return launch_lds(beam_bf16_kernel<V, 1>, grid, block, a, s);
         if (a.beamcap <= 128) return launch_lds(beam_bf16_kernel<V, 2>, grid, block, a, s);
         return launch_lds(beam_bf16_kernel<V, 4>, grid, block, a, s);
+    }
+    if (a.kstride > 64) {
+        if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1, 8, 4>, grid, block, a, s);
+        return launch_lds(beam_kernel<V, 4, 8, 4>, grid, block, a, s);
     }
     if (r4) return launch_lds(beam_kernel<V, 1, 4>, grid, block, a, s);
     if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1>, grid, block, a, s);

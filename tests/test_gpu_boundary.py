@@ -131,3 +131,44 @@ def test_graph_from_adjacency_lists(oracle, form):
         ri, rd = r.sortitems()
         assert set(ri.tolist()) == set(oi[j, :ks[j]].tolist())
         np.testing.assert_array_equal(np.sort(rd), od[j, :ks[j]])
+
+
+@pytest.mark.parametrize("dim", [136, 150, 176, 200, 250])
+def test_any_dim_up_to_256(oracle, dim):
+    """Dimensions that pad to 192 or 256 (rows padded with zeros; the canonical distance's
+    16 runs cover the padded row, in the GPU and the oracle alike): exhaustive k-NN, beam
+    search over an oracle-built graph, and rerank!, bit-exact."""
+    ss = _ss()
+    rng = np.random.default_rng(dim)
+    X = rng.standard_normal((1500, dim)).astype(np.float32)
+    Q = rng.standard_normal((60, dim)).astype(np.float32)
+    E = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(X))
+    ids, d = ss.searchbatch(E, ss.GenericContext(), ss.MatrixDatabase.from_rows(Q), 10)
+    gi, gd = oracle.exhaustive("SqL2", X, Q, 10)
+    np.testing.assert_array_equal(ids.T, gi)
+    np.testing.assert_array_equal(d.T, gd)
+    g = oracle.build_graph("SqL2", X, parallel_block=64, seed=2)
+    G = ss.SearchGraph(ss.SqL2, ss.MatrixDatabase.from_rows(X),
+                       ss.StaticAdjList.from_csr(g["offs"], g["adj"]), g["hints"],
+                       ss.BeamSearch(g["bsize"], g["delta"], g["maxvisits"]))
+    ctx = ss.SearchGraphContext()
+    ids, d = ss.searchbatch(G, ctx, ss.MatrixDatabase.from_rows(Q), 10)
+    bi, bd, bcd, bcb = oracle.beamsearch("SqL2", X, g["offs"], g["adj"], g["hints"], Q, 10,
+                                         g["bsize"], g["delta"], g["maxvisits"])
+    np.testing.assert_array_equal(ids.T, bi)
+    np.testing.assert_array_equal(d.T, bd)
+    np.testing.assert_array_equal(ctx.costdists, bcd)
+    I = np.asfortranarray(rng.integers(1, 1501, (12, 60)).astype(np.uint32))
+    D = np.zeros(I.shape, np.float32, order="F")
+    ri, rd = oracle.rerank("SqL2", X, Q, np.ascontiguousarray(I.T), np.ascontiguousarray(D.T))
+    ss.rerank_(ss.SqL2, ss.MatrixDatabase.from_rows(X), ss.MatrixDatabase.from_rows(Q), I, D)
+    np.testing.assert_array_equal(I.T, ri)
+    np.testing.assert_array_equal(D.T, rd)
+
+
+def test_dim_above_256_is_refused():
+    ss = _ss()
+    X = np.zeros((100, 257), np.float32)
+    with pytest.raises(Exception):
+        E = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(X))
+        ss.searchbatch(E, ss.GenericContext(), ss.MatrixDatabase.from_rows(X[:2]), 1)

@@ -63,6 +63,63 @@ def test_fp32_beamsearch_dim128_bit_exact(oracle, g128, bsize, delta, beamcap, m
     np.testing.assert_array_equal(ctx.costblocks, gcb)
 
 
+@pytest.mark.parametrize("k,bsize,beamcap,maxvisits", [
+    (65, 12, 32, 10**6),
+    (100, 16, 32, 10**6),
+    (256, 64, 256, 10**6),
+    (128, 32, 256, 700),
+])
+def test_fp32_beamsearch_large_k_bit_exact(oracle, g128, k, bsize, beamcap, maxvisits):
+    """k beyond one register (the result queue in four: k <= 256), as KnnSorted(k) of any
+    size in the reference."""
+    import similaritysearch as ss
+    X, Q, G, offs = g128
+    G.algo = ss.BeamSearch(bsize, 1.1, maxvisits)
+    ctx = ss.SearchGraphContext(beamcap=beamcap)
+    ids, d = ss.searchbatch(G, ctx, ss.MatrixDatabase.from_rows(Q), k)
+    gi, gd, gcd, gcb = oracle.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, Q, k, bsize,
+                                         1.1, maxvisits, beamcap=beamcap, nthreads=8)
+    np.testing.assert_array_equal(ids.T, gi)
+    np.testing.assert_array_equal(d.T, gd)
+    np.testing.assert_array_equal(ctx.costdists, gcd)
+    np.testing.assert_array_equal(ctx.costblocks, gcb)
+
+
+def test_large_k_queue_vector_and_allknn(oracle, g128, allknn128):
+    """Per-query k up to 200 in the queue-vector form, and allknn with k = 100."""
+    import similaritysearch as ss
+    X, Q, G, offs = g128
+    rng = np.random.default_rng(231)
+    ks = rng.choice([1, 10, 100, 200], len(Q)).astype(np.int32)
+    G.algo = ss.BeamSearch(16, 1.1, 10**6)
+    ctx = ss.SearchGraphContext()
+    res = ss.searchbatch_knns(G, ctx, ss.MatrixDatabase.from_rows(Q), ks)
+    gi, gd, gcd, gcb = oracle.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, Q, 200, 16,
+                                         1.1, 10**6, kq=ks, nthreads=8)
+    for j, (ij, dj) in enumerate(res):
+        np.testing.assert_array_equal(ij, gi[j, :ks[j]])
+        np.testing.assert_array_equal(dj, gd[j, :ks[j]])
+    np.testing.assert_array_equal(ctx.costdists, gcd)
+    Ga, _, _, Xa, ga = allknn128
+    ids, d = ss.allknn(Ga, ss.SearchGraphContext(), 100)
+    ai, ad = oracle.allknn_graph("SqL2", Xa, ga["offs"], ga["adj"], 100, 16, 1.1, 10**6, nthreads=8)
+    np.testing.assert_array_equal(ids.T, ai)
+    np.testing.assert_array_equal(d.T, ad)
+
+
+def test_large_k_limits(g128):
+    """k > 64 is refused in the BF16 form, and k > 256 everywhere."""
+    import similaritysearch as ss
+    from similaritysearch import _lib
+    X, Q, G, offs = g128
+    G.algo = ss.BeamSearch(16, 1.1, 10**6)
+    with pytest.raises(_lib.SSGError):
+        ss.searchbatch(G, ss.SearchGraphContext(precision="bf16", rerank=14),
+                       ss.MatrixDatabase.from_rows(Q[:8]), 100)
+    with pytest.raises(_lib.SSGError):
+        ss.searchbatch(G, ss.SearchGraphContext(), ss.MatrixDatabase.from_rows(Q[:8]), 257)
+
+
 @pytest.mark.parametrize("bsize,beamcap,rerank,k", [
     (16, 32, 14, 10),
     (32, 32, 14, 10),
@@ -197,7 +254,7 @@ def allknn128(oracle):
                        ss.StaticAdjList.from_csr(g["offs"], g["adj"]), g["hints"],
                        ss.BeamSearch(16, 1.1, 10**6), device=0)
     ai, ad = oracle.allknn_graph("SqL2", X, g["offs"], g["adj"], 8, 16, 1.1, 10**6, nthreads=8)
-    return G, ai, ad
+    return G, ai, ad, X, g
 
 
 @pytest.mark.parametrize("vbuckets", [0, 1, 32, 512, -1])
@@ -210,7 +267,7 @@ def test_visited_table_sizes_bit_exact(oracle, g128, allknn128, vbuckets):
     X, Q, G, offs = g128
     L = _lib.load()
     h = G._handle()
-    Ga, ai, ad = allknn128
+    Ga, ai, ad, _, _ = allknn128
     for hh in (h, Ga._handle()):
         _lib.check(L.ssg_set_launch(hh, C.byref(_lib.LaunchParams(32, 8, vbuckets))), "set launch")
     try:

@@ -61,11 +61,11 @@ def test_noalloc_argument_errors(index):
     from similaritysearch.device import DeviceArray
     L = _lib.load()
     dQ = DeviceArray.from_numpy(np.zeros((4, 64), np.float32))
-    dI, dD = DeviceArray((4, 80), np.uint32), DeviceArray((4, 80), np.float32)
+    dI, dD = DeviceArray((4, 300), np.uint32), DeviceArray((4, 300), np.float32)
     bp = ss.BeamParams(16, 1.1, 10**6, 32, 0, 0, 1)
-    rc = L.ssg_beamsearch_noalloc_dev(index._handle(), dQ.ptr, 4, 80, 80, C.byref(bp), None, dI.ptr,
+    rc = L.ssg_beamsearch_noalloc_dev(index._handle(), dQ.ptr, 4, 300, 300, C.byref(bp), None, dI.ptr,
                                       dD.ptr, None, None, None)
-    assert rc == -5  # SSG_E_LIMIT: k beyond the register queue
+    assert rc == -5  # SSG_E_LIMIT: k beyond the register queues
     rc = L.ssg_beamsearch_noalloc_dev(index._handle(), None, 4, 10, 10, C.byref(bp), None, dI.ptr,
                                       dD.ptr, None, None, None)
     assert rc == -1
