@@ -23,9 +23,10 @@
  * memory; _dev functions take device pointers and enqueue on `stream`
  * (a hipStream_t, NULL = the handle's own stream) without synchronising.
  *
- * Size limits (SSG_E_LIMIT beyond them; the reference has none): dim <= 256 (rows pad
- * to roundup(dim, 16) up to 128, else to 192 or 256); n < 2^32; beam search
- * k <= kstride <= 256 (<= 64 in the BF16 form);
+ * Size limits (SSG_E_LIMIT beyond them; the reference has none): dim <= 1024 (rows pad
+ * to roundup(dim, 16) up to 128, else to 192, 256, 384, 512, 768 or 1024; above 256 the
+ * exact exhaustive search takes the one-block-per-query scan, and the BF16 and precision
+ * forms are refused); n < 2^32; beam search k <= kstride <= 256 (<= 64 in the BF16 form);
  * exact exhaustive k <= 3840 (k > 32 takes a one-block-per-query scan); precision form
  * k <= ncand <= 256 with dim <= 128; rerank! columns <= 1024 rows; hsp_queries
  * columns <= 256 rows; neardup blocks <= 1024 objects when filtering.

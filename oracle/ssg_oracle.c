@@ -24,9 +24,9 @@
 /* Follows src/dist/vecs.jl:93-102 (SqL2), :59-67 (L2), src/dist/cos.jl:78-85 */
 /* (NormCosine).  The reference sums in FP32 in a CPU-width-dependent order  */
 /* (@fastmath @simd); the contract here fixes one order: the padded dim Dp   */
-/* (roundup(dim, 16) up to 128, else 192 or 256: or_padded_dim) is cut into  */
-/* 16 contiguous runs of Dp/16 elements,                                     */
-/* each run is accumulated in FP64 with fma in increasing index, and the 16  */
+/* (roundup(dim, 16) up to 128, else 192, 256, 384, 512, 768 or 1024;        */
+/* or_padded_dim) is cut into 16 contiguous runs of Dp/16 elements, each     */
+/* run is accumulated in FP64 with fma in increasing index, and the 16       */
 /* partial sums are combined by the xor-butterfly 8,4,2,1 (the GPU's 16-lane */
 /* reduction).  The FP64 result is rounded once to Float32.                  */
 /* Pinned by: closed forms of test/testdistances.jl:34-76 and an FP64        */
@@ -34,9 +34,12 @@
 /* ------------------------------------------------------------------------ */
 int or_padded_dim(int dim)
 {
+    static const int sizes[] = {192, 256, 384, 512, 768, 1024};
     const int dp = (dim + 15) & ~15;
     if (dp <= 128) return dp;
-    return dp <= 192 ? 192 : dp <= 256 ? 256 : dp;
+    for (int i = 0; i < 6; i++)
+        if (dp <= sizes[i]) return sizes[i];
+    return dp;
 }
 
 static double tree16(int metric, const float* a, const float* b, int dim)

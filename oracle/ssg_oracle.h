@@ -70,7 +70,7 @@ void or_beamsearch_bf16(int metric, const float* X, int64_t n, int dim, const in
                         uint32_t* ids, float* dists, int64_t* costdists, int64_t* costblocks,
                         int nthreads);
 /* The padded dimension of the canonical distance's 16 runs: roundup(dim, 16) up to 128,
- * else 192 or 256 (the sizes the GPU kernels are built for). */
+ * else 192, 256, 384, 512, 768 or 1024 (the sizes the GPU kernels are built for). */
 int or_padded_dim(int dim);
 /* BF16 rounding (nearest, ties to even) and the BF16 form's FP32 search distance of row x
  * (rounded to BF16) to the FP32 query q */

@@ -180,6 +180,8 @@ int ssg_beam_run(ssg_handle* h, const float* dQ, int32_t qdim, int64_t m, int32_
         return ssg_fail(SSG_E_ARG, "precision must be SSG_F32 or SSG_BF16");
     if (bp->precision == SSG_BF16 && (allknn || internal))
         return ssg_fail(SSG_E_ARG, "the BF16 form serves searchbatch! only");
+    if (bp->precision == SSG_BF16 && h->dp > 256)
+        return ssg_fail(SSG_E_LIMIT, "the BF16 form is limited to dim <= 256");
     if (bp->expand < 0 || bp->expand > 4) return ssg_fail(SSG_E_ARG, "expand must be in 0..4");
     if (bp->expand > 1 && bp->precision != SSG_BF16)
         return ssg_fail(SSG_E_ARG, "wide steps (expand > 1) belong to the BF16 form");
@@ -306,8 +308,9 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
         HIPCHK(launch_fill_f32(w.qnorm + m, m_pad - m, 0.f, s), "fill qnorm");
         HIPCHK(launch_pad_rows(dQ, m, h->dim, w.qpad, h->dp, w.qnorm, s), "pad queries");
     }
-    if (k > EXH_KC) {
-        // more than stage 1 keeps per slice: the one-block-per-query scan
+    if (k > EXH_KC || h->dp > 256) {
+        // more than stage 1 keeps per slice, or rows above 256 dims: the one-block-per-query
+        // scan
         HIPCHK(launch_exh_wide(h->X.as<float>(), n_eff, h->dp, h->metric, w.qpad, m, k, dids, ddists,
                                kstride, s), "exhaustive k > 32");
         return SSG_OK;
@@ -766,10 +769,10 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     if (metric < SSG_SQL2 || metric > SSG_NORMCOS)
         return ssg_fail(SSG_E_ARG, "ssg_upload_db: unknown distance");
     // rows are padded to the next size the kernels are built for (roundup(dim, 16) up to
-    // 128, else 192 or 256); the canonical distance's 16 runs cover the padded row
-    int32_t dp = (dim + 15) / 16 * 16;
-    if (dp > 128) dp = dp <= 192 ? 192 : 256;
-    if (dim > 256) return ssg_fail(SSG_E_LIMIT, "ssg_upload_db: dim is limited to 256");
+    // 128, else 192, 256, 384, 512, 768 or 1024); the canonical distance's 16 runs cover
+    // the padded row
+    if (dim > 1024) return ssg_fail(SSG_E_LIMIT, "ssg_upload_db: dim is limited to 1024");
+    const int32_t dp = ssg_padded_dim(dim);
     if (n >= (int64_t)0xffffffffLL) return ssg_fail(SSG_E_LIMIT, "ssg_upload_db: n must fit UInt32 ids");
     HIPCHK(hipSetDevice(h->device), "hipSetDevice");
     HIPCHK(hipStreamSynchronize(h->stream), "sync");  // no launch may still read the old database

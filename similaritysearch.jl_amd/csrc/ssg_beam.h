@@ -14,7 +14,7 @@
 //   * res (k <= 64, or k <= 256 in four registers) and the beam (<= 256 slots) live
 //     in registers (RegQueue).
 //   * Neighbour lists are expanded 64 ids at a time: one coalesced id load, one
-//     visited-byte load per id, then the unvisited ids are compacted in LDS and
+//     visited test per id (below), then the unvisited ids are compacted in LDS and
 //     their vectors gathered in batches (one child per 16-lane group per round,
 //     8 rounds = 32 children), with
 //     the child degrees loaded in the same round trip.  Every row index of a
@@ -102,9 +102,9 @@ struct Searcher {
     // ---- LDS visited table ----
     // Object c < 2^hb (15 <= hb <= 24) maps bijectively to h = c * 0x9E3779B1 mod 2^hb;
     // its bucket is h * vnb >> hb (vnb >= 2^(hb-15), so a bucket covers at most 2^15
-    // consecutive h) and its tag (h mod 2^15) + 1, which with the bucket names c.  A bucket holds 8 tags in
-    // 16 bytes (0 = free, VT_ERASED = an erased tag, occupied for good), filled left to
-    // right and never emptied.  A mark goes to the bucket while it has a free slot and to
+    // consecutive h) and its tag (h mod 2^15) + 1, which with the bucket names c.  A
+    // bucket holds 8 tags in 16 bytes (0 = free, VT_ERASED = an erased tag, occupied for
+    // good), filled left to right and never emptied.  A mark goes to the bucket while it has a free slot and to
     // the HBM epoch byte otherwise, so c is visited iff its tag is in its bucket, or its
     // bucket is full and its byte holds the epoch.
     static constexpr uint32_t VT_ERASED = 0xFFFFu;
@@ -624,12 +624,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void b
     beam_body<V, NB, 1, R>(a);
 }
 
+// Gather rounds per batch for rows above 256 dims (V > 16), where each round's rows take
+// V registers per lane: 2 rounds up to 512 dims, 1 beyond.
+template <int V>
+constexpr int wide_rounds()
+{
+    return V <= 32 ? 2 : 1;
+}
+
 // the same search issued by index! and optimize_index! (a separate symbol keeps
 // their launches apart from query traffic in kernel traces)
 template <int V, int NB>
 __global__ __launch_bounds__(256) void beam_build_kernel(BeamArgs a)
 {
-    beam_body<V, NB, 0, 8>(a);
+    beam_body<V, NB, 0, (V > 16 ? wide_rounds<V>() : 8)>(a);
 }
 
 // Launch with the visited tables as dynamic LDS (WAVES * vnb * 16 bytes per block).  vnb < 0
@@ -674,26 +682,42 @@ hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
     const dim3 grid((unsigned)(nslots / WAVES)), block(256);
     // the 4- and 10-round variants exist for one beam register (beam caps <= 64); 10 rounds
     // (40 children per batch) only in the BF16 form, where its rows still leave 4 waves per SIMD
-    const bool r4 = a.rounds == 4 && a.beamcap <= 64;
-    const bool r10 = a.rounds == 10 && a.beamcap <= 64;
-    if (a.internal) {
-        if (a.beamcap <= 64) return launch_lds(beam_build_kernel<V, 1>, grid, block, a, s);
-        return launch_lds(beam_build_kernel<V, 4>, grid, block, a, s);
+    if constexpr (V > 16) {
+        // rows above 256 dims: the FP32 form only, with fewer gather rounds
+        constexpr int RW = wide_rounds<V>();
+        if (a.prec == 1) return hipErrorInvalidValue;
+        if (a.internal) {
+            if (a.beamcap <= 64) return launch_lds(beam_build_kernel<V, 1>, grid, block, a, s);
+            return launch_lds(beam_build_kernel<V, 4>, grid, block, a, s);
+        }
+        if (a.kstride > 64) {
+            if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1, RW, 4>, grid, block, a, s);
+            return launch_lds(beam_kernel<V, 4, RW, 4>, grid, block, a, s);
+        }
+        if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1, RW>, grid, block, a, s);
+        return launch_lds(beam_kernel<V, 4, RW>, grid, block, a, s);
+    } else {
+        const bool r4 = a.rounds == 4 && a.beamcap <= 64;
+        const bool r10 = a.rounds == 10 && a.beamcap <= 64;
+        if (a.internal) {
+            if (a.beamcap <= 64) return launch_lds(beam_build_kernel<V, 1>, grid, block, a, s);
+            return launch_lds(beam_build_kernel<V, 4>, grid, block, a, s);
+        }
+        if (a.prec == 1) {
+            if (r4) return launch_lds(beam_bf16_kernel<V, 1, 4>, grid, block, a, s);
+            if (r10) return launch_lds(beam_bf16_kernel<V, 1, 10>, grid, block, a, s);
+            if (a.beamcap <= 64) return launch_lds(beam_bf16_kernel<V, 1>, grid, block, a, s);
+            if (a.beamcap <= 128) return launch_lds(beam_bf16_kernel<V, 2>, grid, block, a, s);
+            return launch_lds(beam_bf16_kernel<V, 4>, grid, block, a, s);
+        }
+        if (a.kstride > 64) {
+            if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1, 8, 4>, grid, block, a, s);
+            return launch_lds(beam_kernel<V, 4, 8, 4>, grid, block, a, s);
+        }
+        if (r4) return launch_lds(beam_kernel<V, 1, 4>, grid, block, a, s);
+        if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1>, grid, block, a, s);
+        return launch_lds(beam_kernel<V, 4>, grid, block, a, s);
     }
-    if (a.prec == 1) {
-        if (r4) return launch_lds(beam_bf16_kernel<V, 1, 4>, grid, block, a, s);
-        if (r10) return launch_lds(beam_bf16_kernel<V, 1, 10>, grid, block, a, s);
-        if (a.beamcap <= 64) return launch_lds(beam_bf16_kernel<V, 1>, grid, block, a, s);
-        if (a.beamcap <= 128) return launch_lds(beam_bf16_kernel<V, 2>, grid, block, a, s);
-        return launch_lds(beam_bf16_kernel<V, 4>, grid, block, a, s);
-    }
-    if (a.kstride > 64) {
-        if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1, 8, 4>, grid, block, a, s);
-        return launch_lds(beam_kernel<V, 4, 8, 4>, grid, block, a, s);
-    }
-    if (r4) return launch_lds(beam_kernel<V, 1, 4>, grid, block, a, s);
-    if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1>, grid, block, a, s);
-    return launch_lds(beam_kernel<V, 4>, grid, block, a, s);
 }
 
 }  // namespace ssg

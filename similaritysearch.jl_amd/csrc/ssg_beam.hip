@@ -145,6 +145,10 @@ hipError_t launch_beam(const BeamArgs& a, int nslots, hipStream_t s)
     case 8: return launch_v<8>(a, nslots, s);
     case 12: return launch_v<12>(a, nslots, s);
     case 16: return launch_v<16>(a, nslots, s);
+    case 24: return launch_v<24>(a, nslots, s);
+    case 32: return launch_v<32>(a, nslots, s);
+    case 48: return launch_v<48>(a, nslots, s);
+    case 64: return launch_v<64>(a, nslots, s);
     default: return hipErrorInvalidValue;
     }
 }

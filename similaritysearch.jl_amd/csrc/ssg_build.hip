@@ -133,7 +133,7 @@ static hipError_t launch_block_filter(const float* X, int dp, int metric, int64_
     switch (dp / 16) {
 #define SSG_BF(VV) \
     case VV: hipLaunchKernelGGL((block_filter_kernel<VV>), grid, block, 0, s, X, metric, sp0, nb, ks, tids, td, has_tmp, out, cnt); break;
-        SSG_BF(1) SSG_BF(2) SSG_BF(3) SSG_BF(4) SSG_BF(5) SSG_BF(6) SSG_BF(7) SSG_BF(8) SSG_BF(12) SSG_BF(16)
+        SSG_BF(1) SSG_BF(2) SSG_BF(3) SSG_BF(4) SSG_BF(5) SSG_BF(6) SSG_BF(7) SSG_BF(8) SSG_BF(12) SSG_BF(16) SSG_BF(24) SSG_BF(32) SSG_BF(48) SSG_BF(64)
 #undef SSG_BF
     default: return hipErrorInvalidValue;
     }

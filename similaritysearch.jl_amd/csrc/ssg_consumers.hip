@@ -391,7 +391,7 @@ hipError_t launch_nearest_center(const float* X, int32_t dp, int32_t metric, con
     switch (dp / 16) {
 #define SSG_NC(VV) \
     case VV: hipLaunchKernelGGL((nearest_center_kernel<VV>), grid, block, 0, s, X, metric, items, mb, F, nf, pos, pdist); break;
-        SSG_NC(1) SSG_NC(2) SSG_NC(3) SSG_NC(4) SSG_NC(5) SSG_NC(6) SSG_NC(7) SSG_NC(8) SSG_NC(12) SSG_NC(16)
+        SSG_NC(1) SSG_NC(2) SSG_NC(3) SSG_NC(4) SSG_NC(5) SSG_NC(6) SSG_NC(7) SSG_NC(8) SSG_NC(12) SSG_NC(16) SSG_NC(24) SSG_NC(32) SSG_NC(48) SSG_NC(64)
 #undef SSG_NC
     default: return hipErrorInvalidValue;
     }
@@ -418,7 +418,7 @@ hipError_t launch_pair_dist(const float* X, int32_t dp, int32_t metric, const ui
     switch (dp / 16) {
 #define SSG_PD(VV) \
     case VV: hipLaunchKernelGGL((pair_dist_kernel<VV>), grid, block, 0, s, X, metric, ids, mb, P); break;
-        SSG_PD(1) SSG_PD(2) SSG_PD(3) SSG_PD(4) SSG_PD(5) SSG_PD(6) SSG_PD(7) SSG_PD(8) SSG_PD(12) SSG_PD(16)
+        SSG_PD(1) SSG_PD(2) SSG_PD(3) SSG_PD(4) SSG_PD(5) SSG_PD(6) SSG_PD(7) SSG_PD(8) SSG_PD(12) SSG_PD(16) SSG_PD(24) SSG_PD(32) SSG_PD(48) SSG_PD(64)
 #undef SSG_PD
     default: return hipErrorInvalidValue;
     }
@@ -474,7 +474,7 @@ static hipError_t launch_hsp_nb(const float* X, int32_t dp, int32_t metric, int6
         hipLaunchKernelGGL((hsp_kernel<VV, NB>), grid, block, lds, s, X, metric, m, k, ids, dists,   \
                            out_ids, out_dists);                                                      \
         break;
-        SSG_HSP(1) SSG_HSP(2) SSG_HSP(3) SSG_HSP(4) SSG_HSP(5) SSG_HSP(6) SSG_HSP(7) SSG_HSP(8) SSG_HSP(12) SSG_HSP(16)
+        SSG_HSP(1) SSG_HSP(2) SSG_HSP(3) SSG_HSP(4) SSG_HSP(5) SSG_HSP(6) SSG_HSP(7) SSG_HSP(8) SSG_HSP(12) SSG_HSP(16) SSG_HSP(24) SSG_HSP(32) SSG_HSP(48) SSG_HSP(64)
 #undef SSG_HSP
     default: return hipErrorInvalidValue;
     }

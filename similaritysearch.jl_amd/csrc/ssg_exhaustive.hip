@@ -1039,7 +1039,7 @@ hipError_t launch_exh_wide(const float* X, int64_t n, int32_t dp, int32_t metric
     switch (dp / 16) {
 #define SSG_EW(VV) \
     case VV: hipLaunchKernelGGL((exh_wide_kernel<VV>), grid, block, lds, s, X, n, metric, Qp, k, P2, ids, dists, kstride); break;
-        SSG_EW(1) SSG_EW(2) SSG_EW(3) SSG_EW(4) SSG_EW(5) SSG_EW(6) SSG_EW(7) SSG_EW(8) SSG_EW(12) SSG_EW(16)
+        SSG_EW(1) SSG_EW(2) SSG_EW(3) SSG_EW(4) SSG_EW(5) SSG_EW(6) SSG_EW(7) SSG_EW(8) SSG_EW(12) SSG_EW(16) SSG_EW(24) SSG_EW(32) SSG_EW(48) SSG_EW(64)
 #undef SSG_EW
     default: return hipErrorInvalidValue;
     }
@@ -1056,7 +1056,7 @@ hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float
         switch (dp / 16) {
 #define SSG_RW(VV) \
     case VV: hipLaunchKernelGGL((rerank_wide_kernel<VV>), grid, block, lds, s, X, metric, Q, qdim, m, k, ids, dists); break;
-            SSG_RW(1) SSG_RW(2) SSG_RW(3) SSG_RW(4) SSG_RW(5) SSG_RW(6) SSG_RW(7) SSG_RW(8) SSG_RW(12) SSG_RW(16)
+            SSG_RW(1) SSG_RW(2) SSG_RW(3) SSG_RW(4) SSG_RW(5) SSG_RW(6) SSG_RW(7) SSG_RW(8) SSG_RW(12) SSG_RW(16) SSG_RW(24) SSG_RW(32) SSG_RW(48) SSG_RW(64)
 #undef SSG_RW
         default: return hipErrorInvalidValue;
         }
@@ -1065,7 +1065,7 @@ hipError_t launch_rerank(const float* X, int32_t dp, int32_t metric, const float
     switch (dp / 16) {
 #define SSG_RR(VV) \
     case VV: hipLaunchKernelGGL((rerank_kernel<VV>), grid, block, 0, s, X, metric, Q, qdim, m, k, ids, dists); break;
-        SSG_RR(1) SSG_RR(2) SSG_RR(3) SSG_RR(4) SSG_RR(5) SSG_RR(6) SSG_RR(7) SSG_RR(8) SSG_RR(12) SSG_RR(16)
+        SSG_RR(1) SSG_RR(2) SSG_RR(3) SSG_RR(4) SSG_RR(5) SSG_RR(6) SSG_RR(7) SSG_RR(8) SSG_RR(12) SSG_RR(16) SSG_RR(24) SSG_RR(32) SSG_RR(48) SSG_RR(64)
 #undef SSG_RR
     default: return hipErrorInvalidValue;
     }

@@ -179,6 +179,10 @@ hipError_t launch_fft(const FftArgs& a, int dp, int blocks, hipStream_t s)
     case 8: return fft_rounds<8>(a, blocks, s);
     case 12: return fft_rounds<12>(a, blocks, s);
     case 16: return fft_rounds<16>(a, blocks, s);
+    case 24: return fft_rounds<24>(a, blocks, s);
+    case 32: return fft_rounds<32>(a, blocks, s);
+    case 48: return fft_rounds<48>(a, blocks, s);
+    case 64: return fft_rounds<64>(a, blocks, s);
     default: return hipErrorInvalidValue;
     }
 }

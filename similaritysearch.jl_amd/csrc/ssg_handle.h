@@ -11,6 +11,17 @@
 #include "../../include/ssg.h"
 #include "ssg_kernels.h"
 
+// Padded row size of dimension dim: roundup(dim, 16) up to 128, else the next of 192, 256,
+// 384, 512, 768 and 1024 (the row sizes the kernels are built for; 0 above 1024).
+inline int32_t ssg_padded_dim(int64_t dim)
+{
+    const int32_t dp = (int32_t)((dim + 15) / 16 * 16);
+    if (dp <= 128) return dp;
+    for (int32_t s : {192, 256, 384, 512, 768, 1024})
+        if (dp <= s) return s;
+    return 0;
+}
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;

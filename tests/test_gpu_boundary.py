@@ -133,11 +133,12 @@ def test_graph_from_adjacency_lists(oracle, form):
         np.testing.assert_array_equal(np.sort(rd), od[j, :ks[j]])
 
 
-@pytest.mark.parametrize("dim", [136, 150, 176, 200, 250])
-def test_any_dim_up_to_256(oracle, dim):
-    """Dimensions that pad to 192 or 256 (rows padded with zeros; the canonical distance's
-    16 runs cover the padded row, in the GPU and the oracle alike): exhaustive k-NN, beam
-    search over an oracle-built graph, and rerank!, bit-exact."""
+@pytest.mark.parametrize("dim", [136, 150, 176, 200, 250, 300, 512, 700, 1000, 1024])
+def test_any_dim_up_to_1024(oracle, dim):
+    """Dimensions that pad to 192, 256, 384, 512, 768 or 1024 (rows padded with zeros; the
+    canonical distance's 16 runs cover the padded row, in the GPU and the oracle alike):
+    exhaustive k-NN (above 256 dims the one-block-per-query scan), beam search over an
+    oracle-built graph, and rerank!, bit-exact."""
     ss = _ss()
     rng = np.random.default_rng(dim)
     X = rng.standard_normal((1500, dim)).astype(np.float32)
@@ -166,9 +167,22 @@ def test_any_dim_up_to_256(oracle, dim):
     np.testing.assert_array_equal(D.T, rd)
 
 
-def test_dim_above_256_is_refused():
+def test_bf16_form_above_256_dims_is_refused(oracle):
     ss = _ss()
-    X = np.zeros((100, 257), np.float32)
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((300, 300)).astype(np.float32)
+    g = oracle.build_graph("SqL2", X, parallel_block=64, seed=2)
+    G = ss.SearchGraph(ss.SqL2, ss.MatrixDatabase.from_rows(X),
+                       ss.StaticAdjList.from_csr(g["offs"], g["adj"]), g["hints"],
+                       ss.BeamSearch(g["bsize"], g["delta"], g["maxvisits"]))
+    with pytest.raises(Exception):
+        ss.searchbatch(G, ss.SearchGraphContext(precision="bf16", rerank=14),
+                       ss.MatrixDatabase.from_rows(X[:4]), 10)
+
+
+def test_dim_above_1024_is_refused():
+    ss = _ss()
+    X = np.zeros((100, 1025), np.float32)
     with pytest.raises(Exception):
         E = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(X))
         ss.searchbatch(E, ss.GenericContext(), ss.MatrixDatabase.from_rows(X[:2]), 1)

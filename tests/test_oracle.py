@@ -27,15 +27,17 @@ def test_distance_known_answers(oracle):
 
 
 def test_padded_dim_rule(oracle):
-    """The canonical distance's runs cover roundup(dim, 16) up to 128, else 192 or 256: the
-    row sizes the GPU kernels are built for."""
+    """The canonical distance's runs cover roundup(dim, 16) up to 128, else 192, 256, 384,
+    512, 768 or 1024: the row sizes the GPU kernels are built for."""
     f = oracle.lib().or_padded_dim
-    got = {d: f(d) for d in (1, 16, 17, 100, 128, 129, 150, 176, 192, 193, 240, 256)}
+    got = {d: f(d) for d in (1, 16, 17, 100, 128, 129, 150, 176, 192, 193, 240, 256, 257, 384,
+                             385, 512, 513, 768, 769, 1024)}
     assert got == {1: 16, 16: 16, 17: 32, 100: 112, 128: 128, 129: 192, 150: 192, 176: 192,
-                   192: 192, 193: 256, 240: 256, 256: 256}
+                   192: 192, 193: 256, 240: 256, 256: 256, 257: 384, 384: 384, 385: 512,
+                   512: 512, 513: 768, 768: 768, 769: 1024, 1024: 1024}
 
 
-@pytest.mark.parametrize("dim", [1, 4, 15, 16, 17, 32, 96, 100, 128, 150, 200, 256])
+@pytest.mark.parametrize("dim", [1, 4, 15, 16, 17, 32, 96, 100, 128, 150, 200, 256, 300, 1000])
 @pytest.mark.parametrize("metric", ["SqL2", "L2", "NormCosine"])
 def test_distance_is_rounded_fp64(oracle, dim, metric):
     """canonical distance = float32 of the exact value (within one float32 ulp)."""
