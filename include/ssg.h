@@ -153,6 +153,25 @@ typedef struct {
 /* index!(G::SearchGraph, ctx) over the uploaded database: builds the graph on the device
  * and installs it in the handle, together with its hints and the tuned BeamSearch. */
 int ssg_build_graph(ssg_handle* h, const ssg_build_params* p);
+/* append_items!(G::SearchGraph, ctx, db) (src/searchgraph/insertions.jl:27-34): appends the
+ * m objects of X (column-major (dim, m)) to the database of a graph that ssg_build_graph
+ * built on this handle and continues index! from length(index) + 1 with the same build
+ * parameters and the same generator of draws (blocks of parallel_block objects start at
+ * the first new object, callbacks fire as their positions say). */
+int ssg_append_items_graph(ssg_handle* h, const float* X, int64_t m);
+/* neardup(dist, X, eps; recall < 1, k, blocksize, filterblocks) (src/neardup.jl:71-76): the
+ * same loop with the net held in a SearchGraph grown by append_items! (index! with the build
+ * parameters bp, whose opt_minrecall carries MinRecall(recall); continued over each block's
+ * new centers).  Each block searches the net as it stands with KnnSorted(k) queues (the
+ * graph's current BeamSearch; the nearest result decides, 0 < k <= 64).  *costdists counts
+ * the block searches' evaluations (costdists of the beam searches) and the block filters';
+ * index!'s own searches and its optimiser's are not counted.  net: a handle holding an empty
+ * database of h's dimension and distance (ssg_upload_db with n = 0); it returns holding the
+ * net's centers (rows in map order) and their graph (the reference's idx).  Outputs as
+ * ssg_neardup. */
+int ssg_neardup_graph(ssg_handle* h, ssg_handle* net, float eps, int32_t blocksize, int32_t filterblocks,
+                      int32_t k, const ssg_build_params* bp, uint32_t* map, int64_t* nmap, int32_t* nn,
+                      float* dist, int64_t* costdists);
 /* Size of the installed graph and its current BeamSearch parameters. */
 int ssg_graph_info(ssg_handle* h, int64_t* nnz, int64_t* nhints, ssg_beam_params* algo);
 /* Copy the installed graph out in StaticAdjList form (offset[n] inclusive ends, 1-based

@@ -86,6 +86,9 @@ def lib(baseline=False):
         L.or_build_graph.restype = C.c_int
         L.or_build_graph.argtypes = [C.c_int, _p, _i64, C.c_int, C.POINTER(BuildParams),
                                      C.POINTER(Graph)]
+        L.or_build_graph_staged.restype = C.c_int
+        L.or_build_graph_staged.argtypes = [C.c_int, _p, _i64, C.c_int, C.POINTER(BuildParams), _p,
+                                            C.c_int, C.POINTER(Graph)]
         L.or_free.argtypes = [_p]
         L.or_hsp_queries.argtypes = [C.c_int, _p, C.c_int, _i64, _i32, _p, _p, _p, _p, C.c_int]
         L.or_fp8_mfma.argtypes = [_p, _i64, _p, _i64, C.c_int, _p, C.c_int, C.c_int, C.c_int, _p, _p]
@@ -272,14 +275,21 @@ def random_hints(offs, adj, logbase=1.1, seed=0):
 
 def build_graph(metric, X, parallel_block=1, nb_logbase=1.3, nb_minsize=2, hints_logbase=1.1,
                 cb_logbase=1.5, cb_starting=256, opt_minrecall=0.97, opt_numqueries=32,
-                opt_ksearch=10, beamcap=32, seed=0):
-    """index! restated; returns dict(offs, adj, hints, bsize, delta, maxvisits)."""
+                opt_ksearch=10, beamcap=32, seed=0, stages=None):
+    """index! restated; returns dict(offs, adj, hints, bsize, delta, maxvisits).  stages
+    (increasing object counts ending at n): index! over the first stages[0] objects, then
+    append_items! up to each later count (the graph built by successive calls)."""
     X = _f32c(X)
     p = BuildParams(nb_logbase, nb_minsize, parallel_block, hints_logbase, cb_logbase,
                     cb_starting, opt_minrecall, opt_numqueries, opt_ksearch, beamcap, seed)
     g = Graph()
-    rc = lib().or_build_graph(METRICS.get(metric, metric), _ptr(X), X.shape[0], X.shape[1],
-                              C.byref(p), C.byref(g))
+    if stages is None:
+        rc = lib().or_build_graph(METRICS.get(metric, metric), _ptr(X), X.shape[0], X.shape[1],
+                                  C.byref(p), C.byref(g))
+    else:
+        ends = np.ascontiguousarray(stages, np.int64)
+        rc = lib().or_build_graph_staged(METRICS.get(metric, metric), _ptr(X), X.shape[0],
+                                         X.shape[1], C.byref(p), _ptr(ends), len(ends), C.byref(g))
     if rc != 0:
         raise ValueError("empty database")
     n = X.shape[0]
@@ -491,14 +501,18 @@ def closest_of_rows(ids, d, samedata):
     return best
 
 
-def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True, nthreads=1):
+def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True, nthreads=1, graph=None):
     """neardup over an exact index (src/neardup.jl:69-256): blocks of blocksize objects;
     a block's objects search the net built so far (nearest in (distance, id) order; a
     near duplicate when within eps), the rest are filtered in order against the block's
     new centers (first center at the least distance), which then join the net.  Returns
     (map, nn, dist, centers, costdists) as in similaritysearch.neardup.  Pinned in
     tests/test_oracle.py: one block is the greedy eps-net in object order (brute force),
-    and blocked runs keep the net invariants."""
+    and blocked runs keep the net invariants.  graph (a dict of build_graph arguments):
+    the recall < 1 form (src/neardup.jl:71-76), the net a SearchGraph built by index! and
+    grown by append_items! of each block's new centers (build_graph with stages), searched
+    by the restated beam search with KnnSorted(k); costdists counts those searches'
+    evaluations instead of the net's length per object."""
     X = _f32c(X)
     eps = np.float32(eps)
     n = len(X)
@@ -530,10 +544,26 @@ def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True, nthreads=1):
                 L[i - 1] = items[best_t]; D[i - 1] = best_d
         return [items[t] for t in tmp]
 
+    ends = []  # net sizes after each append_items! (graph form)
     for r0 in range(0, n, blocksize):
         rng = list(range(r0 + 1, min(n, r0 + blocksize) + 1))
         if not net:
             new = block(rng)
+        elif graph is not None:
+            Xn = X[np.asarray(net) - 1]
+            g = build_graph(metric, Xn, stages=ends, **graph)
+            ids, d, cd, _ = beamsearch(metric, Xn, g["offs"], g["adj"], g["hints"],
+                                       X[r0:r0 + len(rng)], k, g["bsize"], g["delta"],
+                                       g["maxvisits"], beamcap=graph.get("beamcap", 32),
+                                       nthreads=nthreads)
+            cost += int(cd.sum())
+            imap = []
+            for t, j in enumerate(rng):
+                if ids[t, 0] == 0 or d[t, 0] > eps:
+                    imap.append(j)
+                else:
+                    D[j - 1] = d[t, 0]; L[j - 1] = M[int(ids[t, 0]) - 1]
+            new = block(imap) if imap else []
         else:
             # searchbatch! of the block against the net (src/neardup.jl:121-127); each exact
             # search charges the net's length (src/exact/sequential-exhaustive.jl:78)
@@ -547,6 +577,8 @@ def neardup(metric, X, eps, k=8, blocksize=256, filterblocks=True, nthreads=1):
                     D[j - 1] = d[t, 0]; L[j - 1] = M[int(ids[t, 0]) - 1]
             new = block(imap) if imap else []
         net.extend(new)
+        if new:
+            ends.append(len(net))
     return np.asarray(M, np.uint32), L, D, np.asarray(net, np.uint32), cost
 
 

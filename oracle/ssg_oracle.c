@@ -1128,7 +1128,19 @@ static int cb_fires(const or_build_params* p, int64_t n, int64_t m)
 int or_build_graph(int metric, const float* X, int64_t n, int dim, const or_build_params* p,
                    or_graph* out)
 {
-    if (n <= 0) return -1;
+    return or_build_graph_staged(metric, X, n, dim, p, &n, 1, out);
+}
+
+/* index! over X[0 .. ends[0]), then append_items! of X[ends[s-1] .. ends[s]) for each later
+ * stage: each call of the reference's index! continues from length(index) + 1
+ * (insertions.jl:27-34, 96-113), so a block never crosses a stage end; the graph, its
+ * hints, the BeamSearch and the generator of draws carry over. */
+int or_build_graph_staged(int metric, const float* X, int64_t n, int dim, const or_build_params* p,
+                          const int64_t* ends, int nends, or_graph* out)
+{
+    if (n <= 0 || nends < 1 || ends[nends - 1] != n) return -1;
+    for (int st = 0; st < nends; st++)
+        if (ends[st] < 1 || (st > 0 && ends[st] <= ends[st - 1])) return -1;
     alist_t* L = (alist_t*)calloc((size_t)n, sizeof(alist_t));
     csr_t c = {(int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1)), NULL, 0};
     uint32_t* hints = NULL;
@@ -1178,9 +1190,10 @@ int or_build_graph(int metric, const float* X, int64_t n, int dim, const or_buil
             len = node;
         }
     } else {
-        /* _parallel_append_items_loop! insertions.jl:46-79 */
-        for (int64_t sp = 1; sp <= n;) {
-            const int64_t ep = (sp + PB - 1 < n) ? sp + PB - 1 : n;
+        /* _parallel_append_items_loop! insertions.jl:46-79, one call per stage */
+        for (int64_t sp = 1, st = 0; sp <= n;) {
+            while (ends[st] < sp) st++;
+            const int64_t ep = (sp + PB - 1 < ends[st]) ? sp + PB - 1 : ends[st];
             const int32_t ks = nb_size(p, ep);
             if (len > 0) snapshot(L, len, &c);
             gidx_t g = {metric, X, len, dim, c.offs, c.adj, 0};

@@ -130,6 +130,10 @@ typedef struct {
 
 int or_build_graph(int metric, const float* X, int64_t n, int dim, const or_build_params* p,
                    or_graph* out);
+/* The same build in stages: index! over the first ends[0] objects, then append_items! of the
+ * objects up to ends[1], ends[2], ... (ends increasing, ends[nends - 1] == n). */
+int or_build_graph_staged(int metric, const float* X, int64_t n, int dim, const or_build_params* p,
+                          const int64_t* ends, int nends, or_graph* out);
 void or_free(void* p);
 
 /* FP8 matrix-core products of the precision form: D[i n + j] is the restated value (before

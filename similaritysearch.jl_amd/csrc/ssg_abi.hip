@@ -781,6 +781,7 @@ int ssg_upload_db(ssg_handle* h, const float* X, int32_t dim, int64_t n, int32_t
     h->metric = metric;
     h->n = n;
     h->n_pad = (n + 63) / 64 * 64;
+    h->b_valid = false;
     h->has_graph = false;
     h->lists.clear();
     h->hints_host.clear();
@@ -843,38 +844,37 @@ static hipError_t grow_rows(ssg_handle* h, int64_t n2, hipStream_t s)
     return e;
 }
 
-int ssg_append_db(ssg_handle* h, const float* X, int64_t m)
+int ssg_append_rows(ssg_handle* h, const float* X, int64_t m, bool dev)
 {
-    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
-    std::lock_guard<std::mutex> lk(h->mu);
-    if (!h->X.p) return ssg_fail(SSG_E_STATE, "ssg_append_db: no database uploaded");
-    if (m < 0 || (m > 0 && !X)) return ssg_fail(SSG_E_ARG, "ssg_append_db: bad arguments");
-    if (h->has_graph)
-        return ssg_fail(SSG_E_STATE, "ssg_append_db: the handle holds a graph (inserting into a "
-                                     "SearchGraph is index!, not an append)");
-    if (h->n + m >= (int64_t)0xffffffffLL) return ssg_fail(SSG_E_LIMIT, "ssg_append_db: n must fit UInt32 ids");
+    if (h->n + m >= (int64_t)0xffffffffLL) return ssg_fail(SSG_E_LIMIT, "append: n must fit UInt32 ids");
     if (m == 0) return SSG_OK;
     HIPCHK(hipSetDevice(h->device), "hipSetDevice");
     HIPCHK(hipStreamSynchronize(h->stream), "sync");  // no launch may still read the old rows
     const int64_t n = h->n, n2 = n + m, dp = h->dp, dim = h->dim;
     HIPCHK(grow_rows(h, n2, h->stream), "grow X");
-    DevBuf tmp;
-    const int64_t chunk = std::max<int64_t>(1, (int64_t)(256 << 20) / (dim * 4));
-    HIPCHK(tmp.ensure((size_t)std::min(chunk, m) * dim * sizeof(float)), "alloc staging");
-    for (int64_t r = 0; r < m; r += chunk) {
-        const int64_t rows = std::min(chunk, m - r);
-        hipError_t e = hipMemcpyAsync(tmp.p, X + r * dim, (size_t)rows * dim * sizeof(float),
-                                      hipMemcpyHostToDevice, h->stream);
-        if (e == hipSuccess)
-            e = launch_pad_rows(tmp.as<float>(), rows, (int32_t)dim, h->X.as<float>() + (n + r) * dp,
-                                (int32_t)dp, h->xnorm.as<float>() + n + r, h->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-        if (e != hipSuccess) {
-            tmp.release();
-            return ssg_hipfail(e, "append X");
+    if (dev) {
+        // padded rows: a copy, with the norms (zero padding adds exactly nothing to them)
+        HIPCHK(launch_pad_rows(X, m, (int32_t)dp, h->X.as<float>() + n * dp, (int32_t)dp,
+                               h->xnorm.as<float>() + n, h->stream), "append X");
+        HIPCHK(hipStreamSynchronize(h->stream), "sync");
+    } else {
+        DevBuf tmp;
+        const int64_t chunk = std::max<int64_t>(1, (int64_t)(256 << 20) / (dim * 4));
+        HIPCHK(tmp.ensure((size_t)std::min(chunk, m) * dim * sizeof(float)), "alloc staging");
+        for (int64_t r = 0; r < m; r += chunk) {
+            const int64_t rows = std::min(chunk, m - r);
+            hipError_t e = hipMemcpyAsync(tmp.p, X + r * dim, (size_t)rows * dim * sizeof(float),
+                                          hipMemcpyHostToDevice, h->stream);
+            if (e == hipSuccess)
+                e = launch_pad_rows(tmp.as<float>(), rows, (int32_t)dim, h->X.as<float>() + (n + r) * dp,
+                                    (int32_t)dp, h->xnorm.as<float>() + n + r, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) {
+                tmp.release();
+                return ssg_hipfail(e, "append X");
+            }
         }
     }
-    tmp.release();
     std::vector<float> nr((size_t)m);
     HIPCHK(hipMemcpy(nr.data(), h->xnorm.as<float>() + n, (size_t)m * sizeof(float),
                      hipMemcpyDeviceToHost), "read norms");
@@ -889,6 +889,18 @@ int ssg_append_db(ssg_handle* h, const float* X, int64_t m)
     h->xe_fmt = 0;
     ssg_query_layout_stale(h);
     return SSG_OK;
+}
+
+int ssg_append_db(ssg_handle* h, const float* X, int64_t m)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->X.p) return ssg_fail(SSG_E_STATE, "ssg_append_db: no database uploaded");
+    if (m < 0 || (m > 0 && !X)) return ssg_fail(SSG_E_ARG, "ssg_append_db: bad arguments");
+    if (h->has_graph)
+        return ssg_fail(SSG_E_STATE, "ssg_append_db: the handle holds a graph (inserting into a "
+                                     "SearchGraph is ssg_append_items_graph)");
+    return ssg_append_rows(h, X, m, false);
 }
 
 int ssg_neardup(ssg_handle* h, float eps, int32_t blocksize, int32_t filterblocks, uint32_t* map,
@@ -1064,6 +1076,136 @@ int ssg_neardup(ssg_handle* h, float eps, int32_t blocksize, int32_t filterblock
     return SSG_OK;
 }
 
+// neardup with recall < 1 (src/neardup.jl:71-76): the net is a SearchGraph grown by
+// append_items! (index! continued over each block's new centers, with OptimizeParameters
+// (MinRecall(recall)) among its callbacks); every block searches the net as it stands
+// (searchbatch! with KnnSorted(k) queues, the nearest result decides), then its remaining
+// objects are filtered in order against the block's new centers as in the exact form.
+int ssg_neardup_graph(ssg_handle* h, ssg_handle* g, float eps, int32_t blocksize, int32_t filterblocks,
+                      int32_t k, const ssg_build_params* bp, uint32_t* map, int64_t* nmap, int32_t* nn,
+                      float* dist, int64_t* costdists)
+{
+    if (!h || !g || !bp) return ssg_fail(SSG_E_ARG, "null argument");
+    if (g == h) return ssg_fail(SSG_E_ARG, "ssg_neardup_graph: the net needs its own handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    std::lock_guard<std::mutex> lkg(g->mu);
+    if (g->n != 0 || g->dim != h->dim || g->metric != h->metric || g->device != h->device)
+        return ssg_fail(SSG_E_ARG, "ssg_neardup_graph: the net handle needs an empty database of "
+                                   "the same dimension and distance on the same device");
+    if (!h->X.p) return ssg_fail(SSG_E_STATE, "ssg_neardup_graph: no database uploaded");
+    if (!map || !nmap || !nn || !dist || !costdists) return ssg_fail(SSG_E_ARG, "null buffer");
+    if (blocksize < 1) return ssg_fail(SSG_E_ARG, "ssg_neardup_graph: blocksize must be >= 1");
+    if (k < 1 || k > 64) return ssg_fail(SSG_E_ARG, "ssg_neardup_graph: k must be in 1..64");
+    if (h->dp > 256) return ssg_fail(SSG_E_LIMIT, "ssg_neardup_graph: dim is limited to 256");
+    const int64_t n = h->n;
+    *nmap = 0;
+    *costdists = 0;
+    if (n == 0) return SSG_OK;
+    const int32_t bs = (int32_t)std::min<int64_t>(blocksize, n);
+    if (filterblocks && bs > 1024) return ssg_fail(SSG_E_LIMIT, "ssg_neardup_graph: blocksize is limited to 1024");
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    hipStream_t s = h->stream;
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    int rc = ssg_graph_begin(g, bp);
+    if (rc != SSG_OK) return rc;
+    const int64_t dp = h->dp;
+    DevBuf rid, rd, rcd, items, P, cover, cdist, fresh_d, rows;
+    HIPCHK(rid.ensure((size_t)bs * k * 4), "alloc");
+    HIPCHK(rd.ensure((size_t)bs * k * 4), "alloc");
+    HIPCHK(rcd.ensure((size_t)bs * 8), "alloc");
+    HIPCHK(items.ensure((size_t)bs * 4), "alloc");
+    if (filterblocks) HIPCHK(P.ensure((size_t)bs * bs * 4), "alloc");
+    HIPCHK(cover.ensure((size_t)bs * 4), "alloc");
+    HIPCHK(cdist.ensure((size_t)bs * 4), "alloc");
+    HIPCHK(fresh_d.ensure((size_t)bs * 4), "alloc");
+    HIPCHK(rows.ensure((size_t)bs * dp * 4 + (size_t)bs * 4), "alloc");
+    std::vector<uint32_t> M, imap, fresh, sid((size_t)bs * k);
+    std::vector<float> sd((size_t)bs * k), hcd(bs);
+    std::vector<int64_t> scd(bs);
+    std::vector<int32_t> hcov(bs);
+    int64_t cost = 0;
+    for (int64_t r0 = 0; r0 < n; r0 += bs) {
+        const int32_t mb = (int32_t)std::min<int64_t>(bs, n - r0);
+        imap.clear();
+        if (g->n == 0) {
+            for (int32_t t = 0; t < mb; t++) imap.push_back((uint32_t)(r0 + t));
+        } else {
+            // searchbatch!(idx, ctx, X[range], knns) over the net graph (neardup.jl:121-127)
+            rc = ssg_beam_run(g, h->X.as<float>() + r0 * dp, (int32_t)dp, mb, k, k, &g->algo, nullptr,
+                              rid.as<uint32_t>(), rd.as<float>(), rcd.as<int64_t>(), nullptr, s, false,
+                              g->n, true);
+            if (rc != SSG_OK) return rc;
+            HIPCHK(hipMemcpyAsync(sid.data(), rid.p, (size_t)mb * k * 4, hipMemcpyDeviceToHost, s), "copy");
+            HIPCHK(hipMemcpyAsync(sd.data(), rd.p, (size_t)mb * k * 4, hipMemcpyDeviceToHost, s), "copy");
+            HIPCHK(hipMemcpyAsync(scd.data(), rcd.p, (size_t)mb * 8, hipMemcpyDeviceToHost, s), "copy");
+            HIPCHK(hipStreamSynchronize(s), "sync");
+            for (int32_t t = 0; t < mb; t++) {
+                cost += scd[t];
+                const int64_t j = r0 + t;
+                const uint32_t pid = sid[(size_t)t * k];  // 1-based net position, 0 = none
+                const float d = sd[(size_t)t * k];
+                if (pid == 0 || d > eps) {
+                    imap.push_back((uint32_t)j);
+                } else {
+                    dist[j] = d;
+                    nn[j] = (int32_t)M[pid - 1];
+                }
+            }
+        }
+        if (imap.empty()) continue;
+        const int32_t mi = (int32_t)imap.size();
+        fresh.clear();
+        if (!filterblocks) {
+            for (uint32_t j : imap) {
+                M.push_back(j + 1);
+                nn[j] = (int32_t)(j + 1);
+                dist[j] = 0.f;
+                fresh.push_back(j);
+            }
+        } else {
+            HIPCHK(hipMemcpyAsync(items.p, imap.data(), (size_t)mi * 4, hipMemcpyHostToDevice, s), "copy");
+            HIPCHK(launch_pair_dist(h->X.as<float>(), (int32_t)dp, h->metric, items.as<uint32_t>(), mi,
+                                    P.as<float>(), s), "pair distances");
+            HIPCHK(launch_greedy_net(P.as<float>(), mi, eps, cover.as<int32_t>(), cdist.as<float>(), s),
+                   "greedy net");
+            HIPCHK(hipMemcpyAsync(hcov.data(), cover.p, (size_t)mi * 4, hipMemcpyDeviceToHost, s), "copy");
+            HIPCHK(hipMemcpyAsync(hcd.data(), cdist.p, (size_t)mi * 4, hipMemcpyDeviceToHost, s), "copy");
+            HIPCHK(hipStreamSynchronize(s), "sync");
+            int64_t T = 0;  // centers of this block so far (add_distance_evaluations!, :240)
+            for (int32_t a = 0; a < mi; a++) {
+                const uint32_t j = imap[a];
+                if (a > 0) cost += T;
+                if (hcov[a] == a) {
+                    T++;
+                    M.push_back(j + 1);
+                    nn[j] = (int32_t)(j + 1);
+                    dist[j] = 0.f;
+                    fresh.push_back(j);
+                } else {
+                    nn[j] = (int32_t)(imap[hcov[a]] + 1);
+                    dist[j] = hcd[a];
+                }
+            }
+        }
+        // append_items!(idx, ctx, X[tmp]) (:255): the new centers' rows join the net and
+        // index! continues over them
+        const int64_t nf = (int64_t)fresh.size();
+        HIPCHK(hipMemcpyAsync(fresh_d.p, fresh.data(), (size_t)nf * 4, hipMemcpyHostToDevice, s), "copy");
+        HIPCHK(launch_gather_padded_rows(h->X.as<float>(), h->xnorm.as<float>(), (int32_t)dp,
+                                         fresh_d.as<uint32_t>(), nf, rows.as<float>(),
+                                         rows.as<float>() + (size_t)bs * dp, 0, s), "gather net rows");
+        HIPCHK(hipStreamSynchronize(s), "sync");
+        rc = ssg_append_rows(g, rows.as<float>(), nf, true);
+        if (rc == SSG_OK) rc = ssg_build_continue(g);
+        if (rc != SSG_OK) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    std::copy(M.begin(), M.end(), map);
+    *nmap = (int64_t)M.size();
+    *costdists = cost;
+    return SSG_OK;
+}
+
 int ssg_upload_graph(ssg_handle* h, const int64_t* offset, int64_t n, const uint32_t* end_point,
                      int64_t nnz, const uint32_t* hints, int64_t nhints)
 {
@@ -1091,6 +1233,7 @@ int ssg_upload_graph(ssg_handle* h, const int64_t* offset, int64_t n, const uint
     for (int64_t i = 0; i < nhints; i++)
         if (hints[i] < 1 || (int64_t)hints[i] > n) return ssg_fail(SSG_E_ARG, "ssg_upload_graph: hint out of range");
     HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    h->b_valid = false;  // not built by index! here: append_items! cannot continue it
     h->lists.swap(lists);
     h->hints_host.assign(hints, hints + nhints);
     return ssg_install_lists(h);

@@ -282,7 +282,9 @@ struct Builder {
     ssg_handle* h;
     const ssg_build_params* p;
     hipStream_t s;
-    std::vector<uint32_t> seg_start, seg_cap, dev_len;
+    std::vector<uint32_t>& seg_start;
+    std::vector<uint32_t>& seg_cap;
+    std::vector<uint32_t>& dev_len;
     std::vector<uint32_t> pos, val, nidx;
     std::vector<uint2> nval;
 
@@ -399,38 +401,57 @@ struct Builder {
 
 extern "C" {
 
-int ssg_build_graph(ssg_handle* h, const ssg_build_params* p)
+static int check_build_params(const ssg_build_params* p, int64_t n)
 {
-    if (!h || !p) return ssg_fail(SSG_E_ARG, "null argument");
-    std::lock_guard<std::mutex> lk(h->mu);
-    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
-    const int64_t n = h->n;
-    if (!h->X.p || n <= 0) return ssg_fail(SSG_E_STATE, "index! needs a non-empty database");
     if (p->parallel_block < 2) return ssg_fail(SSG_E_ARG, "the GPU builder needs parallel_block >= 2");
     if (p->beamcap < 1 || p->beamcap > 256) return ssg_fail(SSG_E_ARG, "beamcap must be in 1..256");
     if (!(p->nb_logbase > 1.f) || p->nb_minsize < 0) return ssg_fail(SSG_E_ARG, "bad Neighborhood");
     if (nb_size(p, n) > 64) return ssg_fail(SSG_E_LIMIT, "neighborhood size above 64");
     if (p->opt_minrecall > 0 && (p->opt_ksearch < 1 || p->opt_ksearch > EXH_KC || p->opt_numqueries < 1))
         return ssg_fail(SSG_E_ARG, "bad OptimizeParameters");
+    return SSG_OK;
+}
+
+// The insertion loop of index! (src/searchgraph/insertions.jl:46-79, 96-113) over the objects
+// h->b_len + 1 .. n with the parameters and generator state the handle keeps: blocks of
+// parallel_block objects from the first uninserted one, so a build in several calls has the
+// block boundaries of the reference's index! after each append_items!.
+static int build_continue(ssg_handle* h)
+{
+    const int64_t n = h->n;
+    const ssg_build_params* p = &h->b_params;
+    if (!h->X.p || n <= 0) return ssg_fail(SSG_E_STATE, "index! needs a non-empty database");
+    int rc = check_build_params(p, n);
+    if (rc) return rc;
     hipStream_t s = h->stream;
-    Builder B{h, p, s, {}, {}, {}, {}, {}, {}, {}};
-    h->lists.assign((size_t)n, {});
-    h->hints_host.clear();
-    ssg_query_layout_stale(h);
-    h->algo = ssg_beam_params{4, 1.0f, 1000000, p->beamcap};
-    B.seg_start.assign((size_t)n, 0);
-    B.seg_cap.assign((size_t)n, 0);
-    B.dev_len.assign((size_t)n, 0);
-    HIPCHK(h->node.ensure((size_t)n * sizeof(uint2)), "alloc node table");
-    HIPCHK(launch_zero_u32(h->node.as<unsigned int>(), 2 * n, s), "zero node table");
-    h->adj.release();
-    HIPCHK(h->adj.ensure((size_t)n * 16 * 4), "alloc adjacency pool");
-    h->pool_used = 0;
-    h->nhints = 0;
+    const int64_t n0 = h->b_len;
+    Builder B{h, p, s, h->b_seg_start, h->b_seg_cap, h->b_dev_len, {}, {}, {}, {}};
+    if (n0 == 0) {
+        h->lists.assign((size_t)n, {});
+        h->hints_host.clear();
+        h->algo = ssg_beam_params{4, 1.0f, 1000000, p->beamcap};
+        B.seg_start.assign((size_t)n, 0);
+        B.seg_cap.assign((size_t)n, 0);
+        B.dev_len.assign((size_t)n, 0);
+        HIPCHK(h->node.ensure((size_t)n * sizeof(uint2)), "alloc node table");
+        HIPCHK(launch_zero_u32(h->node.as<unsigned int>(), 2 * n, s), "zero node table");
+        h->adj.release();
+        HIPCHK(h->adj.ensure((size_t)n * 16 * 4), "alloc adjacency pool");
+        h->pool_used = 0;
+        h->nhints = 0;
+        rc = B.upload_hints();
+        if (rc) return rc;
+    } else {
+        h->lists.resize((size_t)n);
+        B.seg_start.resize((size_t)n, 0);
+        B.seg_cap.resize((size_t)n, 0);
+        B.dev_len.resize((size_t)n, 0);
+        HIPCHK(h->node.grow((size_t)n * sizeof(uint2), (size_t)n0 * sizeof(uint2), s), "grow node table");
+        HIPCHK(launch_zero_u32(h->node.as<unsigned int>() + 2 * n0, 2 * (n - n0), s), "zero node table");
+    }
     h->has_dups = 0;
     h->has_graph = true;
-    int rc = B.upload_hints();
-    if (rc) return rc;
+    ssg_query_layout_stale(h);
     const int PB = p->parallel_block;
     const int ksmax = nb_size(p, n);
     HIPCHK(h->bIds.ensure((size_t)PB * ksmax * 4), "alloc");
@@ -441,9 +462,9 @@ int ssg_build_graph(ssg_handle* h, const ssg_build_params* p)
     std::vector<int32_t> cnts((size_t)PB);
     std::vector<uint8_t> mark((size_t)n, 0);
     std::vector<uint32_t> changed, Lc;
-    Rng rng{p->seed};
-    int64_t len = 0;
-    for (int64_t sp = 1; sp <= n;) {
+    Rng rng{h->b_rng};
+    int64_t len = n0;
+    for (int64_t sp = n0 + 1; sp <= n;) {
         const int64_t ep = std::min<int64_t>(n, sp + PB - 1);
         const int nb = (int)(ep - sp + 1);
         const int ks = nb_size(p, ep);
@@ -505,11 +526,13 @@ int ssg_build_graph(ssg_handle* h, const ssg_build_params* p)
             }
         }
         len = ep;
+        h->b_len = len;
         if (cb_fires(p, sp, ep)) {
             HIPCHK(hipStreamSynchronize(s), "sync");
             rc = B.callbacks(len, rng);
             if (rc) return rc;
         }
+        h->b_rng = rng.s;
         sp = ep + 1;
     }
     HIPCHK(hipStreamSynchronize(s), "sync");
@@ -530,6 +553,52 @@ int ssg_build_graph(ssg_handle* h, const ssg_build_params* p)
     h->has_dups = dups;
     ssg_query_layout_stale(h);
     return SSG_OK;
+}
+
+int ssg_graph_begin(ssg_handle* h, const ssg_build_params* p)
+{
+    const int rc = check_build_params(p, std::max<int64_t>(h->n, 1));
+    if (rc) return rc;
+    h->b_params = *p;
+    h->b_len = 0;
+    h->b_rng = p->seed;
+    h->b_valid = true;
+    return SSG_OK;
+}
+
+int ssg_build_continue(ssg_handle* h)
+{
+    if (!h->b_valid) return ssg_fail(SSG_E_STATE, "the graph was not built by index! on this handle");
+    const int rc = build_continue(h);
+    if (rc) h->b_valid = false;  // an interrupted insertion loop cannot be continued
+    return rc;
+}
+
+int ssg_build_graph(ssg_handle* h, const ssg_build_params* p)
+{
+    if (!h || !p) return ssg_fail(SSG_E_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(h->mu);
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    if (!h->X.p || h->n <= 0) return ssg_fail(SSG_E_STATE, "index! needs a non-empty database");
+    const int rc = check_build_params(p, h->n);
+    if (rc) return rc;
+    ssg_graph_begin(h, p);
+    return ssg_build_continue(h);
+}
+
+int ssg_append_items_graph(ssg_handle* h, const float* X, int64_t m)
+{
+    if (!h) return ssg_fail(SSG_E_ARG, "null handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    HIPCHK(hipSetDevice(h->device), "hipSetDevice");
+    if (m < 0 || (m > 0 && !X)) return ssg_fail(SSG_E_ARG, "append_items!: bad arguments");
+    if (!h->has_graph || !h->b_valid || h->b_len != h->n)
+        return ssg_fail(SSG_E_STATE, "append_items!: the handle's graph was not built by index! here");
+    if (check_build_params(&h->b_params, h->n + m))
+        return ssg_fail(SSG_E_LIMIT, "append_items!: neighborhood size above 64 at the new size");
+    int rc = ssg_append_rows(h, X, m, false);
+    if (rc) return rc;
+    return ssg_build_continue(h);
 }
 
 int ssg_optimize_minrecall(ssg_handle* h, const float* Q, int64_t m, int32_t ksearch,

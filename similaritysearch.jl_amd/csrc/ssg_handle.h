@@ -83,6 +83,15 @@ struct ssg_handle {
     std::vector<std::vector<uint32_t>> lists;  // host copy of the lists (0-based)
     std::vector<uint32_t> hints_host;          // 1-based
     ssg_beam_params algo{4, 1.0f, 1000000, 32};
+    // index! state kept between calls, so append_items! continues the insertion loop
+    // (src/searchgraph/insertions.jl:27-34, index! from length(index) + 1): the build
+    // parameters, objects inserted so far, the draws' generator state and each list's
+    // segment of the device pool (start, capacity, length on the device)
+    bool b_valid = false;
+    ssg_build_params b_params{};
+    int64_t b_len = 0;
+    uint64_t b_rng = 0;
+    std::vector<uint32_t> b_seg_start, b_seg_cap, b_dev_len;
     // beam scratch
     DevBuf vis, epochs, counter;
     int64_t vis_stride = 0;
@@ -177,3 +186,12 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad, const float* 
 
 // Install a frozen CSR graph (0-based lists) on the device.
 int ssg_install_lists(ssg_handle* h);
+extern "C" {
+// Append m rows to the database: host rows of dim floats (column-major (dim, m)), or
+// device rows already padded to dp (dev = true).  Encoded copies are dropped.
+int ssg_append_rows(ssg_handle* h, const float* X, int64_t m, bool dev);
+// index! over the objects after h->b_len (the whole database on the first call).
+int ssg_build_continue(ssg_handle* h);
+// Start an index! state with parameters p over the handle's (possibly empty) database.
+int ssg_graph_begin(ssg_handle* h, const ssg_build_params* p);
+}

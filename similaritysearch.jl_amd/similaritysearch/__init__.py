@@ -412,18 +412,42 @@ class SearchGraph(_GpuIndex):
                              numqueries, ksearch, ctx.beamcap, seed, verbose)
         L = _lib.load()
         _lib.check(L.ssg_build_graph(h, C.byref(p)), "index!")
+        G._download()
+        return G
+
+    def _download(self):
+        """The device graph, its hints and BeamSearch copied into adj / hints / algo."""
+        L = _lib.load()
+        h = self._handle()
         nnz, nh, bp = C.c_int64(), C.c_int64(), BeamParams()
         _lib.check(L.ssg_graph_info(h, C.byref(nnz), C.byref(nh), C.byref(bp)), "graph info")
-        n = len(G.db)
+        n = len(self.db)
         offset = np.zeros(n, np.int64)
         end_point = np.zeros(max(nnz.value, 1), np.uint32)
         hints = np.zeros(max(nh.value, 1), np.uint32)
         _lib.check(L.ssg_download_graph(h, offset.ctypes.data, end_point.ctypes.data,
                                         hints.ctypes.data), "graph download")
-        G.adj = StaticAdjList(offset, end_point[:nnz.value])
-        G.hints = hints[:nh.value].copy()
-        G.algo = BeamSearch(bp.bsize, bp.delta, bp.maxvisits)
-        return G
+        self.adj = StaticAdjList(offset, end_point[:nnz.value])
+        self.hints = hints[:nh.value].copy()
+        self.algo = BeamSearch(bp.bsize, bp.delta, bp.maxvisits)
+
+    def append_items(self, ctx, U):
+        """append_items!(G, ctx, U) (src/searchgraph/insertions.jl:27-34) for a graph that
+        SearchGraph.build made: U's objects get ids n+1..n+len(U) and index! continues over
+        them on the device with the build's parameters (ssg_append_items_graph)."""
+        U = _as_db(U)
+        m = len(U)
+        if m == 0:
+            return self
+        if U.dim != self.db.dim:
+            raise ValueError("appended objects differ in dimension")
+        rows = np.ascontiguousarray(U.rows, np.float32)
+        up = _rows_for(self.dist, rows)
+        _lib.check(_lib.load().ssg_append_items_graph(self._handle(), up.ctypes.data, m),
+                   "append_items!")
+        self.db = MatrixDatabase.from_rows(np.concatenate([self.db.rows, rows]))
+        self._download()
+        return self
 
     def set_launch(self, slots_per_cu=32, rounds=8, vbuckets=-1):
         """Launch shape of this index's searches (ssg_set_launch; results do not depend on it)."""

@@ -110,7 +110,10 @@ SIGNATURES = {
     "ssg_graph_destroy": (C.c_int, [_p]),
     "ssg_rerank": (C.c_int, [_p, _p, _i64, _i32, _p, _p]),
     "ssg_append_db": (C.c_int, [_p, _p, _i64]),
+    "ssg_append_items_graph": (C.c_int, [_p, _p, _i64]),
     "ssg_neardup": (C.c_int, [_p, C.c_float, _i32, _i32, _p, _p, _p, _p, _p]),
+    "ssg_neardup_graph": (C.c_int, [_p, _p, C.c_float, _i32, _i32, _i32, C.POINTER(BuildParams),
+                                    _p, _p, _p, _p, _p]),
     "ssg_rerank_dev": (C.c_int, [_p, _p, _i64, _i32, _p, _p, _p]),
     "ssg_hsp_queries": (C.c_int, [_p, _i64, _i32, _p, _p, _p, _p]),
     "ssg_hsp_queries_dev": (C.c_int, [_p, _i64, _i32, _p, _p, _p, _p, _p]),

@@ -9,6 +9,8 @@ and only the reference's sequential bookkeeping runs on the host.
 - neardup: src/neardup.jl:69-256
 - hsp_queries: src/hsp.jl:8-33, 77-101
 """
+import ctypes as C
+
 import numpy as np
 
 from . import (ExhaustiveSearch, GenericContext, MatrixDatabase, SearchGraph, _as_db, _lib,
@@ -119,7 +121,8 @@ def _nearest_rows(E, Q, k):
     return ids[0], dists[0]
 
 
-def neardup(dist, X, eps, k=8, blocksize=256, filterblocks=True):
+def neardup(dist, X, eps, k=8, blocksize=256, filterblocks=True, recall=1.0, parallel_block=128,
+            seed=0):
     """neardup(dist, X, ϵ; k, blocksize, filterblocks) with an exact index
     (src/neardup.jl:69-85 with recall = 1.0, then neardup_ :94-171 and neardup_block!
     :186-256), run on the device by ssg_neardup.  X is processed in blocks of blocksize
@@ -129,14 +132,22 @@ def neardup(dist, X, eps, k=8, blocksize=256, filterblocks=True):
     the new centers join the net.  k only sizes the reference's searches; with an exact
     index their nearest entry does not depend on it.
 
-    Returns a dict with the reference's fields: idx (an ExhaustiveSearch over the net),
-    map (net position -> object id), nn (each object's covering object, 1-based; itself
-    for a center), dist (the distance to it), centers (the net, in construction order),
-    costdists (distance evaluations of the block filters)."""
+    With recall < 1 (neardup.jl:71-76) the net is a SearchGraph grown by append_items!
+    (index! with OptimizeParameters(MinRecall(recall)) among its callbacks, parallel_block
+    and seed as in SearchGraph.build), searched with KnnSorted(k) queues by
+    ssg_neardup_graph; costdists then counts the block searches' and filters' evaluations.
+
+    Returns a dict with the reference's fields: idx (an ExhaustiveSearch or SearchGraph
+    over the net), map (net position -> object id), nn (each object's covering object,
+    1-based; itself for a center), dist (the distance to it), centers (the net, in
+    construction order), costdists (distance evaluations of the block searches and
+    filters)."""
     X = _as_db(X)
     n = len(X)
     if n and not 1 <= blocksize:
         raise ValueError("blocksize must be >= 1")
+    if recall < 1.0:
+        return _neardup_graph(dist, X, eps, k, blocksize, filterblocks, recall, parallel_block, seed)
     E = ExhaustiveSearch(dist, X)
     M = np.zeros(max(n, 1), np.uint32)
     L = np.zeros(n, np.int32)
@@ -150,6 +161,35 @@ def neardup(dist, X, eps, k=8, blocksize=256, filterblocks=True):
     E.close()
     M = M[:int(nm[0])].copy()
     net = ExhaustiveSearch(dist, MatrixDatabase.from_rows(X.rows[M.astype(np.int64) - 1]))
+    return {"idx": net, "map": M, "nn": L, "dist": D, "centers": M.copy(),
+            "costdists": int(cost[0])}
+
+
+def _neardup_graph(dist, X, eps, k, blocksize, filterblocks, recall, parallel_block, seed):
+    from . import SearchGraph, BeamSearch, _GpuIndex
+    n = len(X)
+    E = ExhaustiveSearch(dist, X)
+    net = SearchGraph.__new__(SearchGraph)
+    _GpuIndex.__init__(net, dist, MatrixDatabase.from_rows(np.zeros((0, X.dim), np.float32)))
+    net.adj, net.hints, net.algo = None, np.zeros(0, np.uint32), BeamSearch()
+    M = np.zeros(max(n, 1), np.uint32)
+    L = np.zeros(n, np.int32)
+    D = np.zeros(n, np.float32)
+    nm = np.zeros(1, np.int64)
+    cost = np.zeros(1, np.int64)
+    p = _lib.BuildParams(1.3, 2, int(parallel_block), 1.1, 1.5, 256, float(recall), 32, 10, 32,
+                         int(seed), 0)
+    if n:
+        _lib.check(_lib.load().ssg_neardup_graph(E._handle(), net._handle(), float(np.float32(eps)),
+                                                 int(blocksize), int(bool(filterblocks)), int(k),
+                                                 C.byref(p), M.ctypes.data, nm.ctypes.data,
+                                                 L.ctypes.data, D.ctypes.data, cost.ctypes.data),
+                   "neardup")
+    E.close()
+    M = M[:int(nm[0])].copy()
+    net.db = MatrixDatabase.from_rows(np.ascontiguousarray(X.rows[M.astype(np.int64) - 1]))
+    if len(M):
+        net._download()
     return {"idx": net, "map": M, "nn": L, "dist": D, "centers": M.copy(),
             "costdists": int(cost[0])}
 

@@ -263,3 +263,34 @@ def test_neardup_several_batches(oracle, metric, eps, n, blocksize, filterblocks
     np.testing.assert_array_equal(R["dist"], D)
     assert R["costdists"] == cost
     assert (R["nn"][n // 2:n // 2 + 300] != np.arange(n // 2 + 1, n // 2 + 301)).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("metric,eps,blocksize,pb", [("SqL2", 0.003, 200, 32), ("NormCosine", 2e-4, 256, 64)])
+def test_neardup_recall_below_one_matches_oracle(oracle, metric, eps, blocksize, pb):
+    """neardup with recall < 1 (neardup.jl:71-76): the net is a SearchGraph grown by
+    append_items! of each block's new centers and searched with KnnSorted(8); the GPU
+    (ssg_neardup_graph) equals the oracle's restatement in map, nn, dist and costdists, and
+    the returned idx is the net's graph over the centers."""
+    ss = _ss()
+    X = _data(2400, 3, 131, metric)
+    X[1500:1600] = X[:100] + np.float32(1e-5)
+    R = ss.neardup(_dist(ss, metric), _db(X), eps, blocksize=blocksize, recall=0.9,
+                   parallel_block=pb, seed=3)
+    M, L, D, centers, cost = oracle.neardup(metric, X, eps, blocksize=blocksize, nthreads=8,
+                                            graph=dict(parallel_block=pb, opt_minrecall=0.9, seed=3))
+    assert len(centers) > 64
+    np.testing.assert_array_equal(R["map"], M)
+    np.testing.assert_array_equal(R["nn"], L)
+    np.testing.assert_array_equal(R["dist"], D)
+    assert R["costdists"] == cost
+    # idx: index! over the first block's centers, then append_items! of each later block's
+    # (the stage ends are the net sizes at the block boundaries)
+    G = R["idx"]
+    blk = (M.astype(np.int64) - 1) // blocksize
+    ends = [int(e) for e in np.nonzero(np.diff(blk))[0] + 1] + [len(M)]
+    Xc = X[M.astype(np.int64) - 1]
+    g = oracle.build_graph(metric, Xc, parallel_block=pb, opt_minrecall=0.9, seed=3, stages=ends)
+    np.testing.assert_array_equal(G.adj.offset, g["offs"][1:])
+    np.testing.assert_array_equal(G.adj.end_point, g["adj"])
+    np.testing.assert_array_equal(G.hints, g["hints"])

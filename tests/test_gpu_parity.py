@@ -224,3 +224,40 @@ def test_gpu_optimize_matches_oracle(oracle, small_graph):
     assert (algo.bsize, np.float32(algo.delta), algo.maxvisits) == \
         (o["bsize"], np.float32(o["delta"]), o["maxvisits"])
     assert rec == pytest.approx(o["recall"])
+
+
+@pytest.mark.parametrize("stages", [[1000, 1700, 3000], [640, 1280, 3000], [300, 301, 2999, 3000]])
+def test_gpu_append_items_graph_matches_oracle(oracle, stages):
+    """append_items! on a SearchGraph (insertions.jl:27-34): index! continues from
+    length(index) + 1 over the new objects; the graph, hints and tuned BeamSearch after each
+    append equal the oracle's build with the same stages; searches over the grown graph
+    match the oracle's."""
+    ss = _ss()
+    X = _uniform(3000, 8, 107)
+    G = ss.SearchGraph.build(ss.SqL2, ss.MatrixDatabase.from_rows(X[:stages[0]]), parallel_block=64,
+                             seed=7)
+    for a, b in zip(stages[:-1], stages[1:]):
+        G.append_items(ss.SearchGraphContext(), ss.MatrixDatabase.from_rows(X[a:b]))
+    g = oracle.build_graph("SqL2", X, parallel_block=64, seed=7, stages=stages)
+    np.testing.assert_array_equal(G.adj.offset, g["offs"][1:])
+    np.testing.assert_array_equal(G.adj.end_point, g["adj"])
+    np.testing.assert_array_equal(G.hints, g["hints"])
+    assert (G.algo.bsize, np.float32(G.algo.delta), G.algo.maxvisits) == \
+        (g["bsize"], np.float32(g["delta"]), g["maxvisits"])
+    Q = _uniform(200, 8, 108)
+    ctx = ss.SearchGraphContext()
+    ids, d = ss.searchbatch(G, ctx, ss.MatrixDatabase.from_rows(Q), 10)
+    bi, bd, bcd, bcb = oracle.beamsearch("SqL2", X, g["offs"], g["adj"], g["hints"], Q, 10,
+                                         g["bsize"], g["delta"], g["maxvisits"])
+    np.testing.assert_array_equal(ids.T, bi)
+    np.testing.assert_array_equal(d.T, bd)
+    np.testing.assert_array_equal(ctx.costdists, bcd)
+
+
+def test_gpu_append_items_needs_an_index_built_here(oracle, small_graph):
+    """A graph uploaded as CSR (not built by index! on the handle) cannot be continued."""
+    ss = _ss()
+    X, g = small_graph
+    G = _graph_index(ss, X, g)
+    with pytest.raises(Exception):
+        G.append_items(ss.SearchGraphContext(), ss.MatrixDatabase.from_rows(X[:10]))

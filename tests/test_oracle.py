@@ -808,3 +808,52 @@ def test_hsp_queries_hand_case(oracle):
     # 4: d(4,1) = 9, d(4,2) = 4, neither < 1, kept.  Sorted: 2 (1.0), 4 (1.0), 1 (4.0)
     np.testing.assert_array_equal(hi[0], [2, 4, 1, 0])
     np.testing.assert_array_equal(hd[0], np.array([1.0, 1.0, 4.0, np.inf], np.float32))
+
+
+def test_staged_build_on_block_boundaries_is_one_build(oracle):
+    """index! continued after append_items! starts a block at the first new object; when
+    every stage ends on a block boundary the blocks, draws and callbacks are those of a
+    single index!, so the graphs are identical (and a stage off a boundary changes them)."""
+    from similaritysearch import datasets
+    X = datasets.gmm(3000, 8, seed=3)
+    one = oracle.build_graph("SqL2", X, parallel_block=64, seed=1)
+    st = oracle.build_graph("SqL2", X, parallel_block=64, seed=1, stages=[640, 1280, 3000])
+    for key in ("offs", "adj", "hints"):
+        np.testing.assert_array_equal(one[key], st[key])
+    assert (one["bsize"], one["delta"], one["maxvisits"]) == (st["bsize"], st["delta"], st["maxvisits"])
+    off = oracle.build_graph("SqL2", X, parallel_block=64, seed=1, stages=[1000, 3000])
+    assert len(off["adj"]) != len(one["adj"]) or not np.array_equal(off["adj"], one["adj"])
+
+
+def test_neardup_graph_form_small_net_equals_exact(oracle):
+    """While the net has fewer than 64 centers the beam search is brute force over it
+    (beamsearch.jl:94-101) and charges the net's size per object, as the exact search does:
+    the recall < 1 form then equals the exact form in every output."""
+    rng = np.random.default_rng(74)
+    C0 = rng.random((30, 4), dtype=np.float32)
+    X = (C0[rng.integers(0, 30, 700)] + rng.normal(0, 1e-3, (700, 4))).astype(np.float32)
+    eps = np.float32(0.01)
+    a = oracle.neardup("SqL2", X, eps, blocksize=100)
+    b = oracle.neardup("SqL2", X, eps, blocksize=100,
+                       graph=dict(parallel_block=16, opt_minrecall=0.9, seed=2))
+    assert len(a[3]) < 64
+    for x, y in zip(a[:4], b[:4]):
+        np.testing.assert_array_equal(x, y)
+    assert a[4] == b[4]
+
+
+def test_neardup_graph_form_invariants(oracle):
+    """With a larger net (approximate searches): centers cover themselves at distance 0,
+    every other object lies within eps of a center, at the recorded distance."""
+    rng = np.random.default_rng(75)
+    X = rng.random((1500, 3), dtype=np.float32)
+    eps = np.float32(0.004)
+    M, L, D, centers, cost = oracle.neardup("SqL2", X, eps, blocksize=200,
+                                            graph=dict(parallel_block=32, opt_minrecall=0.9, seed=4))
+    assert len(centers) > 200
+    np.testing.assert_array_equal(np.sort(centers), np.nonzero(L == np.arange(1, 1501))[0] + 1)
+    other = L != np.arange(1, 1501)
+    assert np.all(D[other] <= eps) and np.all(D[~other] == 0)
+    assert set(L[other].tolist()) <= set(centers.tolist())
+    d = oracle.dist_batch("SqL2", X, X)
+    np.testing.assert_array_equal(D[other], d[np.nonzero(other)[0], L[other] - 1])
