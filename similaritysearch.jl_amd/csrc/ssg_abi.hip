@@ -341,6 +341,17 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
     HIPCHK(launch_zero_u32(w.flags, 1, s), "zero flags");
     ExhArgs a1 = a;
     a1.m = m_pad;
+    // Short slices spend most of stage 1 filling and compacting candidate lists from a
+    // threshold of -Inf.  Then each query's threshold starts at its k-th best score over
+    // 2048 rows spread over the database (a lower bound of the k-th best over all rows,
+    // less the rounding allowance), and the main pass keeps only rows above it.
+    const int64_t rps = ((n_eff + ns - 1) / ns + 63) / 64 * 64;
+    if (h->dp <= 128 && n_eff >= 32768 && rps < 32768) {
+        HIPCHK(h->seed_tau.ensure((size_t)m_pad * sizeof(float)), "alloc seed");
+        HIPCHK(exh_launch_seed_sample(h->X.as<float>(), h->xnorm.as<float>(), h->dp, n_eff, w, m, m_pad, k,
+                                      h->metric, h->xnmax, h->seed_tau.as<float>(), s), "seed");
+        w.tau0 = h->seed_tau.as<float>();
+    }
     HIPCHK(exh_launch_stage1(a1, w, ns, s), "exhaustive stage 1");
     HIPCHK(exh_launch_stage2(a, w, h->xnmax, ns, dids, ddists, kstride, s), "exhaustive stage 2");
     unsigned int nf = 0;

@@ -264,7 +264,8 @@ __global__ __launch_bounds__(256, 2) void exh_stage1_l(const float* __restrict__
                                                        int metric, const float* __restrict__ qpad,
                                                        const float* __restrict__ qnorm, int64_t m,
                                                        int64_t rows_per_slice, int nslices, uint2* list,
-                                                       float* cand_d, uint32_t* cand_i, float* slice_tau)
+                                                       float* cand_d, uint32_t* cand_i, float* slice_tau,
+                                                       const float* __restrict__ tau0)
 {
     constexpr int KS = DP / 4;            // MFMA k-steps
     constexpr int LS = DP + 4;            // LDS row stride in floats (staggers banks)
@@ -316,7 +317,9 @@ __global__ __launch_bounds__(256, 2) void exh_stage1_l(const float* __restrict__
     uint32_t wp[NCT];
 #pragma unroll
     for (int ct = 0; ct < NCT; ct++) {
-        tau[ct] = -__builtin_inff();
+        // the starting threshold: a seed pass's (rows at or below it cannot be among the k
+        // nearest, and slice_tau below reports it as the bound of the dropped rows)
+        tau[ct] = tau0 ? tau0[min(qb0 + 32 * w + 16 * ct + col, m - 1)] : -__builtin_inff();
         wp[ct] = (uint32_t)(((32 * w + 16 * ct + col) * qstride + g * SC) * (int64_t)sizeof(uint2));
     }
 
@@ -456,6 +459,47 @@ __global__ __launch_bounds__(256, 2) void exh_stage1_l(const float* __restrict__
     lds_order();
     for (int ql = 0; ql < 32; ql++) {
         const int* cq = &s_cnt[w][64 * (ql >> 4) + (ql & 15)];
+        static_assert(EXH_KC == 32, "two lists of at most EXH_KC entries fill one wave");
+        // Two queries whose lists hold at most kc entries (nothing to select): each half of
+        // the wave writes one of them out in list order (sub-list 0 first), as the
+        // compaction would, without its round trip through the list.
+        if (ql + 1 < 32) {
+            const int* cn = &s_cnt[w][64 * ((ql + 1) >> 4) + ((ql + 1) & 15)];
+            if (cq[0] + cq[16] + cq[32] + cq[48] <= kc && cn[0] + cn[16] + cn[32] + cn[48] <= kc) {
+                const int qq = ql + (lane >> 5), i = lane & 31;
+                const int* cc = &s_cnt[w][64 * (qq >> 4) + (qq & 15)];
+                const int c0 = cc[0], c1 = cc[16], c2 = cc[32], c3 = cc[48];
+                const int tot = c0 + c1 + c2 + c3;
+                uint2 v = make_uint2(0u, 0xffffffffu);
+                if (i < tot) {
+                    int g = 0, pos = i;
+                    if (pos >= c0) {
+                        pos -= c0;
+                        g = 1;
+                        if (pos >= c1) {
+                            pos -= c1;
+                            g = 2;
+                            if (pos >= c2) {
+                                pos -= c2;
+                                g = 3;
+                            }
+                        }
+                    }
+                    v = list_of(32 * w + qq)[g * SC + pos];
+                }
+                const float drop = s_tau[w][qq];
+                const int64_t q = qb0 + 32 * w + qq;
+                const float qn = q < m ? qnorm[q] : 0.f;
+                auto dist_of = [&](float sc) { return metric == M_NORMCOS ? 1.0f - sc : qn - 2.0f * sc; };
+                const int64_t base = (q * nslices + sl) * EXH_KC;
+                cand_d[base + i] = i < tot ? dist_of(__uint_as_float(v.x)) : __builtin_inff();
+                cand_i[base + i] = i < tot ? v.y : 0xffffffffu;
+                if (i == 0)
+                    slice_tau[q * nslices + sl] = drop == -__builtin_inff() ? __builtin_inff() : dist_of(drop);
+                ql++;
+                continue;
+            }
+        }
         int kept = 0;
         const float tk = compact_list_j<8>(list_of(32 * w + ql), SC, cq[0], cq[16], cq[32], cq[48], kc, true,
                                            s_hist[w], lane, &kept);
@@ -628,6 +672,162 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
         const unsigned int p = atomicAdd(&flags[0], 1u);
         flags[1 + p] = (unsigned int)q;
     }
+}
+
+// Seed of the starting thresholds: one wave per query selects the k-th smallest approximate
+// distance among a stage-1 pass's candidates (radix select on the order-preserving keys)
+// and turns it back into a score, lowered by a margin for the rounding of that conversion
+// (a lower threshold only lets more rows through).  Fewer than k candidates: -Inf.
+__global__ __launch_bounds__(256) void exh_seed_kernel(const float* __restrict__ cand_d,
+                                                       const uint32_t* __restrict__ cand_i,
+                                                       const float* __restrict__ qnorm, int nslices,
+                                                       int kc, int64_t m_valid, int64_t m_pad, int k,
+                                                       int metric, float* tau0)
+{
+    constexpr int PER = 32;  // entries per lane (nslices * kc <= 2048)
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const int64_t q = (int64_t)blockIdx.x * 4 + w;
+    if (q >= m_pad) return;
+    if (q >= m_valid) {
+        if (lane == 0) tau0[q] = -__builtin_inff();
+        return;
+    }
+    const int total = nslices * kc;
+    uint32_t key[PER];
+    int nvalid = 0;
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const int e = lane + 64 * t;
+        key[t] = 0xffffffffu;
+        if (e < total && cand_i[q * total + e] != 0xffffffffu) key[t] = sort_key(cand_d[q * total + e]);
+        nvalid += key[t] != 0xffffffffu ? 1 : 0;
+    }
+    for (int o = 32; o >= 1; o >>= 1) nvalid += __shfl_xor(nvalid, o);
+    if (nvalid < k) {
+        if (lane == 0) tau0[q] = -__builtin_inff();
+        return;
+    }
+    uint32_t prefix = 0;
+    int want = k;
+    for (int bit = 31; bit >= 0; bit--) {
+        const uint32_t hi_mask = (bit == 31) ? 0u : (0xffffffffu << (bit + 1));
+        int c = 0;
+#pragma unroll
+        for (int t = 0; t < PER; t++)
+            c += (key[t] != 0xffffffffu && (key[t] & hi_mask) == prefix && !((key[t] >> bit) & 1u)) ? 1 : 0;
+        for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+        if (c < want) {
+            want -= c;
+            prefix |= (1u << bit);
+        }
+    }
+    if (lane == 0) {
+        const float dk = key_float(prefix);
+        const float qn = qnorm[q];
+        const float sk = metric == M_NORMCOS ? 1.0f - dk : 0.5f * (qn - dk);
+        tau0[q] = __builtin_isfinite(sk) ? sk - (fabsf(sk) + qn + 1.0f) * 9.5367431640625e-07f : -__builtin_inff();
+    }
+}
+
+// Seed of the starting thresholds from a sample: a block takes 4 queries and scores S <=
+// 2048 rows spread over the database (row i * stride) against all four, each thread 8
+// rows, in FP32 with the score's form (x.q - |x|^2/2, or x.q); wave w then selects query
+// w's k-th largest score (radix select over LDS) and lowers it by the certificate's error
+// allowance, so that it lies below the matrix-core score of every one of those k rows:
+// each of them, and so each of the k nearest, passes the main pass's threshold.  Fewer
+// than k rows: -Inf.
+constexpr int SEED_S = 2048;
+__global__ __launch_bounds__(256) void exh_seed_sample_kernel(const float* __restrict__ X,
+                                                              const float* __restrict__ xnorm, int dp,
+                                                              int64_t stride, int S,
+                                                              const float* __restrict__ qpad,
+                                                              const float* __restrict__ qnorm,
+                                                              int64_t m_valid, int64_t m_pad, int k,
+                                                              int metric, float xnmax, float* tau0)
+{
+    __shared__ __attribute__((aligned(16))) float s_q[4][128];
+    __shared__ uint32_t s_key[4][SEED_S];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t q0 = (int64_t)blockIdx.x * 4;
+    for (int e = tid; e < 4 * dp; e += 256) {
+        const int64_t q = min(q0 + e / dp, m_pad - 1);
+        s_q[e / dp][e % dp] = qpad[q * dp + e % dp];
+    }
+    __syncthreads();
+    for (int r = tid; r < SEED_S; r += 256) {
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        if (r < S) {
+            const int64_t row = (int64_t)r * stride;
+            const float4* xp = reinterpret_cast<const float4*>(X + row * dp);
+            const float h = metric == M_NORMCOS ? 0.f : -0.5f * xnorm[row];
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] = h;
+            for (int u = 0; u < dp / 4; u++) {
+                const float4 a = xp[u];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const float4 b = reinterpret_cast<const float4*>(s_q[t])[u];
+                    acc[t] = __builtin_fmaf(a.x, b.x, acc[t]);
+                    acc[t] = __builtin_fmaf(a.y, b.y, acc[t]);
+                    acc[t] = __builtin_fmaf(a.z, b.z, acc[t]);
+                    acc[t] = __builtin_fmaf(a.w, b.w, acc[t]);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) s_key[t][r] = r < S ? score_key(acc[t]) : 0u;  // 0: below every score
+    }
+    __syncthreads();
+    const int64_t q = q0 + w;
+    if (q >= m_pad) return;
+    if (q >= m_valid || S < k) {
+        if (lane == 0) tau0[q] = -__builtin_inff();
+        return;
+    }
+    constexpr int PER = SEED_S / 64;
+    uint32_t key[PER];
+#pragma unroll
+    for (int t = 0; t < PER; t++) key[t] = s_key[w][lane + 64 * t];
+    // k-th largest key: bit by bit from the top
+    uint32_t prefix = 0;
+    int want = k;
+    for (int bit = 31; bit >= 0; bit--) {
+        const uint32_t cand = prefix | (1u << bit);
+        const uint32_t hi_mask = 0xffffffffu << bit;
+        int c = 0;
+#pragma unroll
+        for (int t = 0; t < PER; t++) c += (key[t] & hi_mask) == cand ? 1 : 0;
+        for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
+        // keys with the current prefix and this bit set are above the rest
+        if (c >= want) prefix = cand;
+        else want -= c;
+    }
+    if (lane == 0) {
+        const float sk = key_score(prefix);
+        const float err = (float)(dp + 8) * 2.384185791015625e-07f * (xnmax + qnorm[q]);
+        tau0[q] = __builtin_isfinite(sk) ? sk - err : -__builtin_inff();
+    }
+}
+
+hipError_t exh_launch_seed_sample(const float* X, const float* xnorm, int32_t dp, int64_t n,
+                                  const ExhWork& w, int64_t m_valid, int64_t m_pad, int k, int metric,
+                                  float xnmax, float* tau0, hipStream_t s)
+{
+    if (dp > 128) return hipErrorInvalidValue;
+    const int S = (int)std::min<int64_t>(n, SEED_S);
+    const int64_t stride = n / S;
+    hipLaunchKernelGGL(exh_seed_sample_kernel, dim3((unsigned)((m_pad + 3) / 4)), dim3(256), 0, s, X, xnorm,
+                       dp, stride, S, w.qpad, w.qnorm, m_valid, m_pad, k, metric, xnmax, tau0);
+    return hipGetLastError();
+}
+
+hipError_t exh_launch_seed(const ExhWork& w, int nslices, int64_t m_valid, int64_t m_pad, int k,
+                           int metric, float* tau0, hipStream_t s)
+{
+    if (nslices * EXH_KC > 2048) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(exh_seed_kernel, dim3((unsigned)((m_pad + 3) / 4)), dim3(256), 0, s, w.cand_d,
+                       w.cand_i, w.qnorm, nslices, EXH_KC, m_valid, m_pad, k, metric, tau0);
+    return hipGetLastError();
 }
 
 // Stage 3a: exact rescans.  Block (f, part): part of the rows for flagged query f;
@@ -876,7 +1076,7 @@ static void stage1_dp(dim3 grid, hipStream_t s, const ExhArgs& a, const ExhWork&
 {
     if constexpr (DP <= 128) {
         hipLaunchKernelGGL((exh_stage1_l<DP>), grid, dim3(256), 0, s, a.X, a.xnorm, a.n, a.metric, w.qpad,
-                           w.qnorm, w.m_valid, rps, ns, w.list, w.cand_d, w.cand_i, w.slice_tau);
+                           w.qnorm, w.m_valid, rps, ns, w.list, w.cand_d, w.cand_i, w.slice_tau, w.tau0);
     } else {
         // 64-query blocks; query rows up to roundup(m_valid, 64) (the padded rows of X when
         // the queries are the database itself)

@@ -104,7 +104,7 @@ struct ssg_handle {
     // occupancy unchanged, 0 = visited bytes in HBM only)
     int32_t vbuckets = -1;
     // exhaustive work
-    DevBuf qpad, qnorm, cand_d, cand_i, slice_tau, flags, part_i, part_d, xList;
+    DevBuf qpad, qnorm, cand_d, cand_i, slice_tau, flags, part_i, part_d, xList, seed_tau;
     // host-call staging
     DevBuf sQ, sIds, sD, sCd, sCb, sK, sB, sH, sHc, sO;
     DevBuf cIds, cD;  // hsp_queries outputs (host entry point)

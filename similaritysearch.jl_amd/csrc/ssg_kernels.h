@@ -103,8 +103,19 @@ struct ExhWork {
     int32_t nslices_cap;
     int64_t m_valid;       // queries actually present (qpad may hold fewer than m_pad rows)
     uint2* list;           // dp <= 128: m_pad * nslices * 512 candidate-list entries
+    const float* tau0;     // nullable, dp <= 128: per-query starting threshold (score units)
 };
 hipError_t exh_launch_stage1(const ExhArgs& a, const ExhWork& w, int nslices, hipStream_t s);
+// Starting thresholds from a stage-1 pass over a subset of the rows: tau0[q] = the score of
+// query q's k-th best candidate of that pass, lowered by a rounding margin (-Inf with fewer
+// than k candidates); rows m_valid .. m_pad get -Inf.
+hipError_t exh_launch_seed(const ExhWork& w, int nslices, int64_t m_valid, int64_t m_pad, int k,
+                           int metric, float* tau0, hipStream_t s);
+// The same from min(n, 2048) rows spread over the database, scored directly in FP32 and
+// lowered by the certificate's error allowance (xnmax: the largest squared row norm).
+hipError_t exh_launch_seed_sample(const float* X, const float* xnorm, int32_t dp, int64_t n,
+                                  const ExhWork& w, int64_t m_valid, int64_t m_pad, int k, int metric,
+                                  float xnmax, float* tau0, hipStream_t s);
 hipError_t exh_launch_stage2(const ExhArgs& a, const ExhWork& w, float xnmax, int nslices,
                              uint32_t* ids, float* dists, int kstride, hipStream_t s);
 hipError_t exh_launch_rescan(const ExhArgs& a, const ExhWork& w, int f0, int nf, uint32_t* part_i,

@@ -48,6 +48,34 @@ def test_exhaustive_matches_oracle(oracle, metric, dim, n, m, k):
     np.testing.assert_array_equal(d, gd)
 
 
+@pytest.mark.parametrize("metric,dim,n,m,k,order", [
+    ("SqL2", 32, 40000, 300, 10, "random"),
+    ("SqL2", 32, 40000, 300, 1, "far_first"),
+    ("L2", 128, 36000, 200, 32, "random"),
+    ("NormCosine", 96, 50000, 150, 10, "far_first"),
+    ("SqL2", 16, 40000, 257, 10, "duplicates"),
+])
+def test_exhaustive_seeded_thresholds(oracle, metric, dim, n, m, k, order):
+    """Short slices (n >= 32768 over many slices) take a seed pass over the first n/16 rows
+    whose k-th best score starts each query's threshold.  Bit-exact whatever the rows'
+    order: far_first puts the rows farthest from every query first (a weak seed), and
+    duplicates repeats rows so that distances tie at the threshold."""
+    ss = _ss()
+    X = _unit(n, dim, 13) if metric == "NormCosine" else _uniform(n, dim, 13)
+    Q = _unit(m, dim, 14) if metric == "NormCosine" else _uniform(m, dim, 14)
+    if order == "far_first":
+        c = Q.mean(axis=0)
+        X = X[np.argsort(-((X - c) ** 2).sum(axis=1), kind="stable")]
+    elif order == "duplicates":
+        X = np.repeat(X[: n // 8], 8, axis=0)
+        Q = X[np.random.default_rng(15).integers(0, n, m)] + np.float32(1e-3)
+    X = np.ascontiguousarray(X, np.float32)
+    Q = np.ascontiguousarray(Q, np.float32)
+    ids, d, gi, gd = _exh(ss, oracle, metric, X, Q, k)
+    np.testing.assert_array_equal(ids, gi)
+    np.testing.assert_array_equal(d, gd)
+
+
 def test_exhaustive_fewer_objects_than_k(oracle):
     ss = _ss()
     X = _uniform(5, 16, 3)
