@@ -101,9 +101,9 @@ def _traffic(name, key, applies, scale=1):
 
 def _kname(prec, beamcap, rounds):
     """The beam kernel instantiation a launch runs (ssg_beam.h launch_v): beam registers
-    1/2/4 by beam cap, gather rounds 4 or 10 only with one beam register (10: BF16 form)."""
+    1/2/4 by beam cap, gather rounds 4, 8, or 10 with one beam register in the BF16 form."""
     nb = 1 if beamcap <= 64 else 2 if (prec and beamcap <= 128) else 4
-    r = rounds if nb == 1 and (rounds == 4 or (rounds == 10 and prec)) else 8
+    r = rounds if rounds == 4 or (rounds == 10 and prec and nb == 1) else 8
     return f"{'beam_bf16_kernel' if prec else 'beam_kernel'}<8, {nb}, {r}>"
 
 

@@ -256,8 +256,8 @@ int ssg_graph_destroy(ssg_graph* graph);
 /* Launch shape of the handle's beam searches (no reference counterpart): persistent wave
  * slots per CU (each wave owns a visited slot of n bytes; 4..64, default 32), gather
  * rounds per batch (each round gathers one row per 16-lane group: 8, the default; 4, fewer
- * registers; 10, in the BF16 form, 40 children per batch; the 4- and 10-round kernels serve
- * beam caps <= 64, others run 8) and the LDS visited table of each wave (vbuckets buckets
+ * registers; 10, in the BF16 form, 40 children per batch, for beam caps <= 64, others run
+ * 8) and the LDS visited table of each wave (vbuckets buckets
  * of eight 16-bit tags, 16 bytes each: a visited mark goes to the table while its bucket
  * has room, else to the wave's HBM slot; -1, the default: the largest table that leaves
  * the kernel's occupancy unchanged; 0: every mark in HBM; 1..2048: that many buckets,

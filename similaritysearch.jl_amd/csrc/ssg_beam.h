@@ -680,8 +680,9 @@ template <int V>
 hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
 {
     const dim3 grid((unsigned)(nslots / WAVES)), block(256);
-    // the 4- and 10-round variants exist for one beam register (beam caps <= 64); 10 rounds
-    // (40 children per batch) only in the BF16 form, where its rows still leave 4 waves per SIMD
+    // 4 rounds for every beam cap (fewer registers: the 4-register FP32 kernel fits 3 waves
+    // per SIMD with them, 2 with 8 rounds); 10 rounds (40 children per batch) for one beam
+    // register in the BF16 form, where its rows still leave 4 waves per SIMD
     if constexpr (V > 16) {
         // rows above 256 dims: the FP32 form only, with fewer gather rounds
         constexpr int RW = wide_rounds<V>();
@@ -697,26 +698,32 @@ hipError_t launch_v(const BeamArgs& a, int nslots, hipStream_t s)
         if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1, RW>, grid, block, a, s);
         return launch_lds(beam_kernel<V, 4, RW>, grid, block, a, s);
     } else {
-        const bool r4 = a.rounds == 4 && a.beamcap <= 64;
+        const bool r4 = a.rounds == 4;
         const bool r10 = a.rounds == 10 && a.beamcap <= 64;
         if (a.internal) {
             if (a.beamcap <= 64) return launch_lds(beam_build_kernel<V, 1>, grid, block, a, s);
             return launch_lds(beam_build_kernel<V, 4>, grid, block, a, s);
         }
         if (a.prec == 1) {
-            if (r4) return launch_lds(beam_bf16_kernel<V, 1, 4>, grid, block, a, s);
             if (r10) return launch_lds(beam_bf16_kernel<V, 1, 10>, grid, block, a, s);
-            if (a.beamcap <= 64) return launch_lds(beam_bf16_kernel<V, 1>, grid, block, a, s);
-            if (a.beamcap <= 128) return launch_lds(beam_bf16_kernel<V, 2>, grid, block, a, s);
-            return launch_lds(beam_bf16_kernel<V, 4>, grid, block, a, s);
+            if (a.beamcap <= 64)
+                return r4 ? launch_lds(beam_bf16_kernel<V, 1, 4>, grid, block, a, s)
+                          : launch_lds(beam_bf16_kernel<V, 1>, grid, block, a, s);
+            if (a.beamcap <= 128)
+                return r4 ? launch_lds(beam_bf16_kernel<V, 2, 4>, grid, block, a, s)
+                          : launch_lds(beam_bf16_kernel<V, 2>, grid, block, a, s);
+            return r4 ? launch_lds(beam_bf16_kernel<V, 4, 4>, grid, block, a, s)
+                      : launch_lds(beam_bf16_kernel<V, 4>, grid, block, a, s);
         }
         if (a.kstride > 64) {
             if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1, 8, 4>, grid, block, a, s);
             return launch_lds(beam_kernel<V, 4, 8, 4>, grid, block, a, s);
         }
-        if (r4) return launch_lds(beam_kernel<V, 1, 4>, grid, block, a, s);
-        if (a.beamcap <= 64) return launch_lds(beam_kernel<V, 1>, grid, block, a, s);
-        return launch_lds(beam_kernel<V, 4>, grid, block, a, s);
+        if (a.beamcap <= 64)
+            return r4 ? launch_lds(beam_kernel<V, 1, 4>, grid, block, a, s)
+                      : launch_lds(beam_kernel<V, 1>, grid, block, a, s);
+        return r4 ? launch_lds(beam_kernel<V, 4, 4>, grid, block, a, s)
+                  : launch_lds(beam_kernel<V, 4>, grid, block, a, s);
     }
 }
 

@@ -163,7 +163,7 @@ int ssg_optimize_launch(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
         for (int32_t sp : {8, 16, 24, 32, 48, 64}) {
             for (int32_t rd : {4, 8, 10}) {
                 if (rd == 10 && knobs[kn].precision != SSG_BF16) continue;  // BF16 kernels only
-                if (rd != 8 && knobs[kn].beamcap > 64) continue;  // one-register beams only
+                if (rd == 10 && knobs[kn].beamcap > 64) continue;  // one-register beams only
                 for (int32_t vb : {-1, 0}) {
                     h->slots_per_cu = sp;
                     h->rounds = rd;

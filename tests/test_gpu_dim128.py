@@ -243,6 +243,32 @@ def test_launch_shapes_are_bit_exact(oracle, g128, prec, shape):
         _lib.check(L.ssg_set_launch(h, C.byref(_lib.LaunchParams(32, 8, -1))), "set launch")
 
 
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+@pytest.mark.parametrize("beamcap", [128, 256])
+def test_wide_beams_4_rounds_bit_exact(oracle, g128, prec, beamcap):
+    """The 4-round kernels of the 2- and 4-register beams (fewer registers, more waves
+    per SIMD) give the oracle's bits."""
+    import similaritysearch as ss
+    from similaritysearch import _lib
+    X, Q, G, offs = g128
+    L = _lib.load()
+    h = G._handle()
+    _lib.check(L.ssg_set_launch(h, C.byref(_lib.LaunchParams(32, 4, -1))), "set launch")
+    try:
+        G.algo = ss.BeamSearch(100, 1.05, 10**6)
+        rr = 14 if prec == "bf16" else None
+        ctx = ss.SearchGraphContext(beamcap=beamcap, precision=prec, rerank=rr or 0)
+        ids, d = ss.searchbatch(G, ctx, ss.MatrixDatabase.from_rows(Q), 10)
+        gi, gd, gcd, gcb = oracle.beamsearch("SqL2", X, offs, G.adj.end_point, G.hints, Q, 10, 100,
+                                             1.05, 10**6, beamcap=beamcap, nthreads=8, bf16_rerank=rr)
+        np.testing.assert_array_equal(ids.T, gi)
+        np.testing.assert_array_equal(d.T, gd)
+        np.testing.assert_array_equal(ctx.costdists, gcd)
+        np.testing.assert_array_equal(ctx.costblocks, gcb)
+    finally:
+        _lib.check(L.ssg_set_launch(h, C.byref(_lib.LaunchParams(32, 8, -1))), "set launch")
+
+
 @pytest.fixture(scope="module")
 def allknn128(oracle):
     """A 3,000 x 128 graph from the oracle's builder and the oracle's allknn over it."""
