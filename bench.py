@@ -100,11 +100,12 @@ def _traffic(name, key, applies, scale=1):
 
 
 def _kname(prec, beamcap, rounds):
-    """The beam kernel instantiation a launch runs (ssg_beam.h launch_v): beam registers
-    1/2/4 by beam cap, gather rounds 4, 8, or 10 with one beam register in the BF16 form."""
+    """The beam kernel instantiation a launch runs (ssg_beam.h launch_v) for k <= 64: beam
+    registers 1/2/4 by beam cap, gather rounds 4, 8, or 10 with one beam register in the
+    BF16 form; the FP32 kernel's last parameter is its one result register."""
     nb = 1 if beamcap <= 64 else 2 if (prec and beamcap <= 128) else 4
     r = rounds if rounds == 4 or (rounds == 10 and prec and nb == 1) else 8
-    return f"{'beam_bf16_kernel' if prec else 'beam_kernel'}<8, {nb}, {r}>"
+    return f"beam_bf16_kernel<8, {nb}, {r}>" if prec else f"beam_kernel<8, {nb}, {r}, 1>"
 
 
 def _capture(L, _lib, st, fn):
