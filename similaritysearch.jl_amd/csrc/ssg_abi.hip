@@ -499,7 +499,9 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     // at or below it, which removes most appends, and the rerank stage certifies each
     // query (the ncand-th best candidate must score above the threshold).  Queries it
     // cannot certify are searched again from no threshold.
-    const int32_t T = h->prec_sample > 0 ? h->prec_sample : std::max(16, 3 * kc / 16);
+    // T = kc / 8 puts the threshold near rank 2 kc: a query's kc-th best lies below it
+    // unless 16 sampled rows of 128 fall in its kc best (binomial tail 0.6% at kc = 128).
+    const int32_t T = h->prec_sample > 0 ? h->prec_sample : std::max(16, kc / 8);
     if (!sample) {
         HIPCHK(prec_launch_stage1(a, s), "precision candidate pass");
         HIPCHK(prec_launch_stage2(a, o, h->dp, s), "precision rerank");
@@ -529,7 +531,7 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     as.tau0 = h->pTau0.as<float>();
     if (seed) {
         // Seed pass: the T2-th best score of every 16th tile of the sample (about rank
-        // 16 T2 = 128 of the sample for T = 24) starts the sample pass's threshold, so it
+        // 16 T2 = 80 of the sample for T = 16) starts the sample pass's threshold, so it
         // appends about 16 T2 rows per query instead of filling its lists from nothing.
         // A query with fewer than T sample rows above its seed gets no threshold from the
         // sample pass (its main pass then starts from none): slower, still exact.
@@ -565,7 +567,8 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     if (nf == 0) return SSG_OK;
     // redo of the flagged queries, gathered, over enough slices to fill the device
     const int64_t mf_pad = ((int64_t)nf + PREC_QB - 1) / PREC_QB * PREC_QB;
-    int32_t nsf = (int32_t)std::max<int64_t>(1, (2 * (int64_t)h->num_cu + mf_pad / PREC_QB - 1) / (mf_pad / PREC_QB));
+    int32_t nsf = (int32_t)std::max<int64_t>(
+        1, ((int64_t)h->num_cu * (8 / PREC_W) + mf_pad / PREC_QB - 1) / (mf_pad / PREC_QB));
     nsf = std::max(1, std::min<int32_t>(nsf, 2048 / kc));
     int64_t rpsf = (n + nsf - 1) / nsf;
     rpsf = (rpsf + 127) / 128 * 128;

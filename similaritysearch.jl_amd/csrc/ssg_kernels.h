@@ -127,7 +127,8 @@ constexpr int EXH_WIDE_KMAX = 3840;  // k of the one-block-per-query path (k > E
 constexpr int EXH_QT = 128;       // queries per block of exhaustive stage 1 (query padding)
 
 // Precision form of exhaustive k-NN: BF16 MFMA candidate pass + exact rerank.
-constexpr int PREC_QB = 256;          // queries per candidate-pass block
+constexpr int PREC_W = 8;             // waves (of 64 queries) per candidate-pass block
+constexpr int PREC_QB = 64 * PREC_W;  // queries per candidate-pass block
 constexpr int PREC_NCAND_MAX = 256;   // candidates per query
 constexpr int PREC_CAP_MAX = 1024;    // list capacity per (query, slice): prec_list_cap(ncand)
 // 4 sub-lists of a power-of-two number of entries each, room for the kept entries of a

@@ -12,12 +12,14 @@
 //   v_mfma_scale_f32_16x16x128_f8f6f4, whose E8M0 scale operands multiply the product by
 //   2^-e of each side exactly.  FP8 rows are padded to 128 values.
 //
-// Candidate pass (prec_stage1): block = 256 queries x one DB slice, 4 waves of 64
-//   queries.  A wave keeps its queries' encoded rows as the B operands of the matrix-core
-//   product in registers for the whole slice; DB rows are staged in
-//   LDS in tiles of 128 rows (BF16) or 256 rows (FP8) (double-buffered, one barrier per
-//   tile, the next tile prefetched into registers while the current one is multiplied)
-//   and shared by the 4 waves.  The accumulator starts at -||x||^2/2 (SqL2, L2) or 0
+// Candidate pass (prec_stage1): block = 512 queries x one DB slice, 8 waves of 64
+//   queries (one block per CU: each row staged in LDS serves 512 queries).  A wave keeps
+//   its queries' encoded rows as the B operands of the matrix-core product in registers
+//   for the whole slice; DB rows are staged in LDS in tiles of 128 rows (BF16) or 256
+//   rows (FP8) (double-buffered, one barrier per tile, the next tile prefetched into
+//   registers while the current one is multiplied) and shared by the 8 waves.  Within a
+//   tile the products of one 16-row block run while the previous block's scores are
+//   tested, and the operand pieces of the block after are read from LDS meanwhile.  The accumulator starts at -||x||^2/2 (SqL2, L2) or 0
 //   (NormCosine), so the MFMA result is the score s = x.q - ||x||^2/2 (or x.q): larger
 //   is nearer, and an invalid row starts at -Inf.  A lane holds 4 rows x 1 query per 16x16 tile; a tile
 //   is tested with two max operations and one compare against the query's threshold,
@@ -84,7 +86,7 @@ template <> struct PrecOp<true> {
 // instance uses are what the kernel has to keep clear around the call.  F8 selects the
 // FP8 form.
 template <int KB, int J, bool F8>
-__global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
+__global__ __launch_bounds__(PREC_QB, 8 / PREC_W) void prec_stage1(PrecArgs a)
 {
     typedef PrecOp<F8> Op;
     typedef typename Op::T OpT;
@@ -99,14 +101,14 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
     constexpr int LSB = SWZ ? RB : RB + 16;  // LDS row stride in bytes
     constexpr int NCT = 4;                   // 16-query column tiles per wave
     constexpr int U4 = RB / 16;              // 16-byte units per row
-    constexpr int PF = TR * U4 / 256;        // 16-byte units per thread per tile
-    static_assert(TR * U4 % 256 == 0, "tile must split evenly over the block");
+    constexpr int PF = TR * U4 / PREC_QB;    // 16-byte units per thread per tile
+    static_assert(TR * U4 % PREC_QB == 0, "tile must split evenly over the block");
     static_assert(!SWZ || (U4 == 8 && KB == 1), "the swizzle covers one 128-byte row");
     __shared__ __attribute__((aligned(16))) uint8_t s_x[2][TR * LSB];
     __shared__ __attribute__((aligned(16))) float s_h[2][TR];
-    __shared__ __attribute__((aligned(16))) uint32_t s_hist[4][256];
-    __shared__ int s_cnt[4][NCT * 64];   // sub-list counts of each wave's queries
-    __shared__ float s_tau[4][NCT * 16]; // their thresholds
+    __shared__ __attribute__((aligned(16))) uint32_t s_hist[PREC_W][256];
+    __shared__ int s_cnt[PREC_W][NCT * 64];   // sub-list counts of each wave's queries
+    __shared__ float s_tau[PREC_W][NCT * 16]; // their thresholds
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int col = lane & 15, g = lane >> 4;
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
             pfx = a.xnorm[min(t + (tid & (TR - 1)), r1 - 1)];
 #define SSG_PF(I, R)                                                                        \
             if (I < PF) {                                                                   \
-                const int idx = tid + 256 * I;                                              \
+                const int idx = tid + PREC_QB * I;                                          \
                 const int row = idx / U4, c = idx % U4;                                     \
                 const int64_t gr = min(t + row, r1 - 1); /* rows past the slice: see s_h */ \
                 R = reinterpret_cast<const uint4*>(a.Xe + gr * RB)[c];                      \
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
         auto stage = [&](int b) {
 #define SSG_ST(I, R)                                                                        \
             if (I < PF) {                                                                   \
-                const int idx = tid + 256 * I;                                              \
+                const int idx = tid + PREC_QB * I;                                          \
                 const int row = idx / U4, c = idx % U4;                                     \
                 const int pc = SWZ ? (c ^ (row & 7)) : c;                                   \
                 *reinterpret_cast<uint4*>(&s_x[b][row * LSB + pc * 16]) = R;                \
@@ -199,93 +201,128 @@ __global__ __launch_bounds__(256, 2) void prec_stage1(PrecArgs a)
             if (more) prefetch(t0 + step);
             const uint8_t* xb = s_x[b];
             const float* hb = s_h[b];
-            // MFMA chains of one 16-row block for the wave's 4 query tiles
-            auto chain = [&](int rb, floatx4 (&acc)[NCT]) {
-                const floatx4 c0 = *reinterpret_cast<const floatx4*>(&hb[rb * 16 + 4 * g]);
+            // One 16-row block: the lane's operand pieces and start values (LDS reads),
+            // then the MFMA chains for the wave's 4 query tiles.  The products of block
+            // rb + 1 run while the scores of block rb are tested, and (up to 3 pieces per
+            // lane) the pieces of block rb + 2 are read meanwhile, so the LDS latency is
+            // behind matrix-core work.
+            struct Frag {
                 OpT af[KB];
+                floatx4 c0;
+            };
+            auto frag = [&](int rb, Frag& f) {
+                f.c0 = *reinterpret_cast<const floatx4*>(&hb[rb * 16 + 4 * g]);
                 const int r = rb * 16 + col;
                 if constexpr (SWZ) {
                     // the lane's 32 bytes are chunks 2g and 2g + 1 of its row
                     const uint8_t* rp = xb + r * LSB;
                     const intx4 lo = *reinterpret_cast<const intx4*>(rp + ((2 * g) ^ (r & 7)) * 16);
                     const intx4 hi = *reinterpret_cast<const intx4*>(rp + ((2 * g + 1) ^ (r & 7)) * 16);
-                    af[0] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+                    f.af[0] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
                 } else {
                     const uint8_t* ap = xb + r * LSB + g * KB * PB;
 #pragma unroll
-                    for (int s = 0; s < KB; s++) af[s] = Op::ld(ap + PB * s);
+                    for (int s = 0; s < KB; s++) f.af[s] = Op::ld(ap + PB * s);
                 }
+            };
+            auto chain = [&](const Frag& f, floatx4 (&acc)[NCT]) {
 #pragma unroll
                 for (int ct = 0; ct < NCT; ct++) {
-                    floatx4 c = c0;
+                    floatx4 c = f.c0;
 #pragma unroll
-                    for (int s = 0; s < KB; s++) c = Op::mma(af[s], bq[ct][s], c, sx, sq);
+                    for (int s = 0; s < KB; s++) c = Op::mma(f.af[s], bq[ct][s], c, sx, sq);
                     acc[ct] = c;
                 }
             };
+            constexpr int NRB = TR / 16;
+            constexpr bool LA = KB < 4;  // read ahead (the 4-piece form has no registers for it)
             floatx4 cur[NCT], nxt[NCT];
-            chain(0, cur);
+            Frag fr;
+            frag(0, fr);
+            chain(fr, cur);
+            if (LA) frag(1, fr);
 #pragma unroll
-            for (int rb = 0; rb < TR / 16; rb++) {
-                if (rb + 1 < TR / 16) chain(rb + 1, nxt);
+            for (int rb = 0; rb < NRB; rb++) {
+                if (rb + 1 < NRB) {
+                    if (!LA) frag(rb + 1, fr);
+                    chain(fr, nxt);
+                    if (LA && rb + 2 < NRB) frag(rb + 2, fr);
+                }
                 // Each 16-query tile is tested on its own (the maximum of its 4 x 16 scores
-                // against each query's threshold; 256 scores per decision), and the append
-                // path of a tile runs only when one of its scores passes.
-                const uint32_t rowb = (uint32_t)(t0 + rb * 16 + 4 * g);
-                uint64_t qfull = 0;  // bit 16 * ct + col: that query's list is compacted now
+                // against each query's threshold; 256 scores per decision).  The fast path
+                // is one test of all 4 tiles; the append path of a tile runs only when one
+                // of its scores passes.
+                float mx[NCT];
+                bool anyp = false;
 #pragma unroll
                 for (int ct = 0; ct < NCT; ct++) {
-                    const float mx = __builtin_elementwise_maximum(
+                    mx[ct] = __builtin_elementwise_maximum(
                         __builtin_elementwise_maximum(cur[ct][0], cur[ct][1]),
                         __builtin_elementwise_maximum(cur[ct][2], cur[ct][3]));
-                    if (__builtin_expect(__ballot(mx > tau[ct]) != 0, 0)) {
-                        const float t = tau[ct];
-                        uint32_t o = wp[ct];
-                        // No branches: every row's entry is stored, a row that does not
-                        // pass at an offset past the buffer's extent (2^31), which the
-                        // hardware drops, and only passing rows advance the offset.
+                    anyp |= mx[ct] > tau[ct];
+                }
+                if (__builtin_expect(__ballot(anyp) != 0, 0)) {
+                    const uint32_t rowb = (uint32_t)(t0 + rb * 16 + 4 * g);
+                    uint64_t qfull = 0;  // bit 16 * ct + col: that query's list is compacted now
 #pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const bool p = cur[ct][r] > t;
-                            const u32x2 e = {__float_as_uint(cur[ct][r]), rowb + r};
-                            __builtin_amdgcn_raw_buffer_store_b64(e, lrs, (int)(p ? o : 0x80000000u), 0, 0);
-                            o += p ? 8u : 0u;
+                    for (int ct = 0; ct < NCT; ct++) {
+                        if (__ballot(mx[ct] > tau[ct]) != 0) {
+                            const float t = tau[ct];
+                            uint32_t o = wp[ct];
+                            // No branches: every row's entry is stored, a row that does not
+                            // pass at an offset past the buffer's extent (2^31), which the
+                            // hardware drops, and only passing rows advance the offset.
+#pragma unroll
+                            for (int r = 0; r < 4; r++) {
+                                const bool p = cur[ct][r] > t;
+                                const u32x2 e = {__float_as_uint(cur[ct][r]), rowb + r};
+                                __builtin_amdgcn_raw_buffer_store_b64(e, lrs, (int)(p ? o : 0x80000000u), 0, 0);
+                                o += p ? 8u : 0u;
+                            }
+                            wp[ct] = o;
+                            // A lane adds at most 4 entries per query and row block, so a
+                            // sub-list with at most sc - 5 entries cannot fill up on the next
+                            // (and the offset's low bits never wrap to 0).
+                            const uint64_t f = __ballot((o & smask) > full_at);
+                            qfull |= ((f | (f >> 16) | (f >> 32) | (f >> 48)) & 0xffffull) << (16 * ct);
                         }
-                        wp[ct] = o;
-                        // A lane adds at most 4 entries per query and row block, so a
-                        // sub-list with at most sc - 5 entries cannot fill up on the next
-                        // (and the offset's low bits never wrap to 0).
-                        const uint64_t f = __ballot((o & smask) > full_at);
-                        qfull |= ((f | (f >> 16) | (f >> 32) | (f >> 48)) & 0xffffull) << (16 * ct);
+                    }
+                    if (__builtin_expect(qfull != 0, 0)) {
+                        // counts and thresholds go through LDS so that the loop indexes no
+                        // register array
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ct++) {
+                            s_cnt[w][64 * ct + lane] = (int)((wp[ct] & smask) >> 3);
+                            if (g == 0) s_tau[w][16 * ct + col] = tau[ct];
+                        }
+                        wave_sync();  // also makes this wave's appends visible to the reads
+                        while (qfull) {
+                            const int ql = __builtin_ctzll(qfull);
+                            qfull &= qfull - 1;
+                            int* cq = &s_cnt[w][64 * (ql >> 4) + (ql & 15)];
+                            int kept = 0;
+                            const float nt = compact_list_j<J>(list_of(64 * w + ql), sc, cq[0], cq[16], cq[32],
+                                                               cq[48], kc, false, s_hist[w], lane, &kept);
+                            if (lane < 4) cq[16 * lane] = (kept + 3 - lane) >> 2;
+                            if (lane == 0) s_tau[w][ql] = nt;
+                            lds_order();
+                        }
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ct++) {
+                            wp[ct] = (wp[ct] & ~smask) + (uint32_t)s_cnt[w][64 * ct + lane] * 8u;
+                            tau[ct] = s_tau[w][16 * ct + col];
+                        }
+                    }
+                    // The next block's products are an operand of this path too (at its
+                    // end, when they are done), so that the compiler issues them before the
+                    // branch instead of sinking them next to their test, which would wait
+                    // for each product.
+                    if (rb + 1 < NRB) {
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ct++) asm volatile("" : "+v"(nxt[ct]));
                     }
                 }
-                if (__builtin_expect(qfull != 0, 0)) {
-                    // counts and thresholds go through LDS so that the loop indexes no
-                    // register array
-#pragma unroll
-                    for (int ct = 0; ct < NCT; ct++) {
-                        s_cnt[w][64 * ct + lane] = (int)((wp[ct] & smask) >> 3);
-                        if (g == 0) s_tau[w][16 * ct + col] = tau[ct];
-                    }
-                    wave_sync();  // also makes this wave's appends visible to the reads
-                    while (qfull) {
-                        const int ql = __builtin_ctzll(qfull);
-                        qfull &= qfull - 1;
-                        int* cq = &s_cnt[w][64 * (ql >> 4) + (ql & 15)];
-                        int kept = 0;
-                        const float nt = compact_list_j<J>(list_of(64 * w + ql), sc, cq[0], cq[16], cq[32], cq[48],
-                                                      kc, false, s_hist[w], lane, &kept);
-                        if (lane < 4) cq[16 * lane] = (kept + 3 - lane) >> 2;
-                        if (lane == 0) s_tau[w][ql] = nt;
-                        lds_order();
-                    }
-#pragma unroll
-                    for (int ct = 0; ct < NCT; ct++) {
-                        wp[ct] = (wp[ct] & ~smask) + (uint32_t)s_cnt[w][64 * ct + lane] * 8u;
-                        tau[ct] = s_tau[w][16 * ct + col];
-                    }
-                }
-                if (rb + 1 < TR / 16) {
+                if (rb + 1 < NRB) {
 #pragma unroll
                     for (int ct = 0; ct < NCT; ct++) cur[ct] = nxt[ct];
                 }
@@ -605,7 +642,7 @@ hipError_t launch_rows_to_bf16(const float* src, int64_t rows, int32_t dp, uint1
 int32_t prec_slices(int64_t n, int64_t m, int num_cu, int kc)
 {
     const int64_t qtiles = (m + PREC_QB - 1) / PREC_QB;
-    const int64_t cap = (int64_t)num_cu * 2;
+    const int64_t cap = (int64_t)num_cu * (8 / PREC_W);  // resident blocks
     int64_t smax = (n + 4 * PQ_TR - 1) / (4 * PQ_TR);
     if (smax > (int64_t)PQ_JS * 64 / kc) smax = (int64_t)PQ_JS * 64 / kc;
     if (smax < 1) smax = 1;
@@ -629,7 +666,7 @@ int32_t prec_slices(int64_t n, int64_t m, int num_cu, int kc)
 
 hipError_t prec_launch_stage1(const PrecArgs& a, hipStream_t s)
 {
-    const dim3 grid((unsigned)(a.m_pad / PREC_QB), (unsigned)a.nslices), block(256);
+    const dim3 grid((unsigned)(a.m_pad / PREC_QB), (unsigned)a.nslices), block(PREC_QB);
     // list capacities: 128 (kc <= 64; the sample pass), 512 (kc <= 128), 1024
     if (a.cap != 128 && a.cap != 512 && a.cap != 1024) return hipErrorInvalidValue;
     if (a.kc + 64 > a.cap) return hipErrorInvalidValue;
