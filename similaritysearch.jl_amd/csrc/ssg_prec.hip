@@ -15,8 +15,8 @@
 // Candidate pass (prec_stage1): block = 512 queries x one DB slice, 8 waves of 64
 //   queries (one block per CU: each row staged in LDS serves 512 queries).  A wave keeps
 //   its queries' encoded rows as the B operands of the matrix-core product in registers
-//   for the whole slice; DB rows are staged in LDS in tiles of 128 rows (BF16) or 256
-//   rows (FP8) (double-buffered, one barrier per tile, the next tile prefetched into
+//   for the whole slice; DB rows are staged in LDS in tiles of 256 rows (128 for the
+//   largest BF16 lists) (double-buffered, one barrier per tile, the next tile prefetched into
 //   registers while the current one is multiplied) and shared by the 8 waves.  Within a
 //   tile the products of one 16-row block run while the previous block's scores are
 //   tested, and the operand pieces of the block after are read from LDS meanwhile.  The accumulator starts at -||x||^2/2 (SqL2, L2) or 0
@@ -92,12 +92,13 @@ __global__ __launch_bounds__(PREC_QB, 8 / PREC_W) void prec_stage1(PrecArgs a)
     typedef typename Op::T OpT;
     constexpr int PB = Op::PB;
     constexpr int RB = 4 * PB * KB;          // encoded row bytes
-    // FP8 rows are 128 bytes: tiles of 256 rows (half the barriers per row), stored
-    // unpadded with the 8 16-byte chunks of row r at positions c ^ (r & 7), so that the 8
-    // rows a quarter-wave reads, or the chunks a row is written in, fall in distinct
-    // banks.  BF16 rows: 128-row tiles, rows padded by 16 bytes.
+    // FP8 rows are 128 bytes, stored unpadded with the 8 16-byte chunks of row r at
+    // positions c ^ (r & 7), so that the 8 rows a quarter-wave reads, or the chunks a row
+    // is written in, fall in distinct banks.  BF16 rows are padded by 16 bytes.
     constexpr bool SWZ = F8;
-    constexpr int TR = F8 ? 2 * PQ_TR : PQ_TR;  // DB rows per LDS tile
+    // DB rows per LDS tile: 256 (128 for the 1024-entry lists of rows of 3-4 pieces,
+    // which leave no registers for the larger prefetch)
+    constexpr int TR = (F8 || J < 16 || KB < 3) ? 2 * PQ_TR : PQ_TR;
     constexpr int LSB = SWZ ? RB : RB + 16;  // LDS row stride in bytes
     constexpr int NCT = 4;                   // 16-query column tiles per wave
     constexpr int U4 = RB / 16;              // 16-byte units per row
