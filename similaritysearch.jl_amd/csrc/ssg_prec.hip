@@ -15,8 +15,8 @@
 // Candidate pass (prec_stage1): block = 512 queries x one DB slice, 8 waves of 64
 //   queries (one block per CU: each row staged in LDS serves 512 queries).  A wave keeps
 //   its queries' encoded rows as the B operands of the matrix-core product in registers
-//   for the whole slice; DB rows are staged in LDS in tiles of 256 rows (128 for the
-//   largest BF16 lists) (double-buffered, one barrier per tile, the next tile prefetched into
+//   for the whole slice; DB rows are staged in LDS in tiles of 512 (FP8) or 256 (BF16)
+//   rows (double-buffered, one barrier per tile, the next tile prefetched into
 //   registers while the current one is multiplied) and shared by the 8 waves.  Within a
 //   tile the products of one 16-row block run while the previous block's scores are
 //   tested, and the operand pieces of the block after are read from LDS meanwhile.  The accumulator starts at -||x||^2/2 (SqL2, L2) or 0
@@ -96,9 +96,9 @@ __global__ __launch_bounds__(PREC_QB, 8 / PREC_W) void prec_stage1(PrecArgs a)
     // positions c ^ (r & 7), so that the 8 rows a quarter-wave reads, or the chunks a row
     // is written in, fall in distinct banks.  BF16 rows are padded by 16 bytes.
     constexpr bool SWZ = F8;
-    // DB rows per LDS tile: 256 (128 for the 1024-entry lists of rows of 3-4 pieces,
-    // which leave no registers for the larger prefetch)
-    constexpr int TR = (F8 || J < 16 || KB < 3) ? 2 * PQ_TR : PQ_TR;
+    // DB rows per LDS tile, as many as LDS and the prefetch registers allow: FP8 512 (256
+    // for 1024-entry lists), BF16 256 (128 for 1024-entry lists of rows of 3-4 pieces)
+    constexpr int TR = (F8 && J < 16) ? 4 * PQ_TR : (F8 || J < 16 || KB < 3) ? 2 * PQ_TR : PQ_TR;
     constexpr int LSB = SWZ ? RB : RB + 16;  // LDS row stride in bytes
     constexpr int NCT = 4;                   // 16-query column tiles per wave
     constexpr int U4 = RB / 16;              // 16-byte units per row
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(PREC_QB, 8 / PREC_W) void prec_stage1(PrecArgs a)
         uint4 pf0, pf1, pf2, pf3, pf4, pf5, pf6, pf7;  // the next tile (PF <= 8 units per thread)
         float pfx = 0.f;                                // the norm of its row tid % TR
         int64_t pft = r0;                               // its first row
-        static_assert(PF <= 8 && TR <= 256, "prefetch registers");
+        static_assert(PF <= 8 && TR <= PREC_QB, "prefetch registers; one row norm per thread");
         // Every load is unconditional (clamped indices) and nothing consumes it before
         // stage(): a conditional load consumed in its branch would make the wave wait for
         // all of its loads in flight, the tile prefetch included.
