@@ -304,9 +304,7 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
         HIPCHK(h->qnorm.ensure((size_t)m_pad * sizeof(float)), "alloc qnorm");
         w.qpad = h->qpad.as<float>();
         w.qnorm = h->qnorm.as<float>();
-        HIPCHK(launch_fill_f32(w.qpad + m * h->dp, (m_pad - m) * h->dp, 0.f, s), "fill qpad");
-        HIPCHK(launch_fill_f32(w.qnorm + m, m_pad - m, 0.f, s), "fill qnorm");
-        HIPCHK(launch_pad_rows(dQ, m, h->dim, w.qpad, h->dp, w.qnorm, s), "pad queries");
+        HIPCHK(launch_pad_rows(dQ, m, h->dim, w.qpad, h->dp, w.qnorm, s, m_pad), "pad queries");
     }
     if (k > EXH_KC || h->dp > 256) {
         // more than stage 1 keeps per slice, or rows above 256 dims: the one-block-per-query

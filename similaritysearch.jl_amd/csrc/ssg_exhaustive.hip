@@ -531,6 +531,7 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
 {
     constexpr int PER = 32;  // entries per lane (nslices * kc <= 2048)
     __shared__ uint32_t s_sel[4][SEL];
+    __shared__ __attribute__((aligned(16))) uint32_t s_hist[4][256];
     const int lane = lane_id(), w = threadIdx.x >> 6;
     const int g = lane >> 4, j = lane & 15;
     const int64_t q = (int64_t)blockIdx.x * 4 + w;
@@ -562,25 +563,22 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
     for (int t = 0; t < PER; t++) nvalid += key[t] != 0xffffffffu ? 1 : 0;
     for (int o = 32; o >= 1; o >>= 1) nvalid += __shfl_xor(nvalid, o);
 
-    // radix select: T = SEL-th smallest key (or the largest valid one)
+    // radix select: T = SEL-th smallest key (or the largest valid one), as the SEL-th
+    // largest of the complemented keys (8-bit digits, LDS histograms)
     const int want0 = nvalid < SEL ? nvalid : SEL;
     uint32_t T = 0xffffffffu;
     if (want0 > 0) {
-        uint32_t prefix = 0;
-        int want = want0;
-        for (int bit = 31; bit >= 0; bit--) {
-            const uint32_t hi_mask = (bit == 31) ? 0u : (0xffffffffu << (bit + 1));
-            int c = 0;
+        uint32_t ik[PER];
+        bool act[PER];
 #pragma unroll
-            for (int t = 0; t < PER; t++)
-                c += (key[t] != 0xffffffffu && (key[t] & hi_mask) == prefix && !((key[t] >> bit) & 1u)) ? 1 : 0;
-            for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
-            if (c < want) {
-                want -= c;
-                prefix |= (1u << bit);
-            }
+        for (int t = 0; t < PER; t++) {
+            act[t] = key[t] != 0xffffffffu;
+            ik[t] = ~key[t];
         }
-        T = prefix;
+        uint32_t kand, diff;
+        key_bits<PER>(ik, act, &kand, &diff);
+        int above = 0;
+        T = ~kth_largest<PER>(ik, act, want0, 4, s_hist[w], lane, &above, kand, diff);
     }
     // gather the selection in index order: keys < T, then keys == T while room
     int nsel = 0;
@@ -613,11 +611,14 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
     float myd = __builtin_inff();
     double mys = 0.0;
     uint32_t myid = 0xffffffffu;
-    for (int r0 = 0; r0 < nsel; r0 += 16) {
-        float xv[4][V];
+    if constexpr (V <= 4) {
+        // short rows: every selected row's piece is loaded before any is used, so the
+        // wave waits for one round of loads instead of SEL / 16
+        static_assert(SEL == 64, "16 rows per 16-lane group");
+        float xv[16][V];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int u = r0 + 4 * r + g;
+        for (int r = 0; r < 16; r++) {
+            const int u = 4 * r + g;
             if (u < nsel) {
                 load_run<V>(X + (size_t)s_sel[w][u] * dp + j * V, xv[r]);
             } else {
@@ -626,16 +627,43 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
             }
         }
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const double s = group16_sum(run_partial<V>(metric, xv[r], qv));
-            const float d = finish_dist(metric, s);
-            // candidate u = r0 + 4r + g goes to lane u
+        for (int r = 0; r < 16; r++) {
+            const double sr = group16_sum(run_partial<V>(metric, xv[r], qv));
+            const float d = finish_dist(metric, sr);
+            // candidate u = 4r + g goes to lane u
             const float dt = __shfl(d, (lane & 3) * 16);
-            const double st = __shfl(s, (lane & 3) * 16);
-            if (lane >= r0 && lane < r0 + 16 && ((lane - r0) >> 2) == r && lane < nsel) {
+            const double st = __shfl(sr, (lane & 3) * 16);
+            if ((lane >> 2) == r && lane < nsel) {
                 myd = dt;
                 mys = st;
                 myid = s_sel[w][lane];
+            }
+        }
+    } else {
+        for (int r0 = 0; r0 < nsel; r0 += 16) {
+            float xv[4][V];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int u = r0 + 4 * r + g;
+                if (u < nsel) {
+                    load_run<V>(X + (size_t)s_sel[w][u] * dp + j * V, xv[r]);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < V; t++) xv[r][t] = 0.f;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const double s = group16_sum(run_partial<V>(metric, xv[r], qv));
+                const float d = finish_dist(metric, s);
+                // candidate u = r0 + 4r + g goes to lane u
+                const float dt = __shfl(d, (lane & 3) * 16);
+                const double st = __shfl(s, (lane & 3) * 16);
+                if (lane >= r0 && lane < r0 + 16 && ((lane - r0) >> 2) == r && lane < nsel) {
+                    myd = dt;
+                    mys = st;
+                    myid = s_sel[w][lane];
+                }
             }
         }
     }
@@ -674,61 +702,6 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
     }
 }
 
-// Seed of the starting thresholds: one wave per query selects the k-th smallest approximate
-// distance among a stage-1 pass's candidates (radix select on the order-preserving keys)
-// and turns it back into a score, lowered by a margin for the rounding of that conversion
-// (a lower threshold only lets more rows through).  Fewer than k candidates: -Inf.
-__global__ __launch_bounds__(256) void exh_seed_kernel(const float* __restrict__ cand_d,
-                                                       const uint32_t* __restrict__ cand_i,
-                                                       const float* __restrict__ qnorm, int nslices,
-                                                       int kc, int64_t m_valid, int64_t m_pad, int k,
-                                                       int metric, float* tau0)
-{
-    constexpr int PER = 32;  // entries per lane (nslices * kc <= 2048)
-    const int lane = lane_id(), w = threadIdx.x >> 6;
-    const int64_t q = (int64_t)blockIdx.x * 4 + w;
-    if (q >= m_pad) return;
-    if (q >= m_valid) {
-        if (lane == 0) tau0[q] = -__builtin_inff();
-        return;
-    }
-    const int total = nslices * kc;
-    uint32_t key[PER];
-    int nvalid = 0;
-#pragma unroll
-    for (int t = 0; t < PER; t++) {
-        const int e = lane + 64 * t;
-        key[t] = 0xffffffffu;
-        if (e < total && cand_i[q * total + e] != 0xffffffffu) key[t] = sort_key(cand_d[q * total + e]);
-        nvalid += key[t] != 0xffffffffu ? 1 : 0;
-    }
-    for (int o = 32; o >= 1; o >>= 1) nvalid += __shfl_xor(nvalid, o);
-    if (nvalid < k) {
-        if (lane == 0) tau0[q] = -__builtin_inff();
-        return;
-    }
-    uint32_t prefix = 0;
-    int want = k;
-    for (int bit = 31; bit >= 0; bit--) {
-        const uint32_t hi_mask = (bit == 31) ? 0u : (0xffffffffu << (bit + 1));
-        int c = 0;
-#pragma unroll
-        for (int t = 0; t < PER; t++)
-            c += (key[t] != 0xffffffffu && (key[t] & hi_mask) == prefix && !((key[t] >> bit) & 1u)) ? 1 : 0;
-        for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
-        if (c < want) {
-            want -= c;
-            prefix |= (1u << bit);
-        }
-    }
-    if (lane == 0) {
-        const float dk = key_float(prefix);
-        const float qn = qnorm[q];
-        const float sk = metric == M_NORMCOS ? 1.0f - dk : 0.5f * (qn - dk);
-        tau0[q] = __builtin_isfinite(sk) ? sk - (fabsf(sk) + qn + 1.0f) * 9.5367431640625e-07f : -__builtin_inff();
-    }
-}
-
 // Seed of the starting thresholds from a sample: a block takes 4 queries and scores S <=
 // 2048 rows spread over the database (row i * stride) against all four, each thread 8
 // rows, in FP32 with the score's form (x.q - |x|^2/2, or x.q); wave w then selects query
@@ -747,6 +720,7 @@ __global__ __launch_bounds__(256) void exh_seed_sample_kernel(const float* __res
 {
     __shared__ __attribute__((aligned(16))) float s_q[4][128];
     __shared__ uint32_t s_key[4][SEED_S];
+    __shared__ __attribute__((aligned(16))) uint32_t s_hist[4][256];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t q0 = (int64_t)blockIdx.x * 4;
     for (int e = tid; e < 4 * dp; e += 256) {
@@ -788,20 +762,14 @@ __global__ __launch_bounds__(256) void exh_seed_sample_kernel(const float* __res
     uint32_t key[PER];
 #pragma unroll
     for (int t = 0; t < PER; t++) key[t] = s_key[w][lane + 64 * t];
-    // k-th largest key: bit by bit from the top
-    uint32_t prefix = 0;
-    int want = k;
-    for (int bit = 31; bit >= 0; bit--) {
-        const uint32_t cand = prefix | (1u << bit);
-        const uint32_t hi_mask = 0xffffffffu << bit;
-        int c = 0;
+    // k-th largest key (8-bit digits, LDS histograms)
+    bool act[PER];
 #pragma unroll
-        for (int t = 0; t < PER; t++) c += (key[t] & hi_mask) == cand ? 1 : 0;
-        for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o);
-        // keys with the current prefix and this bit set are above the rest
-        if (c >= want) prefix = cand;
-        else want -= c;
-    }
+    for (int t = 0; t < PER; t++) act[t] = true;
+    uint32_t kand, diff;
+    key_bits<PER>(key, act, &kand, &diff);
+    int above = 0;
+    const uint32_t prefix = kth_largest<PER>(key, act, k, 4, s_hist[w], lane, &above, kand, diff);
     if (lane == 0) {
         const float sk = key_score(prefix);
         const float err = (float)(dp + 8) * 2.384185791015625e-07f * (xnmax + qnorm[q]);
@@ -818,15 +786,6 @@ hipError_t exh_launch_seed_sample(const float* X, const float* xnorm, int32_t dp
     const int64_t stride = n / S;
     hipLaunchKernelGGL(exh_seed_sample_kernel, dim3((unsigned)((m_pad + 3) / 4)), dim3(256), 0, s, X, xnorm,
                        dp, stride, S, w.qpad, w.qnorm, m_valid, m_pad, k, metric, xnmax, tau0);
-    return hipGetLastError();
-}
-
-hipError_t exh_launch_seed(const ExhWork& w, int nslices, int64_t m_valid, int64_t m_pad, int k,
-                           int metric, float* tau0, hipStream_t s)
-{
-    if (nslices * EXH_KC > 2048) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(exh_seed_kernel, dim3((unsigned)((m_pad + 3) / 4)), dim3(256), 0, s, w.cand_d,
-                       w.cand_i, w.qnorm, nslices, EXH_KC, m_valid, m_pad, k, metric, tau0);
     return hipGetLastError();
 }
 
@@ -901,26 +860,36 @@ __global__ __launch_bounds__(64) void exh_rescan_merge(const unsigned int* flags
 }
 
 // Pad rows to dp floats and compute FP32 squared norms.
-__global__ void pad_rows_kernel(const float* __restrict__ src, int64_t rows, int dim,
-                                float* __restrict__ dst, int dp, float* norms)
+// One thread per row, one wave per block (rows are few and far apart: more CUs issue
+// the scattered reads).  Rows rows .. rows_pad - 1 are zero.  The norm is summed in
+// element order.
+__global__ __launch_bounds__(64) void pad_rows_kernel(const float* __restrict__ src, int64_t rows,
+                                                      int64_t rows_pad, int dim, float* __restrict__ dst, int dp,
+                                                      float* norms)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
+    if (r >= rows_pad) return;
     float s = 0.f;
-    for (int i = 0; i < dp; i++) {
-        const float v = i < dim ? src[r * dim + i] : 0.f;
-        dst[r * dp + i] = v;
-        s = __builtin_fmaf(v, v, s);
+    if (r < rows) {
+#pragma unroll 8
+        for (int i = 0; i < dp; i++) {
+            const float v = i < dim ? src[r * dim + i] : 0.f;
+            dst[r * dp + i] = v;
+            s = __builtin_fmaf(v, v, s);
+        }
+    } else {
+        for (int i = 0; i < dp; i++) dst[r * dp + i] = 0.f;
     }
     if (norms) norms[r] = s;
 }
 
 hipError_t launch_pad_rows(const float* src, int64_t rows, int32_t dim, float* dst, int32_t dp,
-                           float* norms, hipStream_t s)
+                           float* norms, hipStream_t s, int64_t rows_pad)
 {
-    if (rows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, src,
-                       rows, dim, dst, dp, norms);
+    if (rows_pad < rows) rows_pad = rows;
+    if (rows_pad <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)((rows_pad + 63) / 64)), dim3(64), 0, s, src,
+                       rows, rows_pad, dim, dst, dp, norms);
     return hipGetLastError();
 }
 

@@ -106,13 +106,10 @@ struct ExhWork {
     const float* tau0;     // nullable, dp <= 128: per-query starting threshold (score units)
 };
 hipError_t exh_launch_stage1(const ExhArgs& a, const ExhWork& w, int nslices, hipStream_t s);
-// Starting thresholds from a stage-1 pass over a subset of the rows: tau0[q] = the score of
-// query q's k-th best candidate of that pass, lowered by a rounding margin (-Inf with fewer
-// than k candidates); rows m_valid .. m_pad get -Inf.
-hipError_t exh_launch_seed(const ExhWork& w, int nslices, int64_t m_valid, int64_t m_pad, int k,
-                           int metric, float* tau0, hipStream_t s);
-// The same from min(n, 2048) rows spread over the database, scored directly in FP32 and
-// lowered by the certificate's error allowance (xnmax: the largest squared row norm).
+// Starting thresholds: tau0[q] = query q's k-th best score among min(n, 2048) rows spread
+// over the database, scored directly in FP32 and lowered by the certificate's error
+// allowance (xnmax: the largest squared row norm); -Inf with fewer than k rows, and for
+// rows m_valid .. m_pad.
 hipError_t exh_launch_seed_sample(const float* X, const float* xnorm, int32_t dp, int64_t n,
                                   const ExhWork& w, int64_t m_valid, int64_t m_pad, int k, int metric,
                                   float xnmax, float* tau0, hipStream_t s);
@@ -240,7 +237,9 @@ hipError_t launch_nearest_center(const float* X, int32_t dp, int32_t metric, con
                                  hipStream_t s);
 hipError_t launch_gather_padded_rows(const float* X, const float* xn, int32_t dp, const uint32_t* ids,
                                      int64_t m, float* Y, float* yn, int64_t n0, hipStream_t s);
+// dst rows of dp values: rows 0 .. rows - 1 from src (dim values, zero padded) with their
+// squared norms, rows .. rows_pad - 1 zero.
 hipError_t launch_pad_rows(const float* src, int64_t rows, int32_t dim, float* dst, int32_t dp,
-                           float* norms, hipStream_t s);
+                           float* norms, hipStream_t s, int64_t rows_pad = 0);
 
 }  // namespace ssg
