@@ -541,6 +541,8 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
 
     uint32_t key[PER];
     uint32_t cid[PER];
+    // both loads of an entry are issued together (a distance load that waited for its id
+    // would serialise the wave's 32 round trips)
 #pragma unroll
     for (int t = 0; t < PER; t++) {
         const int e = lane + 64 * t;
@@ -548,8 +550,9 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
         cid[t] = 0xffffffffu;
         if (e < total) {
             const uint32_t id = cand_i[q * total + e];
+            const float d = cand_d[q * total + e];
             if (id != 0xffffffffu) {
-                key[t] = sort_key(cand_d[q * total + e]);
+                key[t] = sort_key(d);
                 cid[t] = id;
             }
         }

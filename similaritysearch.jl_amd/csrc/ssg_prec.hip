@@ -372,6 +372,9 @@ __global__ __launch_bounds__(256) void prec_stage2(PrecArgs a, PrecOut o)
     uint32_t key[PQ_JS], cid[PQ_JS];
     bool act[PQ_JS], keep[PQ_JS];
     int total = 0;
+    // an entry's count and list loads are issued together (a list load that waited for
+    // the count would serialise the wave's round trips); entries past a slice's count
+    // are read but not used
 #pragma unroll
     for (int j = 0; j < PQ_JS; j++) {
         const int e = lane + 64 * j;
@@ -379,11 +382,14 @@ __global__ __launch_bounds__(256) void prec_stage2(PrecArgs a, PrecOut o)
         act[j] = false;
         key[j] = 0u;
         cid[j] = 0u;
-        if (s < ns && pos < a.list_n[q * ns + s]) {
+        if (s < ns) {
+            const int cnt = a.list_n[q * ns + s];
             const uint2 v = a.list[(q * ns + s) * (int64_t)a.cap + pos];
-            act[j] = true;
-            key[j] = score_key(__uint_as_float(v.x));
-            cid[j] = v.y;
+            if (pos < cnt) {
+                act[j] = true;
+                key[j] = score_key(__uint_as_float(v.x));
+                cid[j] = v.y;
+            }
         }
         total += __popcll(__ballot(act[j]));
     }
