@@ -116,6 +116,9 @@ _MIN_RECALL = {"bf16": 0.99, "fp8": 0.95}
     ("SqL2", 64, 5000, 257, 10, 64, 0),
     ("SqL2", 128, 4000, 100, 10, 40, 5),
     ("SqL2", 20, 3000, 64, 1, 32, 2),
+    # lists of 1024 entries (ncand > 128), compacted several times per slice
+    ("NormCosine", 96, 6000, 300, 32, 200, 0),
+    ("SqL2", 128, 4000, 100, 10, 160, 2),
 ])
 def test_candidates_and_rerank(oracle, metric, dim, n, m, k, ncand, slices, fmt):
     ss = _ss()
