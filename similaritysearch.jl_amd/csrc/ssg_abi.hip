@@ -563,34 +563,64 @@ static int ssg_prec_run(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
     HIPCHK(hipMemcpyAsync(&nf, flags, sizeof(unsigned int), hipMemcpyDeviceToHost, s), "read flags");
     HIPCHK(hipStreamSynchronize(s), "sync");
     if (nf == 0) return SSG_OK;
-    // redo of the flagged queries, gathered, over enough slices to fill the device
-    const int64_t mf_pad = ((int64_t)nf + PREC_QB - 1) / PREC_QB * PREC_QB;
-    int32_t nsf = (int32_t)std::max<int64_t>(
-        1, ((int64_t)h->num_cu * (8 / PREC_W) + mf_pad / PREC_QB - 1) / (mf_pad / PREC_QB));
-    nsf = std::max(1, std::min<int32_t>(nsf, 2048 / kc));
-    int64_t rpsf = (n + nsf - 1) / nsf;
-    rpsf = (rpsf + 127) / 128 * 128;
-    nsf = (int32_t)((n + rpsf - 1) / rpsf);
-    HIPCHK(h->pQf.ensure((size_t)mf_pad * rowb), "alloc redo queries");
-    HIPCHK(h->pList.grow((size_t)mf_pad * nsf * cap * sizeof(uint2), 0, s), "alloc candidate lists");
-    HIPCHK(h->pListN.grow((size_t)mf_pad * nsf * sizeof(int32_t), 0, s), "alloc candidate counts");
-    HIPCHK(launch_gather_rows_u16(reinterpret_cast<const uint16_t*>(a.Qe), flags + 1, flags, mf_pad, rowb / 2,
-                                  h->pQf.as<uint16_t>(), s), "gather");
-    PrecArgs af = a;
-    af.Qe = h->pQf.as<uint8_t>();
-    af.m_pad = mf_pad;
-    af.rows_per_slice = rpsf;
-    af.nslices = nsf;
-    af.list = h->pList.as<uint2>();
-    af.list_n = h->pListN.as<int32_t>();
-    af.tau0 = nullptr;
-    PrecOut of = o;
-    of.m = nf;
-    of.qmap = flags + 1;
-    of.flags = nullptr;
-    HIPCHK(prec_launch_stage1(af, s), "precision candidate pass (redo)");
-    HIPCHK(prec_launch_stage2(af, of, h->dp, s), "precision rerank (redo)");
-    return SSG_OK;
+    // Redo of the queries in idx[0 .. cnt) (count: the same number on the device), gathered,
+    // over enough slices to fill the device.  tau_q (nullable): one starting threshold per
+    // gathered query for all of its slices; then the redo certifies its queries into
+    // fl_out as the main pass does.
+    auto redo = [&](const uint32_t* idx, const unsigned int* count, unsigned int cnt, const float* tau_q,
+                    unsigned int* fl_out) -> int {
+        const int64_t mf_pad = ((int64_t)cnt + PREC_QB - 1) / PREC_QB * PREC_QB;
+        int32_t nsf = (int32_t)std::max<int64_t>(
+            1, ((int64_t)h->num_cu * (8 / PREC_W) + mf_pad / PREC_QB - 1) / (mf_pad / PREC_QB));
+        nsf = std::max(1, std::min<int32_t>(nsf, 2048 / kc));
+        int64_t rpsf = (n + nsf - 1) / nsf;
+        rpsf = (rpsf + 127) / 128 * 128;
+        nsf = (int32_t)((n + rpsf - 1) / rpsf);
+        HIPCHK(h->pQf.ensure((size_t)mf_pad * rowb), "alloc redo queries");
+        HIPCHK(h->pList.grow((size_t)mf_pad * nsf * cap * sizeof(uint2), 0, s), "alloc candidate lists");
+        HIPCHK(h->pListN.grow((size_t)mf_pad * nsf * sizeof(int32_t), 0, s), "alloc candidate counts");
+        HIPCHK(launch_gather_rows_u16(reinterpret_cast<const uint16_t*>(a.Qe), idx, count, mf_pad, rowb / 2,
+                                      h->pQf.as<uint16_t>(), s), "gather");
+        PrecArgs af = a;
+        af.Qe = h->pQf.as<uint8_t>();
+        af.m_pad = mf_pad;
+        af.rows_per_slice = rpsf;
+        af.nslices = nsf;
+        af.list = h->pList.as<uint2>();
+        af.list_n = h->pListN.as<int32_t>();
+        af.tau0 = nullptr;
+        if (tau_q) {
+            HIPCHK(h->pTauF.ensure((size_t)mf_pad * nsf * sizeof(float)), "alloc redo thresholds");
+            HIPCHK(launch_spread_f32(tau_q, cnt, mf_pad, nsf, h->pTauF.as<float>(), s), "redo thresholds");
+            af.tau0 = h->pTauF.as<float>();
+        }
+        PrecOut of = o;
+        of.m = cnt;
+        of.qmap = idx;
+        of.flags = fl_out;
+        HIPCHK(prec_launch_stage1(af, s), "precision candidate pass (redo)");
+        HIPCHK(prec_launch_stage2(af, of, h->dp, s), "precision rerank (redo)");
+        return SSG_OK;
+    };
+    if (!seed) return redo(flags + 1, flags, nf, nullptr, nullptr);
+    // With seed thresholds (about rank 16 * 16 * T2 of the slice, far below the 128th best
+    // of all but a few queries in a million), the redo of each query starts at the largest
+    // of its slices' seeds instead of nothing: a fraction of the appends.  The queries that
+    // redo cannot certify are searched once more from no threshold.
+    HIPCHK(h->pFlags2.ensure((size_t)(nf + 1) * sizeof(unsigned int)), "alloc redo flags");
+    HIPCHK(h->pTauQ.ensure((size_t)nf * sizeof(float)), "alloc redo thresholds");
+    unsigned int* flags2 = h->pFlags2.as<unsigned int>();
+    HIPCHK(launch_max_slices_f32(h->pTauSeed.as<float>(), ns, flags + 1, nf, h->pTauQ.as<float>(), s), "seeds");
+    HIPCHK(launch_zero_u32(flags2, 1, s), "zero flags");
+    if (int rc = redo(flags + 1, flags, nf, h->pTauQ.as<float>(), flags2)) return rc;
+    unsigned int nf2 = 0;
+    HIPCHK(hipMemcpyAsync(&nf2, flags2, sizeof(unsigned int), hipMemcpyDeviceToHost, s), "read flags");
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    if (nf2 == 0) return SSG_OK;
+    // flags2 holds positions among the redone queries: map them to query rows
+    HIPCHK(h->pIdx3.ensure((size_t)nf2 * sizeof(uint32_t)), "alloc last queries");
+    HIPCHK(launch_compose_u32(flags + 1, flags2 + 1, nf2, h->pIdx3.as<uint32_t>(), s), "compose");
+    return redo(h->pIdx3.as<uint32_t>(), flags2, nf2, nullptr, nullptr);
 }
 
 // Frozen lists -> node table + packed pool on the device.

@@ -123,6 +123,7 @@ struct ssg_handle {
     // queries, per-(query, slice) candidate lists and their lengths; prec_slices > 0 forces
     // the number of DB slices
     DevBuf Xe, pQe, pList, pListN, pTau0, pFlags, pQf;
+    DevBuf pFlags2, pTauQ, pTauF, pIdx3;  // the seeded redo: its flags, thresholds, last queries
     DevBuf Xs, xns;  // rows of every 16th tile of Xe, packed, and their norms (sample pass)
     DevBuf Xs2, xns2, pTauSeed;  // every 16th tile of Xs (seed pass) and its thresholds
     DevBuf xAmax;    // FP8 max |value| bits: [0] and [1] the rows, [2] the last queries

@@ -182,6 +182,13 @@ inline int64_t prec_sample_rows(int64_t n, int32_t step)
 }
 hipError_t launch_sample_rows(const uint16_t* Xe, const float* xnorm, int64_t ns_rows, int32_t step,
                               int32_t width, uint16_t* Xs, float* xns, hipStream_t s);
+// Redo of uncertified queries: out[i] = max over the ns slices of src[idx[i] * ns + s], the
+// next float below it;
+// out[i * nsf + s] = v[i] (i < cnt) or -Inf (cnt <= i < rows); out[i] = a[b[i]].
+hipError_t launch_max_slices_f32(const float* src, int32_t ns, const uint32_t* idx, int64_t cnt, float* out,
+                                 hipStream_t s);
+hipError_t launch_spread_f32(const float* v, int64_t cnt, int64_t rows, int32_t nsf, float* out, hipStream_t s);
+hipError_t launch_compose_u32(const uint32_t* a, const uint32_t* b, int64_t cnt, uint32_t* out, hipStream_t s);
 hipError_t launch_gather_rows_u16(const uint16_t* src, const uint32_t* idx, const unsigned int* count,
                                   int64_t max_rows, int32_t width, uint16_t* dst, hipStream_t s);
 int32_t prec_slices(int64_t n, int64_t m, int num_cu, int kc);

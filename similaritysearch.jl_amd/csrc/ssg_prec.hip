@@ -550,6 +550,65 @@ hipError_t launch_gather_rows_u16(const uint16_t* src, const uint32_t* idx, cons
     return hipGetLastError();
 }
 
+// Redo helpers: out[i] = the largest of src[idx[i] * ns .. + ns), one float step lower
+// (i < cnt); out[i * nsf + s]
+// = v[i] for i < cnt, -Inf for cnt <= i < rows; out[i] = a[b[i]] (i < cnt).
+__global__ void max_slices_kernel(const float* __restrict__ src, int32_t ns, const uint32_t* __restrict__ idx,
+                                  int64_t cnt, float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    float t = -__builtin_inff();
+    for (int s = 0; s < ns; s++) t = fmaxf(t, src[(int64_t)idx[i] * ns + s]);
+    // one step down, so that rows scoring exactly the seed (ties, frequent among FP8
+    // scores) still pass
+    if (t > -__builtin_inff()) {
+        const uint32_t u = __float_as_uint(t);
+        t = t > 0.f ? __uint_as_float(u - 1u) : t < 0.f ? __uint_as_float(u + 1u) : -__uint_as_float(1u);
+    }
+    out[i] = t;
+}
+
+__global__ void spread_kernel(const float* __restrict__ v, int64_t cnt, int64_t rows, int32_t nsf,
+                              float* __restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * nsf) return;
+    const int64_t i = e / nsf;
+    out[e] = i < cnt ? v[i] : -__builtin_inff();
+}
+
+__global__ void compose_kernel(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, int64_t cnt,
+                               uint32_t* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) out[i] = a[b[i]];
+}
+
+hipError_t launch_max_slices_f32(const float* src, int32_t ns, const uint32_t* idx, int64_t cnt, float* out,
+                                 hipStream_t s)
+{
+    if (cnt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(max_slices_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, src, ns, idx, cnt,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_spread_f32(const float* v, int64_t cnt, int64_t rows, int32_t nsf, float* out, hipStream_t s)
+{
+    const int64_t tot = rows * nsf;
+    if (tot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(spread_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, v, cnt, rows, nsf, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_compose_u32(const uint32_t* a, const uint32_t* b, int64_t cnt, uint32_t* out, hipStream_t s)
+{
+    if (cnt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(compose_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, a, b, cnt, out);
+    return hipGetLastError();
+}
+
 // The rows of every step-th 128-row tile (tiles 0, step, 2 step, ...) of the encoded rows
 // (width 16-bit words each) and their norms, packed: sample row i is row
 // (i / 128) * 128 * step + i % 128.  Rows at or past n are not copied (the caller passes

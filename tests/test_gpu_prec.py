@@ -272,6 +272,39 @@ def test_sampled_thresholds_do_not_change_results(oracle, slices, fmt):
 
 
 @pytest.mark.parametrize("fmt", ["bf16", "fp8"])
+def test_seeded_redo_and_last_resort(oracle, fmt):
+    """Slices of at least 256 Ki rows get seed thresholds.  With T = 2 the main pass cannot
+    certify most queries; they are redone from the largest of their slices' seeds (one
+    float step lower), and a query the redo cannot certify either is searched once more
+    from no threshold: 100 copies of it among the rows, 8 of them in rows 0-7 (the first
+    tile, in the seed sample), so its seed is its own score and the redo keeps only the
+    100 copies of the 128 candidates it needs.  The results equal the search without
+    sampling and hold the pinned properties of the candidate pass."""
+    ss = _ss()
+    from similaritysearch import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(21)
+    n, dim = 270000, 16
+    X = rng.random((n, dim), dtype=np.float32)
+    v = rng.random(dim, dtype=np.float32)
+    pos = np.sort(np.concatenate([np.arange(8), 8 + rng.choice(n - 8, 92, replace=False)]))
+    X[pos] = v
+    Q = np.concatenate([v[None, :], rng.random((299, dim), dtype=np.float32)]).astype(np.float32)
+    E = ss.ExhaustiveSearch(ss.SqL2, ss.MatrixDatabase.from_rows(X))
+    h = E._handle()
+    _lib.check(L.ssg_set_option(h, _lib.SSG_OPT_PREC_SLICES, 1), "option")
+    outs = []
+    for t in (-1, 2):
+        _lib.check(L.ssg_set_option(h, _lib.SSG_OPT_PREC_SAMPLE, t), "option")
+        outs.append(ss.exhaustive_candidates(E, 10, 128, ss.MatrixDatabase.from_rows(Q), fmt))
+    for a, b in zip(outs[0], outs[1]):
+        np.testing.assert_array_equal(a, b)
+    ids, d, ci, cd = (o.T for o in outs[1])
+    np.testing.assert_array_equal(ids[0], pos[:10] + 1)
+    _check(oracle, "SqL2", X, Q[:6], 10, 128, ids[:6], d[:6], ci[:6], cd[:6], fmt, Q)
+
+
+@pytest.mark.parametrize("fmt", ["bf16", "fp8"])
 def test_ties_prefer_lower_ids(oracle, fmt):
     """300 copies of one vector among random rows: all have the same approximate and
     exact distance to it, so the candidate pass must keep the lowest ids (a compaction
