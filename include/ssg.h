@@ -299,9 +299,9 @@ int ssg_optimize_launch(ssg_handle* h, const float* dQ, int64_t m, int32_t k, in
  * default, chooses it from the device size); results do not depend on it.
  * SSG_OPT_PREC_SAMPLE: the precision form's sampled thresholds (a pass over every 16th
  * 128-row tile sets each query's starting threshold at its T-th best sampled score; queries
- * the result cannot certify are searched again without it).  0 (default): used on slices of
- * at least 131072 rows with T = max(16, 3 ncand / 16); -1: never; T in 1..256: always,
- * with that T.  Results do not depend on it. */
+ * the result cannot certify are searched again from a lower threshold, or from none).
+ * 0 (default): used on slices of at least 131072 rows with T = max(16, ncand / 8); -1:
+ * never; T in 1..256: always, with that T.  Results do not depend on it. */
 enum { SSG_OPT_QUERY_LAYOUT = 1, SSG_OPT_PREC_SLICES = 2, SSG_OPT_PREC_SAMPLE = 3 };
 int ssg_set_option(ssg_handle* h, int32_t option, int64_t value);
 
