@@ -541,21 +541,25 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
 
     uint32_t key[PER];
     uint32_t cid[PER];
-    // both loads of an entry are issued together (a distance load that waited for its id
-    // would serialise the wave's 32 round trips)
+    // Buffer loads over the query's entries (reads past them return 0), all issued before
+    // any is used: plain loads were sunk into the branch of their use, which serialised
+    // the wave's 64 round trips.
+    const __amdgpu_buffer_rsrc_t ri = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(cand_i + q * total), 0, total * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(cand_d + q * total), 0, total * 4, 0x00020000);
 #pragma unroll
     for (int t = 0; t < PER; t++) {
         const int e = lane + 64 * t;
-        key[t] = 0xffffffffu;
-        cid[t] = 0xffffffffu;
-        if (e < total) {
-            const uint32_t id = cand_i[q * total + e];
-            const float d = cand_d[q * total + e];
-            if (id != 0xffffffffu) {
-                key[t] = sort_key(d);
-                cid[t] = id;
-            }
-        }
+        cid[t] = __builtin_amdgcn_raw_buffer_load_b32(ri, 4 * e, 0, 0);
+        key[t] = __builtin_amdgcn_raw_buffer_load_b32(rd, 4 * e, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const int e = lane + 64 * t;
+        const bool ok = e < total && cid[t] != 0xffffffffu;
+        key[t] = ok ? sort_key(__uint_as_float(key[t])) : 0xffffffffu;
+        cid[t] = ok ? cid[t] : 0xffffffffu;
     }
     float tmin = __builtin_inff();
     for (int s = lane; s < nslices; s += 64) tmin = fminf(tmin, slice_tau[q * nslices + s]);
@@ -713,8 +717,9 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
 // each of them, and so each of the k nearest, passes the main pass's threshold.  Fewer
 // than k rows: -Inf.
 constexpr int SEED_S = 2048;
+template <int DP>
 __global__ __launch_bounds__(256) void exh_seed_sample_kernel(const float* __restrict__ X,
-                                                              const float* __restrict__ xnorm, int dp,
+                                                              const float* __restrict__ xnorm,
                                                               int64_t stride, int S,
                                                               const float* __restrict__ qpad,
                                                               const float* __restrict__ qnorm,
@@ -726,21 +731,35 @@ __global__ __launch_bounds__(256) void exh_seed_sample_kernel(const float* __res
     __shared__ __attribute__((aligned(16))) uint32_t s_hist[4][256];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t q0 = (int64_t)blockIdx.x * 4;
-    for (int e = tid; e < 4 * dp; e += 256) {
-        const int64_t q = min(q0 + e / dp, m_pad - 1);
-        s_q[e / dp][e % dp] = qpad[q * dp + e % dp];
+    constexpr int U = DP / 4;                             // float4 pieces per row
+    constexpr int RU = DP <= 32 ? 4 : DP <= 64 ? 2 : 1;  // rows whose loads are in flight together
+    static_assert((SEED_S / 256) % RU == 0, "rows per thread");
+    for (int e = tid; e < 4 * DP; e += 256) {
+        const int64_t q = min(q0 + e / DP, m_pad - 1);
+        s_q[e / DP][e % DP] = qpad[q * DP + e % DP];
     }
     __syncthreads();
-    for (int r = tid; r < SEED_S; r += 256) {
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-        if (r < S) {
-            const int64_t row = (int64_t)r * stride;
-            const float4* xp = reinterpret_cast<const float4*>(X + row * dp);
-            const float h = metric == M_NORMCOS ? 0.f : -0.5f * xnorm[row];
+    for (int r0 = tid; r0 < SEED_S; r0 += 256 * RU) {
+        // every row's loads first (rows past S clamped to the last sampled row)
+        float4 xa[RU][U];
+        float h[RU];
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc[t] = h;
-            for (int u = 0; u < dp / 4; u++) {
-                const float4 a = xp[u];
+        for (int i = 0; i < RU; i++) {
+            const int64_t row = (int64_t)min(r0 + 256 * i, S - 1) * stride;
+            const float4* xp = reinterpret_cast<const float4*>(X + row * DP);
+#pragma unroll
+            for (int u = 0; u < U; u++) xa[i][u] = xp[u];
+            h[i] = metric == M_NORMCOS ? 0.f : -0.5f * xnorm[row];
+        }
+#pragma unroll
+        for (int i = 0; i < RU; i++) {
+            const int r = r0 + 256 * i;
+            float acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] = h[i];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const float4 a = xa[i][u];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
                     const float4 b = reinterpret_cast<const float4*>(s_q[t])[u];
@@ -750,9 +769,9 @@ __global__ __launch_bounds__(256) void exh_seed_sample_kernel(const float* __res
                     acc[t] = __builtin_fmaf(a.w, b.w, acc[t]);
                 }
             }
-        }
 #pragma unroll
-        for (int t = 0; t < 4; t++) s_key[t][r] = r < S ? score_key(acc[t]) : 0u;  // 0: below every score
+            for (int t = 0; t < 4; t++) s_key[t][r] = r < S ? score_key(acc[t]) : 0u;  // 0: below every score
+        }
     }
     __syncthreads();
     const int64_t q = q0 + w;
@@ -775,7 +794,7 @@ __global__ __launch_bounds__(256) void exh_seed_sample_kernel(const float* __res
     const uint32_t prefix = kth_largest<PER>(key, act, k, 4, s_hist[w], lane, &above, kand, diff);
     if (lane == 0) {
         const float sk = key_score(prefix);
-        const float err = (float)(dp + 8) * 2.384185791015625e-07f * (xnmax + qnorm[q]);
+        const float err = (float)(DP + 8) * 2.384185791015625e-07f * (xnmax + qnorm[q]);
         tau0[q] = __builtin_isfinite(sk) ? sk - err : -__builtin_inff();
     }
 }
@@ -787,8 +806,15 @@ hipError_t exh_launch_seed_sample(const float* X, const float* xnorm, int32_t dp
     if (dp > 128) return hipErrorInvalidValue;
     const int S = (int)std::min<int64_t>(n, SEED_S);
     const int64_t stride = n / S;
-    hipLaunchKernelGGL(exh_seed_sample_kernel, dim3((unsigned)((m_pad + 3) / 4)), dim3(256), 0, s, X, xnorm,
-                       dp, stride, S, w.qpad, w.qnorm, m_valid, m_pad, k, metric, xnmax, tau0);
+    const dim3 grid((unsigned)((m_pad + 3) / 4));
+    switch (dp) {
+#define SSG_SS(D) \
+    case D: hipLaunchKernelGGL((exh_seed_sample_kernel<D>), grid, dim3(256), 0, s, X, xnorm, stride, S, w.qpad, \
+                               w.qnorm, m_valid, m_pad, k, metric, xnmax, tau0); break;
+        SSG_SS(16) SSG_SS(32) SSG_SS(48) SSG_SS(64) SSG_SS(80) SSG_SS(96) SSG_SS(112) SSG_SS(128)
+#undef SSG_SS
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -874,7 +900,7 @@ __global__ __launch_bounds__(64) void pad_rows_kernel(const float* __restrict__ 
     if (r >= rows_pad) return;
     float s = 0.f;
     if (r < rows) {
-#pragma unroll 8
+#pragma unroll 32
         for (int i = 0; i < dp; i++) {
             const float v = i < dim ? src[r * dim + i] : 0.f;
             dst[r * dp + i] = v;

@@ -372,25 +372,19 @@ __global__ __launch_bounds__(256) void prec_stage2(PrecArgs a, PrecOut o)
     uint32_t key[PQ_JS], cid[PQ_JS];
     bool act[PQ_JS], keep[PQ_JS];
     int total = 0;
-    // an entry's count and list loads are issued together (a list load that waited for
-    // the count would serialise the wave's round trips); entries past a slice's count
-    // are read but not used
+    // Every load is unconditional (clamped slice) and only selected afterwards: a load
+    // inside a branch, or one that waited for its slice's count, would serialise the
+    // wave's round trips.  Entries past a slice's count are read but not used.
 #pragma unroll
     for (int j = 0; j < PQ_JS; j++) {
         const int e = lane + 64 * j;
         const int s = e / kc, pos = e - s * kc;
-        act[j] = false;
-        key[j] = 0u;
-        cid[j] = 0u;
-        if (s < ns) {
-            const int cnt = a.list_n[q * ns + s];
-            const uint2 v = a.list[(q * ns + s) * (int64_t)a.cap + pos];
-            if (pos < cnt) {
-                act[j] = true;
-                key[j] = score_key(__uint_as_float(v.x));
-                cid[j] = v.y;
-            }
-        }
+        const int sc = s < ns ? s : 0;
+        const int cnt = a.list_n[q * ns + sc];
+        const uint2 v = a.list[(q * ns + sc) * (int64_t)a.cap + pos];
+        act[j] = s < ns && pos < cnt;
+        key[j] = act[j] ? score_key(__uint_as_float(v.x)) : 0u;
+        cid[j] = act[j] ? v.y : 0u;
         total += __popcll(__ballot(act[j]));
     }
     const int nsel = total < o.ncand ? total : o.ncand;
