@@ -889,27 +889,34 @@ __global__ __launch_bounds__(64) void exh_rescan_merge(const unsigned int* flags
 }
 
 // Pad rows to dp floats and compute FP32 squared norms.
-// One thread per row, one wave per block (rows are few and far apart: more CUs issue
-// the scattered reads).  Rows rows .. rows_pad - 1 are zero.  The norm is summed in
-// element order.
-__global__ __launch_bounds__(64) void pad_rows_kernel(const float* __restrict__ src, int64_t rows,
-                                                      int64_t rows_pad, int dim, float* __restrict__ dst, int dp,
-                                                      float* norms)
+// 64 rows per block of 256 threads, in chunks of 128 values: the block reads and writes
+// each chunk coalesced (rows past `rows` and values past `dim` are zero) and stages it in
+// LDS, where thread t < 64 continues row t's squared norm, summed in element order.
+__global__ __launch_bounds__(256) void pad_rows_kernel(const float* __restrict__ src, int64_t rows,
+                                                       int64_t rows_pad, int dim, float* __restrict__ dst, int dp,
+                                                       float* norms)
 {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows_pad) return;
+    constexpr int RB = 64, CH = 128;
+    __shared__ float s_v[RB][CH + 1];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * RB;
     float s = 0.f;
-    if (r < rows) {
-#pragma unroll 32
-        for (int i = 0; i < dp; i++) {
-            const float v = i < dim ? src[r * dim + i] : 0.f;
-            dst[r * dp + i] = v;
-            s = __builtin_fmaf(v, v, s);
+    for (int c0 = 0; c0 < dp; c0 += CH) {
+        const int cw = min(CH, dp - c0);
+        for (int e = tid; e < RB * cw; e += 256) {
+            const int rl = e / cw, i = c0 + e % cw;
+            const int64_t r = r0 + rl;
+            float v = 0.f;
+            if (r < rows && i < dim) v = src[r * dim + i];
+            if (r < rows_pad) dst[r * dp + i] = v;
+            s_v[rl][i - c0] = v;
         }
-    } else {
-        for (int i = 0; i < dp; i++) dst[r * dp + i] = 0.f;
+        __syncthreads();
+        if (tid < RB)
+            for (int i = 0; i < cw; i++) s = __builtin_fmaf(s_v[tid][i], s_v[tid][i], s);
+        __syncthreads();
     }
-    if (norms) norms[r] = s;
+    if (norms && tid < RB && r0 + tid < rows_pad) norms[r0 + tid] = s;
 }
 
 hipError_t launch_pad_rows(const float* src, int64_t rows, int32_t dim, float* dst, int32_t dp,
@@ -917,8 +924,8 @@ hipError_t launch_pad_rows(const float* src, int64_t rows, int32_t dim, float* d
 {
     if (rows_pad < rows) rows_pad = rows;
     if (rows_pad <= 0) return hipSuccess;
-    hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)((rows_pad + 63) / 64)), dim3(64), 0, s, src,
-                       rows, rows_pad, dim, dst, dp, norms);
+    hipLaunchKernelGGL(pad_rows_kernel, dim3((unsigned)((rows_pad + 63) / 64)), dim3(256), 0, s, src, rows,
+                       rows_pad, dim, dst, dp, norms);
     return hipGetLastError();
 }
 
