@@ -336,7 +336,6 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
     a.qdim = h->dim;
     a.m = m;
     a.k = k;
-    HIPCHK(launch_zero_u32(w.flags, 1, s), "zero flags");
     ExhArgs a1 = a;
     a1.m = m_pad;
     // Short slices spend most of stage 1 filling and compacting candidate lists from a

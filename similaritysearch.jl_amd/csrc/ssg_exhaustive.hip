@@ -84,11 +84,13 @@ __global__ __launch_bounds__(256) void exh_stage1_lds(const float* __restrict__ 
                                                   int metric, const float* __restrict__ qpad,
                                                   const float* __restrict__ qnorm, int64_t rows_per_slice,
                                                   int nslices, float* cand_d, uint32_t* cand_i,
-                                                  float* slice_tau)
+                                                  float* slice_tau, unsigned int* fail_count)
 {
     constexpr int LS = DP + 4;  // LDS row stride (floats)
     constexpr int KS = DP / 4;  // MFMA k-steps
     constexpr int QT = TR;  // 64 queries per block, staged through the row tile
+    // the merge that follows on the stream counts its failed certificates from zero
+    if (fail_count && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *fail_count = 0u;
     __shared__ float s_x[TR * LS];
     __shared__ float s_xn[TR];
     __shared__ float s_cd[QT][CAP];
@@ -265,8 +267,10 @@ __global__ __launch_bounds__(256, 2) void exh_stage1_l(const float* __restrict__
                                                        const float* __restrict__ qnorm, int64_t m,
                                                        int64_t rows_per_slice, int nslices, uint2* list,
                                                        float* cand_d, uint32_t* cand_i, float* slice_tau,
-                                                       const float* __restrict__ tau0)
+                                                       const float* __restrict__ tau0, unsigned int* fail_count)
 {
+    // the merge that follows on the stream counts its failed certificates from zero
+    if (fail_count && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *fail_count = 0u;
     constexpr int KS = DP / 4;            // MFMA k-steps
     constexpr int LS = DP + 4;            // LDS row stride in floats (staggers banks)
     constexpr int NCT = 2;                // 16-query column tiles per wave
@@ -1081,13 +1085,14 @@ static void stage1_dp(dim3 grid, hipStream_t s, const ExhArgs& a, const ExhWork&
 {
     if constexpr (DP <= 128) {
         hipLaunchKernelGGL((exh_stage1_l<DP>), grid, dim3(256), 0, s, a.X, a.xnorm, a.n, a.metric, w.qpad,
-                           w.qnorm, w.m_valid, rps, ns, w.list, w.cand_d, w.cand_i, w.slice_tau, w.tau0);
+                           w.qnorm, w.m_valid, rps, ns, w.list, w.cand_d, w.cand_i, w.slice_tau, w.tau0,
+                           w.flags);
     } else {
         // 64-query blocks; query rows up to roundup(m_valid, 64) (the padded rows of X when
         // the queries are the database itself)
         const dim3 g64((unsigned)((w.m_valid + TR - 1) / TR), grid.y);
         hipLaunchKernelGGL((exh_stage1_lds<DP, EXH_KC>), g64, dim3(256), 0, s, a.X, a.xnorm, a.n, a.metric,
-                           w.qpad, w.qnorm, rps, ns, w.cand_d, w.cand_i, w.slice_tau);
+                           w.qpad, w.qnorm, rps, ns, w.cand_d, w.cand_i, w.slice_tau, w.flags);
     }
 }
 
