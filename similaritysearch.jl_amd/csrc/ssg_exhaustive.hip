@@ -715,11 +715,14 @@ __global__ __launch_bounds__(256) void exh_stage2(const float* __restrict__ X, i
 
 // Seed of the starting thresholds from a sample: a block takes 4 queries and scores S <=
 // 2048 rows spread over the database (row i * stride) against all four, each thread 8
-// rows, in FP32 with the score's form (x.q - |x|^2/2, or x.q); wave w then selects query
-// w's k-th largest score (radix select over LDS) and lowers it by the certificate's error
-// allowance, so that it lies below the matrix-core score of every one of those k rows:
-// each of them, and so each of the k nearest, passes the main pass's threshold.  Fewer
-// than k rows: -Inf.
+// rows, in FP32 with the score's form (x.q - |x|^2/2, or x.q).  Lane j of wave w holds
+// query w's scores of sampled rows j, j + 64, ...; the wave takes the k-th largest of the
+// 128 values formed by each lane's two largest.  At least k sampled scores reach that
+// value, so it is at most the k-th largest sampled score (equal to it unless three of the
+// k largest share a lane), and lowered by the certificate's error allowance it lies below
+// the matrix-core score of each of the k best sampled rows: each of them, and so each of
+// the k nearest rows, passes the main pass's threshold.  Fewer than k rows, or k outside
+// 1 .. 128: -Inf.
 constexpr int SEED_S = 2048;
 template <int DP>
 __global__ __launch_bounds__(256) void exh_seed_sample_kernel(const float* __restrict__ X,
@@ -732,7 +735,6 @@ __global__ __launch_bounds__(256) void exh_seed_sample_kernel(const float* __res
 {
     __shared__ __attribute__((aligned(16))) float s_q[4][128];
     __shared__ uint32_t s_key[4][SEED_S];
-    __shared__ __attribute__((aligned(16))) uint32_t s_hist[4][256];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int64_t q0 = (int64_t)blockIdx.x * 4;
     constexpr int U = DP / 4;                             // float4 pieces per row
@@ -780,22 +782,30 @@ __global__ __launch_bounds__(256) void exh_seed_sample_kernel(const float* __res
     __syncthreads();
     const int64_t q = q0 + w;
     if (q >= m_pad) return;
-    if (q >= m_valid || S < k) {
+    if (q >= m_valid || S < k || k < 1 || k > 128) {
         if (lane == 0) tau0[q] = -__builtin_inff();
         return;
     }
     constexpr int PER = SEED_S / 64;
-    uint32_t key[PER];
+    uint32_t m1 = 0u, m2 = 0u;  // this lane's two largest keys, m1 >= m2
 #pragma unroll
-    for (int t = 0; t < PER; t++) key[t] = s_key[w][lane + 64 * t];
-    // k-th largest key (8-bit digits, LDS histograms)
-    bool act[PER];
+    for (int t = 0; t < PER; t++) {
+        const uint32_t x = s_key[w][lane + 64 * t];
+        m2 = max(m2, min(m1, x));
+        m1 = max(m1, x);
+    }
+    // ranks in the wave's 128 values, ordered by value (larger first), then lane, then
+    // the lane's first before its second: a permutation of 0 .. 127
+    int r1 = 0, r2 = 0;
 #pragma unroll
-    for (int t = 0; t < PER; t++) act[t] = true;
-    uint32_t kand, diff;
-    key_bits<PER>(key, act, &kand, &diff);
-    int above = 0;
-    const uint32_t prefix = kth_largest<PER>(key, act, k, 4, s_hist[w], lane, &above, kand, diff);
+    for (int j = 0; j < 64; j++) {
+        const uint32_t o1 = rl_u(m1, j), o2 = rl_u(m2, j);
+        r1 += ((o1 > m1 || (o1 == m1 && j < lane)) ? 1 : 0) + ((o2 > m1 || (o2 == m1 && j < lane)) ? 1 : 0);
+        r2 += ((o1 > m2 || (o1 == m2 && j <= lane)) ? 1 : 0) + ((o2 > m2 || (o2 == m2 && j < lane)) ? 1 : 0);
+    }
+    const uint64_t b1 = __ballot(r1 == k - 1), b2 = __ballot(r2 == k - 1);
+    const uint32_t prefix = b1 ? rl_u(m1, __ffsll((unsigned long long)b1) - 1)
+                               : rl_u(m2, __ffsll((unsigned long long)b2) - 1);
     if (lane == 0) {
         const float sk = key_score(prefix);
         const float err = (float)(DP + 8) * 2.384185791015625e-07f * (xnmax + qnorm[q]);
