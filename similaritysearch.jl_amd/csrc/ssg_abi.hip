@@ -339,9 +339,9 @@ int ssg_exh_run(ssg_handle* h, const float* dQ, const float* qpad_in, const floa
     ExhArgs a1 = a;
     a1.m = m_pad;
     // Short slices spend most of stage 1 filling and compacting candidate lists from a
-    // threshold of -Inf.  Then each query's threshold starts at its k-th best score over
-    // 2048 rows spread over the database (a lower bound of the k-th best over all rows,
-    // less the rounding allowance), and the main pass keeps only rows above it.
+    // threshold of -Inf.  Then each query's threshold starts at a lower bound of its k-th
+    // best score over 2048 rows spread over the database (and so of the k-th best over all
+    // rows), less the rounding allowance, and the main pass keeps only rows above it.
     const int64_t rps = ((n_eff + ns - 1) / ns + 63) / 64 * 64;
     if (h->dp <= 128 && n_eff >= 32768 && rps < 32768) {
         HIPCHK(h->seed_tau.ensure((size_t)m_pad * sizeof(float)), "alloc seed");

@@ -106,9 +106,10 @@ struct ExhWork {
     const float* tau0;     // nullable, dp <= 128: per-query starting threshold (score units)
 };
 hipError_t exh_launch_stage1(const ExhArgs& a, const ExhWork& w, int nslices, hipStream_t s);
-// Starting thresholds: tau0[q] = query q's k-th best score among min(n, 2048) rows spread
-// over the database, scored directly in FP32 and lowered by the certificate's error
-// allowance (xnmax: the largest squared row norm); -Inf with fewer than k rows, and for
+// Starting thresholds: tau0[q] = a lower bound of query q's k-th best score among
+// min(n, 2048) rows spread over the database (the k-th largest of each lane's two best),
+// scored directly in FP32 and lowered by the certificate's error allowance (xnmax: the
+// largest squared row norm); -Inf with fewer than k rows, k outside 1 .. 128, and for
 // rows m_valid .. m_pad.
 hipError_t exh_launch_seed_sample(const float* X, const float* xnorm, int32_t dp, int64_t n,
                                   const ExhWork& w, int64_t m_valid, int64_t m_pad, int k, int metric,
