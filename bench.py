@@ -716,9 +716,12 @@ def exhaustive_fp32(args, compact=False):
                      "oracle_match_200": ok,
                      "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": MFMA_FP32_PEAK_TFLOPS,
                                   "unit": "TFLOP/s", "frac": round(tf / MFMA_FP32_PEAK_TFLOPS, 4),
-                                  "traffic": _traffic("exhaustive_fp32_1m128_traffic.json",
-                                                      "hbm_bytes_per_call",
-                                                      name == "gmm_1m128" and n == 1_000_000),
+                                  "traffic": (_traffic("exhaustive_fp32_1m128_traffic.json",
+                                                       "hbm_bytes_per_call",
+                                                       name == "gmm_1m128" and n == 1_000_000)
+                                              or _traffic("exhaustive_fp32_100k32_traffic.json",
+                                                          "hbm_bytes_per_call",
+                                                          name == "uniform_100k32" and n == 100_000)),
                                   "kernel": "exh_stage1_l<%d> + exh_stage2 (whole call)" % ((dim + 15) // 16 * 16),
                                   "kernel_ms": round(ms, 3), "flops_per_launch": 2.0 * dim * n * m}}
         if name == "uniform_100k32" and not args.no_cpu:
